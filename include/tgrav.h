@@ -1,0 +1,161 @@
+/*
+ * tgrav -- MI355X (gfx950) softened direct-sum gravity field, C ABI.
+ *
+ * Drop-in replacement for the CPU field evaluation behind Torch's gas/star
+ * gravity bridge.  Units are the caller's; the default constant is
+ * G = 4.300917270e-3 pc (km/s)^2 / Msun.  All arrays are FP64 SoA of equal
+ * length; ptr_kind says whether they are host or device (hipMalloc'd, e.g. a
+ * PyTorch-ROCm tensor's data_ptr()) pointers.
+ *
+ * Pair kernel (Plummer): eps2_ij = eps_s_j^2 + eps_t_i^2,
+ *   a_i   =  G sum_j m_j (x_j - x_i) (|x_j - x_i|^2 + eps2_ij)^(-3/2)
+ *   phi_i = -G sum_j m_j             (|x_j - x_i|^2 + eps2_ij)^(-1/2)
+ * A pair with |x_j - x_i|^2 + eps2_ij == 0 contributes nothing (self-exclusion).
+ *
+ * Conventions (mirroring the AMUSE legacy-function boundary):
+ *   - every entry point returns 0 on success and a negative TGRAV_E* code on
+ *     failure (AMUSE's ERROR_CODE, src/amuse/torch_amuse_flash/interface.py:1436-1440);
+ *     tgrav_last_error() then describes the failure (thread-local string);
+ *   - the caller owns all input/output memory; the context owns its scratch;
+ *   - a context is bound to one device and one HIP stream and is not thread-safe
+ *     (like the single-threaded AMUSE worker); separate contexts may run concurrently;
+ *   - with TGRAV_HOST pointers a call is synchronous (outputs are valid on return);
+ *     with TGRAV_DEVICE pointers work is enqueued on the context's stream and the
+ *     caller synchronises (tgrav_synchronize, or stream order).
+ */
+#ifndef TGRAV_H
+#define TGRAV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TGRAV_PLAIN_FP64 0   /* FP64 throughout; matches the FP64 oracle to <= 1e-12 */
+#define TGRAV_MIXED 1        /* FP64 displacements, FP32 bulk, compensated sums, FP64 kick */
+
+#define TGRAV_HOST 0
+#define TGRAV_DEVICE 1
+
+#define TGRAV_OK 0
+#define TGRAV_EINVAL -1      /* bad argument: n < 0, NULL required pointer, bad mode */
+#define TGRAV_ENOSOURCES -2  /* evaluation before tgrav_set_sources */
+#define TGRAV_EHIP -3        /* a HIP runtime call or kernel launch failed */
+#define TGRAV_ENONFINITE -4  /* validation found NaN/Inf in an input */
+#define TGRAV_ENOMEM -5      /* device allocation failed */
+#define TGRAV_ELIMIT -6      /* size beyond what this build supports */
+
+typedef struct tgrav_ctx tgrav_ctx;
+typedef struct tgrav_batch tgrav_batch;
+
+/* One problem of a batched evaluation (device pointers).  eps arrays may be NULL
+ * (= 0); any output may be NULL; v* may be NULL unless the batch kicks. */
+typedef struct tgrav_problem {
+  int64_t n_sources;
+  const double *sx, *sy, *sz, *sm, *se;
+  int64_t n_targets;
+  const double *tx, *ty, *tz, *te;
+  double *ax, *ay, *az, *phi;
+  double *vx, *vy, *vz;
+} tgrav_problem;
+
+/* ---- context ---------------------------------------------------------- */
+int tgrav_version(void);
+const char* tgrav_last_error(void);
+/* Replaces the worker start-up of the AMUSE plugin (interface.F90 initialize_code) for
+ * this path: binds a device, creates a default-priority stream. */
+int tgrav_create(tgrav_ctx** out, int device);
+int tgrav_destroy(tgrav_ctx* ctx);
+/* Run on an existing HIP stream (e.g. torch.cuda.current_stream().cuda_stream);
+ * NULL restores the context's own stream. */
+int tgrav_set_stream(tgrav_ctx* ctx, void* hip_stream);
+void* tgrav_get_stream(tgrav_ctx* ctx);
+int tgrav_synchronize(tgrav_ctx* ctx);
+/* Gravitational constant; the reference reads FLASH's `Newton`
+ * (Gravity_getAccelAtPoint.F90:134). */
+int tgrav_set_constant(tgrav_ctx* ctx, double G);
+/* Validation of inputs for NaN/Inf (default on).  With device pointers it costs a
+ * device->host round trip per call. */
+int tgrav_set_validation(tgrav_ctx* ctx, int on);
+/* Kernel configuration: variant index (see tgrav_variant_info) and the number of
+ * source chunks (0 = automatic).  Used by the autotuner; -1 leaves a field as is. */
+int tgrav_set_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
+int tgrav_num_variants(int mode);
+int tgrav_variant_info(int mode, int variant, int* targets_per_lane, int* threads,
+                       int* tile_sources);
+
+/* ---- the field -------------------------------------------------------- */
+/* Source set: the stellar side (stars, for the forward direction of
+ * get_gravity_at_point, src/torch_mainloop.py:128-140) or the gas cells as point
+ * masses m = rho dV (for the reverse direction, Gravity_getAccelAtPoint.F90:168-204).
+ * eps may be NULL (= 0).  The arrays are copied into the context (packed FP64). */
+int tgrav_set_sources(tgrav_ctx* ctx, int64_t n, const double* x, const double* y,
+                      const double* z, const double* m, const double* eps, int ptr_kind);
+
+/* Acceleration and potential at n points from the source set.  Replaces the per-point
+ * loops of get_accel_gas_on_particles (interface.F90:1130-1216) and
+ * get_potential_at_point (interface.F90:1073-1127) -> Gravity_getAccelAtPoint
+ * (Gravity_getAccelAtPoint.F90:32-419) / Gravity_getPotentialAtPoint
+ * (Gravity_getPotentialAtPoint.F90:33-305), and the stellar-side AMUSE
+ * get_gravity_at_point / get_potential_at_point(eps, x, y, z) (interface.py:221-243).
+ * eps_t is the per-point softening (may be NULL = 0); any of ax/ay/az/phi may be NULL
+ * (phi == NULL skips the potential). */
+int tgrav_eval(tgrav_ctx* ctx, int64_t n, const double* eps_t, const double* x,
+               const double* y, const double* z, double* ax, double* ay, double* az,
+               double* phi, int mode, int ptr_kind);
+
+/* Evaluation with the bridge kick fused: v += dt * a in FP64 (the gas kick of
+ * interface.F90:1793-1795 and the star kick of interface.F90:1432-1444; the bridge
+ * passes dt/2, src/torch_mainloop.py:287,289).  ax..phi may be NULL. */
+int tgrav_eval_kick(tgrav_ctx* ctx, int64_t n, const double* eps_t, const double* x,
+                    const double* y, const double* z, double dt, double* vx, double* vy,
+                    double* vz, double* ax, double* ay, double* az, double* phi, int mode,
+                    int ptr_kind);
+
+/* Standalone FP64 kick v += dt * a (.deprecated/bridge_multiples.py:612-616). */
+int tgrav_kick(tgrav_ctx* ctx, int64_t n, double dt, const double* ax, const double* ay,
+               const double* az, double* vx, double* vy, double* vz, int ptr_kind);
+
+/* Block-structured targets: nblk cubic leaf blocks of nb^3 cells, block b with lower
+ * corner (ox,oy,oz)[b] and cell width dx[b]; cell (i,j,k) centre o + (idx+1/2) dx,
+ * output index b*nb^3 + (i*nb + j)*nb + k (the meshgrid('ij')/ravel('C') order of
+ * .deprecated/bridge_multiples.py:566-571, FLASH's get_1blk_cell_coords,
+ * interface.F90:418-437).  eps_b is the per-block target softening (may be NULL).
+ * vx/vy/vz may be NULL (no kick) or per-cell velocities kicked by dt. */
+int tgrav_eval_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const double* ox,
+                      const double* oy, const double* oz, const double* dx,
+                      const double* eps_b, double* ax, double* ay, double* az, double* phi,
+                      double dt, double* vx, double* vy, double* vz, int mode, int ptr_kind);
+
+/* Fused bidirectional pass of Particles_sinkAccelGasOnSinksAndSinksOnGas
+ * (Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:47-411): gas cells (as point
+ * masses, softening ce) and particles (softening pe), symmetric eps2 = ce^2 + pe^2;
+ * returns the acceleration/potential of the cells from the particles and of the
+ * particles from the cells in one pass over the pairs.  Outputs may be NULL. */
+int tgrav_bidirectional(tgrav_ctx* ctx, int64_t nc, const double* cx, const double* cy,
+                        const double* cz, const double* cm, const double* ce, int64_t np,
+                        const double* px, const double* py, const double* pz,
+                        const double* pm, const double* pe, double* cax, double* cay,
+                        double* caz, double* cphi, double* pax, double* pay, double* paz,
+                        double* pphi, int mode, int ptr_kind);
+
+/* ---- batched uneven evaluations --------------------------------------- */
+/* Many independent problems in one persistent launch (device pointers only).  The plan
+ * packs the sources, builds a cost-sorted work list (target tile x source chunk) and
+ * owns the partial-sum scratch.  dt != 0 fuses the FP64 kick v += dt * a. */
+int tgrav_batch_create(tgrav_ctx* ctx, int32_t n_problems, const tgrav_problem* problems,
+                       int mode, double dt, tgrav_batch** out);
+/* Runs the plan.  use_graph != 0 captures the launch sequence (counter reset,
+ * persistent evaluation, chunk reduction + kick) in a hipGraph on first use and
+ * replays it afterwards. */
+int tgrav_batch_run(tgrav_batch* b, int use_graph);
+int tgrav_batch_destroy(tgrav_batch* b);
+/* Work-list statistics: items, total interactions. */
+int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGRAV_H */

@@ -1,0 +1,127 @@
+"""CPU oracle for the softened direct-sum gravity field -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker (or as the timed
+CPU baseline).  The product (``torch_amd``) never imports it.
+
+``liboracle.so`` is built from ``tgrav_oracle.c`` by ``oracle/Makefile``; each C
+function cites the reference file:line it restates.  ``oracle.brute`` holds a
+50-digit ``decimal`` brute force that pins the C restatement independently.
+
+Parity status of each function (what pins it is in tests/test_oracle.py):
+  eval_plummer    pinned: decimal brute force, closed forms, invariants
+  eval_flash      pinned: closed forms (r >= rsoft Newtonian, continuity at rsoft,
+                  r -> 0 limit, a = -grad phi in clean mode)
+  bidirectional   pinned: equals two eval_plummer calls; Newton's third law
+  kick            pinned: exact arithmetic on representable inputs
+  block_centres   pinned: numpy meshgrid('ij').ravel('C') of the same centres
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+SEQ, NEUMAIER, LONGDOUBLE = 0, 1, 2
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        i64, dbl, i32 = ctypes.c_int64, ctypes.c_double, ctypes.c_int
+        L.oracle_eval_plummer.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 4 + [dbl] + [_D] * 4 + [i32, i32]
+        L.oracle_eval_flash.argtypes = [i64] + [_D] * 4 + [i64] + [_D] * 3 + [dbl, dbl, i32, i32] + [_D] * 4 + [i32]
+        L.oracle_bidirectional.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 5 + [dbl] + [_D] * 8
+        L.oracle_kick.argtypes = [i64, dbl] + [_D] * 6
+        L.oracle_block_centres.argtypes = [i64, i32] + [_D] * 7
+        for f in (L.oracle_eval_plummer, L.oracle_eval_flash, L.oracle_bidirectional,
+                  L.oracle_kick, L.oracle_block_centres):
+            f.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _in(a, n=None):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.shape != (n,):
+        raise ValueError(f"expected shape ({n},), got {a.shape}")
+    return a, a.ctypes.data_as(_D)
+
+
+def _out(n):
+    a = np.zeros(n, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+def eval_plummer(sx, sy, sz, sm, se, tx, ty, tz, te, G, mode=SEQ, threads=1):
+    """(ax, ay, az, phi) at the targets from the sources (Plummer kernel)."""
+    ns, nt = len(sx), len(tx)
+    keep = [_in(a, ns) for a in (sx, sy, sz, sm, se)] + [_in(a, nt) for a in (tx, ty, tz, te)]
+    outs = [_out(nt) for _ in range(4)]
+    rc = lib().oracle_eval_plummer(ns, *[k[1] for k in keep[:5]], nt, *[k[1] for k in keep[5:]],
+                                   float(G), *[o[1] for o in outs], int(mode), int(threads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_eval_plummer failed ({rc})")
+    return tuple(o[0] for o in outs)
+
+
+def eval_flash(sx, sy, sz, sm, tx, ty, tz, G, rsoft, kernel="spline", quirks=False, threads=1):
+    ns, nt = len(sx), len(tx)
+    keep = [_in(a, ns) for a in (sx, sy, sz, sm)] + [_in(a, nt) for a in (tx, ty, tz)]
+    outs = [_out(nt) for _ in range(4)]
+    typ = {"spline": 1, "linear": 2}[kernel]
+    rc = lib().oracle_eval_flash(ns, *[k[1] for k in keep[:4]], nt, *[k[1] for k in keep[4:]],
+                                 float(G), float(rsoft), typ, int(bool(quirks)),
+                                 *[o[1] for o in outs], int(threads))
+    if rc != 0:
+        raise RuntimeError(f"oracle_eval_flash failed ({rc})")
+    return tuple(o[0] for o in outs)
+
+
+def bidirectional(cx, cy, cz, cm, ce, px, py, pz, pm, pe, G):
+    """Returns ((pax,pay,paz,pphi) on particles, (cax,cay,caz,cphi) on cells)."""
+    nc, np_ = len(cx), len(px)
+    keep = [_in(a, nc) for a in (cx, cy, cz, cm, ce)] + [_in(a, np_) for a in (px, py, pz, pm, pe)]
+    po = [_out(np_) for _ in range(4)]
+    co = [_out(nc) for _ in range(4)]
+    rc = lib().oracle_bidirectional(nc, *[k[1] for k in keep[:5]], np_, *[k[1] for k in keep[5:]],
+                                    float(G), *[o[1] for o in po], *[o[1] for o in co])
+    if rc != 0:
+        raise RuntimeError(f"oracle_bidirectional failed ({rc})")
+    return tuple(o[0] for o in po), tuple(o[0] for o in co)
+
+
+def kick(dt, ax, ay, az, vx, vy, vz):
+    """Returns new (vx, vy, vz) = v + dt*a."""
+    n = len(ax)
+    a = [_in(v, n) for v in (ax, ay, az)]
+    v = [np.array(q, dtype=np.float64, copy=True) for q in (vx, vy, vz)]
+    rc = lib().oracle_kick(n, float(dt), *[q[1] for q in a], *[q.ctypes.data_as(_D) for q in v])
+    if rc != 0:
+        raise RuntimeError("oracle_kick failed")
+    return tuple(v)
+
+
+def block_centres(ox, oy, oz, dx, nb=8):
+    nblk = len(ox)
+    keep = [_in(a, nblk) for a in (ox, oy, oz, dx)]
+    outs = [_out(nblk * nb ** 3) for _ in range(3)]
+    rc = lib().oracle_block_centres(nblk, nb, *[k[1] for k in keep], *[o[1] for o in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_block_centres failed")
+    return tuple(o[0] for o in outs)

@@ -1,0 +1,263 @@
+/*
+ * tgrav oracle -- CPU FP64 restatement of Torch's softened direct-sum gravity field.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product path (torch_amd/libtgrav.so)
+ * never calls into it.
+ *
+ * What it restates (all FP64, as FLASH builds promote `real` to double,
+ * support/Makefile.h.in:13):
+ *
+ *   oracle_eval_plummer   -- the direct-sum loop nest of
+ *       src/flash/source/physics/Gravity/GravityMain/Poisson/BHTree/Couple_AMUSE/
+ *           Gravity_getAccelAtPoint.F90:168-248   (acceleration, isolated branch)
+ *           Gravity_getPotentialAtPoint.F90:152-227 (potential, isolated branch)
+ *       targets outer, sources inner, one accumulator per output, with the
+ *       Plummer pair kernel used by the AMUSE stellar side (src/torch_mainloop.py:130,
+ *       epsilon_squared): eps2_ij = eps_s_j^2 + eps_t_i^2,
+ *           a_i   =  G sum_j m_j d_ij (|d_ij|^2 + eps2_ij)^(-3/2),   d_ij = x_j - x_i
+ *           phi_i = -G sum_j m_j      (|d_ij|^2 + eps2_ij)^(-1/2)
+ *       (the reference's prefactor = -G*m times (x - x_c)/r^3 is the same sign).
+ *       A pair with |d|^2 + eps2 == 0 contributes nothing (self-exclusion).
+ *   oracle_eval_flash     -- the FLASH spline / linear softening kernels with a global
+ *       softening radius (Gravity_getAccelAtPoint.F90:140-145, 206-241;
+ *       Gravity_getPotentialAtPoint.F90:190-222), with or without the reference's
+ *       potential quirks (wrong sign inside the radius, `pgpot = slope`).
+ *   oracle_bidirectional  -- the fused gas<->particle pass of
+ *       src/flash/source/Particles/ParticlesMain/active/Sink/
+ *           Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:204-316
+ *       (cells outer, particles inner, particle accelerations scattered in the
+ *       inner loop, cell acceleration stored per cell), Plummer kernel.
+ *   oracle_kick           -- v += dt * a  (interface.F90:1793-1795, 1432-1444;
+ *       src/torch_mainloop.py:287,289 call it with dt/2).
+ *   oracle_block_centres  -- implicit cell centres of 8^3 (nb^3) leaf blocks in the
+ *       C order of .deprecated/bridge_multiples.py:566-571 (meshgrid 'ij', ravel 'C').
+ *
+ * Accumulation modes: ORACLE_SEQ (plain sequential FP64, the reference's order),
+ * ORACLE_NEUMAIER (compensated FP64), ORACLE_LONGDOUBLE (x87 80-bit for the
+ * pair math and the sums; used to measure the FP64 paths' own rounding).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <omp.h>
+
+#define ORACLE_SEQ 0
+#define ORACLE_NEUMAIER 1
+#define ORACLE_LONGDOUBLE 2
+
+typedef struct { double s, c; } nsum;
+
+static inline void nadd(nsum* a, double v) {
+  double t = a->s + v;
+  if (fabs(a->s) >= fabs(v)) a->c += (a->s - t) + v;
+  else a->c += (v - t) + a->s;
+  a->s = t;
+}
+
+static inline double sq(double v) { return v * v; }
+
+int oracle_version(void) { return 3; }
+
+/* Plummer direct sum.  Any of se/te may be NULL (= 0).  Any of ax/ay/az/phi may be NULL. */
+int oracle_eval_plummer(int64_t ns, const double* sx, const double* sy, const double* sz,
+                        const double* sm, const double* se,
+                        int64_t nt, const double* tx, const double* ty, const double* tz,
+                        const double* te, double G,
+                        double* ax, double* ay, double* az, double* phi,
+                        int mode, int nthreads) {
+  if (ns < 0 || nt < 0) return -1;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  for (int64_t i = 0; i < nt; ++i) {
+    const double xi = tx[i], yi = ty[i], zi = tz[i];
+    const double e2t = te ? te[i] * te[i] : 0.0;
+    double gx = 0, gy = 0, gz = 0, gp = 0;
+    if (mode == ORACLE_SEQ) {
+      for (int64_t j = 0; j < ns; ++j) {
+        const double dx = sx[j] - xi, dy = sy[j] - yi, dz = sz[j] - zi;
+        const double e2 = (se ? se[j] * se[j] : 0.0) + e2t;
+        const double r2 = dx * dx + dy * dy + dz * dz + e2;
+        if (r2 == 0.0) continue;
+        const double rinv = 1.0 / sqrt(r2);
+        const double mr = sm[j] * rinv;
+        const double w = mr * rinv * rinv;
+        gx += w * dx; gy += w * dy; gz += w * dz; gp += mr;
+      }
+    } else if (mode == ORACLE_NEUMAIER) {
+      nsum sx_ = {0, 0}, sy_ = {0, 0}, sz_ = {0, 0}, sp_ = {0, 0};
+      for (int64_t j = 0; j < ns; ++j) {
+        const double dx = sx[j] - xi, dy = sy[j] - yi, dz = sz[j] - zi;
+        const double e2 = (se ? se[j] * se[j] : 0.0) + e2t;
+        const double r2 = dx * dx + dy * dy + dz * dz + e2;
+        if (r2 == 0.0) continue;
+        const double rinv = 1.0 / sqrt(r2);
+        const double mr = sm[j] * rinv;
+        const double w = mr * rinv * rinv;
+        nadd(&sx_, w * dx); nadd(&sy_, w * dy); nadd(&sz_, w * dz); nadd(&sp_, mr);
+      }
+      gx = sx_.s + sx_.c; gy = sy_.s + sy_.c; gz = sz_.s + sz_.c; gp = sp_.s + sp_.c;
+    } else {
+      long double lx = 0, ly = 0, lz = 0, lp = 0;
+      const long double e2tl = te ? (long double)te[i] * (long double)te[i] : 0.0L;
+      for (int64_t j = 0; j < ns; ++j) {
+        const long double dx = (long double)sx[j] - xi, dy = (long double)sy[j] - yi,
+                          dz = (long double)sz[j] - zi;
+        const long double e2 = (se ? (long double)se[j] * (long double)se[j] : 0.0L) + e2tl;
+        const long double r2 = dx * dx + dy * dy + dz * dz + e2;
+        if (r2 == 0.0L) continue;
+        const long double rinv = 1.0L / sqrtl(r2);
+        const long double mr = (long double)sm[j] * rinv;
+        const long double w = mr * rinv * rinv;
+        lx += w * dx; ly += w * dy; lz += w * dz; lp += mr;
+      }
+      const long double Gl = G;
+      if (ax) ax[i] = (double)(Gl * lx);
+      if (ay) ay[i] = (double)(Gl * ly);
+      if (az) az[i] = (double)(Gl * lz);
+      if (phi) phi[i] = (double)(-Gl * lp);
+      continue;
+    }
+    if (ax) ax[i] = G * gx;
+    if (ay) ay[i] = G * gy;
+    if (az) az[i] = G * gz;
+    if (phi) phi[i] = -G * gp;
+  }
+  return 0;
+}
+
+/* FLASH softening kernels with the global radius `rsoft` (Gravity_getAccelAtPoint.F90:140-145).
+ * type 1 = spline, 2 = linear.  quirks != 0 reproduces the reference potential as written:
+ * the spline branch's pot_kernel (with its -q^4/10 term) multiplied by prefactor = -G m
+ * (Gravity_getPotentialAtPoint.F90:203-211,225) and `pgpot = slope` in the linear branch
+ * (:215-217).  quirks == 0 uses the standard M4-spline potential (Price & Monaghan 2007,
+ * eq. A2 with h = rsoft/2) and the potential of the linear force, (r^2 - 3 rsoft^2)/(2 rsoft^3),
+ * both continuous with -1/r at r = rsoft.  The reference leaves the spline force kernel
+ * unassigned for q <= 1e-5 (a stale value from an earlier pair); both modes use its r -> 0
+ * limit there, h^-3 (4/3 - 6/5 q^2 + 1/2 q^3), times d (so a coincident pair adds 0).
+ * Sum order and sign follow Gravity_getAccelAtPoint.F90:212-246: d = x_t - x_c,
+ * g += prefactor * k * d with prefactor = -G m. */
+int oracle_eval_flash(int64_t ns, const double* sx, const double* sy, const double* sz,
+                      const double* sm, int64_t nt, const double* tx, const double* ty,
+                      const double* tz, double G, double rsoft, int type, int quirks,
+                      double* ax, double* ay, double* az, double* phi, int nthreads) {
+  if (ns < 0 || nt < 0 || !(rsoft > 0) || (type != 1 && type != 2)) return -1;
+  if (nthreads < 1) nthreads = 1;
+  const double hinv = 2.0 / rsoft, h2inv = hinv * hinv, slope = 1.0 / (rsoft * rsoft * rsoft);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  for (int64_t i = 0; i < nt; ++i) {
+    double gx = 0, gy = 0, gz = 0, gp = 0;
+    for (int64_t j = 0; j < ns; ++j) {
+      const double pref = -G * sm[j];
+      const double dx = tx[i] - sx[j], dy = ty[i] - sy[j], dz = tz[i] - sz[j];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      double kv, pk;
+      if (r < rsoft) {
+        if (type == 1) {
+          const double q = r * hinv;
+          if (q < 1.0) {
+            if (q > 1.0e-5) kv = h2inv * (4.0 / 3.0 * q - 1.2 * q * q * q + 0.5 * q * q * q * q) / r;
+            else kv = h2inv * hinv * (4.0 / 3.0 - 1.2 * q * q + 0.5 * q * q * q);
+            if (quirks)
+              pk = hinv * (2.0 / 3.0 * q * q - 0.1 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);
+            else
+              pk = -hinv * (2.0 / 3.0 * q * q - 0.3 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);
+          } else {
+            kv = h2inv * (8.0 / 3.0 * q - 3.0 * q * q + 1.2 * q * q * q - 1.0 / 6.0 * pow(q, 4) -
+                          1.0 / (15.0 * q * q)) / r;
+            pk = hinv * (4.0 / 3.0 * q * q - q * q * q + 0.3 * pow(q, 4) - 1.0 / 30.0 * pow(q, 5) -
+                         8.0 / 5.0 + 1.0 / (15.0 * q));
+            if (!quirks) pk = -pk;
+          }
+        } else {
+          kv = slope;
+          pk = quirks ? slope : -(r * r - 3.0 * rsoft * rsoft) / (2.0 * rsoft * rsoft * rsoft);
+        }
+      } else {
+        kv = 1.0 / (r * r * r);
+        pk = 1.0 / r;
+      }
+      gx += pref * (kv * dx); gy += pref * (kv * dy); gz += pref * (kv * dz);
+      gp += pref * pk;
+    }
+    if (ax) ax[i] = gx;
+    if (ay) ay[i] = gy;
+    if (az) az[i] = gz;
+    if (phi) phi[i] = gp;
+  }
+  return 0;
+}
+
+/* Fused bidirectional pass (Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:204-316), Plummer
+ * kernel with the symmetric eps2 = eps_c^2 + eps_p^2.  Cells outer, particles inner; the
+ * particle accelerations are scattered in the inner loop exactly like
+ * particles_global(accxProp,p) there (:296-301), so this restatement is serial.
+ * Outputs: (pax,pay,paz,pphi) on particles from cells, (cax,cay,caz,cphi) on cells from
+ * particles.  Any output may be NULL. */
+int oracle_bidirectional(int64_t nc, const double* cx, const double* cy, const double* cz,
+                         const double* cm, const double* ce,
+                         int64_t np, const double* px, const double* py, const double* pz,
+                         const double* pm, const double* pe, double G,
+                         double* pax, double* pay, double* paz, double* pphi,
+                         double* cax, double* cay, double* caz, double* cphi) {
+  if (nc < 0 || np < 0) return -1;
+  for (int64_t p = 0; p < np; ++p) {
+    if (pax) pax[p] = 0; if (pay) pay[p] = 0; if (paz) paz[p] = 0; if (pphi) pphi[p] = 0;
+  }
+  for (int64_t c = 0; c < nc; ++c) {
+    double gx = 0, gy = 0, gz = 0, gp = 0;
+    const double e2c = ce ? ce[c] * ce[c] : 0.0;
+    for (int64_t p = 0; p < np; ++p) {
+      const double dx = px[p] - cx[c], dy = py[p] - cy[c], dz = pz[p] - cz[c];
+      const double r2 = dx * dx + dy * dy + dz * dz + e2c + (pe ? pe[p] * pe[p] : 0.0);
+      if (r2 == 0.0) continue;
+      const double rinv = 1.0 / sqrt(r2);
+      const double r3 = rinv * rinv * rinv;
+      /* cell -> particle: d points from the particle to the cell, i.e. -d */
+      const double fc = G * cm[c] * r3;
+      if (pax) pax[p] -= fc * dx;
+      if (pay) pay[p] -= fc * dy;
+      if (paz) paz[p] -= fc * dz;
+      if (pphi) pphi[p] -= G * cm[c] * rinv;
+      const double wp = pm[p] * r3;
+      gx += wp * dx; gy += wp * dy; gz += wp * dz; gp += pm[p] * rinv;
+    }
+    if (cax) cax[c] = G * gx;
+    if (cay) cay[c] = G * gy;
+    if (caz) caz[c] = G * gz;
+    if (cphi) cphi[c] = -G * gp;
+  }
+  return 0;
+}
+
+/* v += dt * a, FP64 (the bridge's half kick uses dt/2). */
+int oracle_kick(int64_t n, double dt, const double* ax, const double* ay, const double* az,
+                double* vx, double* vy, double* vz) {
+  if (n < 0) return -1;
+  for (int64_t i = 0; i < n; ++i) {
+    vx[i] += dt * ax[i];
+    vy[i] += dt * ay[i];
+    vz[i] += dt * az[i];
+  }
+  return 0;
+}
+
+/* Cell centres of nblk cubic leaf blocks of nb^3 cells: block b has lower corner
+ * (ox[b],oy[b],oz[b]) and cell width dx[b]; cell (i,j,k) has centre o + (idx + 1/2) dx and
+ * flat index b*nb^3 + (i*nb + j)*nb + k (x slowest, z fastest). */
+int oracle_block_centres(int64_t nblk, int nb, const double* ox, const double* oy,
+                         const double* oz, const double* dx, double* x, double* y, double* z) {
+  if (nblk < 0 || nb <= 0) return -1;
+  const int64_t nc = (int64_t)nb * nb * nb;
+  for (int64_t b = 0; b < nblk; ++b)
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j)
+        for (int k = 0; k < nb; ++k) {
+          const int64_t t = b * nc + ((int64_t)i * nb + j) * nb + k;
+          x[t] = ox[b] + (i + 0.5) * dx[b];
+          y[t] = oy[b] + (j + 0.5) * dx[b];
+          z[t] = oz[b] + (k + 0.5) * dx[b];
+        }
+  return 0;
+}

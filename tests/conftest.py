@@ -1,0 +1,18 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box only)")
+
+
+@pytest.fixture(scope="session")
+def G():
+    from torch_amd.workloads import G_PC_MSUN_KMS
+    return G_PC_MSUN_KMS

@@ -1,0 +1,94 @@
+"""The C ABI library (CPU-side checks; no compute calls without a GPU).
+
+* libtgrav.so loads and exports every entry point include/tgrav.h declares;
+* argument errors are reported through return codes and tgrav_last_error before any
+  device work (AMUSE's ERROR_CODE convention);
+* the Python mirror exposes the reference's method names with the reference's argument
+  order (interface.py:221-243, 548-559).
+"""
+import ctypes
+import inspect
+import os
+import re
+
+import pytest
+
+from torch_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tgrav.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tgrav_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(names) == bound, set(names) ^ bound
+
+
+def test_problem_struct_matches_header():
+    src = open(HEADER).read()
+    body = src[src.index("typedef struct tgrav_problem"):src.index("} tgrav_problem;")]
+    fields = re.findall(r"\b(n_sources|n_targets|[a-z]+)\s*[,;]", body.split("{", 1)[1])
+    assert [f for f, _ in _lib.Problem._fields_] == fields
+    assert ctypes.sizeof(_lib.Problem) == 8 * len(fields)
+
+
+def test_version_and_variants():
+    lib = _lib.load()
+    assert lib.tgrav_version() >= 1
+    for mode in (_lib.PLAIN_FP64, _lib.MIXED):
+        n = lib.tgrav_num_variants(mode)
+        assert n >= 4
+        k, nt, ts = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        for v in range(n):
+            assert lib.tgrav_variant_info(mode, v, ctypes.byref(k), ctypes.byref(nt), ctypes.byref(ts)) == 0
+            assert k.value >= 1 and nt.value % 64 == 0 and ts.value % nt.value == 0
+        assert lib.tgrav_variant_info(mode, n, None, None, None) == _lib.EINVAL
+    assert lib.tgrav_num_variants(7) == _lib.EINVAL
+    assert b"bad mode" in lib.tgrav_last_error()
+
+
+def test_null_context_and_arguments_are_rejected():
+    lib = _lib.load()
+    assert lib.tgrav_create(None, 0) == _lib.EINVAL
+    assert b"out is NULL" in lib.tgrav_last_error()
+    assert lib.tgrav_set_sources(None, 1, None, None, None, None, None, 0) == _lib.EINVAL
+    assert lib.tgrav_eval(None, 1, None, None, None, None, None, None, None, None, 0, 0) == _lib.EINVAL
+    assert lib.tgrav_set_constant(None, 1.0) == _lib.EINVAL
+    assert lib.tgrav_destroy(None) == 0
+    with pytest.raises(_lib.TgravError) as ei:
+        _lib.check(lib.tgrav_set_validation(None, 1), "tgrav_set_validation")
+    assert ei.value.code == _lib.EINVAL and "context is NULL" in str(ei.value)
+
+
+def test_create_without_a_device_fails_cleanly():
+    """Here (no GPU) tgrav_create must return an error code, not crash."""
+    lib = _lib.load()
+    ctx = ctypes.c_void_p()
+    rc = lib.tgrav_create(ctypes.byref(ctx), 0)
+    if rc == 0:
+        lib.tgrav_destroy(ctx)
+        pytest.skip("a GPU is present")
+    assert rc in (_lib.EHIP, _lib.EINVAL)
+    assert ctx.value is None
+    assert lib.tgrav_last_error()
+
+
+def test_python_mirror_keeps_reference_method_names():
+    from torch_amd import GasGravityField, StellarGravityField
+    for cls, meth in ((StellarGravityField, "get_gravity_at_point"),
+                      (StellarGravityField, "get_potential_at_point"),
+                      (GasGravityField, "get_accel_gas_on_particles"),
+                      (GasGravityField, "get_potential_at_point")):
+        params = list(inspect.signature(getattr(cls, meth)).parameters)
+        assert params == ["self", "eps", "x", "y", "z"], (cls, meth, params)

@@ -1,0 +1,341 @@
+"""GPU parity of libtgrav.so against the CPU oracle (needs an MI355X).
+
+Bars: plain FP64 within 1e-12 normwise relative error of the oracle per target (or, for
+the brute-force KATs with cancellation, within the condition-aware FP64 summation bound);
+mixed precision <= 1e-6 median and <= 1e-4 max relative acceleration error against the
+FP64 oracle.  Integer-exact properties (kick, block centres, host vs device pointers,
+determinism) are checked bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import _util
+from tests.test_oracle import kat_error_ok, load_kats
+from torch_amd import GasGravityField, GravityField, StellarGravityField, TgravError, _lib
+from torch_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = W.G_PC_MSUN_KMS
+THREADS = 16
+
+
+@pytest.fixture(scope="module")
+def fp64():
+    f = GravityField("fp64")
+    yield f
+    f.close()
+
+
+@pytest.fixture(scope="module")
+def mixed():
+    f = GravityField("mixed")
+    yield f
+    f.close()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return np.load(os.path.join(HERE, "golden", "c1_oracle.npz"))
+
+
+@pytest.fixture(scope="module")
+def c2():
+    return W.c2(1002)
+
+
+def gpu_eval(f, S, T, eps_t=None):
+    f.set_sources(S["x"], S["y"], S["z"], S["m"], S.get("eps"))
+    return f.evaluate(eps_t, T["x"], T["y"], T["z"])
+
+
+def stars(d, pre):
+    return dict(x=d[pre + "x"], y=d[pre + "y"], z=d[pre + "z"], m=d[pre + "m"], eps=d[pre + "eps"])
+
+
+# ------------------------------------------------------------------ KATs ----
+@pytest.mark.parametrize("mode,k", [("fp64", 64.0), ("mixed", None)])
+def test_brute_force_kats(mode, k):
+    kats, Gk = load_kats()
+    with GravityField(mode, G=Gk) as f:
+        for i, c in enumerate(kats):
+            s, t = c["s"], c["t"]
+            f.set_sources(s[:, 0], s[:, 1], s[:, 2], c["m"], c["se"])
+            got = np.stack(f.evaluate(c["te"], t[:, 0], t[:, 1], t[:, 2]), 1)
+            if k is not None:
+                ok, worst = kat_error_ok(got, c, Gk, k)
+                assert ok, f"case {i}: {worst:.2f}x the FP64 bound"
+            else:
+                # mixed: relative to the sum of |terms| (its condition-aware scale)
+                ok, worst = kat_error_ok(got, c, Gk, 2.0 ** 53 * 2e-6)
+                assert ok, f"case {i}: {worst:.2f}x the mixed bound"
+
+
+def test_single_pair_sweep_fp64(fp64):
+    """One source, targets over 12 decades of distance: the FP64 rsqrt refinement keeps each
+    pair within a few ulp of the closed form."""
+    r = np.geomspace(1e-6, 1e6, 4001)
+    fp64.set_sources([0.0], [0.0], [0.0], [1.5], [0.0])
+    ax, ay, az, phi = fp64.evaluate(0.0, r, np.zeros_like(r), np.zeros_like(r))
+    np.testing.assert_allclose(ax, -G * 1.5 / r ** 2, rtol=8e-16 * 4, atol=0)
+    np.testing.assert_allclose(phi, -G * 1.5 / r, rtol=8e-16 * 4, atol=0)
+    assert np.all(ay == 0) and np.all(az == 0)
+
+
+# ------------------------------------------------------------------- c1 ----
+def test_c1_fp64_matches_oracle(fp64, c1):
+    S, C = stars(c1, "star_"), stars(c1, "cell_")
+    fwd = gpu_eval(fp64, S, C)
+    s = _util.summary(fwd, tuple(c1["fwd"]))
+    print("c1 fwd fp64", s, flush=True)
+    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+    rev = gpu_eval(fp64, C, S)
+    s = _util.summary(rev, tuple(c1["rev"]))
+    print("c1 rev fp64", s, flush=True)
+    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+
+
+def test_c1_mixed_error(mixed, c1):
+    S, C = stars(c1, "star_"), stars(c1, "cell_")
+    for name, got, ref in (("fwd", gpu_eval(mixed, S, C), c1["fwd"]),
+                           ("rev", gpu_eval(mixed, C, S), c1["rev"])):
+        s = _util.summary(got, tuple(ref))
+        print(f"c1 {name} mixed", s, flush=True)
+        assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+        assert s["pot_median"] <= 1e-6 and s["pot_max"] <= 1e-4
+
+
+# ------------------------------------------------------------------- c2 ----
+def test_c2_fp64_matches_oracle_on_subsample(fp64, c2):
+    st, ce = c2["stars"], c2["cells"]
+    fwd = gpu_eval(fp64, st, ce)
+    idx = _util.subsample(len(ce["x"]), 2048, 21)
+    ref = oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], st["eps"], ce["x"][idx], ce["y"][idx],
+                              ce["z"][idx], None, G, mode=oracle.NEUMAIER, threads=THREADS)
+    s = _util.summary(tuple(a[idx] for a in fwd), ref)
+    print("c2 fwd fp64", s, flush=True)
+    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+    rev = gpu_eval(fp64, ce, st)
+    idx = _util.subsample(len(st["x"]), 384, 22)
+    idx = np.union1d(idx, np.where(st["binary"] >= 0)[0][:128])
+    ref = oracle.eval_plummer(ce["x"], ce["y"], ce["z"], ce["m"], ce["eps"], st["x"][idx], st["y"][idx],
+                              st["z"][idx], None, G, mode=oracle.NEUMAIER, threads=THREADS)
+    s = _util.summary(tuple(a[idx] for a in rev), ref)
+    print("c2 rev fp64", s, flush=True)
+    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+
+
+def test_c2_mixed_against_fp64(fp64, mixed, c2):
+    """Full-size mixed error against the (oracle-pinned) FP64 kernel; binary members apart."""
+    st, ce = c2["stars"], c2["cells"]
+    for name, S, T in (("fwd", st, ce), ("rev", ce, st)):
+        ref = gpu_eval(fp64, S, T)
+        got = gpu_eval(mixed, S, T)
+        s = _util.summary(got, ref)
+        print(f"c2 {name} mixed", s, flush=True)
+        assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+        if name == "rev":
+            b = st["binary"] >= 0
+            sb = _util.summary(tuple(a[b] for a in got), tuple(a[b] for a in ref))
+            print("c2 rev mixed binary members", sb, flush=True)
+            assert sb["acc_median"] <= 1e-6 and sb["acc_max"] <= 1e-4
+
+
+# ------------------------------------------------------------ edge cases ----
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("ns,nt", [(1, 1), (3, 63), (257, 65), (1000, 1), (5, 1029), (777, 2049)])
+def test_ragged_sizes(mode, ns, nt):
+    rng = W.rng_for(ns * 7 + nt)
+    S = dict(x=rng.normal(size=ns), y=rng.normal(size=ns), z=rng.normal(size=ns),
+             m=rng.uniform(0.1, 2, ns), eps=rng.uniform(0.001, 0.05, ns))
+    t = rng.normal(size=(3, nt))
+    te = rng.uniform(0, 0.02, nt)
+    with GravityField(mode) as f:
+        got = gpu_eval(f, S, dict(x=t[0], y=t[1], z=t[2]), te)
+    ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], t[0], t[1], t[2], te, G,
+                              mode=oracle.NEUMAIER)
+    s = _util.summary(got, ref)
+    if mode == "fp64":
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, s
+    else:
+        assert s["acc_max"] <= 1e-4 and s["pot_max"] <= 1e-4, s
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_self_exclusion_and_zero_softening(mode):
+    """eps = 0 and coincident source/target: the pair contributes nothing."""
+    x = np.array([0.0, 1.0, 0.0, 0.3])
+    y = np.array([0.0, 0.0, 2.0, -0.4])
+    z = np.zeros(4)
+    m = np.array([1.0, 2.0, 3.0, 0.5])
+    with GravityField(mode) as f:
+        f.set_sources(x, y, z, m, None)
+        got = f.evaluate(None, x, y, z)
+    ref = oracle.eval_plummer(x, y, z, m, None, x, y, z, None, G, mode=oracle.LONGDOUBLE)
+    for g, r in zip(got, ref):
+        assert np.all(np.isfinite(g))
+        np.testing.assert_allclose(g, r, rtol=1e-12 if mode == "fp64" else 1e-6, atol=0)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_empty_inputs(mode):
+    with GravityField(mode) as f:
+        f.set_sources([], [], [], [], None)
+        ax, ay, az, phi = f.evaluate(0.0, np.ones(5), np.ones(5), np.ones(5))
+        assert np.all(ax == 0) and np.all(phi == 0)
+        f.set_sources([0.0], [0.0], [0.0], [1.0], [0.1])
+        out = f.evaluate(None, np.zeros(0), np.zeros(0), np.zeros(0))
+        assert all(o.shape == (0,) for o in out)
+
+
+def test_error_codes():
+    f = GravityField("fp64")
+    with pytest.raises(TgravError) as e:
+        f.evaluate(0.0, [0.0], [0.0], [0.0])
+    assert e.value.code == _lib.ENOSOURCES
+    with pytest.raises(TgravError) as e:
+        f.set_sources([0.0, np.nan], [0.0, 0.0], [0.0, 0.0], [1.0, 1.0])
+    assert e.value.code == _lib.ENONFINITE
+    f.set_sources([0.0], [0.0], [0.0], [1.0])
+    with pytest.raises(TgravError) as e:
+        f.evaluate(0.0, [np.inf], [0.0], [0.0])
+    assert e.value.code == _lib.ENONFINITE
+    with pytest.raises(TgravError) as e:
+        f.evaluate(0.0, [0.0, 1.0], [0.0], [0.0])
+    assert e.value.code == _lib.EINVAL
+    lib = _lib.load()
+    assert lib.tgrav_eval(f.handle, 1, None, None, None, None, None, None, None, None, 0, 0) == _lib.EINVAL
+    assert lib.tgrav_eval(f.handle, 0, None, None, None, None, None, None, None, None, 5, 0) == _lib.EINVAL
+    assert lib.tgrav_eval(f.handle, -1, None, None, None, None, None, None, None, None, 0, 0) == _lib.EINVAL
+    f.close()
+
+
+# --------------------------------------------------- launch-shape choices ----
+def test_chunked_sources_match_single_pass(fp64, c2):
+    """Splitting the source range over workgroups (chunk sums + ordered reduction) agrees
+    with one pass per target to rounding."""
+    ce, st = c2["cells"], c2["stars"]
+    sl = slice(0, 200_000)
+    S = {k: ce[k][sl] for k in ("x", "y", "z", "m", "eps")}
+    T = {k: st[k][:3000] for k in ("x", "y", "z")}
+    outs = []
+    for nsplit in (1, 0, 7, 64):
+        fp64.set_config(nsplit=nsplit)
+        outs.append(gpu_eval(fp64, S, T))
+    fp64.set_config(nsplit=0)
+    for o in outs[1:]:
+        s = _util.summary(o, outs[0])
+        assert s["acc_max"] <= 1e-13 and s["pot_max"] <= 1e-13, s
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_every_variant_agrees(mode, c2):
+    st, ce = c2["stars"], c2["cells"]
+    T = {k: ce[k][:40_000] for k in ("x", "y", "z")}
+    ref = oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], st["eps"], T["x"][:1024], T["y"][:1024],
+                              T["z"][:1024], None, G, mode=oracle.NEUMAIER, threads=THREADS)
+    lib = _lib.load()
+    with GravityField(mode) as f:
+        for v in range(lib.tgrav_num_variants(_lib.PLAIN_FP64 if mode == "fp64" else _lib.MIXED)):
+            f.set_config(variant=v)
+            got = gpu_eval(f, st, T)
+            s = _util.summary(tuple(a[:1024] for a in got), ref)
+            print(mode, "variant", v, s, flush=True)
+            if mode == "fp64":
+                assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+            else:
+                assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_deterministic(mode, c2):
+    st, ce = c2["stars"], c2["cells"]
+    T = {k: ce[k][:100_000] for k in ("x", "y", "z")}
+    with GravityField(mode) as f:
+        a = gpu_eval(f, st, T)
+        b = gpu_eval(f, st, T)
+    for p, q in zip(a, b):
+        assert np.array_equal(p, q)
+
+
+def test_device_pointers_match_host_pointers(fp64, c2):
+    import torch
+    st, ce = c2["stars"], c2["cells"]
+    T = {k: ce[k][:50_000] for k in ("x", "y", "z")}
+    host = gpu_eval(fp64, st, T)
+    d = lambda a: torch.as_tensor(a, device="cuda")  # noqa: E731
+    fp64.set_sources(d(st["x"]), d(st["y"]), d(st["z"]), d(st["m"]), d(st["eps"]))
+    dev = fp64.evaluate(None, d(T["x"]), d(T["y"]), d(T["z"]))
+    torch.cuda.synchronize()
+    for p, q in zip(host, dev):
+        assert np.array_equal(p, q.cpu().numpy())
+
+
+# ------------------------------------------------------------- kick/blocks ----
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_fused_kick_equals_eval_then_kick(mode, c2):
+    st, ce = c2["stars"], c2["cells"]
+    dt = 0.5 * 1.3e-3
+    with GasGravityField(mode) as f:
+        f.set_sources(ce["x"], ce["y"], ce["z"], ce["m"], ce["eps"])
+        vx, vy, vz = st["vx"].copy(), st["vy"].copy(), st["vz"].copy()
+        a = f.evaluate_and_kick(None, st["x"], st["y"], st["z"], dt, vx, vy, vz)
+        b = f.evaluate(None, st["x"], st["y"], st["z"])
+        for p, q in zip(a[:3], b[:3]):
+            assert np.array_equal(p, q)
+        ex = oracle.kick(dt, b[0], b[1], b[2], st["vx"], st["vy"], st["vz"])
+        for p, q in zip((vx, vy, vz), ex):
+            assert np.array_equal(p, q)
+        # standalone kick
+        wx, wy, wz = st["vx"].copy(), st["vy"].copy(), st["vz"].copy()
+        f.kick(dt, b[0], b[1], b[2], wx, wy, wz)
+        for p, q in zip((wx, wy, wz), ex):
+            assert np.array_equal(p, q)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_block_targets_equal_explicit_centres(mode, c2):
+    st, ce = c2["stars"], c2["cells"]
+    o, size = ce["block_origin"][:300], ce["block_size"][:300]
+    dx = size / 8
+    bx, by, bz = oracle.block_centres(o[:, 0], o[:, 1], o[:, 2], dx)
+    with StellarGravityField(mode) as f:
+        f.set_stars(st["x"], st["y"], st["z"], st["m"], st["eps"])
+        blk = f.evaluate_blocks(o, dx, eps=dx)
+        exp = f.evaluate(np.repeat(dx, 512), bx, by, bz)
+        for p, q in zip(blk, exp):
+            assert np.array_equal(p, q)
+        v = [np.zeros(len(bx)) for _ in range(3)]
+        f.evaluate_blocks(o, dx, eps=dx, dt=0.25, vel=v)
+        for p, q in zip(v, exp[:3]):
+            assert np.array_equal(p, 0.25 * q)
+
+
+# ---------------------------------------------------- full-size properties ----
+def test_third_law_at_c2_scale(fp64, c2):
+    """With symmetric softening (stars eps_*, cells eps = dx), the stars' total force on the
+    gas cancels the gas's total force on the stars (the reference's force_error check,
+    interface.F90:1467-1522) -- over all 1.02e10 pairs."""
+    st, ce = c2["stars"], c2["cells"]
+    a_c = gpu_eval(fp64, st, ce, eps_t=ce["eps"])
+    a_s = gpu_eval(fp64, ce, st, eps_t=st["eps"])
+    for q in range(3):
+        f_c = np.sum(ce["m"] * a_c[q])
+        f_s = np.sum(st["m"] * a_s[q])
+        scale = np.sum(np.abs(ce["m"] * a_c[q])) + np.sum(np.abs(st["m"] * a_s[q]))
+        print("third law", q, f_c, f_s, scale, flush=True)
+        assert abs(f_c + f_s) <= 1e-12 * scale
+
+
+def test_superposition_at_c2_scale(fp64, c2):
+    st, ce = c2["stars"], c2["cells"]
+    full = gpu_eval(fp64, ce, st)
+    half = len(ce["x"]) // 2
+    parts = [gpu_eval(fp64, {k: ce[k][sl] for k in ("x", "y", "z", "m", "eps")}, st)
+             for sl in (slice(0, half), slice(half, None))]
+    summed = tuple(p + q for p, q in zip(*parts))
+    s = _util.summary(summed, full)
+    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, s

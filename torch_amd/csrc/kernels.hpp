@@ -1,0 +1,504 @@
+// Device code of the direct-sum field: target loaders, the per-tile interaction loops
+// (FP64 and mixed precision), the per-workgroup driver shared by the grid kernels and
+// the persistent batch kernel, and the small elementwise kernels around them.
+//
+// Execution model (gfx950, wave64):
+//   * a workgroup of NT lanes owns a target tile of K*NT targets (K per lane, held in
+//     VGPRs; target t = tile0 + k*NT + lane, so every load/store is coalesced);
+//   * sources stream through LDS in tiles of NT (one source staged per lane per tile);
+//     every lane then reads the same LDS word (broadcast, no bank conflicts) and updates
+//     its K targets;
+//   * the source range may be a chunk of the full set (split over blockIdx.y or over
+//     work items); chunk partial sums then go to a scratch array and a second kernel
+//     sums them in fixed order, so results are deterministic (no float atomics).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+// ----------------------------------------------------------------- descriptors ----
+struct Tgt {
+  const double *x, *y, *z, *eps;           // explicit targets (eps may be null)
+  const double *ox, *oy, *oz, *bdx, *beps; // block-structured targets
+  int64_t n;
+  int nb;
+  int blocks;
+};
+
+struct Out {
+  double *ax, *ay, *az, *phi;   // final outputs (any may be null)
+  double *vx, *vy, *vz;         // fused kick v += dt * a (null = no kick)
+  double dt;
+  double *part;                 // non-null: write unscaled chunk sums to part[q*pstride + t]
+  int64_t pstride;
+  double G;
+};
+
+// Cell centre of block-structured target t: o + (idx + 1/2) dx with the multiply and the
+// add rounded separately (no FMA contraction) so the centres are bit-identical to the
+// host-side construction (numpy / C: o + (i + 0.5) * dx).
+__device__ __forceinline__ void load_target(const Tgt& T, int64_t t, double& x, double& y,
+                                            double& z, double& e2) {
+  if (!T.blocks) {
+    x = T.x[t];
+    y = T.y[t];
+    z = T.z[t];
+    const double e = T.eps ? T.eps[t] : 0.0;
+    e2 = e * e;
+  } else {
+    const int nb = T.nb;
+    const int64_t nb3 = (int64_t)nb * nb * nb;
+    const int64_t b = t / nb3;
+    const int c = (int)(t - b * nb3);
+    const int i = c / (nb * nb), j = (c / nb) % nb, k = c % nb;
+    const double d = T.bdx[b];
+    x = __dadd_rn(T.ox[b], __dmul_rn((double)i + 0.5, d));
+    y = __dadd_rn(T.oy[b], __dmul_rn((double)j + 0.5, d));
+    z = __dadd_rn(T.oz[b], __dmul_rn((double)k + 0.5, d));
+    const double e = T.beps ? T.beps[b] : 0.0;
+    e2 = e * e;
+  }
+}
+
+// Min/max over the workgroup of NV values per lane: xor-shuffle within the wave, then
+// one LDS slot per wave.  `red` holds 2*NV*(NT/64) doubles and is private to this call.
+template <int NT, int NV>
+__device__ __forceinline__ void block_minmax(double (&mn)[NV], double (&mx)[NV], double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      mn[v] = fmin(mn[v], __shfl_xor(mn[v], off));
+      mx[v] = fmax(mx[v], __shfl_xor(mx[v], off));
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      red[(w * NV + v) * 2] = mn[v];
+      red[(w * NV + v) * 2 + 1] = mx[v];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    mn[v] = red[v * 2];
+    mx[v] = red[v * 2 + 1];
+  }
+#pragma unroll
+  for (int w2 = 1; w2 < NT / 64; ++w2) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      mn[v] = fmin(mn[v], red[(w2 * NV + v) * 2]);
+      mx[v] = fmax(mx[v], red[(w2 * NV + v) * 2 + 1]);
+    }
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------------- FP64 tile loop ----
+// 1/sqrt(r2) in FP64: the hardware estimate (v_rsq_f64) refined by one third-order
+// Newton/Householder step  y <- y + y e (1/2 + 3/8 e),  e = 1 - r2 y^2  (the Taylor
+// series of (1-e)^(-1/2); an estimate with relative error d leaves ~2.5 d^3).
+template <bool GUARD>
+__device__ __forceinline__ double rsqrt_f64(double r2) {
+  double y = __builtin_amdgcn_rsq(r2);
+  if (GUARD) y = r2 > 0.0 ? y : 0.0;   // |d|^2 + eps^2 == 0: the pair contributes 0
+  const double t = r2 * y;
+  const double e = fma(-t, y, 1.0);
+  const double p = fma(e, 0.375, 0.5);
+  const double q = y * e;
+  return fma(q, p, y);
+}
+
+template <int K, int NT, bool GUARD, bool UNIF>
+__device__ __forceinline__ void fp64_tile(const double4* __restrict__ s4,
+                                          const double* __restrict__ se2,
+                                          const double (&tx)[K], const double (&ty)[K],
+                                          const double (&tz)[K], const double (&te2)[K],
+                                          double (&ax)[K], double (&ay)[K], double (&az)[K],
+                                          double (&ap)[K]) {
+#pragma unroll 2
+  for (int j = 0; j < NT; ++j) {
+    const double4 s = s4[j];
+    const double e2 = se2[j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double dx = s.x - tx[k], dy = s.y - ty[k], dz = s.z - tz[k];
+      const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, UNIF ? e2 : e2 + te2[k])));
+      const double y = rsqrt_f64<GUARD>(r2);
+      const double mr = s.w * y;
+      ap[k] += mr;
+      const double w = mr * (y * y);
+      ax[k] = fma(w, dx, ax[k]);
+      ay[k] = fma(w, dy, ay[k]);
+      az[k] = fma(w, dz, az[k]);
+    }
+  }
+}
+
+// ------------------------------------------------------ mixed-precision tile loop ----
+template <int K, int NT, bool GUARD, bool UNIF>
+__device__ __forceinline__ void f32_tile(const float4* __restrict__ s4,
+                                         const float* __restrict__ se2, int j0, int j1,
+                                         const float (&tx)[K], const float (&ty)[K],
+                                         const float (&tz)[K], const float (&te2)[K],
+                                         float (&ax)[K], float (&ay)[K], float (&az)[K],
+                                         float (&ap)[K]) {
+#pragma unroll 4
+  for (int j = j0; j < j1; ++j) {
+    const float4 s = s4[j];
+    const float e2 = se2[j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const float dx = s.x - tx[k], dy = s.y - ty[k], dz = s.z - tz[k];
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, UNIF ? e2 : e2 + te2[k])));
+      float y = __builtin_amdgcn_rsqf(r2);
+      if (GUARD) y = r2 > 0.0f ? y : 0.0f;
+      const float mr = s.w * y;
+      ap[k] += mr;
+      const float w = (mr * y) * y;
+      ax[k] = fmaf(w, dx, ax[k]);
+      ay[k] = fmaf(w, dy, ay[k]);
+      az[k] = fmaf(w, dz, az[k]);
+    }
+  }
+}
+
+// Knuth two-sum: (hi, lo) += v with the rounding error of hi + v kept in lo.
+__device__ __forceinline__ void two_sum(float& hi, float& lo, float v) {
+  const float s = hi + v;
+  const float bp = s - hi;
+  const float err = (hi - (s - bp)) + (v - bp);
+  hi = s;
+  lo += err;
+}
+
+// ------------------------------------------------------------------ LDS layout ----
+template <int NT>
+struct LdsF64 {
+  double4 s4[NT];
+  double se2[NT];
+  double red[2 * 4 * (NT / 64)];
+};
+template <int NT>
+struct LdsF32 {
+  float4 s4[NT];
+  float se2[NT];
+  double red[2 * 4 * (NT / 64)];
+};
+
+// Threshold on the smallest possible eps^2 of a tile below which the FP32 loop guards
+// r2 == 0 (and keeps rsq far from overflow); FP64 guards only an exact zero.
+#define TG_F32_TINY_E2 1e-30
+
+__device__ __forceinline__ void store_result(const Out& O, int64_t t, double sx, double sy,
+                                             double sz, double sp) {
+  if (O.part) {
+    O.part[t] = sx;
+    O.part[O.pstride + t] = sy;
+    O.part[2 * O.pstride + t] = sz;
+    O.part[3 * O.pstride + t] = sp;
+    return;
+  }
+  const double gx = O.G * sx, gy = O.G * sy, gz = O.G * sz;
+  if (O.ax) O.ax[t] = gx;
+  if (O.ay) O.ay[t] = gy;
+  if (O.az) O.az[t] = gz;
+  if (O.phi) O.phi[t] = -O.G * sp;
+  if (O.vx) {
+    O.vx[t] += O.dt * gx;
+    O.vy[t] += O.dt * gy;
+    O.vz[t] += O.dt * gz;
+  }
+}
+
+// ------------------------------------------------------- per-workgroup drivers ----
+template <int K, int NT, bool GUARD, bool UNIF>
+__device__ __forceinline__ void fp64_sources(LdsF64<NT>& L, const double4* __restrict__ src4,
+                                             const double* __restrict__ srce2, int64_t s_begin,
+                                             int64_t s_end, double e2u, const double (&tx)[K],
+                                             const double (&ty)[K], const double (&tz)[K],
+                                             const double (&te2)[K], double (&ax)[K],
+                                             double (&ay)[K], double (&az)[K], double (&ap)[K]) {
+  const int tid = threadIdx.x;
+  for (int64_t base = s_begin; base < s_end; base += NT) {
+    __syncthreads();
+    const int64_t j = base + tid;
+    if (j < s_end) {
+      L.s4[tid] = src4[j];
+      L.se2[tid] = srce2[j] + e2u;
+    } else {   // padding source: zero mass, eps^2 = 1 -> contributes exactly 0
+      L.s4[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+      L.se2[tid] = 1.0;
+    }
+    __syncthreads();
+    fp64_tile<K, NT, GUARD, UNIF>(L.s4, L.se2, tx, ty, tz, te2, ax, ay, az, ap);
+  }
+}
+
+template <int K, int NT>
+__device__ void process_fp64(LdsF64<NT>& L, const Tgt& T, int64_t tile0,
+                             const double4* __restrict__ src4, const double* __restrict__ srce2,
+                             int64_t s_begin, int64_t s_end, double e2s_min, const Out& O) {
+  const int tid = threadIdx.x;
+  double tx[K], ty[K], tz[K], te2[K];
+  double mn[1] = {INFINITY}, mx[1] = {-INFINITY};
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    load_target(T, t < T.n ? t : T.n - 1, tx[k], ty[k], tz[k], te2[k]);
+    if (t < T.n) {
+      mn[0] = fmin(mn[0], te2[k]);
+      mx[0] = fmax(mx[0], te2[k]);
+    }
+  }
+  block_minmax<NT, 1>(mn, mx, L.red);
+  const bool unif = mn[0] == mx[0];
+  const bool guard = !(e2s_min + mn[0] > 0.0);
+  const double e2u = unif ? mn[0] : 0.0;
+  double ax[K], ay[K], az[K], ap[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+  if (unif) {
+    if (guard) fp64_sources<K, NT, true, true>(L, src4, srce2, s_begin, s_end, e2u, tx, ty, tz, te2, ax, ay, az, ap);
+    else fp64_sources<K, NT, false, true>(L, src4, srce2, s_begin, s_end, e2u, tx, ty, tz, te2, ax, ay, az, ap);
+  } else {
+    if (guard) fp64_sources<K, NT, true, false>(L, src4, srce2, s_begin, s_end, 0.0, tx, ty, tz, te2, ax, ay, az, ap);
+    else fp64_sources<K, NT, false, false>(L, src4, srce2, s_begin, s_end, 0.0, tx, ty, tz, te2, ax, ay, az, ap);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    if (t < T.n) store_result(O, t, ax[k], ay[k], az[k], ap[k]);
+  }
+}
+
+// Mixed precision.  The tile's FP64 anchor c is the centre of its targets' bounding box.
+// Sources are staged as fp32(x_s - c) (the FP64 difference rounded once), targets are held
+// as fp32(x_t - c); the FP32 difference of the two is exact whenever they are within a
+// factor of two of each other (Sterbenz), i.e. for exactly the close pairs where a
+// difference of raw FP32 coordinates would cancel catastrophically.  FP32 partial sums
+// over FOLD sources are folded into per-target two-sum (hi, lo) accumulators.
+template <int K, int NT, int FOLD, bool GUARD, bool UNIF>
+__device__ __forceinline__ void f32_sources(LdsF32<NT>& L, const double4* __restrict__ src4,
+                                            const double* __restrict__ srce2, int64_t s_begin,
+                                            int64_t s_end, double e2u, double cx, double cy,
+                                            double cz, const float (&tx)[K], const float (&ty)[K],
+                                            const float (&tz)[K], const float (&te2)[K],
+                                            float (&hx)[K], float (&hy)[K], float (&hz)[K],
+                                            float (&hp)[K], float (&lx)[K], float (&ly)[K],
+                                            float (&lz)[K], float (&lp)[K]) {
+  const int tid = threadIdx.x;
+  for (int64_t base = s_begin; base < s_end; base += NT) {
+    __syncthreads();
+    const int64_t j = base + tid;
+    if (j < s_end) {
+      const double4 s = src4[j];
+      L.s4[tid] = make_float4((float)(s.x - cx), (float)(s.y - cy), (float)(s.z - cz), (float)s.w);
+      L.se2[tid] = (float)(srce2[j] + e2u);
+    } else {
+      L.s4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+      L.se2[tid] = 1.0f;
+    }
+    __syncthreads();
+    const int64_t rem = s_end - base;
+    const int jn = rem < NT ? (int)rem : NT;
+    for (int j0 = 0; j0 < jn; j0 += FOLD) {
+      float ax[K], ay[K], az[K], ap[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
+      f32_tile<K, NT, GUARD, UNIF>(L.s4, L.se2, j0, j0 + FOLD, tx, ty, tz, te2, ax, ay, az, ap);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        two_sum(hx[k], lx[k], ax[k]);
+        two_sum(hy[k], ly[k], ay[k]);
+        two_sum(hz[k], lz[k], az[k]);
+        two_sum(hp[k], lp[k], ap[k]);
+      }
+    }
+  }
+}
+
+template <int K, int NT, int FOLD>
+__device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
+                              const double4* __restrict__ src4, const double* __restrict__ srce2,
+                              int64_t s_begin, int64_t s_end, double e2s_min, const Out& O) {
+  const int tid = threadIdx.x;
+  double px[K], py[K], pz[K], pe2[K];
+  double mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  double mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    load_target(T, t < T.n ? t : T.n - 1, px[k], py[k], pz[k], pe2[k]);
+    if (t < T.n) {
+      mn[0] = fmin(mn[0], px[k]); mx[0] = fmax(mx[0], px[k]);
+      mn[1] = fmin(mn[1], py[k]); mx[1] = fmax(mx[1], py[k]);
+      mn[2] = fmin(mn[2], pz[k]); mx[2] = fmax(mx[2], pz[k]);
+      mn[3] = fmin(mn[3], pe2[k]); mx[3] = fmax(mx[3], pe2[k]);
+    }
+  }
+  block_minmax<NT, 4>(mn, mx, L.red);
+  const double cx = 0.5 * (mn[0] + mx[0]), cy = 0.5 * (mn[1] + mx[1]), cz = 0.5 * (mn[2] + mx[2]);
+  const bool unif = mn[3] == mx[3];
+  const bool guard = !(e2s_min + mn[3] > TG_F32_TINY_E2);
+  const double e2u = unif ? mn[3] : 0.0;
+  float tx[K], ty[K], tz[K], te2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    tx[k] = (float)(px[k] - cx);
+    ty[k] = (float)(py[k] - cy);
+    tz[k] = (float)(pz[k] - cz);
+    te2[k] = (float)pe2[k];
+  }
+  float hx[K], hy[K], hz[K], hp[K], lx[K], ly[K], lz[K], lp[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = hp[k] = lx[k] = ly[k] = lz[k] = lp[k] = 0.f;
+  if (unif) {
+    if (guard) f32_sources<K, NT, FOLD, true, true>(L, src4, srce2, s_begin, s_end, e2u, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
+    else f32_sources<K, NT, FOLD, false, true>(L, src4, srce2, s_begin, s_end, e2u, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
+  } else {
+    if (guard) f32_sources<K, NT, FOLD, true, false>(L, src4, srce2, s_begin, s_end, 0.0, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
+    else f32_sources<K, NT, FOLD, false, false>(L, src4, srce2, s_begin, s_end, 0.0, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    if (t < T.n)
+      store_result(O, t, (double)hx[k] + (double)lx[k], (double)hy[k] + (double)ly[k],
+                   (double)hz[k] + (double)lz[k], (double)hp[k] + (double)lp[k]);
+  }
+}
+
+// ------------------------------------------------------------------ grid kernels ----
+// blockIdx.x = target tile, blockIdx.y = source chunk [c*chunk, min((c+1)*chunk, ns)).
+template <int K, int NT>
+__global__ __launch_bounds__(NT) void k_eval_fp64(Tgt T, const double4* __restrict__ src4,
+                                                  const double* __restrict__ srce2, int64_t ns,
+                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+  __shared__ LdsF64<NT> L;
+  const double e2s_min = *e2s_min_p;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  process_fp64<K, NT>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
+}
+
+template <int K, int NT, int FOLD>
+__global__ __launch_bounds__(NT) void k_eval_mixed(Tgt T, const double4* __restrict__ src4,
+                                                   const double* __restrict__ srce2, int64_t ns,
+                                                   int64_t chunk, const double* e2s_min_p, Out O) {
+  __shared__ LdsF32<NT> L;
+  const double e2s_min = *e2s_min_p;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  process_mixed<K, NT, FOLD>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
+}
+
+// Sum of the nsplit chunk partials of each target in chunk order, then the G scaling,
+// the stores and the fused kick.
+__global__ void k_reduce_chunks(const double* __restrict__ part, int64_t pstride, int nsplit,
+                                int64_t n, Out O) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int c = 0; c < nsplit; ++c) {
+    const double* p = part + (int64_t)c * 4 * pstride;
+    s0 += p[t];
+    s1 += p[pstride + t];
+    s2 += p[2 * pstride + t];
+    s3 += p[3 * pstride + t];
+  }
+  Out o = O;
+  o.part = nullptr;
+  store_result(o, t, s0, s1, s2, s3);
+}
+
+// Pack SoA FP64 sources into {x,y,z,m} + eps^2 and fold the smallest eps^2 into
+// stat[0] (as the bits of a non-negative double, whose unsigned-integer order is the
+// numeric order, so one vector atomicMin per workgroup suffices).  With `check`,
+// stat[1] becomes non-zero if any input is NaN/Inf.  stat must be reset first
+// (k_reset_stats).
+__global__ void k_pack_sources(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, const double* __restrict__ m,
+                               const double* __restrict__ eps, double4* __restrict__ s4,
+                               double* __restrict__ se2, unsigned long long* stat, int check) {
+  __shared__ unsigned long long smin[256];
+  __shared__ int sbad[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long mine = 0x7ff0000000000000ULL;   // +inf
+  int bad = 0;
+  if (i < n) {
+    const double4 v = make_double4(x[i], y[i], z[i], m[i]);
+    const double e = eps ? eps[i] : 0.0;
+    s4[i] = v;
+    const double e2 = e * e;
+    se2[i] = e2;
+    mine = (unsigned long long)__double_as_longlong(e2);
+    if (check)
+      bad = !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w) && isfinite(e));
+  }
+  smin[threadIdx.x] = mine;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
+      sbad[threadIdx.x] |= sbad[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMin(&stat[0], smin[0]);
+    if (sbad[0]) atomicOr(&stat[1], 1ULL);
+  }
+}
+
+__global__ void k_reset_stats(unsigned long long* stat) {
+  if (threadIdx.x == 0) {
+    stat[0] = 0x7ff0000000000000ULL;
+    stat[1] = 0ULL;
+  }
+}
+
+// NaN/Inf check of up to five target arrays (any may be null) into flag[0].
+__global__ void k_check_finite(int64_t n, const double* c0, const double* c1, const double* c2,
+                               const double* c3, const double* c4, unsigned long long* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* cs[5] = {c0, c1, c2, c3, c4};
+  int bad = 0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+    if (cs[q] && !isfinite(cs[q][i])) bad = 1;
+  if (bad) atomicOr(flag, 1ULL);
+}
+
+__global__ void k_kick(int64_t n, double dt, const double* __restrict__ ax,
+                       const double* __restrict__ ay, const double* __restrict__ az,
+                       double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] += dt * ax[i];
+  vy[i] += dt * ay[i];
+  vz[i] += dt * az[i];
+}
+
+__global__ void k_fill(int64_t n, double v, double* a, double* b, double* c, double* d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (a) a[i] = v;
+  if (b) b[i] = v;
+  if (c) c[i] = v;
+  if (d) d[i] = v;
+}
+
+}  // namespace tg

@@ -1,0 +1,300 @@
+"""Host-side mirror of the reference's field-evaluation interface.
+
+The two sides of Torch's gravity bridge expose the direct-sum field under AMUSE
+legacy-function names (all FP64 arrays of one length, ``ERROR_CODE`` on failure):
+
+* the stellar side (the AMUSE N-body code, src/torch_mainloop.py:128-140):
+  ``get_gravity_at_point(eps, x, y, z) -> (ax, ay, az)`` and
+  ``get_potential_at_point(eps, x, y, z) -> phi``
+  (spec shape of src/amuse/torch_amuse_flash/interface.py:221-243, 1436-1446);
+  sources are the stars -> :class:`StellarGravityField`;
+* the gas side (the FLASH worker): ``get_accel_gas_on_particles(eps, x, y, z)`` and
+  ``get_potential_at_point(eps, x, y, z)`` (interface.py:221-231, 548-559;
+  interface.F90:1073-1216); sources are the leaf-block cells as point masses
+  m = rho dV -> :class:`GasGravityField`.
+
+Every evaluation runs on the GPU through libtgrav.so.  Inputs may be numpy arrays (host
+pointers; the call is synchronous and returns numpy arrays) or float64 torch tensors on a
+ROCm device (device pointers; work is enqueued on torch's current stream and torch
+tensors are returned).  Scalars are broadcast.  A non-zero library return code raises
+:class:`TgravError`, the way AMUSE raises on a non-zero ERROR_CODE.
+
+Softening: eps2_ij = eps_source_j^2 + eps_i^2 where eps_i is the per-point ``eps``
+argument (in the reference's FLASH exports ``eps`` is accepted but unused and a global
+radius applies, Gravity_getAccelAtPoint.F90:89-145; here it is honoured).
+Units are those of the inputs; the default G = 4.300917270e-3 pc (km/s)^2 / Msun.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import DEVICE, HOST, MIXED, PLAIN_FP64, TgravError
+
+G_PC_MSUN_KMS = 4.300917270e-3
+MODES = {"fp64": PLAIN_FP64, "plain_fp64": PLAIN_FP64, "mixed": MIXED}
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class _Arrays:
+    """Normalises a group of same-length inputs to one backend (numpy host / torch device)."""
+
+    def __init__(self, n, device_like=None):
+        self.n = n
+        self.dev = device_like
+        self.keep = []
+
+    def inp(self, a, name, allow_none=False):
+        if a is None:
+            if allow_none:
+                return None
+            raise TgravError(_lib.EINVAL, "argument", f"{name} is required")
+        if self.dev is not None:
+            import torch
+            if not torch.is_tensor(a):
+                a = torch.as_tensor(np.asarray(a, dtype=np.float64), device=self.dev)
+            a = a.to(device=self.dev, dtype=torch.float64)
+            if a.dim() == 0:
+                a = a.expand(self.n)
+            a = a.contiguous().reshape(-1)
+        else:
+            a = np.asarray(a, dtype=np.float64)
+            if a.ndim == 0:
+                a = np.full(self.n, float(a))
+            a = np.ascontiguousarray(a).reshape(-1)
+        if a.shape[0] != self.n:
+            raise TgravError(_lib.EINVAL, "argument",
+                             f"{name} has length {a.shape[0]}, expected {self.n}")
+        self.keep.append(a)
+        return a
+
+    def out(self):
+        if self.dev is not None:
+            import torch
+            a = torch.empty(self.n, dtype=torch.float64, device=self.dev)
+        else:
+            a = np.empty(self.n, dtype=np.float64)
+        self.keep.append(a)
+        return a
+
+    @staticmethod
+    def ptr(a):
+        if a is None:
+            return None
+        if _is_torch(a):
+            return ctypes.c_void_p(a.data_ptr())
+        return ctypes.c_void_p(a.ctypes.data)
+
+    @property
+    def kind(self):
+        return DEVICE if self.dev is not None else HOST
+
+
+def _length(*arrs):
+    for a in arrs:
+        if a is not None and np.ndim(a) > 0:
+            return int(a.shape[0]) if hasattr(a, "shape") else len(a)
+    return 1
+
+
+def _torch_device(*arrs):
+    for a in arrs:
+        if a is not None and _is_torch(a) and a.is_cuda:
+            return a.device
+    return None
+
+
+class GravityField:
+    """A source set on one GPU and direct-sum evaluations against it.
+
+    mode: "fp64" (plain FP64, matches the FP64 oracle to <= 1e-12) or "mixed"
+    (FP64 displacements about a per-tile anchor, FP32 rsqrt/FMA bulk, two-sum
+    compensated sums, FP64 result and kick).
+    """
+
+    def __init__(self, mode="fp64", device=0, G=G_PC_MSUN_KMS, validate=True):
+        if mode not in MODES:
+            raise ValueError(f"mode must be one of {sorted(MODES)}")
+        self.lib = _lib.load()
+        self.mode = MODES[mode]
+        self.device = device
+        ctx = ctypes.c_void_p()
+        _lib.check(self.lib.tgrav_create(ctypes.byref(ctx), int(device)), "tgrav_create")
+        self._ctx = ctx
+        self.G = G
+        _lib.check(self.lib.tgrav_set_constant(ctx, float(G)), "tgrav_set_constant")
+        _lib.check(self.lib.tgrav_set_validation(ctx, int(bool(validate))), "tgrav_set_validation")
+        self.n_sources = None
+
+    # -- lifetime ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self.lib.tgrav_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def _bind_stream(self, arrays):
+        if arrays.kind == DEVICE:
+            import torch
+            s = torch.cuda.current_stream(arrays.dev).cuda_stream
+            _lib.check(self.lib.tgrav_set_stream(self._ctx, ctypes.c_void_p(s)), "tgrav_set_stream")
+        else:
+            _lib.check(self.lib.tgrav_set_stream(self._ctx, None), "tgrav_set_stream")
+
+    def set_config(self, variant=-1, nsplit=-1, mode=None):
+        m = self.mode if mode is None else MODES[mode]
+        _lib.check(self.lib.tgrav_set_config(self._ctx, m, int(variant), int(nsplit)),
+                   "tgrav_set_config")
+
+    # -- sources ----------------------------------------------------------------
+    def set_sources(self, x, y, z, mass, eps=None):
+        n = _length(x, y, z, mass)
+        A = _Arrays(n, _torch_device(x, y, z, mass, eps))
+        xs = [A.inp(x, "x"), A.inp(y, "y"), A.inp(z, "z"), A.inp(mass, "mass"),
+              A.inp(eps, "eps", allow_none=True)]
+        self._bind_stream(A)
+        rc = self.lib.tgrav_set_sources(self._ctx, n, *[A.ptr(a) for a in xs], A.kind)
+        _lib.check(rc, "tgrav_set_sources")
+        self.n_sources = n
+
+    # -- evaluation ---------------------------------------------------------------
+    def evaluate(self, eps, x, y, z, acceleration=True, potential=True):
+        """(ax, ay, az, phi) at the points; unrequested outputs are None."""
+        n = _length(x, y, z, eps)
+        A = _Arrays(n, _torch_device(x, y, z, eps))
+        e = A.inp(eps, "eps", allow_none=True)
+        xs = [A.inp(x, "x"), A.inp(y, "y"), A.inp(z, "z")]
+        outs = [A.out() if acceleration else None for _ in range(3)] + \
+               [A.out() if potential else None]
+        self._bind_stream(A)
+        rc = self.lib.tgrav_eval(self._ctx, n, A.ptr(e), *[A.ptr(a) for a in xs],
+                                 *[A.ptr(o) for o in outs], self.mode, A.kind)
+        _lib.check(rc, "tgrav_eval")
+        return tuple(outs)
+
+    def evaluate_and_kick(self, eps, x, y, z, dt, vx, vy, vz, potential=False):
+        """v += dt * a(x) in FP64, fused with the evaluation; returns (ax, ay, az, phi) and
+        updates vx/vy/vz in place (numpy float64 arrays or float64 device tensors)."""
+        n = _length(x, y, z, eps)
+        A = _Arrays(n, _torch_device(x, y, z, eps, vx))
+        e = A.inp(eps, "eps", allow_none=True)
+        xs = [A.inp(x, "x"), A.inp(y, "y"), A.inp(z, "z")]
+        for v, name in ((vx, "vx"), (vy, "vy"), (vz, "vz")):
+            self._check_inplace(v, n, name, A)
+        outs = [A.out() for _ in range(3)] + [A.out() if potential else None]
+        self._bind_stream(A)
+        rc = self.lib.tgrav_eval_kick(self._ctx, n, A.ptr(e), *[A.ptr(a) for a in xs], float(dt),
+                                      A.ptr(vx), A.ptr(vy), A.ptr(vz), *[A.ptr(o) for o in outs],
+                                      self.mode, A.kind)
+        _lib.check(rc, "tgrav_eval_kick")
+        return tuple(outs)
+
+    def evaluate_blocks(self, origin, dx, nb=8, eps=None, potential=True, dt=0.0, vel=None):
+        """Field at the nb^3 cell centres of leaf blocks (lower corners origin[n,3], cell
+        width dx[n]); outputs in C order b*nb^3 + (i*nb + j)*nb + k.  vel=(vx,vy,vz)
+        per-cell arrays are kicked by dt in place."""
+        nblk = int(origin.shape[0])
+        ncell = nblk * nb ** 3
+        A = _Arrays(nblk, _torch_device(origin, dx, eps))
+        ox, oy, oz = (A.inp(origin[:, q], "origin") for q in range(3))
+        d = A.inp(dx, "dx")
+        e = A.inp(eps, "eps", allow_none=True)
+        B = _Arrays(ncell, A.dev)
+        outs = [B.out() for _ in range(3)] + [B.out() if potential else None]
+        v = [None, None, None]
+        if vel is not None:
+            for q, name in enumerate(("vx", "vy", "vz")):
+                self._check_inplace(vel[q], ncell, name, B)
+                v[q] = vel[q]
+        self._bind_stream(A)
+        rc = self.lib.tgrav_eval_blocks(self._ctx, nblk, int(nb), A.ptr(ox), A.ptr(oy), A.ptr(oz),
+                                        A.ptr(d), A.ptr(e), *[B.ptr(o) for o in outs], float(dt),
+                                        *[B.ptr(q) for q in v], self.mode, A.kind)
+        _lib.check(rc, "tgrav_eval_blocks")
+        return tuple(outs)
+
+    def kick(self, dt, ax, ay, az, vx, vy, vz):
+        """v += dt * a in FP64, in place (the bridge passes dt/2)."""
+        n = _length(ax)
+        A = _Arrays(n, _torch_device(ax, vx))
+        a = [A.inp(ax, "ax"), A.inp(ay, "ay"), A.inp(az, "az")]
+        for v, name in ((vx, "vx"), (vy, "vy"), (vz, "vz")):
+            self._check_inplace(v, n, name, A)
+        self._bind_stream(A)
+        rc = self.lib.tgrav_kick(self._ctx, n, float(dt), *[A.ptr(q) for q in a],
+                                 A.ptr(vx), A.ptr(vy), A.ptr(vz), A.kind)
+        _lib.check(rc, "tgrav_kick")
+
+    @staticmethod
+    def _check_inplace(v, n, name, A):
+        if A.kind == DEVICE:
+            import torch
+            ok = torch.is_tensor(v) and v.dtype == torch.float64 and v.is_contiguous() \
+                and v.device == A.dev and v.numel() == n
+        else:
+            ok = isinstance(v, np.ndarray) and v.dtype == np.float64 and \
+                v.flags.c_contiguous and v.size == n and v.flags.writeable
+        if not ok:
+            raise TgravError(_lib.EINVAL, "argument",
+                             f"{name} must be a writable contiguous float64 array of length {n} "
+                             "on the same device as the positions")
+
+    def synchronize(self):
+        _lib.check(self.lib.tgrav_synchronize(self._ctx), "tgrav_synchronize")
+
+
+class StellarGravityField(GravityField):
+    """Stars as sources: the stellar side's AMUSE GravityFieldInterface methods."""
+
+    def set_stars(self, x, y, z, mass, eps=None):
+        self.set_sources(x, y, z, mass, eps)
+
+    def get_gravity_at_point(self, eps, x, y, z):
+        ax, ay, az, _ = self.evaluate(eps, x, y, z, potential=False)
+        return ax, ay, az
+
+    def get_potential_at_point(self, eps, x, y, z):
+        return self.evaluate(eps, x, y, z, acceleration=False)[3]
+
+
+class GasGravityField(GravityField):
+    """Gas cells as point masses (m = rho dV, Gravity_getAccelAtPoint.F90:188-204) as
+    sources: the FLASH worker's exports."""
+
+    def set_cells(self, x, y, z, rho, dx, eps=None):
+        """Cells with centre (x,y,z), density rho and width dx; softening eps (default:
+        the cell width)."""
+        mass = rho * dx * dx * dx
+        self.set_sources(x, y, z, mass, dx if eps is None else eps)
+
+    def set_blocks(self, origin, dx, rho, nb=8, eps=None):
+        """Leaf blocks (lower corners origin[n,3], cell width dx[n]) with densities
+        rho[n*nb^3] in C order; softening eps per cell (default: the cell width)."""
+        from .blocks import block_centres
+        x, y, z, dxc = block_centres(origin, dx, nb)
+        self.set_cells(x, y, z, rho, dxc, eps)
+
+    def get_accel_gas_on_particles(self, eps, x, y, z):
+        ax, ay, az, _ = self.evaluate(eps, x, y, z, potential=False)
+        return ax, ay, az
+
+    def get_potential_at_point(self, eps, x, y, z):
+        return self.evaluate(eps, x, y, z, acceleration=False)[3]
