@@ -25,3 +25,41 @@ def summary(got, ref):
 def subsample(n, k, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     return np.sort(rng.choice(n, size=min(k, n), replace=False))
+
+
+def mixed_error_bound(s, m, se, t, te, G, fold=64):
+    """Analytic per-target bound of the mixed-precision kernel's error when all targets share
+    one tile (anchor c = centre of their bounding box):
+
+    * coordinates are stored as fp32(x - c), each off by <= 2^-24 |x - c|, so a displacement
+      is off by delta_ij <= 2^-24 (|x_s - c| + |x_t - c|) (the FP32 difference itself is exact
+      for close pairs, Sterbenz, and within 2^-24 |d| otherwise);
+    * the Plummer acceleration kernel m d / r^3 has Jacobian norm <= 2 m / r^3 and the potential
+      gradient m |d| / r^3 <= m / r^2;
+    * FP32 rsqrt/products/fold sums add <= (8 + fold) 2^-24 of the sum of |terms|.
+    Returns (acc_bound[nt], pot_bound[nt])."""
+    u = 2.0 ** -24
+    c = 0.5 * (t.min(0) + t.max(0))
+    d = s[None, :, :] - t[:, None, :]
+    dd = (d ** 2).sum(-1)
+    r2 = dd + (se[None, :] ** 2 if se is not None else 0) + (te[:, None] ** 2 if te is not None else 0)
+    ok = r2 > 0
+    r = np.sqrt(np.where(ok, r2, 1.0))
+    delta = u * (np.linalg.norm(s - c, axis=1)[None, :] + np.linalg.norm(t - c, axis=1)[:, None]) + \
+        u * np.sqrt(dd)
+    gm = G * m[None, :]
+    acc = np.where(ok, gm * (2.0 / r ** 3 * delta + (8 + fold) * u * np.sqrt(dd) / r ** 3), 0).sum(1)
+    pot = np.where(ok, gm * (delta / r ** 2 + (8 + fold) * u / r), 0).sum(1)
+    return acc, pot
+
+
+def within_mixed_bound(got, exp, s, m, se, t, te, G):
+    ab, pb = mixed_error_bound(s, m, se, t, te, G)
+    ga = np.stack(got[:3], 1) if isinstance(got, tuple) else got[:, :3]
+    ea = np.stack(exp[:3], 1) if isinstance(exp, tuple) else exp[:, :3]
+    gp = got[3] if isinstance(got, tuple) else got[:, 3]
+    ep = exp[3] if isinstance(exp, tuple) else exp[:, 3]
+    err_a = np.linalg.norm(ga - ea, axis=1)
+    err_p = np.abs(gp - ep)
+    ratio = max(float(np.max(err_a / (ab + 1e-300))), float(np.max(err_p / (pb + 1e-300))))
+    return ratio <= 1.0, ratio

@@ -69,9 +69,10 @@ def test_brute_force_kats(mode, k):
                 ok, worst = kat_error_ok(got, c, Gk, k)
                 assert ok, f"case {i}: {worst:.2f}x the FP64 bound"
             else:
-                # mixed: relative to the sum of |terms| (its condition-aware scale)
-                ok, worst = kat_error_ok(got, c, Gk, 2.0 ** 53 * 2e-6)
-                assert ok, f"case {i}: {worst:.2f}x the mixed bound"
+                ok, worst = _util.within_mixed_bound(got, c["expected"], s, c["m"], c["se"], t,
+                                                     c["te"], Gk)
+                print(f"KAT {i} mixed: error / analytic bound = {worst:.3f}", flush=True)
+                assert ok, f"case {i}: {worst:.2f}x the analytic mixed-precision bound"
 
 
 def test_single_pair_sweep_fp64(fp64):
@@ -175,9 +176,14 @@ def test_self_exclusion_and_zero_softening(mode):
         f.set_sources(x, y, z, m, None)
         got = f.evaluate(None, x, y, z)
     ref = oracle.eval_plummer(x, y, z, m, None, x, y, z, None, G, mode=oracle.LONGDOUBLE)
-    for g, r in zip(got, ref):
-        assert np.all(np.isfinite(g))
-        np.testing.assert_allclose(g, r, rtol=1e-12 if mode == "fp64" else 1e-6, atol=0)
+    assert all(np.all(np.isfinite(g)) for g in got)
+    if mode == "fp64":
+        for g, r in zip(got, ref):
+            np.testing.assert_allclose(g, r, rtol=1e-12, atol=0)
+    else:
+        P = np.stack([x, y, z], 1)
+        ok, worst = _util.within_mixed_bound(got, ref, P, m, None, P, None, G)
+        assert ok, worst
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
@@ -228,7 +234,8 @@ def test_chunked_sources_match_single_pass(fp64, c2):
     fp64.set_config(nsplit=0)
     for o in outs[1:]:
         s = _util.summary(o, outs[0])
-        assert s["acc_max"] <= 1e-13 and s["pot_max"] <= 1e-13, s
+        print("chunked vs single pass", s, flush=True)
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, s
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])

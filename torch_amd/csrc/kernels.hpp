@@ -208,10 +208,10 @@ __device__ __forceinline__ void store_result(const Out& O, int64_t t, double sx,
   if (O.ay) O.ay[t] = gy;
   if (O.az) O.az[t] = gz;
   if (O.phi) O.phi[t] = -O.G * sp;
-  if (O.vx) {
-    O.vx[t] += O.dt * gx;
-    O.vy[t] += O.dt * gy;
-    O.vz[t] += O.dt * gz;
+  if (O.vx) {   // v + dt*a rounded twice (no FMA), as the reference's FP64 kick loops
+    O.vx[t] = __dadd_rn(O.vx[t], __dmul_rn(O.dt, gx));
+    O.vy[t] = __dadd_rn(O.vy[t], __dmul_rn(O.dt, gy));
+    O.vz[t] = __dadd_rn(O.vz[t], __dmul_rn(O.dt, gz));
   }
 }
 
@@ -487,9 +487,9 @@ __global__ void k_kick(int64_t n, double dt, const double* __restrict__ ax,
                        double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  vx[i] += dt * ax[i];
-  vy[i] += dt * ay[i];
-  vz[i] += dt * az[i];
+  vx[i] = __dadd_rn(vx[i], __dmul_rn(dt, ax[i]));
+  vy[i] = __dadd_rn(vy[i], __dmul_rn(dt, ay[i]));
+  vz[i] = __dadd_rn(vz[i], __dmul_rn(dt, az[i]));
 }
 
 __global__ void k_fill(int64_t n, double v, double* a, double* b, double* c, double* d) {

@@ -228,6 +228,12 @@ static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t
 static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
   const int64_t nt = T.n, ns = c->ns;
   if (nt == 0) return 0;
+  // Every pointer a kernel dereferences must be set before anything is launched.
+  if (T.blocks ? (!T.ox || !T.oy || !T.oz || !T.bdx) : (!T.x || !T.y || !T.z))
+    return fail(TGRAV_EINVAL, "internal: target arrays not bound");
+  if (O.vx && (!O.vy || !O.vz)) return fail(TGRAV_EINVAL, "internal: partial velocity set");
+  if (ns > 0 && (!c->s4.p || !c->se2.p || !c->stat.p))
+    return fail(TGRAV_EINVAL, "internal: source buffers missing");
   if (ns == 0) {   // no sources: zero field, kick adds nothing
     tg::k_fill<<<grid1(nt, 256), 256, 0, c->stream>>>(nt, 0.0, O.ax, O.ay, O.az, O.phi);
     LAUNCHCHECK("k_fill");
@@ -412,19 +418,36 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
   return 0;
 }
 
+// Points the target descriptor at the (device) arrays tin[].
+static void bind_targets(tg::Tgt& T, const double* const* tin, int ntin) {
+  if (T.blocks) {
+    T.ox = tin[0];
+    T.oy = tin[1];
+    T.oz = tin[2];
+    T.bdx = tin[3];
+    T.beps = ntin > 4 ? tin[4] : nullptr;
+  } else {
+    T.x = tin[0];
+    T.y = tin[1];
+    T.z = tin[2];
+    T.eps = tin[3];
+  }
+}
+
 // Shared body of tgrav_eval / tgrav_eval_kick / tgrav_eval_blocks.
-static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[4], int ntin, double* outs[4],
+static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[5], int ntin, double* outs[4],
                        double dt, double* vel[3], int mode, int kind) {
   int rc;
   const int64_t n = T.n;
+  const int64_t nin = T.blocks ? n / ((int64_t)T.nb * T.nb * T.nb) : n;
   const bool kick = vel[0] != nullptr;
   if (kind == TGRAV_HOST) {
     if (c->validate)
       for (int q = 0; q < ntin; ++q)
-        if (!host_finite(T.blocks ? n / ((int64_t)T.nb * T.nb * T.nb) : n, tin[q]))
+        if (!host_finite(nin, tin[q]))
           return fail(TGRAV_ENONFINITE, "non-finite value in target array %d", q);
-    const int64_t nin = T.blocks ? n / ((int64_t)T.nb * T.nb * T.nb) : n;
     if ((rc = stage_in(c, 0, nin, tin, ntin))) return rc;
+    bind_targets(T, tin, ntin);
     double* dout[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int q = 0; q < 4; ++q)
       if (outs[q]) {
@@ -438,11 +461,6 @@ static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[4], int ntin, 
         dvel[q] = c->vel[q].as<double>();
         HIPCHECK(hipMemcpyAsync(dvel[q], vel[q], (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
       }
-    if (T.blocks) {
-      T.ox = tin[0]; T.oy = tin[1]; T.oz = tin[2]; T.bdx = tin[3]; T.beps = ntin > 4 ? tin[4] : nullptr;
-    } else {
-      T.x = tin[0]; T.y = tin[1]; T.z = tin[2]; T.eps = tin[3];
-    }
     tg::Out O{dout[0], dout[1], dout[2], dout[3], dvel[0], dvel[1], dvel[2], dt, nullptr, 0, c->G};
     if ((rc = launch_eval(c, mode, T, O))) return rc;
     for (int q = 0; q < 4; ++q)
@@ -456,10 +474,10 @@ static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[4], int ntin, 
   }
   if (c->validate && ntin > 0) {
     bool bad = false;
-    const int64_t nin = T.blocks ? n / ((int64_t)T.nb * T.nb * T.nb) : n;
     if ((rc = device_check_finite(c, nin, tin, ntin, &bad))) return rc;
     if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in a target array");
   }
+  bind_targets(T, tin, ntin);
   tg::Out O{outs[0], outs[1], outs[2], outs[3], vel[0], vel[1], vel[2], dt, nullptr, 0, c->G};
   return launch_eval(c, mode, T, O);
 }
@@ -481,7 +499,7 @@ int tgrav_eval(tgrav_ctx* c, int64_t n, const double* eps_t, const double* x, co
   if (!x || !y || !z) return fail(TGRAV_EINVAL, "target x, y and z are required");
   tg::Tgt T{};
   T.n = n;
-  const double* tin[4] = {x, y, z, eps_t};
+  const double* tin[5] = {x, y, z, eps_t, nullptr};
   double* outs[4] = {ax, ay, az, phi};
   double* vel[3] = {nullptr, nullptr, nullptr};
   return eval_common(c, T, tin, 4, outs, 0.0, vel, mode, kind);
@@ -499,7 +517,7 @@ int tgrav_eval_kick(tgrav_ctx* c, int64_t n, const double* eps_t, const double* 
   if (!isfinite(dt)) return fail(TGRAV_EINVAL, "dt must be finite");
   tg::Tgt T{};
   T.n = n;
-  const double* tin[4] = {x, y, z, eps_t};
+  const double* tin[5] = {x, y, z, eps_t, nullptr};
   double* outs[4] = {ax, ay, az, phi};
   double* vel[3] = {vx, vy, vz};
   return eval_common(c, T, tin, 4, outs, dt, vel, mode, kind);
