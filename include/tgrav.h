@@ -67,9 +67,11 @@ const char* tgrav_last_error(void);
  * this path: binds a device, creates a default-priority stream. */
 int tgrav_create(tgrav_ctx** out, int device);
 int tgrav_destroy(tgrav_ctx* ctx);
-/* Run on an existing HIP stream (e.g. torch.cuda.current_stream().cuda_stream);
- * NULL restores the context's own stream. */
+/* Run on an existing HIP stream of the same HIP runtime (e.g.
+ * torch.cuda.current_stream().cuda_stream); NULL is HIP's null stream. */
 int tgrav_set_stream(tgrav_ctx* ctx, void* hip_stream);
+/* Back to the context's own (default-priority, non-blocking) stream, the initial state. */
+int tgrav_use_own_stream(tgrav_ctx* ctx);
 void* tgrav_get_stream(tgrav_ctx* ctx);
 int tgrav_synchronize(tgrav_ctx* ctx);
 /* Gravitational constant; the reference reads FLASH's `Newton`

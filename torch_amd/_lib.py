@@ -37,6 +37,7 @@ SIGNATURES = [
     ("tgrav_create", _I32, [ctypes.POINTER(_P), _I32]),
     ("tgrav_destroy", _I32, [_P]),
     ("tgrav_set_stream", _I32, [_P, _P]),
+    ("tgrav_use_own_stream", _I32, [_P]),
     ("tgrav_get_stream", _P, [_P]),
     ("tgrav_synchronize", _I32, [_P]),
     ("tgrav_set_constant", _I32, [_P, _D]),
@@ -72,13 +73,40 @@ class TgravError(RuntimeError):
         super().__init__(f"{where}: {ERROR_NAMES.get(code, code)}: {msg}")
 
 
+def _share_torch_hip_runtime():
+    """Make libtgrav.so bind to the HIP runtime PyTorch uses, so one process holds one HIP
+    runtime whatever the import order.  PyTorch-ROCm ships its own libamdhip64.so with the
+    same SONAME as /opt/rocm's; loading torch's copy first (by absolute path, without
+    importing torch) makes the dynamic loader reuse it for libtgrav.so, and later makes
+    torch find the same file already loaded.  Two runtimes in one process would each open
+    the GPU and could not share streams."""
+    if os.environ.get("TGRAV_HIP_RUNTIME", "") == "system":
+        return None
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.origin:
+        return None
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if not os.path.exists(path):
+        return None
+    ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    return path
+
+
+HIP_RUNTIME = None
+
+
 def load():
     """Load libtgrav.so (built in-tree).  Raises if it is missing."""
-    global _lib
+    global _lib, HIP_RUNTIME
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m torch_amd.build`")
+    HIP_RUNTIME = _share_torch_hip_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         f = getattr(lib, name)

@@ -53,9 +53,12 @@ __device__ __forceinline__ void load_target(const Tgt& T, int64_t t, double& x, 
     const int c = (int)(t - b * nb3);
     const int i = c / (nb * nb), j = (c / nb) % nb, k = c % nb;
     const double d = T.bdx[b];
-    x = __dadd_rn(T.ox[b], __dmul_rn((double)i + 0.5, d));
-    y = __dadd_rn(T.oy[b], __dmul_rn((double)j + 0.5, d));
-    z = __dadd_rn(T.oz[b], __dmul_rn((double)k + 0.5, d));
+    {
+#pragma clang fp contract(off)
+      x = T.ox[b] + ((double)i + 0.5) * d;
+      y = T.oy[b] + ((double)j + 0.5) * d;
+      z = T.oz[b] + ((double)k + 0.5) * d;
+    }
     const double e = T.beps ? T.beps[b] : 0.0;
     e2 = e * e;
   }
@@ -194,6 +197,15 @@ struct LdsF32 {
 // r2 == 0 (and keeps rsq far from overflow); FP64 guards only an exact zero.
 #define TG_F32_TINY_E2 1e-30
 
+// v + dt*a with the product and the sum rounded separately (no FMA contraction), as the
+// reference's FP64 kick loops compute it (interface.F90:1793-1795).
+__device__ __forceinline__ void kick_one(const Out& O, int64_t t, double gx, double gy, double gz) {
+#pragma clang fp contract(off)
+  O.vx[t] = O.vx[t] + O.dt * gx;
+  O.vy[t] = O.vy[t] + O.dt * gy;
+  O.vz[t] = O.vz[t] + O.dt * gz;
+}
+
 __device__ __forceinline__ void store_result(const Out& O, int64_t t, double sx, double sy,
                                              double sz, double sp) {
   if (O.part) {
@@ -208,11 +220,7 @@ __device__ __forceinline__ void store_result(const Out& O, int64_t t, double sx,
   if (O.ay) O.ay[t] = gy;
   if (O.az) O.az[t] = gz;
   if (O.phi) O.phi[t] = -O.G * sp;
-  if (O.vx) {   // v + dt*a rounded twice (no FMA), as the reference's FP64 kick loops
-    O.vx[t] = __dadd_rn(O.vx[t], __dmul_rn(O.dt, gx));
-    O.vy[t] = __dadd_rn(O.vy[t], __dmul_rn(O.dt, gy));
-    O.vz[t] = __dadd_rn(O.vz[t], __dmul_rn(O.dt, gz));
-  }
+  if (O.vx) kick_one(O, t, gx, gy, gz);
 }
 
 // ------------------------------------------------------- per-workgroup drivers ----
@@ -487,9 +495,12 @@ __global__ void k_kick(int64_t n, double dt, const double* __restrict__ ax,
                        double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  vx[i] = __dadd_rn(vx[i], __dmul_rn(dt, ax[i]));
-  vy[i] = __dadd_rn(vy[i], __dmul_rn(dt, ay[i]));
-  vz[i] = __dadd_rn(vz[i], __dmul_rn(dt, az[i]));
+  {
+#pragma clang fp contract(off)
+    vx[i] = vx[i] + dt * ax[i];
+    vy[i] = vy[i] + dt * ay[i];
+    vz[i] = vz[i] + dt * az[i];
+  }
 }
 
 __global__ void k_fill(int64_t n, double v, double* a, double* b, double* c, double* d) {

@@ -321,7 +321,13 @@ int tgrav_destroy(tgrav_ctx* c) {
 
 int tgrav_set_stream(tgrav_ctx* c, void* s) {
   if (!c) return fail(TGRAV_EINVAL, "context is NULL");
-  c->stream = s ? (hipStream_t)s : c->own;
+  c->stream = (hipStream_t)s;
+  return 0;
+}
+
+int tgrav_use_own_stream(tgrav_ctx* c) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  c->stream = c->own;
   return 0;
 }
 

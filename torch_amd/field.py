@@ -152,12 +152,15 @@ class GravityField:
         return self._ctx
 
     def _bind_stream(self, arrays):
+        """Device tensors: run on torch's current stream (0 is the null stream) so work is
+        ordered with the caller's.  Host arrays: the context's own stream (calls sync)."""
         if arrays.kind == DEVICE:
             import torch
             s = torch.cuda.current_stream(arrays.dev).cuda_stream
-            _lib.check(self.lib.tgrav_set_stream(self._ctx, ctypes.c_void_p(s)), "tgrav_set_stream")
+            _lib.check(self.lib.tgrav_set_stream(self._ctx, ctypes.c_void_p(s or None)),
+                       "tgrav_set_stream")
         else:
-            _lib.check(self.lib.tgrav_set_stream(self._ctx, None), "tgrav_set_stream")
+            _lib.check(self.lib.tgrav_use_own_stream(self._ctx), "tgrav_use_own_stream")
 
     def set_config(self, variant=-1, nsplit=-1, mode=None):
         m = self.mode if mode is None else MODES[mode]
