@@ -155,6 +155,10 @@ int tgrav_batch_run(tgrav_batch* b, int use_graph);
 int tgrav_batch_destroy(tgrav_batch* b);
 /* Work-list statistics: items, total interactions. */
 int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions);
+/* Batch plan shape for plans created afterwards: interactions per work item
+ * (0 = total / (8 x CUs)) and persistent workgroups per CU (0 = occupancy limit).
+ * The kernel variant is the one tgrav_set_config chose for the mode.  Set by the autotuner. */
+int tgrav_set_batch_config(tgrav_ctx* ctx, double item_interactions, int wg_per_cu);
 
 #ifdef __cplusplus
 }

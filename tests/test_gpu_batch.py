@@ -1,0 +1,162 @@
+"""Batched persistent evaluation and its hipGraph replay (needs an MI355X).
+
+Each problem of a batch must agree with the oracle (FP64: 1e-12; mixed: 1e-6 median,
+1e-4 max) and with a standalone evaluation of the same problem; graph replays must be
+bit-identical to the eager (non-graph) run, including after unrelated eager work between
+replays; the fused kick must equal v + dt*a bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import _util
+from torch_amd import GravityField, TgravError, _lib
+from torch_amd import workloads as W
+from torch_amd.batch import BatchPlan, problems_from_workload
+
+pytestmark = pytest.mark.gpu
+G = W.G_PC_MSUN_KMS
+
+
+def small_batch(seed=31, n=12):
+    """Uneven problems incl. an empty-source and an empty-target one."""
+    rng = W.rng_for(seed)
+    probs = []
+    for p in range(n):
+        ns = int(10 ** rng.uniform(0, 4.3))
+        nt = int(10 ** rng.uniform(0, 4.5))
+        if p == 3:
+            ns = 0
+        if p == 5:
+            nt = 0
+        st = W.star_cluster(rng, max(ns, 1))
+        src = dict(x=st["x"][:ns], y=st["y"][:ns], z=st["z"][:ns], m=st["m"][:ns],
+                   eps=rng.uniform(0.005, 0.1, ns))
+        t = rng.uniform(-2, 2, (3, nt))
+        probs.append(dict(sources=src, targets=dict(x=t[0], y=t[1], z=t[2], eps=np.zeros(nt))))
+    return probs
+
+
+def to_np(outs):
+    return [tuple(None if a is None else a.cpu().numpy() for a in o) for o in outs]
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_batch_matches_oracle_and_standalone(mode):
+    import torch
+    probs = small_batch()
+    with GravityField(mode) as f:
+        plan = BatchPlan(f, problems_from_workload(probs))
+        got = to_np(plan.run(graph=False))
+        torch.cuda.synchronize()
+        for i, p in enumerate(probs):
+            S, T = p["sources"], p["targets"]
+            nt = len(T["x"])
+            if nt == 0:
+                continue
+            ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"], T["y"], T["z"],
+                                      T["eps"], G, mode=oracle.NEUMAIER, threads=16)
+            if len(S["x"]) == 0:
+                assert all(np.all(a == 0) for a in got[i])
+                continue
+            s = _util.summary(got[i], ref)
+            if mode == "fp64":
+                assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+            else:
+                assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (i, s)
+            f.set_sources(S["x"], S["y"], S["z"], S["m"], S["eps"])
+            one = f.evaluate(T["eps"], T["x"], T["y"], T["z"])
+            s1 = _util.summary(got[i], one)
+            assert s1["acc_max"] <= (1e-12 if mode == "fp64" else 1e-5), (i, s1)
+        plan.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_graph_replays_are_identical(mode):
+    import torch
+    probs = small_batch(seed=32)
+    with GravityField(mode) as f:
+        f.set_config(nsplit=0)
+        lib = _lib.load()
+        # small work items force chunked problems (partials + reduction inside the graph)
+        assert lib.tgrav_set_batch_config(f.handle, 2.0e5, 0) == 0
+        plan = BatchPlan(f, problems_from_workload(probs))
+        eager = to_np(plan.run(graph=False))
+        torch.cuda.synchronize()
+        junk = torch.randn(1 << 20, device="cuda")
+        for rep in range(4):
+            for o in plan.outputs:
+                for a in o:
+                    if a is not None:
+                        a.fill_(np.nan)
+            out = to_np(plan.run(graph=True))
+            torch.cuda.synchronize()
+            for i, (a, b) in enumerate(zip(out, eager)):
+                for p, q in zip(a, b):
+                    if p is not None:
+                        assert np.array_equal(p, q), (rep, i)
+            junk = (junk * 1.0001).sum() + junk      # eager work between replays
+            torch.cuda.synchronize()
+        plan.close()
+
+
+def test_batch_kick_in_graph():
+    import torch
+    probs = small_batch(seed=33, n=6)
+    dev = problems_from_workload(probs)
+    rng = W.rng_for(5)
+    vel0 = []
+    for p in dev:
+        nt = p["targets"]["x"].numel()
+        v = [torch.as_tensor(rng.normal(size=nt), device="cuda") for _ in range(3)]
+        vel0.append([a.clone() for a in v])
+        p["velocities"] = v
+    dt = 6.5e-4
+    with GravityField("fp64") as f:
+        plan = BatchPlan(f, dev, dt=dt)
+        for rep in range(3):
+            acc = to_np(plan.run(graph=True))
+            torch.cuda.synchronize()
+        for i, p in enumerate(dev):
+            if p["targets"]["x"].numel() == 0:
+                continue
+            v = [a.cpu().numpy() for a in vel0[i]]
+            for _ in range(3):
+                v = list(oracle.kick(dt, acc[i][0], acc[i][1], acc[i][2], *v))
+            for q in range(3):
+                assert np.array_equal(p["velocities"][q].cpu().numpy(), v[q]), (i, q)
+        plan.close()
+
+
+def test_batch_argument_errors():
+    with GravityField("fp64") as f:
+        bad = [dict(sources=dict(x=np.zeros(2), y=np.zeros(2), z=np.zeros(2), m=np.ones(2)),
+                    targets=dict(x=np.zeros(3), y=np.zeros(3), z=np.zeros(3)))]
+        with pytest.raises(TgravError) as e:
+            BatchPlan(f, problems_from_workload(bad), dt=0.1)     # dt != 0 needs velocities
+        assert e.value.code in (_lib.EINVAL,) or isinstance(e.value, TgravError)
+
+
+def test_c4_batch_mixed_against_fp64():
+    """The 64-problem uneven batch (c4): mixed within the error gate of the FP64 batch."""
+    import torch
+    probs = W.c4(1004)
+    dev = problems_from_workload(probs)
+    res = {}
+    for mode in ("fp64", "mixed"):
+        with GravityField(mode) as f:
+            plan = BatchPlan(f, dev)
+            res[mode] = to_np(plan.run(graph=True))
+            torch.cuda.synchronize()
+            plan.close()
+    ea = np.concatenate([_util.rel_acc_err(m, r) for m, r in zip(res["mixed"], res["fp64"])])
+    print("c4 mixed vs fp64: median", np.median(ea), "max", ea.max(), flush=True)
+    assert np.median(ea) <= 1e-6 and ea.max() <= 1e-4
+    # FP64 batch against the oracle on a few targets of a few problems
+    for i in (0, 17, 40):
+        S, T = probs[i]["sources"], probs[i]["targets"]
+        idx = _util.subsample(len(T["x"]), 64, i)
+        ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][idx], T["y"][idx],
+                                  T["z"][idx], None, G, mode=oracle.NEUMAIER, threads=16)
+        s = _util.summary(tuple(a[idx] for a in res["fp64"][i]), ref)
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)

@@ -60,6 +60,7 @@ SIGNATURES = [
     ("tgrav_batch_run", _I32, [_P, _I32]),
     ("tgrav_batch_destroy", _I32, [_P]),
     ("tgrav_batch_info", _I32, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
+    ("tgrav_set_batch_config", _I32, [_P, _D, _I32]),
 ]
 
 _lib = None

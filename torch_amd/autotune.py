@@ -1,0 +1,204 @@
+"""Autotuner for the launch shape of the direct-sum kernels, timed on the device.
+
+Tunable knobs (all exposed by the C ABI):
+  * kernel variant: targets per lane K and workgroup size NT (tgrav_variant_info lists them);
+  * source split of a single evaluation (tgrav_set_config's nsplit; 0 = built-in heuristic);
+  * batch plans: interactions per work item and persistent workgroups per CU
+    (tgrav_set_batch_config).
+Every candidate is run on synthetic inputs of the requested size and timed with HIP events
+on the stream the kernels use; the fastest is kept.  Results are cached in a JSON file keyed
+by device, mode and log2 size buckets, so a tuned shape is reused for every problem of
+similar size.  ``apply_eval`` / ``apply_batch`` look the cache up and configure a field.
+
+    python -m torch_amd.autotune --eval 10000 1024000 --eval 1024000 10000 --batch c4
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from .field import MODES, GravityField
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.environ.get("TGRAV_TUNE_CACHE", os.path.join(HERE, "tune_cache.json"))
+
+
+def _bucket(n):
+    return int(round(math.log2(max(float(n), 1.0))))
+
+
+def device_key():
+    import torch
+    p = torch.cuda.get_device_properties(0)
+    return f"{getattr(p, 'gcnArchName', p.name).split(':')[0]}/{p.multi_processor_count}cu"
+
+
+def load_cache(path=CACHE):
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return {}
+
+
+def save_cache(cache, path=CACHE):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(cache, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def eval_key(mode, ns, nt):
+    return f"{device_key()}|eval|{mode}|s{_bucket(ns)}|t{_bucket(nt)}"
+
+
+def batch_key(mode, problems):
+    ns = [len(p["sources"]["x"]) for p in problems]
+    nt = [len(p["targets"]["x"]) for p in problems]
+    inter = sum(a * b for a, b in zip(ns, nt))
+    return (f"{device_key()}|batch|{mode}|p{len(problems)}|i{_bucket(inter)}"
+            f"|s{_bucket(np.median(ns))}|t{_bucket(np.median(nt))}")
+
+
+def variants(mode):
+    lib = _lib.load()
+    m = MODES[mode]
+    out = []
+    for v in range(lib.tgrav_num_variants(m)):
+        k, nt, ts = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.tgrav_variant_info(m, v, ctypes.byref(k), ctypes.byref(nt), ctypes.byref(ts))
+        out.append(dict(variant=v, k=k.value, threads=nt.value, tile_sources=ts.value))
+    return out
+
+
+def _time(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    return float(np.median(ts))
+
+
+def _synthetic(ns, nt, seed=0):
+    import torch
+    from . import workloads as W
+    rng = W.rng_for(seed)
+    st = W.star_cluster(rng, ns)
+    t = rng.uniform(-2.0, 2.0, (3, nt))
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    return [d(st[k]) for k in ("x", "y", "z", "m", "eps")], [d(t[q]) for q in range(3)]
+
+
+def tune_eval(mode, ns, nt, reps=5, nsplits=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256), log=print):
+    """Best (variant, nsplit) for one evaluation of nt targets from ns sources."""
+    S, T = _synthetic(ns, nt)
+    table = []
+    with GravityField(mode, validate=False) as f:
+        f.set_sources(*S)
+        for v in variants(mode):
+            for sp in nsplits:
+                tile = v["k"] * v["threads"]
+                if sp > 1 and sp * v["tile_sources"] > ns:
+                    continue
+                if sp > 1 and -(-nt // tile) >= 4096:
+                    continue      # plenty of target tiles already
+                f.set_config(variant=v["variant"], nsplit=sp)
+                ms = _time(lambda: f.evaluate(None, *T), reps)
+                table.append(dict(v, nsplit=sp, ms=ms, rate=ns * nt / ms * 1e3))
+                log(f"  {mode} ns={ns} nt={nt} variant={v['variant']} (K={v['k']}, NT={v['threads']})"
+                    f" nsplit={sp}: {ms:.3f} ms")
+    best = min(table, key=lambda r: r["ms"])
+    return best, table
+
+
+def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(0, 1, 2),
+               log=print):
+    """Best (variant, item size, workgroups per CU) for a batch plan of these problems."""
+    import torch
+    from .batch import BatchPlan, problems_from_workload
+    dev = problems_from_workload(problems)
+    lib = _lib.load()
+    table = []
+    with GravityField(mode, validate=False) as f:
+        for v in variants(mode):
+            for it in items:
+                for w in wg_per_cu:
+                    f.set_config(variant=v["variant"])
+                    lib.tgrav_set_batch_config(f.handle, float(it), int(w))
+                    plan = BatchPlan(f, dev)
+                    ms = _time(lambda: plan.run(graph=True), reps)
+                    table.append(dict(v, item=it, wg_per_cu=w, ms=ms, items_n=plan.n_items,
+                                      rate=plan.interactions / ms * 1e3))
+                    log(f"  batch {mode} variant={v['variant']} (K={v['k']}, NT={v['threads']}) "
+                        f"item={it:.1e} wg/cu={w}: {ms:.3f} ms, {plan.n_items} items")
+                    plan.close()
+                    torch.cuda.synchronize()
+        lib.tgrav_set_batch_config(f.handle, 0.0, 0)
+    best = min(table, key=lambda r: r["ms"])
+    return best, table
+
+
+def apply_eval(field, ns, nt, cache=None):
+    """Configure `field` from the cache for an evaluation of this size; returns the entry."""
+    cache = load_cache() if cache is None else cache
+    mode = {v: k for k, v in MODES.items() if k != "plain_fp64"}[field.mode]
+    e = cache.get(eval_key(mode, ns, nt))
+    if e:
+        field.set_config(variant=e["variant"], nsplit=e["nsplit"])
+    return e
+
+
+def apply_batch(field, problems, cache=None):
+    cache = load_cache() if cache is None else cache
+    mode = {v: k for k, v in MODES.items() if k != "plain_fp64"}[field.mode]
+    e = cache.get(batch_key(mode, problems))
+    if e:
+        field.set_config(variant=e["variant"])
+        _lib.check(field.lib.tgrav_set_batch_config(field.handle, float(e["item"]),
+                                                     int(e["wg_per_cu"])), "tgrav_set_batch_config")
+    return e
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--eval", nargs=2, type=int, action="append", default=[],
+                    metavar=("NS", "NT"))
+    ap.add_argument("--batch", action="append", default=[], choices=["c4"])
+    ap.add_argument("--modes", default="mixed,fp64")
+    ap.add_argument("--cache", default=CACHE)
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args()
+    cache = load_cache(args.cache)
+    t0 = time.time()
+    for mode in args.modes.split(","):
+        for ns, nt in args.eval:
+            best, _ = tune_eval(mode, ns, nt, reps=args.reps)
+            cache[eval_key(mode, ns, nt)] = dict(variant=best["variant"], nsplit=best["nsplit"],
+                                                 ms=best["ms"], k=best["k"], threads=best["threads"])
+            print(f"best {mode} eval ns={ns} nt={nt}: {cache[eval_key(mode, ns, nt)]}", flush=True)
+            save_cache(cache, args.cache)
+        for name in args.batch:
+            from . import workloads as W
+            probs = W.c4(1004)
+            best, _ = tune_batch(mode, probs, reps=max(2, args.reps // 2))
+            cache[batch_key(mode, probs)] = dict(variant=best["variant"], item=best["item"],
+                                                 wg_per_cu=best["wg_per_cu"], ms=best["ms"],
+                                                 k=best["k"], threads=best["threads"])
+            print(f"best {mode} batch {name}: {cache[batch_key(mode, probs)]}", flush=True)
+            save_cache(cache, args.cache)
+    print(f"tuned in {time.time() - t0:.1f} s -> {args.cache}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,6 +17,7 @@
 
 #include "../../include/tgrav.h"
 #include "kernels.hpp"
+#include "batch.hpp"
 
 #define TGRAV_VERSION 1
 
@@ -78,6 +79,20 @@ struct DevBuf {
 // ---------------------------------------------------------------- variants ------
 typedef void (*launch_fn)(dim3, dim3, hipStream_t, const tg::Tgt&, const double4*, const double*,
                           int64_t, int64_t, const double*, const tg::Out&);
+typedef void (*batch_fn)(int, hipStream_t, const tg::BProb*, const tg::BItem*, int, unsigned int*,
+                         const double4*, const double*, const unsigned long long*, double*);
+typedef const void* (*kernel_ptr_fn)();
+
+template <int K, int NT, bool MIXED>
+void launch_batch(int grid, hipStream_t s, const tg::BProb* p, const tg::BItem* it, int n,
+                  unsigned int* ctr, const double4* s4, const double* se2,
+                  const unsigned long long* st, double* part) {
+  tg::k_batch_eval<K, NT, MIXED><<<grid, NT, 0, s>>>(p, it, n, ctr, s4, se2, st, part);
+}
+template <int K, int NT, bool MIXED>
+const void* batch_kernel() {
+  return (const void*)tg::k_batch_eval<K, NT, MIXED>;
+}
 
 template <int K, int NT>
 void launch_fp64(dim3 g, dim3 b, hipStream_t s, const tg::Tgt& T, const double4* s4,
@@ -93,18 +108,18 @@ void launch_mixed(dim3 g, dim3 b, hipStream_t s, const tg::Tgt& T, const double4
 struct Variant {
   int k, nt, ts;
   launch_fn fn;
+  batch_fn bfn;
+  kernel_ptr_fn bkernel;
 };
 
-const Variant kFp64Variants[] = {
-    {1, 256, 256, launch_fp64<1, 256>}, {2, 256, 256, launch_fp64<2, 256>},
-    {4, 256, 256, launch_fp64<4, 256>}, {2, 128, 128, launch_fp64<2, 128>},
-    {4, 128, 128, launch_fp64<4, 128>}, {8, 64, 64, launch_fp64<8, 64>},
-};
-const Variant kMixedVariants[] = {
-    {2, 256, 256, launch_mixed<2, 256, 64>}, {4, 256, 256, launch_mixed<4, 256, 64>},
-    {8, 256, 256, launch_mixed<8, 256, 64>}, {4, 128, 128, launch_mixed<4, 128, 64>},
-    {8, 128, 128, launch_mixed<8, 128, 64>}, {16, 64, 64, launch_mixed<16, 64, 64>},
-};
+#define FP64V(K, NT) {K, NT, NT, launch_fp64<K, NT>, launch_batch<K, NT, false>, batch_kernel<K, NT, false>}
+#define MIXV(K, NT) {K, NT, NT, launch_mixed<K, NT, 64>, launch_batch<K, NT, true>, batch_kernel<K, NT, true>}
+const Variant kFp64Variants[] = {FP64V(1, 256), FP64V(2, 256), FP64V(4, 256),
+                                 FP64V(2, 128), FP64V(4, 128), FP64V(8, 64)};
+const Variant kMixedVariants[] = {MIXV(2, 256), MIXV(4, 256), MIXV(8, 256),
+                                  MIXV(4, 128), MIXV(8, 128), MIXV(16, 64)};
+#undef FP64V
+#undef MIXV
 const int kNumFp64 = sizeof(kFp64Variants) / sizeof(kFp64Variants[0]);
 const int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
 const int kDefaultFp64 = 1;
@@ -127,6 +142,8 @@ struct tgrav_ctx {
   int validate = 1;
   int variant[2] = {kDefaultFp64, kDefaultMixed};
   int nsplit[2] = {0, 0};
+  double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
+  int batch_wg_per_cu = 0;   // persistent workgroups per CU (0 = occupancy limit)
   // packed sources; stat[0] = bits of min eps_s^2 (read by the kernels), stat[1] = NaN flag
   int64_t ns = -1;
   DevBuf s4, se2, stat;
@@ -591,13 +608,235 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t, const double*, const double*, con
   return fail(TGRAV_ELIMIT, "tgrav_bidirectional: not built yet");
 }
 
-int tgrav_batch_create(tgrav_ctx* c, int32_t, const tgrav_problem*, int, double, tgrav_batch** out) {
-  if (out) *out = nullptr;
+int tgrav_set_batch_config(tgrav_ctx* c, double item_interactions, int wg_per_cu) {
   if (!c) return fail(TGRAV_EINVAL, "context is NULL");
-  return fail(TGRAV_ELIMIT, "tgrav_batch_create: not built yet");
+  if (!(item_interactions >= 0.0) || wg_per_cu < 0)
+    return fail(TGRAV_EINVAL, "item size and workgroups per CU must be >= 0");
+  c->batch_item = item_interactions;
+  c->batch_wg_per_cu = wg_per_cu;
+  return 0;
 }
-int tgrav_batch_run(tgrav_batch*, int) { return fail(TGRAV_ELIMIT, "tgrav_batch_run: not built yet"); }
-int tgrav_batch_destroy(tgrav_batch*) { return 0; }
-int tgrav_batch_info(tgrav_batch*, int64_t*, double*) { return fail(TGRAV_ELIMIT, "not built yet"); }
+
+}  // extern "C"
+
+struct tgrav_batch {
+  tgrav_ctx* c = nullptr;
+  int mode = 0, nprob = 0, n_items = 0, nred = 0, grid = 0;
+  const Variant* v = nullptr;
+  int64_t total_src = 0, red_total = 0;
+  double inter = 0.0;
+  DevBuf probs, items, srcs, sprefix, s4, se2, stats, part, counter, ids, rprefix;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void batch_free(tgrav_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  DevBuf* bufs[] = {&b->probs, &b->items, &b->srcs, &b->sprefix, &b->s4, &b->se2,
+                    &b->stats, &b->part, &b->counter, &b->ids, &b->rprefix};
+  for (DevBuf* d : bufs) d->release();
+  delete b;
+}
+
+template <class T>
+static int upload(DevBuf& d, const std::vector<T>& v) {
+  int rc = d.ensure(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (rc) return rc;
+  if (!v.empty()) HIPCHECK(hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Enqueues reset -> pack -> persistent eval -> chunk reduction on `s`.
+static int batch_enqueue(tgrav_batch* b, hipStream_t s) {
+  tg::k_batch_reset<<<grid1(std::max(b->nprob, 1), 256), 256, 0, s>>>(
+      b->nprob, b->counter.as<unsigned int>(), b->stats.as<unsigned long long>());
+  LAUNCHCHECK("k_batch_reset");
+  if (b->total_src > 0) {
+    tg::k_batch_pack<<<grid1(b->total_src, 256), 256, 0, s>>>(
+        b->nprob, b->srcs.as<tg::BSrc>(), b->sprefix.as<int64_t>(), b->s4.as<double4>(),
+        b->se2.as<double>(), b->stats.as<unsigned long long>());
+    LAUNCHCHECK("k_batch_pack");
+  }
+  if (b->n_items > 0) {
+    b->v->bfn(b->grid, s, b->probs.as<tg::BProb>(), b->items.as<tg::BItem>(), b->n_items,
+              b->counter.as<unsigned int>(), b->s4.as<double4>(), b->se2.as<double>(),
+              b->stats.as<unsigned long long>(), b->part.as<double>());
+    LAUNCHCHECK("k_batch_eval");
+  }
+  if (b->nred > 0 && b->red_total > 0) {
+    tg::k_batch_reduce<<<grid1(b->red_total, 256), 256, 0, s>>>(
+        b->nred, b->ids.as<int>(), b->rprefix.as<int64_t>(), b->probs.as<tg::BProb>(),
+        b->part.as<double>());
+    LAUNCHCHECK("k_batch_reduce");
+  }
+  return 0;
+}
+
+extern "C" {
+
+int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mode, double dt,
+                       tgrav_batch** out) {
+  if (out) *out = nullptr;
+  if (!c || !out) return fail(TGRAV_EINVAL, "context and out are required");
+  int rc = check_mode(mode);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && !pr)) return fail(TGRAV_EINVAL, "bad problem list");
+  if (!isfinite(dt)) return fail(TGRAV_EINVAL, "dt must be finite");
+  const bool kick = dt != 0.0;
+  for (int p = 0; p < n; ++p) {
+    const tgrav_problem& q = pr[p];
+    if (q.n_sources < 0 || q.n_targets < 0)
+      return fail(TGRAV_EINVAL, "problem %d: negative size", p);
+    if (q.n_sources > 0 && (!q.sx || !q.sy || !q.sz || !q.sm))
+      return fail(TGRAV_EINVAL, "problem %d: source x, y, z and m are required", p);
+    if (q.n_targets > 0 && (!q.tx || !q.ty || !q.tz))
+      return fail(TGRAV_EINVAL, "problem %d: target x, y and z are required", p);
+    if (kick && q.n_targets > 0 && (!q.vx || !q.vy || !q.vz))
+      return fail(TGRAV_EINVAL, "problem %d: velocities are required when dt != 0", p);
+  }
+  if ((rc = set_device(c))) return rc;
+  tgrav_batch* b = new tgrav_batch();
+  b->c = c;
+  b->mode = mode;
+  b->nprob = n;
+  b->v = variant_of(mode, c->variant[mode]);
+  const int64_t T = (int64_t)b->v->k * b->v->nt, NT = b->v->nt;
+
+  double total = 0.0;
+  for (int p = 0; p < n; ++p) total += (double)pr[p].n_sources * (double)pr[p].n_targets;
+  b->inter = total;
+  double C = c->batch_item > 0 ? c->batch_item
+                               : std::max((double)T * NT * 4.0, total / (c->num_cu * 8.0));
+  std::vector<tg::BProb> probs(n);
+  std::vector<tg::BSrc> srcs(n);
+  std::vector<int64_t> sprefix(n + 1, 0);
+  std::vector<tg::BItem> items;
+  std::vector<double> cost;
+  std::vector<int> ids;
+  std::vector<int64_t> rprefix(1, 0);
+  int64_t part_total = 0;
+  for (int p = 0; p < n; ++p) {
+    const tgrav_problem& q = pr[p];
+    tg::BProb& P = probs[p];
+    memset(&P, 0, sizeof P);
+    P.T.x = q.tx; P.T.y = q.ty; P.T.z = q.tz; P.T.eps = q.te; P.T.n = q.n_targets;
+    P.O = tg::Out{q.ax, q.ay, q.az, q.phi, kick ? q.vx : nullptr, kick ? q.vy : nullptr,
+                  kick ? q.vz : nullptr, dt, nullptr, 0, c->G};
+    P.src_off = sprefix[p];
+    P.ns = q.n_sources;
+    srcs[p] = tg::BSrc{q.sx, q.sy, q.sz, q.sm, q.se};
+    sprefix[p + 1] = sprefix[p] + q.n_sources;
+    if (q.n_targets == 0) { P.nchunk = 1; continue; }
+    const int64_t tiles = (q.n_targets + T - 1) / T;
+    if (tiles > 0x7fffffffLL) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
+    if (q.n_sources == 0) {                    // zero field: written by the reduction
+      P.nchunk = 0;
+      ids.push_back(p);
+      rprefix.push_back(rprefix.back() + q.n_targets);
+      continue;
+    }
+    const double per_tile = (double)T * q.n_sources;
+    int64_t nch = per_tile > C ? (int64_t)ceil(per_tile / C) : 1;
+    int64_t ch = (q.n_sources + nch - 1) / nch;
+    ch = (ch + NT - 1) / NT * NT;
+    nch = (q.n_sources + ch - 1) / ch;
+    P.chunk = ch;
+    P.nchunk = (int)nch;
+    if (nch > 1) {
+      P.part_off = part_total;
+      part_total += nch * 4 * q.n_targets;
+      ids.push_back(p);
+      rprefix.push_back(rprefix.back() + q.n_targets);
+    }
+    for (int64_t t = 0; t < tiles; ++t)
+      for (int64_t k = 0; k < nch; ++k) {
+        items.push_back(tg::BItem{p, (int)t, (int)k, 0});
+        const double nt_t = (double)std::min(T, q.n_targets - t * T);
+        const double ns_k = (double)std::min(ch, q.n_sources - k * ch);
+        cost.push_back(nt_t * ns_k);
+      }
+  }
+  if (items.size() > 0x7fffffffULL) { batch_free(b); return fail(TGRAV_ELIMIT, "too many work items"); }
+  std::vector<size_t> order(items.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t z) { return cost[a] > cost[z]; });
+  std::vector<tg::BItem> sorted(items.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  b->n_items = (int)sorted.size();
+  b->nred = (int)ids.size();
+  b->red_total = rprefix.back();
+  b->total_src = sprefix[n];
+
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->v->bkernel(), b->v->nt, 0) != hipSuccess || occ < 1)
+    occ = 1;
+  const int per_cu = c->batch_wg_per_cu > 0 ? std::min(c->batch_wg_per_cu, occ) : occ;
+  b->grid = std::max(1, std::min(b->n_items, per_cu * c->num_cu));
+
+  if ((rc = upload(b->probs, probs)) || (rc = upload(b->items, sorted)) ||
+      (rc = upload(b->srcs, srcs)) || (rc = upload(b->sprefix, sprefix)) ||
+      (rc = upload(b->ids, ids)) || (rc = upload(b->rprefix, rprefix)) ||
+      (rc = b->s4.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double4))) ||
+      (rc = b->se2.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double))) ||
+      (rc = b->stats.ensure(std::max(n, 1) * 2 * sizeof(unsigned long long))) ||
+      (rc = b->part.ensure(std::max<int64_t>(part_total, 1) * sizeof(double))) ||
+      (rc = b->counter.ensure(sizeof(unsigned int)))) {
+    batch_free(b);
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
+    batch_free(b);
+    return fail(TGRAV_EHIP, "cannot create the capture stream");
+  }
+  *out = b;
+  return 0;
+}
+
+int tgrav_batch_run(tgrav_batch* b, int use_graph) {
+  if (!b) return fail(TGRAV_EINVAL, "batch is NULL");
+  tgrav_ctx* c = b->c;
+  int rc = set_device(c);
+  if (rc) return rc;
+  if (!use_graph) return batch_enqueue(b, c->stream);
+  if (!b->exec) {
+    HIPCHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(TGRAV_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      b->exec = nullptr;
+      return fail(TGRAV_EHIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    b->graph = g;
+  }
+  HIPCHECK(hipGraphLaunch(b->exec, c->stream));
+  return 0;
+}
+
+int tgrav_batch_destroy(tgrav_batch* b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->c->device);
+  (void)hipStreamSynchronize(b->c->stream);
+  batch_free(b);
+  return 0;
+}
+
+int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions) {
+  if (!b) return fail(TGRAV_EINVAL, "batch is NULL");
+  if (n_items) *n_items = b->n_items;
+  if (interactions) *interactions = b->inter;
+  return 0;
+}
 
 }  // extern "C"
