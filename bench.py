@@ -56,9 +56,12 @@ def rel_err(got, ref):
 def run_mode(mode, dev, st, ce, steps, warmup):
     import torch
 
-    from torch_amd import GasGravityField, StellarGravityField
+    from torch_amd import GasGravityField, StellarGravityField, autotune
     fwd = StellarGravityField(mode, validate=False)
     rev = GasGravityField(mode, validate=False)
+    cfg = (autotune.apply_eval(fwd, len(st["x"]), len(ce["x"])),
+           autotune.apply_eval(rev, len(ce["x"]), len(st["x"])))
+    log(f"{mode}: tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
 
