@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
 """Pairwise interactions/s of the direct-sum gravity field on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|c5] [--no-cpu]
 
-Workload (c2, seed 1002): 10,000 Plummer/Kroupa stars (20% hard-binary members) and
+Default workload c2 (seed 1002): 10,000 Plummer/Kroupa stars (20% hard-binary members) and
 1,024,000 AMR cell centres (2,000 leaf blocks of 8^3).  One step is one bridge field
 evaluation in both directions: stars -> cells (acceleration + potential at the cell
 centres) and cells as point masses (m = rho dx^3, eps = dx) -> stars, each preceded by
-loading its source set.  Inputs are resident in HBM (device tensors) before timing.
-2 x 1.024e10 pair interactions per step.
+loading its source set.  2 x 1.024e10 pair interactions per step.
+  c5: 100,000 stars <-> 4,194,304 cells (2 x 4.19e11 interactions per step).
+  c4: 64 uneven independent problems in one graph-replayed persistent launch (one step =
+      one replay, acc + pot of all problems).
+Inputs are resident in HBM (device tensors) before timing.
 
 The headline `value` is the mixed-precision rate; the plain-FP64 kernel is timed the same
 way and reported beside it (`fp64_value`, `mixed_over_fp64`).  Mixed-precision error is
 measured over all targets against the FP64 kernel, which is itself checked against the
-CPU oracle on a seeded subsample (the checker leg).  Prints one JSON line on stdout;
-progress goes to stderr.
+CPU oracle on a seeded subsample (the checker leg).  The roofline uses the forward (c2, c5)
+or persistent (c4) evaluation kernel alone, timed with HIP events on its own stream through
+the library's instrumentation hook.  Prints one JSON line on stdout; progress on stderr.
 """
 import argparse
 import json
@@ -27,10 +31,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FLOP_PER_INTERACTION = 21          # acc + pot, FMA = 2
-PEAK_FP32_TFLOPS = 157.3           # MI355X vector FP32 (packed) peak
+FLOP_PER_INTERACTION = 21          # acc + pot: 3 sub, 1 add, 3 FMA, rsqrt, 2 mul, 1 mul, 1 add, 3 FMA
+PEAK_FP32_TFLOPS = 157.3           # MI355X vector FP32 peak (packed FP32 instructions)
 PEAK_FP64_TFLOPS = 78.6            # MI355X vector FP64 peak
-PEAK_FP32_UNPACKED_TFLOPS = 78.6   # one FP32 op per lane per cycle (packed FP32 disabled)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6   # one FP32 FMA per lane per clock (packed FP32 disabled here)
+BYTES_PER_TARGET = 64              # x, y, z, eps in + ax, ay, az, phi out (FP64)
+BYTES_PER_SOURCE = 40              # x, y, z, m, eps (FP64), read once per launch
 
 
 def log(*a):
@@ -53,25 +59,29 @@ def rel_err(got, ref):
     return ea, ep
 
 
-def run_mode(mode, dev, st, ce, steps, warmup):
+def err_stats(ea, ep=None):
+    d = dict(acc_median=float(np.median(ea)), acc_max=float(ea.max()))
+    if ep is not None:
+        d.update(pot_median=float(np.median(ep)), pot_max=float(ep.max()))
+    return d
+
+
+# ----------------------------------------------------------------- c2 / c5 ----
+def run_pair(mode, dev, ns, nc, steps, warmup):
+    """Forward (stars -> cells) and reverse (cells -> stars) per step."""
     import torch
 
     from torch_amd import GasGravityField, StellarGravityField, autotune
     fwd = StellarGravityField(mode, validate=False)
     rev = GasGravityField(mode, validate=False)
-    cfg = (autotune.apply_eval(fwd, len(st["x"]), len(ce["x"])),
-           autotune.apply_eval(rev, len(ce["x"]), len(st["x"])))
+    cfg = (autotune.apply_eval(fwd, ns, nc), autotune.apply_eval(rev, nc, ns))
     log(f"{mode}: tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
 
-    def step(ev=None):
+    def step():
         fwd.set_sources(*S)
-        if ev is not None:
-            ev[0].record()
         of = fwd.evaluate(None, C[0], C[1], C[2])
-        if ev is not None:
-            ev[1].record()
         rev.set_sources(*C)
         orv = rev.evaluate(None, S[0], S[1], S[2])
         return of, orv
@@ -79,73 +89,116 @@ def run_mode(mode, dev, st, ce, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
+    fwd.set_timing(True)
+    rev.set_timing(True)
     t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     wall0 = time.perf_counter()
     t0.record()
-    for i in range(steps):
-        out = step(kev[i])
+    for _ in range(steps):
+        out = step()
     t1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    ms = t0.elapsed_time(t1) / steps
-    kms = float(np.mean([a.elapsed_time(b) for a, b in kev]))
+    kf = [r[0] for r in fwd.kernel_times()]
+    kr = [r[0] for r in rev.kernel_times()]
     res = tuple(tuple(a.cpu().numpy() for a in o) for o in out)
     fwd.close()
     rev.close()
-    return ms, kms, wall, res
+    return dict(ms=t0.elapsed_time(t1) / steps, fwd_kernel_ms=float(np.mean(kf)),
+                rev_kernel_ms=float(np.mean(kr)), wall=wall, res=res, cfg=cfg)
 
 
-def cpu_baseline(st, ce, G, budget_s):
-    """The oracle (C port of the reference loop nest, -O2) on a seeded target subsample of the
-    forward direction: one thread, then all host cores (<= 16)."""
+# ---------------------------------------------------------------------- c4 ----
+def run_batch(mode, probs_dev, steps, warmup):
+    import torch
+
+    from torch_amd import GravityField, autotune
+    from torch_amd.batch import BatchPlan
+    f = GravityField(mode, validate=False)
+    cfg = autotune.apply_batch(f, [{k: {q: v for q, v in p[k].items()} for k in ("sources", "targets")}
+                                   for p in probs_dev])
+    log(f"{mode}: tuned batch shape {cfg}")
+    plan = BatchPlan(f, probs_dev)
+    for _ in range(warmup):
+        plan.run(graph=True)
+    torch.cuda.synchronize()
+    # the persistent kernel alone, eager launch, instrumented
+    f.set_timing(True)
+    plan.run(graph=False)
+    torch.cuda.synchronize()
+    kms = [r[0] for r in f.kernel_times()]
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record()
+    for _ in range(steps):
+        out = plan.run(graph=True)
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    res = [tuple(a.cpu().numpy() for a in o) for o in out]
+    info = dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.mean(kms)) if kms else None,
+                wall=wall, res=res, cfg=cfg, items=plan.n_items, interactions=plan.interactions)
+    plan.close()
+    f.close()
+    return info
+
+
+# ----------------------------------------------------------- checker legs ----
+def cpu_baseline(S, T, G, budget_s, label):
+    """The oracle (C port of the reference loop nest, -O2) on seeded target subsamples:
+    one thread, then all host cores (<= 16)."""
     import oracle
-    ns = len(st["x"])
+    ns = len(S["x"])
     rng = np.random.Generator(np.random.PCG64(4242))
-    probe = np.sort(rng.choice(len(ce["x"]), 200, replace=False))
+    nprobe = min(len(T["x"]), max(8, int(2e6 / ns)))
+    probe = np.sort(rng.choice(len(T["x"]), nprobe, replace=False))
     t = time.perf_counter()
-    oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], st["eps"], ce["x"][probe], ce["y"][probe],
-                        ce["z"][probe], None, G, mode=oracle.SEQ, threads=1)
-    rate1 = 200 * ns / (time.perf_counter() - t)
+    oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][probe], T["y"][probe],
+                        T["z"][probe], None, G, mode=oracle.SEQ, threads=1)
+    rate1 = nprobe * ns / (time.perf_counter() - t)
     out = {}
     cores = max(1, min(16, os.cpu_count() or 1))
     for threads, share in ((1, 0.35), (cores, 0.65)):
-        n = int(min(len(ce["x"]), max(256, budget_s * share * rate1 * (0.8 * threads) / ns)))
-        idx = np.sort(rng.choice(len(ce["x"]), n, replace=False))
+        n = int(min(len(T["x"]), max(16, budget_s * share * rate1 * (0.8 * threads) / ns)))
+        idx = np.sort(rng.choice(len(T["x"]), n, replace=False))
         t = time.perf_counter()
-        oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], st["eps"], ce["x"][idx], ce["y"][idx],
-                            ce["z"][idx], None, G, mode=oracle.SEQ, threads=threads)
+        oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][idx], T["y"][idx],
+                            T["z"][idx], None, G, mode=oracle.SEQ, threads=threads)
         dt = time.perf_counter() - t
         out[threads] = dict(value=n * ns / dt, seconds=dt, targets=n)
-        log(f"cpu oracle {threads} thread(s): {n} targets x {ns} stars in {dt:.2f} s "
+        log(f"cpu oracle {threads} thread(s): {n} targets x {ns} sources in {dt:.2f} s "
             f"= {n * ns / dt:.3e} interactions/s")
-    return out, cores
+    cb = {"value": out[cores]["value"], "unit": "interactions/s", "cores": cores, "kind": "port",
+          "sample": f"{label}: {out[cores]['targets']} seeded targets x {ns} sources "
+                    f"(acc + pot, FP64, OpenMP over targets); one thread: "
+                    f"{out[1]['value']:.3e} interactions/s on {out[1]['targets']} targets",
+          "scalar_value": out[1]["value"]}
+    return cb
 
 
-def oracle_check(st, ce, res64, G):
-    """FP64 kernel vs the compensated oracle on seeded subsamples (the checker)."""
+def oracle_check(pairs, G):
+    """FP64 GPU results vs the compensated oracle on seeded subsamples (the checker).
+    pairs: [(S, T, gpu_result, n_sub)]."""
     import oracle
     rng = np.random.Generator(np.random.PCG64(77))
-    i_f = np.sort(rng.choice(len(ce["x"]), 1024, replace=False))
-    ref = oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], st["eps"], ce["x"][i_f], ce["y"][i_f],
-                              ce["z"][i_f], None, G, mode=oracle.NEUMAIER, threads=16)
-    ea_f, ep_f = rel_err(tuple(a[i_f] for a in res64[0]), ref)
-    i_r = np.sort(rng.choice(len(st["x"]), 96, replace=False))
-    ref = oracle.eval_plummer(ce["x"], ce["y"], ce["z"], ce["m"], ce["eps"], st["x"][i_r], st["y"][i_r],
-                              st["z"][i_r], None, G, mode=oracle.NEUMAIER, threads=16)
-    ea_r, ep_r = rel_err(tuple(a[i_r] for a in res64[1]), ref)
-    return dict(fwd_targets=len(i_f), rev_targets=len(i_r),
-                max_rel_acc=float(max(ea_f.max(), ea_r.max())),
-                max_rel_pot=float(max(ep_f.max(), ep_r.max())))
+    worst_a = worst_p = 0.0
+    n = 0
+    for S, T, res, k in pairs:
+        idx = np.sort(rng.choice(len(T["x"]), min(k, len(T["x"])), replace=False))
+        ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S.get("eps"), T["x"][idx], T["y"][idx],
+                                  T["z"][idx], T.get("eps_t"), G, mode=oracle.NEUMAIER, threads=16)
+        ea, ep = rel_err(tuple(a[idx] for a in res), ref)
+        worst_a, worst_p, n = max(worst_a, ea.max()), max(worst_p, ep.max()), n + len(idx)
+    return dict(targets_checked=n, max_rel_acc=float(worst_a), max_rel_pot=float(worst_p))
 
 
+# -------------------------------------------------------------------- main ----
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--modes", default="mixed,fp64")
@@ -160,83 +213,117 @@ def main():
 
     from torch_amd import workloads as W
     G = W.G_PC_MSUN_KMS
+    modes = args.modes.split(",")
+    head = "mixed" if "mixed" in modes else modes[0]
     t = time.perf_counter()
-    w = W.c2(1002)
-    st, ce = w["stars"], w["cells"]
-    ns, nc = len(st["x"]), len(ce["x"])
-    log(f"c2 generated in {time.perf_counter() - t:.1f} s: {ns} stars, {nc} cells")
     d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
-    dev = dict(sx=d(st["x"]), sy=d(st["y"]), sz=d(st["z"]), sm=d(st["m"]), se=d(st["eps"]),
-               cx=d(ce["x"]), cy=d(ce["y"]), cz=d(ce["z"]), cm=d(ce["m"]), ce=d(ce["eps"]))
-    torch.cuda.synchronize()
-    inter_step = 2.0 * ns * nc
-
+    line = {"metric": metric_name(), "unit": "interactions/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32/f64" if head == "mixed" else "f64", "data": f"synthetic (seeded {args.workload})"}
     results = {}
-    for mode in args.modes.split(","):
-        ms, kms, wall, res = run_mode(mode, dev, st, ce, args.steps, args.warmup)
-        results[mode] = dict(ms=ms, kms=kms, wall=wall, res=res)
-        log(f"{mode}: {ms:.3f} ms/step ({inter_step / ms * 1e3:.3e} interactions/s), "
-            f"forward kernel {kms:.3f} ms, wall {wall:.2f} s")
+    if args.workload in ("c2", "c5"):
+        w = W.c2(1002) if args.workload == "c2" else W.c5(1005)
+        st, ce = w["stars"], w["cells"]
+        ns, nc = len(st["x"]), len(ce["x"])
+        log(f"{args.workload} generated in {time.perf_counter() - t:.1f} s: {ns} stars, {nc} cells")
+        dev = dict(sx=d(st["x"]), sy=d(st["y"]), sz=d(st["z"]), sm=d(st["m"]), se=d(st["eps"]),
+                   cx=d(ce["x"]), cy=d(ce["y"]), cz=d(ce["z"]), cm=d(ce["m"]), ce=d(ce["eps"]))
+        torch.cuda.synchronize()
+        inter_step = 2.0 * ns * nc
+        for mode in modes:
+            r = run_pair(mode, dev, ns, nc, args.steps, args.warmup)
+            results[mode] = r
+            log(f"{mode}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s); "
+                f"eval kernels fwd {r['fwd_kernel_ms']:.3f} ms, rev {r['rev_kernel_ms']:.3f} ms")
+        r = results[head]
+        kern_ms = r["fwd_kernel_ms"]
+        kern_inter = float(ns) * nc
+        kern_bytes = BYTES_PER_TARGET * nc + BYTES_PER_SOURCE * ns
+        name = ("10,000 stars (20% binary members) <-> 1,024,000 AMR cell centres (2,000 leaf "
+                "blocks of 8^3)" if args.workload == "c2" else
+                "100,000 stars (30% binary members) <-> 4,194,304 AMR cell centres (8,192 leaf "
+                "blocks of 8^3)")
+        line["config"] = {"workload": f"{args.workload}: {name}; forward + reverse, acc + pot",
+                          "seed": w["seed"], "mode": head, "n_stars": ns, "n_cells": nc,
+                          "interactions_per_step": inter_step,
+                          "launch_shapes": {m: results[m]["cfg"] for m in results}}
+        kernel_name = f"k_eval_{'mixed' if head == 'mixed' else 'fp64'} (forward, stars -> cells)"
+    else:
+        probs = W.c4(1004)
+        pdev = []
+        for p in probs:
+            pdev.append({side: {k: d(v) for k, v in p[side].items()} for side in ("sources", "targets")})
+        torch.cuda.synchronize()
+        inter_step = float(sum(len(p["sources"]["x"]) * len(p["targets"]["x"]) for p in probs))
+        log(f"c4 generated in {time.perf_counter() - t:.1f} s: {len(probs)} problems, "
+            f"{inter_step:.3e} interactions")
+        for mode in modes:
+            r = run_batch(mode, pdev, args.steps, args.warmup)
+            results[mode] = r
+            log(f"{mode}: {r['ms']:.3f} ms/replay ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
+                f"persistent kernel {r['kernel_ms']:.3f} ms, {r['items']} work items")
+        r = results[head]
+        kern_ms, kern_inter = r["kernel_ms"], inter_step
+        kern_bytes = sum(BYTES_PER_TARGET * len(p["targets"]["x"]) + BYTES_PER_SOURCE * len(p["sources"]["x"])
+                         for p in probs)
+        line["config"] = {"workload": "c4: 64 uneven problems (N_s log-uniform 100-50,000, N_t "
+                                      "log-uniform 1e3-1e6 in 8^3 blocks) in one graph-replayed "
+                                      "persistent launch, acc + pot", "seed": 1004, "mode": head,
+                          "interactions_per_step": inter_step,
+                          "launch_shapes": {m: results[m]["cfg"] for m in results}}
+        kernel_name = f"k_batch_eval<{'mixed' if head == 'mixed' else 'fp64'}> (persistent)"
 
-    head = "mixed" if "mixed" in results else list(results)[0]
-    r = results[head]
-    value = inter_step / (r["ms"] * 1e-3)
-    kern_s = r["kms"] * 1e-3
-    achieved = FLOP_PER_INTERACTION * ns * nc / kern_s / 1e12
+    line["value"] = inter_step / (results[head]["ms"] * 1e-3)
+    line["ms_per_step"] = results[head]["ms"]
+    achieved = FLOP_PER_INTERACTION * kern_inter / (kern_ms * 1e-3) / 1e12
     peak = PEAK_FP32_TFLOPS if head == "mixed" else PEAK_FP64_TFLOPS
-    line = {
-        "metric": metric_name(),
-        "value": value,
-        "unit": "interactions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["ms"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32/f64" if head == "mixed" else "f64",
-        "data": "synthetic (seeded c2 workload)",
-        "config": {"workload": "c2: 10,000 stars (20% binary members) <-> 1,024,000 AMR cell "
-                               "centres (2,000 leaf blocks of 8^3); forward + reverse, acc + pot",
-                   "seed": 1002, "mode": head, "n_stars": ns, "n_cells": nc,
-                   "interactions_per_step": inter_step},
-        "roofline": {"bound": "valu", "kernel": "k_eval_mixed (forward, stars -> cells)"
-                     if head == "mixed" else "k_eval_fp64 (forward)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "flop_per_interaction": FLOP_PER_INTERACTION,
-                     "kernel_ms": r["kms"], "peak_unpacked_fp32": PEAK_FP32_UNPACKED_TFLOPS},
-    }
-    if "fp64" in results:
+    line["roofline"] = {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": peak,
+                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                        "kernel_ms": kern_ms, "flop_per_interaction": FLOP_PER_INTERACTION,
+                        "interactions_per_launch": kern_inter,
+                        "algorithmic_bytes_per_launch": kern_bytes,
+                        "peak_unpacked_fp32": PEAK_FP32_UNPACKED_TFLOPS,
+                        "frac_of_unpacked_fp32": achieved / PEAK_FP32_UNPACKED_TFLOPS}
+    if "fp64" in results and head == "mixed":
         r64 = results["fp64"]
         line["fp64_value"] = inter_step / (r64["ms"] * 1e-3)
-        line["fp64_kernel_ms"] = r64["kms"]
-        line["mixed_over_fp64"] = line["value"] / line["fp64_value"] if head == "mixed" else 1.0
-        line["fp64_roofline_frac"] = FLOP_PER_INTERACTION * ns * nc / (r64["kms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
-        if "mixed" in results:
+        line["mixed_over_fp64"] = line["value"] / line["fp64_value"]
+        k64 = r64["fwd_kernel_ms"] if "fwd_kernel_ms" in r64 else r64["kernel_ms"]
+        line["fp64_kernel_ms"] = k64
+        line["fp64_roofline_frac"] = FLOP_PER_INTERACTION * kern_inter / (k64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
+        if args.workload in ("c2", "c5"):
             errs = {}
-            for k, name in ((0, "fwd"), (1, "rev")):
+            for k, nm in ((0, "fwd"), (1, "rev")):
                 ea, ep = rel_err(results["mixed"]["res"][k], r64["res"][k])
-                errs[name] = dict(acc_median=float(np.median(ea)), acc_max=float(ea.max()),
-                                  pot_median=float(np.median(ep)), pot_max=float(ep.max()))
-                if name == "rev":
+                errs[nm] = err_stats(ea, ep)
+                if nm == "rev":
                     b = st["binary"] >= 0
-                    errs["rev_binary_members"] = dict(acc_median=float(np.median(ea[b])),
-                                                      acc_max=float(ea[b].max()))
-            line["mixed_error_vs_fp64"] = errs
-            line["error_gate_met"] = all(e["acc_median"] <= 1e-6 and e["acc_max"] <= 1e-4
-                                         for e in errs.values())
+                    errs["rev_binary_members"] = err_stats(ea[b])
+        else:
+            pairs = [rel_err(m, q) for m, q in zip(results["mixed"]["res"], r64["res"]) if len(q[0])]
+            errs = {"all_problems": err_stats(np.concatenate([p[0] for p in pairs]),
+                                              np.concatenate([p[1] for p in pairs]))}
+        line["mixed_error_vs_fp64"] = errs
+        line["error_gate_met"] = all(e["acc_median"] <= 1e-6 and e["acc_max"] <= 1e-4 for e in errs.values())
     if not args.no_cpu:
-        if "fp64" in results:
-            line["fp64_vs_oracle"] = oracle_check(st, ce, results["fp64"]["res"], G)
-            log("fp64 vs oracle:", line["fp64_vs_oracle"])
-        cb, cores = cpu_baseline(st, ce, G, args.cpu_seconds)
-        line["cpu_baseline"] = {
-            "value": cb[cores]["value"], "unit": "interactions/s", "cores": cores, "kind": "port",
-            "sample": f"forward c2 direction, {cb[cores]['targets']} seeded cell targets x {ns} stars "
-                      f"(acc + pot, FP64, OpenMP over targets); one thread: "
-                      f"{cb[1]['value']:.3e} interactions/s on {cb[1]['targets']} targets",
-            "scalar_value": cb[1]["value"]}
+        if args.workload in ("c2", "c5"):
+            S = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
+            C = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
+            if "fp64" in results:
+                line["fp64_vs_oracle"] = oracle_check(
+                    [(S, C, results["fp64"]["res"][0], 1024 if args.workload == "c2" else 256),
+                     (C, S, results["fp64"]["res"][1], 96 if args.workload == "c2" else 24)], G)
+                log("fp64 vs oracle:", line["fp64_vs_oracle"])
+            line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, f"forward {args.workload} direction")
+        else:
+            if "fp64" in results:
+                chk = [(probs[i]["sources"], probs[i]["targets"], results["fp64"]["res"][i], 48)
+                       for i in (0, 9, 21, 40, 63)]
+                line["fp64_vs_oracle"] = oracle_check(chk, G)
+                log("fp64 vs oracle:", line["fp64_vs_oracle"])
+            big = max(range(len(probs)), key=lambda i: len(probs[i]["sources"]["x"]) * len(probs[i]["targets"]["x"]))
+            line["cpu_baseline"] = cpu_baseline(probs[big]["sources"], probs[big]["targets"], G,
+                                                args.cpu_seconds, f"c4 problem {big}")
     print(json.dumps(line), flush=True)
 
 

@@ -160,6 +160,16 @@ int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions);
  * The kernel variant is the one tgrav_set_config chose for the mode.  Set by the autotuner. */
 int tgrav_set_batch_config(tgrav_ctx* ctx, double item_interactions, int wg_per_cu);
 
+/* ---- instrumentation (no counterpart in the reference) ------------------ */
+/* on != 0: record HIP events on the context's stream around every evaluation kernel
+ * (k_eval_*: the direct-sum kernel alone, not the chunk reduction), up to 256 launches;
+ * also clears earlier records. */
+int tgrav_set_timing(tgrav_ctx* ctx, int on);
+/* Copies up to `max` records (milliseconds, targets, sources of each timed launch; waits
+ * for them), clears the list and returns the number of records taken (>= 0). */
+int tgrav_timing_read(tgrav_ctx* ctx, int max, double* ms, int64_t* n_targets,
+                      int64_t* n_sources);
+
 #ifdef __cplusplus
 }
 #endif

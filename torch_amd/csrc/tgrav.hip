@@ -150,7 +150,39 @@ struct tgrav_ctx {
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
   DevBuf in[8], outb[4], vel[3], part, flag;
   unsigned long long* host_flag = nullptr;   // pinned
+  // instrumentation: HIP events around each evaluation kernel (tgrav_set_timing)
+  int timing = 0;
+  std::vector<hipEvent_t> ev;                // 2 per slot
+  std::vector<int64_t> ev_nt, ev_ns;
+  int ev_used = 0;
 };
+
+static const int kTimingSlots = 256;
+
+template <class F>
+static int timed_launch_on(tgrav_ctx* c, hipStream_t s, int64_t nt, int64_t ns, F launch) {
+  if (!c->timing || c->ev_used >= kTimingSlots) return launch();
+// Records an event pair around `launch` when timing is on (slots beyond the pool are not timed).
+  if (c->ev.empty()) {
+    c->ev.resize(2 * kTimingSlots, nullptr);
+    for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+    c->ev_nt.resize(kTimingSlots);
+    c->ev_ns.resize(kTimingSlots);
+  }
+  const int k = c->ev_used++;
+  HIPCHECK(hipEventRecord(c->ev[2 * k], s));
+  int rc = launch();
+  if (rc) return rc;
+  HIPCHECK(hipEventRecord(c->ev[2 * k + 1], s));
+  c->ev_nt[k] = nt;
+  c->ev_ns[k] = ns;
+  return 0;
+}
+
+template <class F>
+static int timed_launch(tgrav_ctx* c, int64_t nt, int64_t ns, F launch) {
+  return timed_launch_on(c, c->stream, nt, ns, launch);
+}
 
 // ------------------------------------------------------------------ helpers ------
 static int set_device(tgrav_ctx* c) {
@@ -263,16 +295,20 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
   tg::Out o = O;
   if (nsplit > 1) {
-    int rc = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
-    if (rc) return rc;
+    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+    if (rc0) return rc0;
     o.part = c->part.as<double>();
     o.pstride = nt;
   } else {
     o.part = nullptr;
   }
-  v->fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v->nt), c->stream, T, c->s4.as<double4>(),
-        c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
-  LAUNCHCHECK(mode == TGRAV_MIXED ? "k_eval_mixed" : "k_eval_fp64");
+  int rc = timed_launch(c, nt, ns, [&]() -> int {
+    v->fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v->nt), c->stream, T, c->s4.as<double4>(),
+          c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
+    LAUNCHCHECK(mode == TGRAV_MIXED ? "k_eval_mixed" : "k_eval_fp64");
+    return 0;
+  });
+  if (rc) return rc;
   if (nsplit > 1) {
     tg::Out fin = O;
     fin.part = nullptr;
@@ -330,6 +366,8 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->part.release();
   c->stat.release();
   c->flag.release();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
@@ -608,6 +646,32 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t, const double*, const double*, con
   return fail(TGRAV_ELIMIT, "tgrav_bidirectional: not built yet");
 }
 
+int tgrav_set_timing(tgrav_ctx* c, int on) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  c->timing = on ? 1 : 0;
+  c->ev_used = 0;
+  return 0;
+}
+
+int tgrav_timing_read(tgrav_ctx* c, int max, double* ms, int64_t* nt, int64_t* ns) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  if (max < 0 || (max > 0 && !ms)) return fail(TGRAV_EINVAL, "bad output buffer");
+  int rc = set_device(c);
+  if (rc) return rc;
+  const int n = std::min(max, c->ev_used);
+  for (int k = 0; k < n; ++k) {
+    HIPCHECK(hipEventSynchronize(c->ev[2 * k + 1]));
+    float f = 0.f;
+    HIPCHECK(hipEventElapsedTime(&f, c->ev[2 * k], c->ev[2 * k + 1]));
+    ms[k] = f;
+    if (nt) nt[k] = c->ev_nt[k];
+    if (ns) ns[k] = c->ev_ns[k];
+  }
+  const int total = c->ev_used;
+  c->ev_used = 0;
+  return total;
+}
+
 int tgrav_set_batch_config(tgrav_ctx* c, double item_interactions, int wg_per_cu) {
   if (!c) return fail(TGRAV_EINVAL, "context is NULL");
   if (!(item_interactions >= 0.0) || wg_per_cu < 0)
@@ -651,7 +715,7 @@ static int upload(DevBuf& d, const std::vector<T>& v) {
 }
 
 // Enqueues reset -> pack -> persistent eval -> chunk reduction on `s`.
-static int batch_enqueue(tgrav_batch* b, hipStream_t s) {
+static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   tg::k_batch_reset<<<grid1(std::max(b->nprob, 1), 256), 256, 0, s>>>(
       b->nprob, b->counter.as<unsigned int>(), b->stats.as<unsigned long long>());
   LAUNCHCHECK("k_batch_reset");
@@ -662,10 +726,15 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s) {
     LAUNCHCHECK("k_batch_pack");
   }
   if (b->n_items > 0) {
-    b->v->bfn(b->grid, s, b->probs.as<tg::BProb>(), b->items.as<tg::BItem>(), b->n_items,
-              b->counter.as<unsigned int>(), b->s4.as<double4>(), b->se2.as<double>(),
-              b->stats.as<unsigned long long>(), b->part.as<double>());
-    LAUNCHCHECK("k_batch_eval");
+    auto go = [&]() -> int {
+      b->v->bfn(b->grid, s, b->probs.as<tg::BProb>(), b->items.as<tg::BItem>(), b->n_items,
+                b->counter.as<unsigned int>(), b->s4.as<double4>(), b->se2.as<double>(),
+                b->stats.as<unsigned long long>(), b->part.as<double>());
+      LAUNCHCHECK("k_batch_eval");
+      return 0;
+    };
+    int rc = timed ? timed_launch_on(b->c, s, b->n_items, b->total_src, go) : go();
+    if (rc) return rc;
   }
   if (b->nred > 0 && b->red_total > 0) {
     tg::k_batch_reduce<<<grid1(b->red_total, 256), 256, 0, s>>>(
@@ -801,10 +870,10 @@ int tgrav_batch_run(tgrav_batch* b, int use_graph) {
   tgrav_ctx* c = b->c;
   int rc = set_device(c);
   if (rc) return rc;
-  if (!use_graph) return batch_enqueue(b, c->stream);
+  if (!use_graph) return batch_enqueue(b, c->stream, true);
   if (!b->exec) {
     HIPCHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    rc = batch_enqueue(b, b->cap);
+    rc = batch_enqueue(b, b->cap, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (rc) {

@@ -260,6 +260,20 @@ class GravityField:
                              f"{name} must be a writable contiguous float64 array of length {n} "
                              "on the same device as the positions")
 
+    def set_timing(self, on=True):
+        """Record HIP events around every evaluation kernel launch (instrumentation)."""
+        _lib.check(self.lib.tgrav_set_timing(self._ctx, int(bool(on))), "tgrav_set_timing")
+
+    def kernel_times(self, max_records=256):
+        """[(ms, n_targets, n_sources)] of the launches timed since set_timing/last read."""
+        ms = (ctypes.c_double * max_records)()
+        nt = (ctypes.c_int64 * max_records)()
+        ns = (ctypes.c_int64 * max_records)()
+        n = self.lib.tgrav_timing_read(self._ctx, max_records, ms, nt, ns)
+        if n < 0:
+            _lib.check(n, "tgrav_timing_read")
+        return [(ms[i], nt[i], ns[i]) for i in range(min(n, max_records))]
+
     def synchronize(self):
         _lib.check(self.lib.tgrav_synchronize(self._ctx), "tgrav_synchronize")
 
