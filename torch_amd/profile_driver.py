@@ -1,0 +1,63 @@
+"""Short, fixed GPU workload for counter runs (rocprofv3 --pmc / --kernel-trace).
+
+    python -m torch_amd.profile_driver --workload c2 --runs mixed:fwd fp64:fwd:1 --reps 3 --copy-mb 512
+
+Each run spec is MODE:DIRECTION[:VARIANT[:NSPLIT]] (direction fwd = stars -> cells,
+rev = cells -> stars; variant/nsplit default to the tuned shape).  After the evaluations
+it performs one plain device-to-device copy of --copy-mb MiB (torch copy_), whose
+FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters against a known byte count.
+Everything runs on torch's current stream; the program prints one line per step.
+"""
+import argparse
+import time
+
+import numpy as np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c5"])
+    ap.add_argument("--runs", nargs="+", default=["mixed:fwd", "fp64:fwd"])
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-mb", type=int, default=512)
+    args = ap.parse_args()
+
+    import torch
+
+    from . import autotune, workloads as W
+    from .field import GravityField
+    w = {"c1": W.c1, "c2": W.c2, "c5": W.c5}[args.workload]()
+    st, ce = w["stars"], w["cells"]
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    S = [d(st[k]) for k in ("x", "y", "z", "m", "eps")]
+    C = [d(ce[k]) for k in ("x", "y", "z", "m", "eps")]
+    torch.cuda.synchronize()
+    for spec in args.runs:
+        parts = spec.split(":")
+        mode, direction = parts[0], parts[1]
+        src, tgt = (S, C) if direction == "fwd" else (C, S)
+        with GravityField(mode, validate=False) as f:
+            e = autotune.apply_eval(f, src[0].numel(), tgt[0].numel())
+            if len(parts) > 2:
+                f.set_config(variant=int(parts[2]), nsplit=int(parts[3]) if len(parts) > 3 else 0)
+            f.set_sources(*src)
+            f.set_timing(True)
+            for r in range(args.reps):
+                f.evaluate(None, tgt[0], tgt[1], tgt[2])
+            torch.cuda.synchronize()
+            ks = f.kernel_times()
+            print(f"{spec}: tuned={e} kernel ms={[round(k[0], 3) for k in ks]}", flush=True)
+    if args.copy_mb > 0:
+        n = args.copy_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copy {args.copy_mb} MiB read + {args.copy_mb} MiB write in "
+              f"{(time.perf_counter() - t) * 1e3:.3f} ms", flush=True)
+
+
+if __name__ == "__main__":
+    main()
