@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Pairwise interactions/s of the direct-sum gravity field on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|c5] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|c5|c2fused] [--no-cpu]
 
 Default workload c2 (seed 1002): 10,000 Plummer/Kroupa stars (20% hard-binary members) and
 1,024,000 AMR cell centres (2,000 leaf blocks of 8^3).  One step is one bridge field
@@ -11,6 +11,8 @@ loading its source set.  2 x 1.024e10 pair interactions per step.
   c5: 100,000 stars <-> 4,194,304 cells (2 x 4.19e11 interactions per step).
   c4: 64 uneven independent problems in one graph-replayed persistent launch (one step =
       one replay, acc + pot of all problems).
+  c2fused: c2 with symmetric softening (eps2 = eps_cell^2 + eps_star^2) evaluated by the fused
+      bidirectional pass (one pass gives both directions); two directed passes timed beside it.
 Inputs are resident in HBM (device tensors) before timing.
 
 The headline `value` is the mixed-precision rate; the plain-FP64 kernel is timed the same
@@ -143,6 +145,49 @@ def run_batch(mode, probs_dev, steps, warmup):
     return info
 
 
+# ---------------------------------------------------------------- c2fused ----
+def run_fused(mode, dev, ns, nc, steps, warmup):
+    """One fused pass per step (cells <-> stars, eps2 = eps_cell^2 + eps_star^2); the same
+    fields from two directed evaluations are timed beside it."""
+    import torch
+
+    from torch_amd import GravityField
+    f = GravityField(mode, validate=False)
+    S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
+    C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
+
+    def fused():
+        return f.bidirectional(*C, *S)
+
+    def directed():
+        f.set_sources(*S)
+        a = f.evaluate(C[4], C[0], C[1], C[2])
+        f.set_sources(*C)
+        b = f.evaluate(S[4], S[0], S[1], S[2])
+        return a, b
+
+    res = {}
+    for name, fn in (("fused", fused), ("directed", directed)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        f.set_timing(True)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            out = fn()
+        t1.record()
+        torch.cuda.synchronize()
+        ks = f.kernel_times()
+        f.set_timing(False)
+        res[name] = dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.mean([k[0] for k in ks])),
+                         res=tuple(tuple(a.cpu().numpy() for a in o) for o in out))
+    f.close()
+    r = res["fused"]
+    return dict(ms=r["ms"], fwd_kernel_ms=r["kernel_ms"], res=r["res"], directed_ms=res["directed"]["ms"],
+                directed_res=res["directed"]["res"], cfg=None)
+
+
 # ----------------------------------------------------------- checker legs ----
 def cpu_baseline(S, T, G, budget_s, label):
     """The oracle (C port of the reference loop nest, -O2) on seeded target subsamples:
@@ -198,7 +243,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "c2fused"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--modes", default="mixed,fp64")
@@ -221,8 +266,8 @@ def main():
             "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32/f64" if head == "mixed" else "f64", "data": f"synthetic (seeded {args.workload})"}
     results = {}
-    if args.workload in ("c2", "c5"):
-        w = W.c2(1002) if args.workload == "c2" else W.c5(1005)
+    if args.workload in ("c2", "c5", "c2fused"):
+        w = W.c5(1005) if args.workload == "c5" else W.c2(1002)
         st, ce = w["stars"], w["cells"]
         ns, nc = len(st["x"]), len(ce["x"])
         log(f"{args.workload} generated in {time.perf_counter() - t:.1f} s: {ns} stars, {nc} cells")
@@ -231,23 +276,35 @@ def main():
         torch.cuda.synchronize()
         inter_step = 2.0 * ns * nc
         for mode in modes:
-            r = run_pair(mode, dev, ns, nc, args.steps, args.warmup)
+            if args.workload == "c2fused":
+                r = run_fused(mode, dev, ns, nc, args.steps, args.warmup)
+                log(f"{mode}: fused {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
+                    f"kernel {r['fwd_kernel_ms']:.3f} ms; two directed passes {r['directed_ms']:.3f} ms")
+            else:
+                r = run_pair(mode, dev, ns, nc, args.steps, args.warmup)
+                log(f"{mode}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s); "
+                    f"eval kernels fwd {r['fwd_kernel_ms']:.3f} ms, rev {r['rev_kernel_ms']:.3f} ms")
             results[mode] = r
-            log(f"{mode}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s); "
-                f"eval kernels fwd {r['fwd_kernel_ms']:.3f} ms, rev {r['rev_kernel_ms']:.3f} ms")
         r = results[head]
         kern_ms = r["fwd_kernel_ms"]
-        kern_inter = float(ns) * nc
+        kern_inter = float(ns) * nc * (2 if args.workload == "c2fused" else 1)
         kern_bytes = BYTES_PER_TARGET * nc + BYTES_PER_SOURCE * ns
+        if args.workload == "c2fused":
+            kern_bytes = (BYTES_PER_SOURCE + 32) * (nc + ns)
         name = ("10,000 stars (20% binary members) <-> 1,024,000 AMR cell centres (2,000 leaf "
-                "blocks of 8^3)" if args.workload == "c2" else
+                "blocks of 8^3)" if args.workload != "c5" else
                 "100,000 stars (30% binary members) <-> 4,194,304 AMR cell centres (8,192 leaf "
                 "blocks of 8^3)")
+        if args.workload == "c2fused":
+            name += ", symmetric softening eps2 = eps_cell^2 + eps_star^2, one fused pass per step"
         line["config"] = {"workload": f"{args.workload}: {name}; forward + reverse, acc + pot",
                           "seed": w["seed"], "mode": head, "n_stars": ns, "n_cells": nc,
                           "interactions_per_step": inter_step,
                           "launch_shapes": {m: results[m]["cfg"] for m in results}}
         kernel_name = f"k_eval_{'mixed' if head == 'mixed' else 'fp64'} (forward, stars -> cells)"
+        if args.workload == "c2fused":
+            kernel_name = f"k_bidir<{'mixed' if head == 'mixed' else 'fp64'}> (both directions)"
+            line["directed_two_pass_ms"] = {m: results[m]["directed_ms"] for m in results}
     else:
         probs = W.c4(1004)
         pdev = []
@@ -291,7 +348,7 @@ def main():
         k64 = r64["fwd_kernel_ms"] if "fwd_kernel_ms" in r64 else r64["kernel_ms"]
         line["fp64_kernel_ms"] = k64
         line["fp64_roofline_frac"] = FLOP_PER_INTERACTION * kern_inter / (k64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
-        if args.workload in ("c2", "c5"):
+        if args.workload in ("c2", "c5", "c2fused"):
             errs = {}
             for k, nm in ((0, "fwd"), (1, "rev")):
                 ea, ep = rel_err(results["mixed"]["res"][k], r64["res"][k])
@@ -306,9 +363,11 @@ def main():
         line["mixed_error_vs_fp64"] = errs
         line["error_gate_met"] = all(e["acc_median"] <= 1e-6 and e["acc_max"] <= 1e-4 for e in errs.values())
     if not args.no_cpu:
-        if args.workload in ("c2", "c5"):
+        if args.workload in ("c2", "c5", "c2fused"):
             S = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
             C = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
+            if args.workload == "c2fused":      # symmetric softening: targets carry their eps too
+                S["eps_t"], C["eps_t"] = st["eps"], ce["eps"]
             if "fp64" in results:
                 line["fp64_vs_oracle"] = oracle_check(
                     [(S, C, results["fp64"]["res"][0], 1024 if args.workload == "c2" else 256),

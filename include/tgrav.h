@@ -141,6 +141,9 @@ int tgrav_bidirectional(tgrav_ctx* ctx, int64_t nc, const double* cx, const doub
                         const double* pm, const double* pe, double* cax, double* cay,
                         double* caz, double* cphi, double* pax, double* pay, double* paz,
                         double* pphi, int mode, int ptr_kind);
+/* Fused-pass launch shape (variant, particle chunks; -1 keeps, 0 = automatic chunks). */
+int tgrav_set_bidir_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
+int tgrav_bidir_variant_info(int mode, int variant, int* targets_per_lane, int* threads);
 
 /* ---- batched uneven evaluations --------------------------------------- */
 /* Many independent problems in one persistent launch (device pointers only).  The plan

@@ -1,0 +1,109 @@
+"""Fused bidirectional pass (needs an MI355X): both fields from one pass over the pairs,
+checked against the oracle's restatement of Particles_sinkAccelGasOnSinksAndSinksOnGas,
+against two directed evaluations, and against Newton's third law."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import _util
+from torch_amd import GravityField, _lib
+from torch_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+G = W.G_PC_MSUN_KMS
+
+
+def cells_and_particles(seed, nblk=3, npart=300):
+    rng = W.rng_for(seed)
+    o = rng.uniform(-1.5, 1.0, (nblk, 3))
+    size = rng.choice([0.125, 0.25, 0.5], nblk)
+    x, y, z, dx = W.block_cell_centres(o, size)
+    rho = 50.0 * np.exp(rng.normal(size=x.size))
+    C = dict(x=x, y=y, z=z, m=rho * dx ** 3, eps=dx)
+    st = W.star_cluster(rng, npart, binary_frac=0.2)
+    P = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
+    return C, P
+
+
+def run(f, C, P):
+    return f.bidirectional(C["x"], C["y"], C["z"], C["m"], C["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("nblk,npart", [(1, 1), (1, 300), (3, 257), (5, 1000)])
+def test_matches_oracle(mode, nblk, npart):
+    C, P = cells_and_particles(nblk * 100 + npart, nblk, npart)
+    with GravityField(mode) as f:
+        co, po = run(f, C, P)
+    (pa, pb, pc, pp), (ca, cb, cc, cp) = oracle.bidirectional(
+        C["x"], C["y"], C["z"], C["m"], C["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"], G)
+    for name, got, ref in (("cells", co, (ca, cb, cc, cp)), ("particles", po, (pa, pb, pc, pp))):
+        s = _util.summary(got, ref)
+        print(mode, nblk, npart, name, s, flush=True)
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (name, s)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_every_variant_and_chunking_agree(mode):
+    C, P = cells_and_particles(7, 6, 2000)
+    lib = _lib.load()
+    ref = None
+    with GravityField(mode) as f:
+        for v in range(4):
+            for ns in (1, 0, 5):
+                f.set_bidir_config(variant=v, nsplit=ns)
+                co, po = run(f, C, P)
+                if ref is None:
+                    ref = (co, po)
+                    continue
+                tol = 1e-12 if mode == "fp64" else 1e-5
+                for got, r in ((co, ref[0]), (po, ref[1])):
+                    s = _util.summary(got, r)
+                    assert s["acc_max"] <= tol and s["pot_max"] <= tol, (v, ns, s)
+        assert lib.tgrav_bidir_variant_info(f.mode, 4, None, None) == _lib.EINVAL
+
+
+def test_c2_fused_equals_directed_and_third_law():
+    w = W.c2(1002)
+    st, ce = w["stars"], w["cells"]
+    with GravityField("fp64") as f:
+        co, po = f.bidirectional(ce["x"], ce["y"], ce["z"], ce["m"], ce["eps"],
+                                 st["x"], st["y"], st["z"], st["m"], st["eps"])
+        f.set_sources(st["x"], st["y"], st["z"], st["m"], st["eps"])
+        dc = f.evaluate(ce["eps"], ce["x"], ce["y"], ce["z"])
+        f.set_sources(ce["x"], ce["y"], ce["z"], ce["m"], ce["eps"])
+        dp = f.evaluate(st["eps"], st["x"], st["y"], st["z"])
+    for name, got, ref in (("cells", co, dc), ("stars", po, dp)):
+        s = _util.summary(got, ref)
+        print("c2 fused vs directed", name, s, flush=True)
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+    for q in range(3):
+        fc, fp = np.sum(ce["m"] * co[q]), np.sum(st["m"] * po[q])
+        scale = np.sum(np.abs(ce["m"] * co[q])) + np.sum(np.abs(st["m"] * po[q]))
+        assert abs(fc + fp) <= 1e-12 * scale
+
+
+def test_c2_fused_mixed_error():
+    w = W.c2(1002)
+    st, ce = w["stars"], w["cells"]
+    args = (ce["x"], ce["y"], ce["z"], ce["m"], ce["eps"], st["x"], st["y"], st["z"], st["m"], st["eps"])
+    with GravityField("fp64") as f:
+        ref = f.bidirectional(*args)
+    with GravityField("mixed") as f:
+        got = f.bidirectional(*args)
+    for k, name in ((0, "cells"), (1, "stars")):
+        s = _util.summary(got[k], ref[k])
+        print("c2 fused mixed", name, s, flush=True)
+        assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+
+
+def test_empty_sides():
+    C, P = cells_and_particles(9, 1, 10)
+    with GravityField("fp64") as f:
+        co, po = f.bidirectional(C["x"], C["y"], C["z"], C["m"], C["eps"], [], [], [], [], None)
+        assert all(np.all(a == 0) for a in co) and all(a.size == 0 for a in po)
+        co, po = f.bidirectional([], [], [], [], None, P["x"], P["y"], P["z"], P["m"], P["eps"])
+        assert all(np.all(a == 0) for a in po)

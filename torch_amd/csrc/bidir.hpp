@@ -1,0 +1,236 @@
+// Fused bidirectional gas <-> particle direct sum, the GPU form of
+// Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:204-316: every (cell, particle) pair is
+// evaluated once and feeds both the cell's field (from the particles) and the particle's
+// field (from the cells), with the symmetric softening eps2 = eps_c^2 + eps_p^2.
+//
+// Mapping: a workgroup holds K*NT cells (K per lane, like the directed kernels) and streams
+// the particles through LDS in tiles of NT.  The particle-side sums cannot stay in
+// registers (each lane sees every particle), so each wave owns a private FP64 accumulator
+// copy of the tile in LDS and walks the tile in a rotated order: at step s of a 64-particle
+// block, lane l takes particle (l + s) mod 64.  In every step the 64 lanes touch 64
+// different accumulators, so the read-add-write needs no atomics and the order in which a
+// particle's contributions are added is fixed.  After the tile the wave copies are summed in
+// wave order and written as this cell tile's partial sums; a second kernel adds the cell
+// tiles' partials per particle in tile order.  Everything is deterministic.
+#pragma once
+#include "kernels.hpp"
+
+namespace tg {
+
+struct Cells {
+  const double *x, *y, *z, *m, *eps;
+  int64_t n;
+};
+
+template <int NT, bool MIXED>
+struct LdsBi {
+  typename std::conditional<MIXED, float4, double4>::type p4[NT];   // particle x, y, z, m
+  typename std::conditional<MIXED, float, double>::type pe2[NT];    // particle eps^2 (+ uniform cell eps^2)
+  double acc[NT / 64][NT][4];                                        // per-wave particle sums
+  double red[2 * 4 * (NT / 64)];
+};
+
+// One pair, both directions.  d = x_p - x_c.  Cell side: (cx, cy, cz, cp) += m_p y^3 d, m_p y;
+// particle side: (ux, uy, uz, up) += m_c y^3 d, m_c y (the particle's acceleration is -G u).
+template <class T, bool GUARD>
+__device__ __forceinline__ void bi_pair(T px, T py, T pz, T pm, T r2e, T qx, T qy, T qz, T qm,
+                                        T& cx, T& cy, T& cz, T& cp, T& ux, T& uy, T& uz, T& up) {
+  const T dx = px - qx, dy = py - qy, dz = pz - qz;
+  T r2;
+  T y;
+  if constexpr (sizeof(T) == 4) {
+    r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, r2e)));
+    y = __builtin_amdgcn_rsqf(r2);
+    if (GUARD) y = r2 > 0.0f ? y : 0.0f;
+  } else {
+    r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, r2e)));
+    y = rsqrt_f64<GUARD>(r2);
+  }
+  const T y3 = (y * y) * y;
+  const T wp = pm * y3;
+  cp = fma(pm, y, cp);
+  cx = fma(wp, dx, cx);
+  cy = fma(wp, dy, cy);
+  cz = fma(wp, dz, cz);
+  const T wc = qm * y3;
+  up = fma(qm, y, up);
+  ux = fma(wc, dx, ux);
+  uy = fma(wc, dy, uy);
+  uz = fma(wc, dz, uz);
+}
+
+template <int K, int NT, bool MIXED, bool GUARD, bool UNIF>
+__device__ __forceinline__ void bi_particles(
+    LdsBi<NT, MIXED>& L, const double4* __restrict__ p4, const double* __restrict__ pe2,
+    int64_t p_begin, int64_t p_end, double e2u, double ax_, double ay_, double az_,
+    const typename std::conditional<MIXED, float, double>::type (&qx)[K],
+    const typename std::conditional<MIXED, float, double>::type (&qy)[K],
+    const typename std::conditional<MIXED, float, double>::type (&qz)[K],
+    const typename std::conditional<MIXED, float, double>::type (&qm)[K],
+    const typename std::conditional<MIXED, float, double>::type (&qe2)[K],
+    double (&hx)[K], double (&hy)[K], double (&hz)[K], double (&hp)[K],
+    double* __restrict__ ppart, int64_t pstride) {
+  typedef typename std::conditional<MIXED, float, double>::type T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int64_t base = p_begin; base < p_end; base += NT) {
+    __syncthreads();
+    const int64_t j = base + tid;
+    if (j < p_end) {
+      const double4 s = p4[j];
+      if constexpr (MIXED)
+        L.p4[tid] = make_float4((float)(s.x - ax_), (float)(s.y - ay_), (float)(s.z - az_), (float)s.w);
+      else
+        L.p4[tid] = s;
+      L.pe2[tid] = (T)(pe2[j] + e2u);
+    } else {                      // padding particle: zero mass, eps^2 = 1
+      if constexpr (MIXED) L.p4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+      else L.p4[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+      L.pe2[tid] = (T)1.0;
+    }
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+      L.acc[w][tid][0] = 0.0;
+      L.acc[w][tid][1] = 0.0;
+      L.acc[w][tid][2] = 0.0;
+      L.acc[w][tid][3] = 0.0;
+    }
+    __syncthreads();
+    for (int blk = 0; blk < NT; blk += 64) {
+      T cx[K], cy[K], cz[K], cp[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) cx[k] = cy[k] = cz[k] = cp[k] = (T)0;
+      for (int s = 0; s < 64; ++s) {
+        const int jj = blk + ((lane + s) & 63);
+        const auto P = L.p4[jj];
+        const T e2 = L.pe2[jj];
+        T ux = 0, uy = 0, uz = 0, up = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          bi_pair<T, GUARD>(P.x, P.y, P.z, P.w, UNIF ? e2 : e2 + qe2[k], qx[k], qy[k], qz[k], qm[k],
+                            cx[k], cy[k], cz[k], cp[k], ux, uy, uz, up);
+        double* a = L.acc[wave][jj];
+        a[0] += (double)ux;
+        a[1] += (double)uy;
+        a[2] += (double)uz;
+        a[3] += (double)up;
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        hx[k] += (double)cx[k];
+        hy[k] += (double)cy[k];
+        hz[k] += (double)cz[k];
+        hp[k] += (double)cp[k];
+      }
+    }
+    __syncthreads();
+    if (j < p_end) {              // this cell tile's partial sums for particle j, wave order
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) {
+        s0 += L.acc[w][tid][0];
+        s1 += L.acc[w][tid][1];
+        s2 += L.acc[w][tid][2];
+        s3 += L.acc[w][tid][3];
+      }
+      ppart[j] = s0;
+      ppart[pstride + j] = s1;
+      ppart[2 * pstride + j] = s2;
+      ppart[3 * pstride + j] = s3;
+    }
+  }
+}
+
+// blockIdx.x = cell tile, blockIdx.y = particle chunk.  Cell results go to CO (final, or
+// chunk partials when CO.part is set); particle partials go to
+// ppart[(tile * 4 + q) * np + j].
+template <int K, int NT, bool MIXED>
+__global__ __launch_bounds__(NT) void k_bidir(Cells C, const double4* __restrict__ p4,
+                                              const double* __restrict__ pe2, int64_t np,
+                                              int64_t chunk, const double* e2p_min_p, Out CO,
+                                              double* __restrict__ ppart) {
+  typedef typename std::conditional<MIXED, float, double>::type T;
+  __shared__ LdsBi<NT, MIXED> L;
+  const int tid = threadIdx.x;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t pb = (int64_t)blockIdx.y * chunk;
+  const int64_t pe = pb + chunk < np ? pb + chunk : np;
+  const double e2p_min = *e2p_min_p;
+  double px[K], py[K], pz[K], pm[K], pq[K];
+  double mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  double mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    const bool ok = t < C.n;
+    const int64_t tc = ok ? t : C.n - 1;
+    px[k] = C.x[tc];
+    py[k] = C.y[tc];
+    pz[k] = C.z[tc];
+    pm[k] = ok ? C.m[tc] : 0.0;          // a padding cell pulls on nothing
+    const double e = C.eps ? C.eps[tc] : 0.0;
+    pq[k] = e * e;
+    if (ok) {
+      mn[0] = fmin(mn[0], px[k]); mx[0] = fmax(mx[0], px[k]);
+      mn[1] = fmin(mn[1], py[k]); mx[1] = fmax(mx[1], py[k]);
+      mn[2] = fmin(mn[2], pz[k]); mx[2] = fmax(mx[2], pz[k]);
+      mn[3] = fmin(mn[3], pq[k]); mx[3] = fmax(mx[3], pq[k]);
+    }
+  }
+  block_minmax<NT, 4>(mn, mx, L.red);
+  double ax_ = 0.0, ay_ = 0.0, az_ = 0.0;
+  if (MIXED) {
+    ax_ = 0.5 * (mn[0] + mx[0]);
+    ay_ = 0.5 * (mn[1] + mx[1]);
+    az_ = 0.5 * (mn[2] + mx[2]);
+  }
+  const bool unif = mn[3] == mx[3];
+  const bool guard = MIXED ? !(e2p_min + mn[3] > TG_F32_TINY_E2) : !(e2p_min + mn[3] > 0.0);
+  const double e2u = unif ? mn[3] : 0.0;
+  T qx[K], qy[K], qz[K], qm[K], qe2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    qx[k] = (T)(px[k] - ax_);
+    qy[k] = (T)(py[k] - ay_);
+    qz[k] = (T)(pz[k] - az_);
+    qm[k] = (T)pm[k];
+    qe2[k] = (T)pq[k];
+  }
+  double hx[K], hy[K], hz[K], hp[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
+  double* pp = ppart + (int64_t)blockIdx.x * 4 * np;
+  if (unif) {
+    if (guard) bi_particles<K, NT, MIXED, true, true>(L, p4, pe2, pb, pe, e2u, ax_, ay_, az_, qx, qy, qz, qm, qe2, hx, hy, hz, hp, pp, np);
+    else bi_particles<K, NT, MIXED, false, true>(L, p4, pe2, pb, pe, e2u, ax_, ay_, az_, qx, qy, qz, qm, qe2, hx, hy, hz, hp, pp, np);
+  } else {
+    if (guard) bi_particles<K, NT, MIXED, true, false>(L, p4, pe2, pb, pe, 0.0, ax_, ay_, az_, qx, qy, qz, qm, qe2, hx, hy, hz, hp, pp, np);
+    else bi_particles<K, NT, MIXED, false, false>(L, p4, pe2, pb, pe, 0.0, ax_, ay_, az_, qx, qy, qz, qm, qe2, hx, hy, hz, hp, pp, np);
+  }
+  Out o = CO;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    if (t < C.n) store_result(o, t, hx[k], hy[k], hz[k], hp[k]);
+  }
+}
+
+// Particle side: sum the ntiles cell-tile partials of each particle (tile order), then the
+// particle's field is (-G u, -G up).
+__global__ void k_bidir_reduce(const double* __restrict__ ppart, int ntiles, int64_t np, Out O) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    const double* p = ppart + (int64_t)t * 4 * np;
+    s0 += p[j];
+    s1 += p[np + j];
+    s2 += p[2 * np + j];
+    s3 += p[3 * np + j];
+  }
+  Out o = O;
+  o.part = nullptr;
+  store_result(o, j, -s0, -s1, -s2, s3);
+}
+
+}  // namespace tg

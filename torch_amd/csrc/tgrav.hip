@@ -18,6 +18,7 @@
 #include "../../include/tgrav.h"
 #include "kernels.hpp"
 #include "batch.hpp"
+#include "bidir.hpp"
 
 #define TGRAV_VERSION 1
 
@@ -120,6 +121,25 @@ const Variant kMixedVariants[] = {MIXV(2, 256), MIXV(4, 256), MIXV(8, 256),
                                   MIXV(4, 128), MIXV(8, 128), MIXV(16, 64)};
 #undef FP64V
 #undef MIXV
+typedef void (*bidir_fn)(dim3, dim3, hipStream_t, const tg::Cells&, const double4*, const double*,
+                         int64_t, int64_t, const double*, const tg::Out&, double*);
+template <int K, int NT, bool MIXED>
+void launch_bidir(dim3 g, dim3 b, hipStream_t s, const tg::Cells& C, const double4* p4,
+                  const double* pe2, int64_t np, int64_t chunk, const double* e2m,
+                  const tg::Out& CO, double* ppart) {
+  tg::k_bidir<K, NT, MIXED><<<g, b, 0, s>>>(C, p4, pe2, np, chunk, e2m, CO, ppart);
+}
+struct BiVariant {
+  int k, nt;
+  bidir_fn fn;
+};
+const BiVariant kBiVariants[2][4] = {
+    {{2, 256, launch_bidir<2, 256, false>}, {4, 256, launch_bidir<4, 256, false>},
+     {4, 128, launch_bidir<4, 128, false>}, {8, 128, launch_bidir<8, 128, false>}},
+    {{2, 256, launch_bidir<2, 256, true>}, {4, 256, launch_bidir<4, 256, true>},
+     {4, 128, launch_bidir<4, 128, true>}, {8, 128, launch_bidir<8, 128, true>}}};
+const int kNumBi = 4;
+
 const int kNumFp64 = sizeof(kFp64Variants) / sizeof(kFp64Variants[0]);
 const int kNumMixed = sizeof(kMixedVariants) / sizeof(kMixedVariants[0]);
 const int kDefaultFp64 = 1;
@@ -142,13 +162,16 @@ struct tgrav_ctx {
   int validate = 1;
   int variant[2] = {kDefaultFp64, kDefaultMixed};
   int nsplit[2] = {0, 0};
+  int bvariant[2] = {1, 1};  // fused bidirectional kernel: variant and particle chunks
+  int bnsplit[2] = {0, 0};
   double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
   int batch_wg_per_cu = 0;   // persistent workgroups per CU (0 = occupancy limit)
   // packed sources; stat[0] = bits of min eps_s^2 (read by the kernels), stat[1] = NaN flag
   int64_t ns = -1;
   DevBuf s4, se2, stat;
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
-  DevBuf in[8], outb[4], vel[3], part, flag;
+  DevBuf in[12], outb[8], vel[3], part, flag;
+  DevBuf bp4, bpe2, bstat, bppart;   // fused bidirectional pass: packed particles, partials
   unsigned long long* host_flag = nullptr;   // pinned
   // instrumentation: HIP events around each evaluation kernel (tgrav_set_timing)
   int timing = 0;
@@ -366,6 +389,10 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->part.release();
   c->stat.release();
   c->flag.release();
+  c->bp4.release();
+  c->bpe2.release();
+  c->bstat.release();
+  c->bppart.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
@@ -638,12 +665,143 @@ int tgrav_kick(tgrav_ctx* c, int64_t n, double dt, const double* ax, const doubl
   return 0;
 }
 
-int tgrav_bidirectional(tgrav_ctx* c, int64_t, const double*, const double*, const double*,
-                        const double*, const double*, int64_t, const double*, const double*,
-                        const double*, const double*, const double*, double*, double*, double*,
-                        double*, double*, double*, double*, double*, int, int) {
+int tgrav_set_bidir_config(tgrav_ctx* c, int mode, int variant, int nsplit) {
   if (!c) return fail(TGRAV_EINVAL, "context is NULL");
-  return fail(TGRAV_ELIMIT, "tgrav_bidirectional: not built yet");
+  int rc = check_mode(mode);
+  if (rc) return rc;
+  if (variant >= kNumBi) return fail(TGRAV_EINVAL, "no fused variant %d", variant);
+  if (variant >= 0) c->bvariant[mode] = variant;
+  if (nsplit >= 0) c->bnsplit[mode] = nsplit;
+  return 0;
+}
+
+int tgrav_bidir_variant_info(int mode, int variant, int* k, int* threads) {
+  int rc = check_mode(mode);
+  if (rc) return rc;
+  if (variant < 0 || variant >= kNumBi) return fail(TGRAV_EINVAL, "no fused variant %d", variant);
+  if (k) *k = kBiVariants[mode][variant].k;
+  if (threads) *threads = kBiVariants[mode][variant].nt;
+  return 0;
+}
+
+int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double* cy,
+                        const double* cz, const double* cm, const double* ce, int64_t np,
+                        const double* px, const double* py, const double* pz, const double* pm,
+                        const double* pe, double* cax, double* cay, double* caz, double* cphi,
+                        double* pax, double* pay, double* paz, double* pphi, int mode, int kind) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc;
+  if ((rc = check_mode(mode)) || (rc = check_kind(kind))) return rc;
+  if (nc < 0 || np < 0) return fail(TGRAV_EINVAL, "sizes must be >= 0");
+  if (nc > 0 && (!cx || !cy || !cz || !cm)) return fail(TGRAV_EINVAL, "cell x, y, z and m are required");
+  if (np > 0 && (!px || !py || !pz || !pm))
+    return fail(TGRAV_EINVAL, "particle x, y, z and m are required");
+  if ((rc = set_device(c))) return rc;
+  const double* ca[5] = {cx, cy, cz, cm, ce};
+  const double* pa[5] = {px, py, pz, pm, pe};
+  double* co[4] = {cax, cay, caz, cphi};
+  double* po[4] = {pax, pay, paz, pphi};
+  if (kind == TGRAV_HOST && c->validate) {
+    for (int q = 0; q < 5; ++q)
+      if (!host_finite(nc, ca[q]) || !host_finite(np, pa[q]))
+        return fail(TGRAV_ENONFINITE, "non-finite value in an input array");
+  }
+  if (kind == TGRAV_HOST) {
+    if ((rc = stage_in(c, 0, nc, ca, 5)) || (rc = stage_in(c, 5, np, pa, 5))) return rc;
+    for (int q = 0; q < 4; ++q) {
+      if (co[q]) {
+        if ((rc = c->outb[q].ensure((size_t)std::max<int64_t>(nc, 1) * 8))) return rc;
+      }
+      if (po[q]) {
+        if ((rc = c->outb[4 + q].ensure((size_t)std::max<int64_t>(np, 1) * 8))) return rc;
+      }
+    }
+  } else if (c->validate) {
+    bool bad = false;
+    if ((rc = device_check_finite(c, nc, ca, 5, &bad))) return rc;
+    if (!bad && (rc = device_check_finite(c, np, pa, 5, &bad))) return rc;
+    if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in an input array");
+  }
+  double* dco[4];
+  double* dpo[4];
+  for (int q = 0; q < 4; ++q) {
+    dco[q] = kind == TGRAV_HOST ? (co[q] ? c->outb[q].as<double>() : nullptr) : co[q];
+    dpo[q] = kind == TGRAV_HOST ? (po[q] ? c->outb[4 + q].as<double>() : nullptr) : po[q];
+  }
+  tg::Out CO{dco[0], dco[1], dco[2], dco[3], nullptr, nullptr, nullptr, 0.0, nullptr, 0, c->G};
+  tg::Out PO{dpo[0], dpo[1], dpo[2], dpo[3], nullptr, nullptr, nullptr, 0.0, nullptr, 0, c->G};
+  if (nc > 0 && np == 0) {
+    tg::k_fill<<<grid1(nc, 256), 256, 0, c->stream>>>(nc, 0.0, dco[0], dco[1], dco[2], dco[3]);
+    LAUNCHCHECK("k_fill");
+  }
+  if (np > 0 && nc == 0) {
+    tg::k_fill<<<grid1(np, 256), 256, 0, c->stream>>>(np, 0.0, dpo[0], dpo[1], dpo[2], dpo[3]);
+    LAUNCHCHECK("k_fill");
+  }
+  if (nc > 0 && np > 0) {
+    if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
+        (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
+        (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
+      return rc;
+    tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->bstat.as<unsigned long long>());
+    LAUNCHCHECK("k_reset_stats");
+    tg::k_pack_sources<<<grid1(np, 256), 256, 0, c->stream>>>(
+        np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
+        c->bstat.as<unsigned long long>(), 0);
+    LAUNCHCHECK("k_pack_sources");
+    const BiVariant& v = kBiVariants[mode][c->bvariant[mode]];
+    const int64_t tile = (int64_t)v.k * v.nt;
+    const int64_t ntiles = (nc + tile - 1) / tile;
+    if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many cells");
+    int want = c->bnsplit[mode];
+    if (want <= 0) {
+      const int64_t target_wg = 4LL * c->num_cu;
+      want = ntiles >= target_wg / 2 ? 1
+                                     : (int)std::min<int64_t>((target_wg + ntiles - 1) / ntiles,
+                                                              std::max<int64_t>(1, np / (4LL * v.nt)));
+    }
+    if (want > 65535) want = 65535;
+    int64_t chunk = (np + want - 1) / want;
+    chunk = (chunk + v.nt - 1) / v.nt * v.nt;
+    const int nsplit = (int)((np + chunk - 1) / chunk);
+    const double part_bytes = (double)ntiles * 4 * np * 8 + (nsplit > 1 ? (double)nsplit * 4 * nc * 8 : 0.0);
+    if (part_bytes > 64.0 * (1ULL << 30))
+      return fail(TGRAV_ELIMIT, "fused pass would need %.1f GiB of partial sums; evaluate the two "
+                  "directions separately", part_bytes / (1ULL << 30));
+    if ((rc = c->bppart.ensure((size_t)ntiles * 4 * np * sizeof(double)))) return rc;
+    tg::Out o = CO;
+    if (nsplit > 1) {
+      if ((rc = c->part.ensure((size_t)nsplit * 4 * nc * sizeof(double)))) return rc;
+      o.part = c->part.as<double>();
+      o.pstride = nc;
+    }
+    tg::Cells C{ca[0], ca[1], ca[2], ca[3], ca[4], nc};
+    rc = timed_launch(c, nc, np, [&]() -> int {
+      v.fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v.nt), c->stream, C, c->bp4.as<double4>(),
+           c->bpe2.as<double>(), np, chunk, c->bstat.as<const double>(), o, c->bppart.as<double>());
+      LAUNCHCHECK("k_bidir");
+      return 0;
+    });
+    if (rc) return rc;
+    if (nsplit > 1) {
+      tg::k_reduce_chunks<<<grid1(nc, 256), 256, 0, c->stream>>>(c->part.as<double>(), nc, nsplit,
+                                                                  nc, CO);
+      LAUNCHCHECK("k_reduce_chunks");
+    }
+    tg::k_bidir_reduce<<<grid1(np, 256), 256, 0, c->stream>>>(c->bppart.as<double>(), (int)ntiles,
+                                                               np, PO);
+    LAUNCHCHECK("k_bidir_reduce");
+  }
+  if (kind == TGRAV_HOST) {
+    for (int q = 0; q < 4; ++q) {
+      if (co[q] && nc > 0)
+        HIPCHECK(hipMemcpyAsync(co[q], dco[q], (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+      if (po[q] && np > 0)
+        HIPCHECK(hipMemcpyAsync(po[q], dpo[q], (size_t)np * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
 }
 
 int tgrav_set_timing(tgrav_ctx* c, int on) {

@@ -234,6 +234,31 @@ class GravityField:
         _lib.check(rc, "tgrav_eval_blocks")
         return tuple(outs)
 
+    def bidirectional(self, cx, cy, cz, cm, ce, px, py, pz, pm, pe, potential=True):
+        """One fused pass over all (cell, particle) pairs with eps2 = ce^2 + pe^2
+        (Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:204-316): returns
+        ((cax, cay, caz, cphi) of the cells from the particles,
+         (pax, pay, paz, pphi) of the particles from the cells)."""
+        nc, np_ = _length(cx, cy, cz, cm), _length(px, py, pz, pm)
+        dev = _torch_device(cx, cy, cz, cm, ce, px, py, pz, pm, pe)
+        A, B = _Arrays(nc, dev), _Arrays(np_, dev)
+        ca = [A.inp(cx, "cx"), A.inp(cy, "cy"), A.inp(cz, "cz"), A.inp(cm, "cm"),
+              A.inp(ce, "ce", allow_none=True)]
+        pa = [B.inp(px, "px"), B.inp(py, "py"), B.inp(pz, "pz"), B.inp(pm, "pm"),
+              B.inp(pe, "pe", allow_none=True)]
+        co = [A.out() for _ in range(3)] + [A.out() if potential else None]
+        po = [B.out() for _ in range(3)] + [B.out() if potential else None]
+        self._bind_stream(A)
+        rc = self.lib.tgrav_bidirectional(self._ctx, nc, *[A.ptr(a) for a in ca], np_,
+                                          *[B.ptr(a) for a in pa], *[A.ptr(o) for o in co],
+                                          *[B.ptr(o) for o in po], self.mode, A.kind)
+        _lib.check(rc, "tgrav_bidirectional")
+        return tuple(co), tuple(po)
+
+    def set_bidir_config(self, variant=-1, nsplit=-1):
+        _lib.check(self.lib.tgrav_set_bidir_config(self._ctx, self.mode, int(variant), int(nsplit)),
+                   "tgrav_set_bidir_config")
+
     def kick(self, dt, ax, ay, az, vx, vy, vz):
         """v += dt * a in FP64, in place (the bridge passes dt/2)."""
         n = _length(ax)
