@@ -230,8 +230,10 @@ def oracle_check(pairs, G):
     n = 0
     for S, T, res, k in pairs:
         idx = np.sort(rng.choice(len(T["x"]), min(k, len(T["x"])), replace=False))
+        et = T.get("eps_t")
         ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S.get("eps"), T["x"][idx], T["y"][idx],
-                                  T["z"][idx], T.get("eps_t"), G, mode=oracle.NEUMAIER, threads=16)
+                                  T["z"][idx], None if et is None else et[idx], G, mode=oracle.NEUMAIER,
+                                  threads=16)
         ea, ep = rel_err(tuple(a[idx] for a in res), ref)
         worst_a, worst_p, n = max(worst_a, ea.max()), max(worst_p, ep.max()), n + len(idx)
     return dict(targets_checked=n, max_rel_acc=float(worst_a), max_rel_pot=float(worst_p))
