@@ -75,12 +75,26 @@ def variants(mode):
     return out
 
 
+_FLUSH = []
+
+
+def _flush():
+    """Other work between timed launches (a 256 MiB copy), as in real use: an evaluation is
+    rarely launched right after itself with warm caches."""
+    import torch
+    if not _FLUSH:
+        _FLUSH.append(torch.empty(1 << 25, dtype=torch.float64, device="cuda"))
+        _FLUSH.append(torch.empty(1 << 25, dtype=torch.float64, device="cuda"))
+    _FLUSH[1].copy_(_FLUSH[0])
+
+
 def _time(fn, reps):
     import torch
     fn()
     torch.cuda.synchronize()
     ts = []
     for _ in range(reps):
+        _flush()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
         fn()
