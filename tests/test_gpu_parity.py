@@ -346,3 +346,36 @@ def test_superposition_at_c2_scale(fp64, c2):
     summed = tuple(p + q for p, q in zip(*parts))
     s = _util.summary(summed, full)
     assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, s
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_acceleration_only_path_is_bit_identical(mode, c2):
+    """get_gravity_at_point asks for the acceleration alone: that path skips the potential
+    accumulator and must give the same accelerations bit for bit."""
+    st, ce = c2["stars"], c2["cells"]
+    T = {k: ce[k][:200_000] for k in ("x", "y", "z")}
+    with StellarGravityField(mode) as f:
+        f.set_stars(st["x"], st["y"], st["z"], st["m"], st["eps"])
+        full = f.evaluate(None, T["x"], T["y"], T["z"])
+        acc = f.get_gravity_at_point(0.0, T["x"], T["y"], T["z"])
+        pot = f.get_potential_at_point(0.0, T["x"], T["y"], T["z"])
+    for p, q in zip(full[:3], acc):
+        assert np.array_equal(p, q)
+    assert np.array_equal(full[3], pot)
+
+
+def test_reference_method_names_give_the_oracle_field(c1):
+    """GasGravityField.get_accel_gas_on_particles / get_potential_at_point and
+    StellarGravityField.get_gravity_at_point on the c1 fixture."""
+    S, C = stars(c1, "star_"), stars(c1, "cell_")
+    with GasGravityField("fp64") as g:
+        g.set_sources(C["x"], C["y"], C["z"], C["m"], C["eps"])
+        ax, ay, az = g.get_accel_gas_on_particles(np.zeros(len(S["x"])), S["x"], S["y"], S["z"])
+        phi = g.get_potential_at_point(np.zeros(len(S["x"])), S["x"], S["y"], S["z"])
+    s = _util.summary((ax, ay, az, phi), tuple(c1["rev"]))
+    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+    with StellarGravityField("fp64") as f:
+        f.set_stars(S["x"], S["y"], S["z"], S["m"], S["eps"])
+        gx, gy, gz = f.get_gravity_at_point(0.0, C["x"], C["y"], C["z"])
+    s = _util.summary((gx, gy, gz, c1["fwd"][3]), tuple(c1["fwd"]))
+    assert s["acc_max"] <= 1e-12

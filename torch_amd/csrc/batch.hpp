@@ -124,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
     }
     const int64_t tile0 = (int64_t)w.tile * (K * NT);
     if constexpr (MIXED)
-      process_mixed<K, NT, 64>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
+      process_mixed<K, NT>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
     else
       process_fp64<K, NT>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
   }

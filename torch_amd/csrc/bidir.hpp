@@ -44,7 +44,7 @@ __device__ __forceinline__ void bi_pair(T px, T py, T pz, T pm, T r2e, T qx, T q
     if (GUARD) y = r2 > 0.0f ? y : 0.0f;
   } else {
     r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, r2e)));
-    y = rsqrt_f64<GUARD>(r2);
+    y = rsqrt_f64<GUARD ? F_GUARD : 0>(r2);
   }
   const T y3 = (y * y) * y;
   const T wp = pm * y3;

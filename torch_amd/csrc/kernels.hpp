@@ -33,6 +33,7 @@ struct Out {
   double *part;                 // non-null: write unscaled chunk sums to part[q*pstride + t]
   int64_t pstride;
   double G;
+  int nopot;                    // 1: the potential is not wanted (its accumulator is skipped)
 };
 
 // Cell centre of block-structured target t: o + (idx + 1/2) dx with the multiply and the
@@ -101,14 +102,26 @@ __device__ __forceinline__ void block_minmax(double (&mn)[NV], double (&mx)[NV],
   __syncthreads();
 }
 
+// --------------------------------------------------------------- inner-loop flags ----
+// Per-tile code path, chosen at run time once per workgroup (a uniform branch), compiled
+// as separate loops:
+//   F_GUARD  the tile's smallest possible eps^2 is 0: zero r2 -> the pair contributes 0
+//   F_UNIF   all targets of the tile share eps_t^2: folded into the staged source eps^2
+//   F_NOPOT  the caller did not ask for the potential
+//   F_SEED32 (FP64 only) r2 >= 1e-30 is guaranteed: seed 1/sqrt from v_rsq_f32 instead of the
+//            quarter-rate v_rsq_f64 (both seeds are ~2^-24 accurate; the refinement is the same)
+enum { F_GUARD = 1, F_UNIF = 2, F_NOPOT = 4, F_SEED32 = 8 };
+
 // --------------------------------------------------------------- FP64 tile loop ----
-// 1/sqrt(r2) in FP64: the hardware estimate (v_rsq_f64) refined by one third-order
-// Newton/Householder step  y <- y + y e (1/2 + 3/8 e),  e = 1 - r2 y^2  (the Taylor
-// series of (1-e)^(-1/2); an estimate with relative error d leaves ~2.5 d^3).
-template <bool GUARD>
+// 1/sqrt(r2) in FP64: a hardware estimate refined by one third-order Newton/Householder
+// step  y <- y + y e (1/2 + 3/8 e),  e = 1 - r2 y^2  (the Taylor series of (1-e)^(-1/2);
+// an estimate with relative error d leaves ~2.5 d^3, below FP64 rounding for d ~ 2^-24).
+template <int FL>
 __device__ __forceinline__ double rsqrt_f64(double r2) {
-  double y = __builtin_amdgcn_rsq(r2);
-  if (GUARD) y = r2 > 0.0 ? y : 0.0;   // |d|^2 + eps^2 == 0: the pair contributes 0
+  double y;
+  if (FL & F_SEED32) y = (double)__builtin_amdgcn_rsqf((float)r2);
+  else y = __builtin_amdgcn_rsq(r2);
+  if (FL & F_GUARD) y = r2 > 0.0 ? y : 0.0;   // |d|^2 + eps^2 == 0: the pair contributes 0
   const double t = r2 * y;
   const double e = fma(-t, y, 1.0);
   const double p = fma(e, 0.375, 0.5);
@@ -116,7 +129,7 @@ __device__ __forceinline__ double rsqrt_f64(double r2) {
   return fma(q, p, y);
 }
 
-template <int K, int NT, bool GUARD, bool UNIF>
+template <int K, int NT, int FL>
 __device__ __forceinline__ void fp64_tile(const double4* __restrict__ s4,
                                           const double* __restrict__ se2,
                                           const double (&tx)[K], const double (&ty)[K],
@@ -130,10 +143,10 @@ __device__ __forceinline__ void fp64_tile(const double4* __restrict__ s4,
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const double dx = s.x - tx[k], dy = s.y - ty[k], dz = s.z - tz[k];
-      const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, UNIF ? e2 : e2 + te2[k])));
-      const double y = rsqrt_f64<GUARD>(r2);
+      const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, (FL & F_UNIF) ? e2 : e2 + te2[k])));
+      const double y = rsqrt_f64<FL>(r2);
       const double mr = s.w * y;
-      ap[k] += mr;
+      if (!(FL & F_NOPOT)) ap[k] += mr;
       const double w = mr * (y * y);
       ax[k] = fma(w, dx, ax[k]);
       ay[k] = fma(w, dy, ay[k]);
@@ -143,25 +156,25 @@ __device__ __forceinline__ void fp64_tile(const double4* __restrict__ s4,
 }
 
 // ------------------------------------------------------ mixed-precision tile loop ----
-template <int K, int NT, bool GUARD, bool UNIF>
+template <int K, int NT, int FL>
 __device__ __forceinline__ void f32_tile(const float4* __restrict__ s4,
-                                         const float* __restrict__ se2, int j0, int j1,
+                                         const float* __restrict__ se2,
                                          const float (&tx)[K], const float (&ty)[K],
                                          const float (&tz)[K], const float (&te2)[K],
                                          float (&ax)[K], float (&ay)[K], float (&az)[K],
                                          float (&ap)[K]) {
 #pragma unroll 4
-  for (int j = j0; j < j1; ++j) {
+  for (int j = 0; j < NT; ++j) {
     const float4 s = s4[j];
     const float e2 = se2[j];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const float dx = s.x - tx[k], dy = s.y - ty[k], dz = s.z - tz[k];
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, UNIF ? e2 : e2 + te2[k])));
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, (FL & F_UNIF) ? e2 : e2 + te2[k])));
       float y = __builtin_amdgcn_rsqf(r2);
-      if (GUARD) y = r2 > 0.0f ? y : 0.0f;
+      if (FL & F_GUARD) y = r2 > 0.0f ? y : 0.0f;
       const float mr = s.w * y;
-      ap[k] += mr;
+      if (!(FL & F_NOPOT)) ap[k] += mr;
       const float w = (mr * y) * y;
       ax[k] = fmaf(w, dx, ax[k]);
       ay[k] = fmaf(w, dy, ay[k]);
@@ -194,7 +207,7 @@ struct LdsF32 {
 };
 
 // Threshold on the smallest possible eps^2 of a tile below which the FP32 loop guards
-// r2 == 0 (and keeps rsq far from overflow); FP64 guards only an exact zero.
+// r2 == 0 (and keeps rsq far from overflow) and the FP64 loop keeps the v_rsq_f64 seed.
 #define TG_F32_TINY_E2 1e-30
 
 // v + dt*a with the product and the sum rounded separately (no FMA contraction), as the
@@ -223,14 +236,30 @@ __device__ __forceinline__ void store_result(const Out& O, int64_t t, double sx,
   if (O.vx) kick_one(O, t, gx, gy, gz);
 }
 
+// Code path of a tile from its smallest/largest target eps^2 and the sources' smallest.
+__device__ __forceinline__ int tile_flags(double e2s_min, double e2t_min, double e2t_max,
+                                          const Out& O, bool f32) {
+  int fl = 0;
+  if (e2t_min == e2t_max) fl |= F_UNIF;
+  const double lo = e2s_min + e2t_min;
+  if (f32) {
+    if (!(lo > TG_F32_TINY_E2)) fl |= F_GUARD;
+  } else {
+    if (!(lo > 0.0)) fl |= F_GUARD;
+    else if (lo > TG_F32_TINY_E2) fl |= F_SEED32;
+  }
+  if (O.nopot) fl |= F_NOPOT;
+  return fl;
+}
+
 // ------------------------------------------------------- per-workgroup drivers ----
-template <int K, int NT, bool GUARD, bool UNIF>
+template <int K, int NT, int FL>
 __device__ __forceinline__ void fp64_sources(LdsF64<NT>& L, const double4* __restrict__ src4,
-                                             const double* __restrict__ srce2, int64_t s_begin,
-                                             int64_t s_end, double e2u, const double (&tx)[K],
-                                             const double (&ty)[K], const double (&tz)[K],
-                                             const double (&te2)[K], double (&ax)[K],
-                                             double (&ay)[K], double (&az)[K], double (&ap)[K]) {
+                                          const double* __restrict__ srce2, int64_t s_begin,
+                                          int64_t s_end, double e2u, const double (&tx)[K],
+                                          const double (&ty)[K], const double (&tz)[K],
+                                          const double (&te2)[K], double (&ax)[K],
+                                          double (&ay)[K], double (&az)[K], double (&ap)[K]) {
   const int tid = threadIdx.x;
   for (int64_t base = s_begin; base < s_end; base += NT) {
     __syncthreads();
@@ -243,9 +272,51 @@ __device__ __forceinline__ void fp64_sources(LdsF64<NT>& L, const double4* __res
       L.se2[tid] = 1.0;
     }
     __syncthreads();
-    fp64_tile<K, NT, GUARD, UNIF>(L.s4, L.se2, tx, ty, tz, te2, ax, ay, az, ap);
+    fp64_tile<K, NT, FL>(L.s4, L.se2, tx, ty, tz, te2, ax, ay, az, ap);
   }
 }
+
+#define TG_DISPATCH8(FN, fl, ...)                                             \
+  switch (fl & ~F_SEED32) {                                                   \
+    case 0: FN<0>(__VA_ARGS__); break;                                        \
+    case F_GUARD: FN<F_GUARD>(__VA_ARGS__); break;                            \
+    case F_UNIF: FN<F_UNIF>(__VA_ARGS__); break;                              \
+    case F_UNIF | F_GUARD: FN<F_UNIF | F_GUARD>(__VA_ARGS__); break;          \
+    case F_NOPOT: FN<F_NOPOT>(__VA_ARGS__); break;                            \
+    case F_NOPOT | F_GUARD: FN<F_NOPOT | F_GUARD>(__VA_ARGS__); break;        \
+    case F_NOPOT | F_UNIF: FN<F_NOPOT | F_UNIF>(__VA_ARGS__); break;          \
+    default: FN<F_NOPOT | F_UNIF | F_GUARD>(__VA_ARGS__); break;              \
+  }
+
+#define TG_DISPATCH(FN, fl, ...)                                              \
+  switch (fl) {                                                               \
+    case 0: FN<0>(__VA_ARGS__); break;                                        \
+    case F_GUARD: FN<F_GUARD>(__VA_ARGS__); break;                            \
+    case F_UNIF: FN<F_UNIF>(__VA_ARGS__); break;                              \
+    case F_UNIF | F_GUARD: FN<F_UNIF | F_GUARD>(__VA_ARGS__); break;          \
+    case F_NOPOT: FN<F_NOPOT>(__VA_ARGS__); break;                            \
+    case F_NOPOT | F_GUARD: FN<F_NOPOT | F_GUARD>(__VA_ARGS__); break;        \
+    case F_NOPOT | F_UNIF: FN<F_NOPOT | F_UNIF>(__VA_ARGS__); break;          \
+    case F_NOPOT | F_UNIF | F_GUARD: FN<F_NOPOT | F_UNIF | F_GUARD>(__VA_ARGS__); break; \
+    case F_SEED32: FN<F_SEED32>(__VA_ARGS__); break;                          \
+    case F_SEED32 | F_UNIF: FN<F_SEED32 | F_UNIF>(__VA_ARGS__); break;        \
+    case F_SEED32 | F_NOPOT: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;      \
+    default: FN<F_SEED32 | F_NOPOT | F_UNIF>(__VA_ARGS__); break;             \
+  }
+
+template <int K, int NT>
+struct Fp64Run {
+  template <int FL>
+  static __device__ __forceinline__ void run(LdsF64<NT>& L, const double4* src4, const double* srce2,
+                                             int64_t sb, int64_t se, double e2u,
+                                             const double (&tx)[K], const double (&ty)[K],
+                                             const double (&tz)[K], const double (&te2)[K],
+                                             double (&ax)[K], double (&ay)[K], double (&az)[K],
+                                             double (&ap)[K]) {
+    fp64_sources<K, NT, FL>(L, src4, srce2, sb, se, (FL & F_UNIF) ? e2u : 0.0, tx, ty, tz, te2,
+                            ax, ay, az, ap);
+  }
+};
 
 template <int K, int NT>
 __device__ void process_fp64(LdsF64<NT>& L, const Tgt& T, int64_t tile0,
@@ -264,19 +335,13 @@ __device__ void process_fp64(LdsF64<NT>& L, const Tgt& T, int64_t tile0,
     }
   }
   block_minmax<NT, 1>(mn, mx, L.red);
-  const bool unif = mn[0] == mx[0];
-  const bool guard = !(e2s_min + mn[0] > 0.0);
-  const double e2u = unif ? mn[0] : 0.0;
+  const int fl = tile_flags(e2s_min, mn[0], mx[0], O, false);
   double ax[K], ay[K], az[K], ap[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
-  if (unif) {
-    if (guard) fp64_sources<K, NT, true, true>(L, src4, srce2, s_begin, s_end, e2u, tx, ty, tz, te2, ax, ay, az, ap);
-    else fp64_sources<K, NT, false, true>(L, src4, srce2, s_begin, s_end, e2u, tx, ty, tz, te2, ax, ay, az, ap);
-  } else {
-    if (guard) fp64_sources<K, NT, true, false>(L, src4, srce2, s_begin, s_end, 0.0, tx, ty, tz, te2, ax, ay, az, ap);
-    else fp64_sources<K, NT, false, false>(L, src4, srce2, s_begin, s_end, 0.0, tx, ty, tz, te2, ax, ay, az, ap);
-  }
+  typedef Fp64Run<K, NT> R;
+  TG_DISPATCH(R::template run, fl, L, src4, srce2, s_begin, s_end, mn[0], tx, ty, tz,
+              te2, ax, ay, az, ap)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t t = tile0 + (int64_t)k * NT + tid;
@@ -288,17 +353,17 @@ __device__ void process_fp64(LdsF64<NT>& L, const Tgt& T, int64_t tile0,
 // Sources are staged as fp32(x_s - c) (the FP64 difference rounded once), targets are held
 // as fp32(x_t - c); the FP32 difference of the two is exact whenever they are within a
 // factor of two of each other (Sterbenz), i.e. for exactly the close pairs where a
-// difference of raw FP32 coordinates would cancel catastrophically.  FP32 partial sums
-// over FOLD sources are folded into per-target two-sum (hi, lo) accumulators.
-template <int K, int NT, int FOLD, bool GUARD, bool UNIF>
+// difference of raw FP32 coordinates would cancel catastrophically.  The FP32 partial sums
+// of each LDS tile (NT sources) are folded into per-target two-sum (hi, lo) accumulators.
+template <int K, int NT, int FL>
 __device__ __forceinline__ void f32_sources(LdsF32<NT>& L, const double4* __restrict__ src4,
-                                            const double* __restrict__ srce2, int64_t s_begin,
-                                            int64_t s_end, double e2u, double cx, double cy,
-                                            double cz, const float (&tx)[K], const float (&ty)[K],
-                                            const float (&tz)[K], const float (&te2)[K],
-                                            float (&hx)[K], float (&hy)[K], float (&hz)[K],
-                                            float (&hp)[K], float (&lx)[K], float (&ly)[K],
-                                            float (&lz)[K], float (&lp)[K]) {
+                                         const double* __restrict__ srce2, int64_t s_begin,
+                                         int64_t s_end, double e2u, double cx, double cy,
+                                         double cz, const float (&tx)[K], const float (&ty)[K],
+                                         const float (&tz)[K], const float (&te2)[K],
+                                         float (&hx)[K], float (&hy)[K], float (&hz)[K],
+                                         float (&hp)[K], float (&lx)[K], float (&ly)[K],
+                                         float (&lz)[K], float (&lp)[K]) {
   const int tid = threadIdx.x;
   for (int64_t base = s_begin; base < s_end; base += NT) {
     __syncthreads();
@@ -312,25 +377,36 @@ __device__ __forceinline__ void f32_sources(LdsF32<NT>& L, const double4* __rest
       L.se2[tid] = 1.0f;
     }
     __syncthreads();
-    const int64_t rem = s_end - base;
-    const int jn = rem < NT ? (int)rem : NT;
-    for (int j0 = 0; j0 < jn; j0 += FOLD) {
-      float ax[K], ay[K], az[K], ap[K];
+    float ax[K], ay[K], az[K], ap[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
-      f32_tile<K, NT, GUARD, UNIF>(L.s4, L.se2, j0, j0 + FOLD, tx, ty, tz, te2, ax, ay, az, ap);
+    for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
+    f32_tile<K, NT, FL>(L.s4, L.se2, tx, ty, tz, te2, ax, ay, az, ap);
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        two_sum(hx[k], lx[k], ax[k]);
-        two_sum(hy[k], ly[k], ay[k]);
-        two_sum(hz[k], lz[k], az[k]);
-        two_sum(hp[k], lp[k], ap[k]);
-      }
+    for (int k = 0; k < K; ++k) {
+      two_sum(hx[k], lx[k], ax[k]);
+      two_sum(hy[k], ly[k], ay[k]);
+      two_sum(hz[k], lz[k], az[k]);
+      if (!(FL & F_NOPOT)) two_sum(hp[k], lp[k], ap[k]);
     }
   }
 }
 
-template <int K, int NT, int FOLD>
+template <int K, int NT>
+struct MixRun {
+  template <int FL>
+  static __device__ __forceinline__ void run(LdsF32<NT>& L, const double4* src4, const double* srce2,
+                                             int64_t sb, int64_t se, double e2u, double cx,
+                                             double cy, double cz, const float (&tx)[K],
+                                             const float (&ty)[K], const float (&tz)[K],
+                                             const float (&te2)[K], float (&hx)[K], float (&hy)[K],
+                                             float (&hz)[K], float (&hp)[K], float (&lx)[K],
+                                             float (&ly)[K], float (&lz)[K], float (&lp)[K]) {
+    f32_sources<K, NT, FL>(L, src4, srce2, sb, se, (FL & F_UNIF) ? e2u : 0.0, cx, cy,
+                                       cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
+  }
+};
+
+template <int K, int NT>
 __device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
                               const double4* __restrict__ src4, const double* __restrict__ srce2,
                               int64_t s_begin, int64_t s_end, double e2s_min, const Out& O) {
@@ -351,9 +427,7 @@ __device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
   }
   block_minmax<NT, 4>(mn, mx, L.red);
   const double cx = 0.5 * (mn[0] + mx[0]), cy = 0.5 * (mn[1] + mx[1]), cz = 0.5 * (mn[2] + mx[2]);
-  const bool unif = mn[3] == mx[3];
-  const bool guard = !(e2s_min + mn[3] > TG_F32_TINY_E2);
-  const double e2u = unif ? mn[3] : 0.0;
+  const int fl = tile_flags(e2s_min, mn[3], mx[3], O, true);
   float tx[K], ty[K], tz[K], te2[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -365,13 +439,9 @@ __device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
   float hx[K], hy[K], hz[K], hp[K], lx[K], ly[K], lz[K], lp[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = hp[k] = lx[k] = ly[k] = lz[k] = lp[k] = 0.f;
-  if (unif) {
-    if (guard) f32_sources<K, NT, FOLD, true, true>(L, src4, srce2, s_begin, s_end, e2u, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
-    else f32_sources<K, NT, FOLD, false, true>(L, src4, srce2, s_begin, s_end, e2u, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
-  } else {
-    if (guard) f32_sources<K, NT, FOLD, true, false>(L, src4, srce2, s_begin, s_end, 0.0, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
-    else f32_sources<K, NT, FOLD, false, false>(L, src4, srce2, s_begin, s_end, 0.0, cx, cy, cz, tx, ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp);
-  }
+  typedef MixRun<K, NT> R;
+  TG_DISPATCH8(R::template run, fl, L, src4, srce2, s_begin, s_end, mn[3], cx, cy, cz, tx,
+              ty, tz, te2, hx, hy, hz, hp, lx, ly, lz, lp)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t t = tile0 + (int64_t)k * NT + tid;
@@ -397,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_eval_fp64(Tgt T, const double4* __restri
   process_fp64<K, NT>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
 }
 
-template <int K, int NT, int FOLD>
+template <int K, int NT>
 __global__ __launch_bounds__(NT) void k_eval_mixed(Tgt T, const double4* __restrict__ src4,
                                                    const double* __restrict__ srce2, int64_t ns,
                                                    int64_t chunk, const double* e2s_min_p, Out O) {
@@ -408,7 +478,7 @@ __global__ __launch_bounds__(NT) void k_eval_mixed(Tgt T, const double4* __restr
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  process_mixed<K, NT, FOLD>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
+  process_mixed<K, NT>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
 }
 
 // Sum of the nsplit chunk partials of each target in chunk order, then the G scaling,

@@ -100,10 +100,10 @@ void launch_fp64(dim3 g, dim3 b, hipStream_t s, const tg::Tgt& T, const double4*
                  const double* se2, int64_t ns, int64_t chunk, const double* e2m, const tg::Out& O) {
   tg::k_eval_fp64<K, NT><<<g, b, 0, s>>>(T, s4, se2, ns, chunk, e2m, O);
 }
-template <int K, int NT, int FOLD>
+template <int K, int NT>
 void launch_mixed(dim3 g, dim3 b, hipStream_t s, const tg::Tgt& T, const double4* s4,
                   const double* se2, int64_t ns, int64_t chunk, const double* e2m, const tg::Out& O) {
-  tg::k_eval_mixed<K, NT, FOLD><<<g, b, 0, s>>>(T, s4, se2, ns, chunk, e2m, O);
+  tg::k_eval_mixed<K, NT><<<g, b, 0, s>>>(T, s4, se2, ns, chunk, e2m, O);
 }
 
 struct Variant {
@@ -114,7 +114,7 @@ struct Variant {
 };
 
 #define FP64V(K, NT) {K, NT, NT, launch_fp64<K, NT>, launch_batch<K, NT, false>, batch_kernel<K, NT, false>}
-#define MIXV(K, NT) {K, NT, NT, launch_mixed<K, NT, 64>, launch_batch<K, NT, true>, batch_kernel<K, NT, true>}
+#define MIXV(K, NT) {K, NT, NT, launch_mixed<K, NT>, launch_batch<K, NT, true>, batch_kernel<K, NT, true>}
 const Variant kFp64Variants[] = {FP64V(1, 256), FP64V(2, 256), FP64V(4, 256),
                                  FP64V(2, 128), FP64V(4, 128), FP64V(8, 64)};
 const Variant kMixedVariants[] = {MIXV(2, 256), MIXV(4, 256), MIXV(8, 256),
@@ -549,7 +549,8 @@ static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[5], int ntin, 
         dvel[q] = c->vel[q].as<double>();
         HIPCHECK(hipMemcpyAsync(dvel[q], vel[q], (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
       }
-    tg::Out O{dout[0], dout[1], dout[2], dout[3], dvel[0], dvel[1], dvel[2], dt, nullptr, 0, c->G};
+    tg::Out O{dout[0], dout[1], dout[2], dout[3], dvel[0], dvel[1], dvel[2], dt, nullptr, 0, c->G,
+              outs[3] == nullptr};
     if ((rc = launch_eval(c, mode, T, O))) return rc;
     for (int q = 0; q < 4; ++q)
       if (outs[q])
@@ -566,7 +567,8 @@ static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[5], int ntin, 
     if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in a target array");
   }
   bind_targets(T, tin, ntin);
-  tg::Out O{outs[0], outs[1], outs[2], outs[3], vel[0], vel[1], vel[2], dt, nullptr, 0, c->G};
+  tg::Out O{outs[0], outs[1], outs[2], outs[3], vel[0], vel[1], vel[2], dt, nullptr, 0, c->G,
+            outs[3] == nullptr};
   return launch_eval(c, mode, T, O);
 }
 
@@ -952,7 +954,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     memset(&P, 0, sizeof P);
     P.T.x = q.tx; P.T.y = q.ty; P.T.z = q.tz; P.T.eps = q.te; P.T.n = q.n_targets;
     P.O = tg::Out{q.ax, q.ay, q.az, q.phi, kick ? q.vx : nullptr, kick ? q.vy : nullptr,
-                  kick ? q.vz : nullptr, dt, nullptr, 0, c->G};
+                  kick ? q.vz : nullptr, dt, nullptr, 0, c->G, q.phi == nullptr};
     P.src_off = sprefix[p];
     P.ns = q.n_sources;
     srcs[p] = tg::BSrc{q.sx, q.sy, q.sz, q.sm, q.se};
