@@ -69,22 +69,29 @@ def err_stats(ea, ep=None):
 
 
 # ----------------------------------------------------------------- c2 / c5 ----
-def run_pair(mode, dev, ns, nc, steps, warmup):
-    """Forward (stars -> cells) and reverse (cells -> stars) per step."""
+def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
+    """Forward (stars -> cells) and reverse (cells -> stars) per step.  blocks=True hands the
+    cells over as 8^3 leaf blocks (origin, width, per-cell masses), the format FLASH holds
+    them in; blocks=False as explicit per-cell arrays."""
     import torch
 
     from torch_amd import GasGravityField, StellarGravityField, autotune
     fwd = StellarGravityField(mode, validate=False)
     rev = GasGravityField(mode, validate=False)
-    cfg = (autotune.apply_eval(fwd, ns, nc), autotune.apply_eval(rev, nc, ns))
-    log(f"{mode}: tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
+    cfg = (None, None) if blocks else (autotune.apply_eval(fwd, ns, nc), autotune.apply_eval(rev, nc, ns))
+    log(f"{mode}: {'block' if blocks else 'explicit'} format, tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
+    B = (dev["bo"], dev["bdx"])
 
     def step():
         fwd.set_sources(*S)
-        of = fwd.evaluate(None, C[0], C[1], C[2])
-        rev.set_sources(*C)
+        if blocks:
+            of = fwd.evaluate_blocks(B[0], B[1])
+            rev.set_blocks(B[0], B[1], C[3])
+        else:
+            of = fwd.evaluate(None, C[0], C[1], C[2])
+            rev.set_sources(*C)
         orv = rev.evaluate(None, S[0], S[1], S[2])
         return of, orv
 
@@ -274,7 +281,8 @@ def main():
         ns, nc = len(st["x"]), len(ce["x"])
         log(f"{args.workload} generated in {time.perf_counter() - t:.1f} s: {ns} stars, {nc} cells")
         dev = dict(sx=d(st["x"]), sy=d(st["y"]), sz=d(st["z"]), sm=d(st["m"]), se=d(st["eps"]),
-                   cx=d(ce["x"]), cy=d(ce["y"]), cz=d(ce["z"]), cm=d(ce["m"]), ce=d(ce["eps"]))
+                   cx=d(ce["x"]), cy=d(ce["y"]), cz=d(ce["z"]), cm=d(ce["m"]), ce=d(ce["eps"]),
+                   bo=d(ce["block_origin"]), bdx=d(ce["block_size"] / 8))
         torch.cuda.synchronize()
         inter_step = 2.0 * ns * nc
         for mode in modes:
@@ -283,9 +291,14 @@ def main():
                 log(f"{mode}: fused {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
                     f"kernel {r['fwd_kernel_ms']:.3f} ms; two directed passes {r['directed_ms']:.3f} ms")
             else:
-                r = run_pair(mode, dev, ns, nc, args.steps, args.warmup)
+                r = run_pair(mode, dev, ns, nc, args.steps, args.warmup, blocks=True)
                 log(f"{mode}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s); "
                     f"eval kernels fwd {r['fwd_kernel_ms']:.3f} ms, rev {r['rev_kernel_ms']:.3f} ms")
+                rx = run_pair(mode, dev, ns, nc, args.steps, args.warmup, blocks=False)
+                log(f"{mode} explicit arrays: {rx['ms']:.3f} ms/step "
+                    f"({inter_step / rx['ms'] * 1e3:.3e} interactions/s); eval kernels fwd "
+                    f"{rx['fwd_kernel_ms']:.3f} ms, rev {rx['rev_kernel_ms']:.3f} ms")
+                r["explicit"] = rx
             results[mode] = r
         r = results[head]
         kern_ms = r["fwd_kernel_ms"]
@@ -302,8 +315,16 @@ def main():
         line["config"] = {"workload": f"{args.workload}: {name}; forward + reverse, acc + pot",
                           "seed": w["seed"], "mode": head, "n_stars": ns, "n_cells": nc,
                           "interactions_per_step": inter_step,
-                          "launch_shapes": {m: results[m]["cfg"] for m in results}}
-        kernel_name = f"k_eval_{'mixed' if head == 'mixed' else 'fp64'} (forward, stars -> cells)"
+                          "format": "8^3 leaf blocks (origin, width, cell masses); explicit-array "
+                                    "rates in explicit_arrays"}
+        kernel_name = f"k_eval_grid<{'mixed' if head == 'mixed' else 'fp64'}> (forward, stars -> 8^3 cell blocks)"
+        if "explicit" in results[head]:
+            line["explicit_arrays"] = {m: {"value": inter_step / (results[m]["explicit"]["ms"] * 1e-3),
+                                           "ms_per_step": results[m]["explicit"]["ms"],
+                                           "fwd_kernel_ms": results[m]["explicit"]["fwd_kernel_ms"],
+                                           "rev_kernel_ms": results[m]["explicit"]["rev_kernel_ms"],
+                                           "launch_shapes": results[m]["explicit"]["cfg"]}
+                                       for m in results}
         if args.workload == "c2fused":
             kernel_name = f"k_bidir<{'mixed' if head == 'mixed' else 'fp64'}> (both directions)"
             line["directed_two_pass_ms"] = {m: results[m]["directed_ms"] for m in results}

@@ -95,6 +95,15 @@ int tgrav_variant_info(int mode, int variant, int* targets_per_lane, int* thread
 int tgrav_set_sources(tgrav_ctx* ctx, int64_t n, const double* x, const double* y,
                       const double* z, const double* m, const double* eps, int ptr_kind);
 
+/* Source set given as 8^3 leaf blocks (FLASH's NXB = 8): block b has lower corner
+ * (ox,oy,oz)[b], cell width dx[b] and softening eps[b] (NULL = 0); m holds the nblk*512
+ * cell masses rho dV in the C order b*512 + (i*8 + j)*8 + k -- the leaf-cell loop of
+ * Gravity_getAccelAtPoint.F90:168-204 as a flat array.  Evaluations then walk the cells
+ * column by column (dx, dy shared by the 8 cells of a column).  nb must be 8. */
+int tgrav_set_source_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const double* ox,
+                            const double* oy, const double* oz, const double* dx,
+                            const double* m, const double* eps, int ptr_kind);
+
 /* Acceleration and potential at n points from the source set.  Replaces the per-point
  * loops of get_accel_gas_on_particles (interface.F90:1130-1216) and
  * get_potential_at_point (interface.F90:1073-1127) -> Gravity_getAccelAtPoint
@@ -141,6 +150,10 @@ int tgrav_bidirectional(tgrav_ctx* ctx, int64_t nc, const double* cx, const doub
                         const double* pm, const double* pe, double* cax, double* cay,
                         double* caz, double* cphi, double* pax, double* pay, double* paz,
                         double* pphi, int mode, int ptr_kind);
+/* Block targets with nb == 8 use a column kernel (each lane owns one (i, j) column of 8
+ * cells and shares dx, dy, dx^2 + dy^2 + eps^2 between them); on = 0 falls back to the
+ * generic kernel (same results to rounding; for tests and comparisons).  Default on. */
+int tgrav_set_block_kernel(tgrav_ctx* ctx, int on);
 /* Fused-pass launch shape (variant, particle chunks; -1 keeps, 0 = automatic chunks). */
 int tgrav_set_bidir_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
 int tgrav_bidir_variant_info(int mode, int variant, int* targets_per_lane, int* threads);

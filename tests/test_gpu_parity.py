@@ -305,20 +305,56 @@ def test_fused_kick_equals_eval_then_kick(mode, c2):
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_block_targets_equal_explicit_centres(mode, c2):
+    """Block-structured targets: with the generic kernel the centres rebuilt on the device
+    give the explicit-centre results bit for bit; the 8^3 column kernel agrees with them to
+    the parity bar (its r^2 is summed as (dx^2 + dy^2 + eps^2) + dz^2)."""
     st, ce = c2["stars"], c2["cells"]
     o, size = ce["block_origin"][:300], ce["block_size"][:300]
     dx = size / 8
     bx, by, bz = oracle.block_centres(o[:, 0], o[:, 1], o[:, 2], dx)
     with StellarGravityField(mode) as f:
         f.set_stars(st["x"], st["y"], st["z"], st["m"], st["eps"])
-        blk = f.evaluate_blocks(o, dx, eps=dx)
         exp = f.evaluate(np.repeat(dx, 512), bx, by, bz)
+        f.set_block_kernel(False)
+        blk = f.evaluate_blocks(o, dx, eps=dx)
         for p, q in zip(blk, exp):
             assert np.array_equal(p, q)
+        f.set_block_kernel(True)
+        col = f.evaluate_blocks(o, dx, eps=dx)
+        s = _util.summary(col, exp)
+        print(mode, "column kernel vs generic", s, flush=True)
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
         v = [np.zeros(len(bx)) for _ in range(3)]
         f.evaluate_blocks(o, dx, eps=dx, dt=0.25, vel=v)
-        for p, q in zip(v, exp[:3]):
+        for p, q in zip(v, col[:3]):
             assert np.array_equal(p, 0.25 * q)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_column_kernel_matches_oracle_full_c2(mode, c2):
+    """All 2,000 c2 leaf blocks through the column kernel; oracle on a subsample."""
+    st, ce = c2["stars"], c2["cells"]
+    dxb = ce["block_size"] / 8
+    with StellarGravityField(mode) as f:
+        f.set_stars(st["x"], st["y"], st["z"], st["m"], st["eps"])
+        got = f.evaluate_blocks(ce["block_origin"], dxb)
+        for ns in (1, 3):
+            f.set_config(nsplit=ns)
+            again = f.evaluate_blocks(ce["block_origin"], dxb)
+            s = _util.summary(again, got)
+            assert s["acc_max"] <= (1e-12 if mode == "fp64" else 1e-5), s
+    idx = _util.subsample(len(ce["x"]), 2048, 23)
+    ref = oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], st["eps"], ce["x"][idx], ce["y"][idx],
+                              ce["z"][idx], None, G, mode=oracle.NEUMAIER, threads=THREADS)
+    s = _util.summary(tuple(a[idx] for a in got), ref)
+    print(mode, "column kernel c2", s, flush=True)
+    if mode == "fp64":
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+    else:
+        assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
 
 
 # ---------------------------------------------------- full-size properties ----
@@ -379,3 +415,57 @@ def test_reference_method_names_give_the_oracle_field(c1):
         gx, gy, gz = f.get_gravity_at_point(0.0, C["x"], C["y"], C["z"])
     s = _util.summary((gx, gy, gz, c1["fwd"][3]), tuple(c1["fwd"]))
     assert s["acc_max"] <= 1e-12
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_block_sources(mode, c2):
+    """Cells given as 8^3 source blocks (tgrav_set_source_blocks): with the generic kernel
+    they reproduce the explicit-cell results bit for bit; the column kernel agrees to the
+    parity bar and with the oracle."""
+    st, ce = c2["stars"], c2["cells"]
+    nb = 400
+    sl = slice(0, nb * 512)
+    o, dxb = ce["block_origin"][:nb], ce["block_size"][:nb] / 8
+    with GasGravityField(mode) as g:
+        g.set_sources(ce["x"][sl], ce["y"][sl], ce["z"][sl], ce["m"][sl], ce["eps"][sl])
+        exp = g.evaluate(None, st["x"], st["y"], st["z"])
+        g.set_blocks(o, dxb, ce["m"][sl])
+        g.set_block_kernel(False)
+        gen = g.evaluate(None, st["x"], st["y"], st["z"])
+        for p, q in zip(gen, exp):
+            assert np.array_equal(p, q)
+        g.set_block_kernel(True)
+        col = g.evaluate(None, st["x"], st["y"], st["z"])
+        for ns in (1, 7):
+            g.set_config(nsplit=ns)
+            again = g.evaluate(None, st["x"], st["y"], st["z"])
+            s = _util.summary(again, col)
+            assert s["acc_max"] <= (1e-12 if mode == "fp64" else 1e-5), (ns, s)
+        g.set_config(nsplit=0)
+        few = g.evaluate(0.001, st["x"][:37], st["y"][:37], st["z"][:37])   # ragged, eps_t > 0
+    s = _util.summary(col, exp)
+    print(mode, "block sources: column vs explicit", s, flush=True)
+    if mode == "fp64":
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+    else:
+        assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+    idx = np.union1d(_util.subsample(len(st["x"]), 200, 24), np.arange(37))
+    ref = oracle.eval_plummer(ce["x"][sl], ce["y"][sl], ce["z"][sl], ce["m"][sl], ce["eps"][sl],
+                              st["x"][idx], st["y"][idx], st["z"][idx], None, G, mode=oracle.NEUMAIER,
+                              threads=THREADS)
+    s = _util.summary(tuple(a[idx] for a in col), ref)
+    assert s["acc_max"] <= (1e-12 if mode == "fp64" else 1e-4), s
+    ref37 = oracle.eval_plummer(ce["x"][sl], ce["y"][sl], ce["z"][sl], ce["m"][sl], ce["eps"][sl],
+                                st["x"][:37], st["y"][:37], st["z"][:37], np.full(37, 0.001), G,
+                                mode=oracle.NEUMAIER, threads=THREADS)
+    s = _util.summary(few, ref37)
+    assert s["acc_max"] <= (1e-12 if mode == "fp64" else 1e-4), s
+
+
+def test_block_sources_errors():
+    with GasGravityField("fp64") as g:
+        lib = _lib.load()
+        assert lib.tgrav_set_source_blocks(g.handle, 2, 4, None, None, None, None, None, None, 0) == _lib.EINVAL
+        assert lib.tgrav_set_source_blocks(g.handle, -1, 8, None, None, None, None, None, None, 0) == _lib.EINVAL
+        with pytest.raises(TgravError):
+            g.set_blocks(np.zeros((1, 3)), np.array([0.1]), np.full(512, np.nan))

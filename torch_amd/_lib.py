@@ -47,6 +47,7 @@ SIGNATURES = [
     ("tgrav_variant_info", _I32, [_I32, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32),
                                   ctypes.POINTER(_I32)]),
     ("tgrav_set_sources", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I32]),
+    ("tgrav_set_source_blocks", _I32, [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I32]),
     ("tgrav_eval", _I32, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
     ("tgrav_eval_kick", _I32, [_P, _I64, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P,
                                _I32, _I32]),
@@ -55,6 +56,7 @@ SIGNATURES = [
                                  _P, _P, _P, _I32, _I32]),
     ("tgrav_bidirectional", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P,
                                    _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
+    ("tgrav_set_block_kernel", _I32, [_P, _I32]),
     ("tgrav_set_bidir_config", _I32, [_P, _I32, _I32, _I32]),
     ("tgrav_bidir_variant_info", _I32, [_I32, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     ("tgrav_batch_create", _I32, [_P, ctypes.c_int32, ctypes.POINTER(Problem), _I32, _D,

@@ -1,0 +1,431 @@
+// Leaf-block kernels for 8^3 blocks (FLASH's NXB = NYB = NZB = 8).
+//
+// Block targets (k_eval_grid): one wave = one leaf block.  Lane l owns the cell column
+// (i, j) = (l / 8, l % 8) and its 8 cells along z, so for every source the lane forms
+// dx, dy and dx^2 + dy^2 + eps^2 once and only dz and r^2 per cell:
+//   per interaction  dz (1) + r^2 (1) + rsqrt + m y, y^2, w (3) + phi (1) + a (3)
+//   per source/lane  dx, dy (2) + dx^2 + dy^2 + eps^2 (2), shared by 8 cells
+// i.e. ~9.5 VALU instructions instead of 13 (FP32) per interaction.  Output index
+// b*512 + (i*8 + j)*8 + k is the C order of the block format.
+//
+// Block sources (k_eval_gridsrc): the same sharing with the roles swapped -- the cells of a
+// source block are walked column by column, and every lane forms dx, dy and the partial r^2
+// of its K targets once per column.
+#pragma once
+#include "kernels.hpp"
+
+namespace tg {
+
+template <int NT>
+struct LdsGrid {
+  double4 s4[NT];
+  double se2[NT];
+};
+template <int NT>
+struct LdsGridF {
+  float4 s4[NT];
+  float se2[NT];
+};
+
+template <int FL>
+__device__ __forceinline__ void grid_tile_f64(const double4* __restrict__ s4, const double* __restrict__ se2,
+                                              double tx, double ty, const double (&tz)[8],
+                                              double (&ax)[8], double (&ay)[8], double (&az)[8],
+                                              double (&ap)[8]) {
+#pragma unroll 2
+  for (int j = 0; j < 64; ++j) {
+    const double4 s = s4[j];
+    const double dx = s.x - tx, dy = s.y - ty;
+    const double base = fma(dx, dx, fma(dy, dy, se2[j]));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double dz = s.z - tz[k];
+      const double r2 = fma(dz, dz, base);
+      const double y = rsqrt_f64<FL>(r2);
+      const double mr = s.w * y;
+      if (!(FL & F_NOPOT)) ap[k] += mr;
+      const double w = mr * (y * y);
+      ax[k] = fma(w, dx, ax[k]);
+      ay[k] = fma(w, dy, ay[k]);
+      az[k] = fma(w, dz, az[k]);
+    }
+  }
+}
+
+template <int FL>
+__device__ __forceinline__ void grid_tile_f32(const float4* __restrict__ s4, const float* __restrict__ se2,
+                                              float tx, float ty, const float (&tz)[8],
+                                              float (&ax)[8], float (&ay)[8], float (&az)[8],
+                                              float (&ap)[8]) {
+#pragma unroll 2
+  for (int j = 0; j < 64; ++j) {
+    const float4 s = s4[j];
+    const float dx = s.x - tx, dy = s.y - ty;
+    const float base = fmaf(dx, dx, fmaf(dy, dy, se2[j]));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float dz = s.z - tz[k];
+      const float r2 = fmaf(dz, dz, base);
+      float y = __builtin_amdgcn_rsqf(r2);
+      if (FL & F_GUARD) y = r2 > 0.0f ? y : 0.0f;
+      const float mr = s.w * y;
+      if (!(FL & F_NOPOT)) ap[k] += mr;
+      const float w = (mr * y) * y;
+      ax[k] = fmaf(w, dx, ax[k]);
+      ay[k] = fmaf(w, dy, ay[k]);
+      az[k] = fmaf(w, dz, az[k]);
+    }
+  }
+}
+
+template <int FL>
+__device__ __forceinline__ void grid_sources_f64(LdsGrid<64>& L, const double4* __restrict__ src4,
+                                                 const double* __restrict__ srce2, int64_t sb,
+                                                 int64_t se, double e2t, double tx, double ty,
+                                                 const double (&tz)[8], double (&ax)[8],
+                                                 double (&ay)[8], double (&az)[8], double (&ap)[8]) {
+  const int lane = threadIdx.x;
+  for (int64_t base = sb; base < se; base += 64) {
+    __syncthreads();
+    const int64_t j = base + lane;
+    if (j < se) {
+      L.s4[lane] = src4[j];
+      L.se2[lane] = srce2[j] + e2t;
+    } else {
+      L.s4[lane] = make_double4(0.0, 0.0, 0.0, 0.0);
+      L.se2[lane] = 1.0;
+    }
+    __syncthreads();
+    grid_tile_f64<FL>(L.s4, L.se2, tx, ty, tz, ax, ay, az, ap);
+  }
+}
+
+template <int FL>
+__device__ __forceinline__ void grid_sources_f32(LdsGridF<64>& L, const double4* __restrict__ src4,
+                                                 const double* __restrict__ srce2, int64_t sb,
+                                                 int64_t se, double e2t, double cx, double cy,
+                                                 double cz, float tx, float ty, const float (&tz)[8],
+                                                 float (&hx)[8], float (&hy)[8], float (&hz)[8],
+                                                 float (&hp)[8], float (&lx)[8], float (&ly)[8],
+                                                 float (&lz)[8], float (&lp)[8]) {
+  const int lane = threadIdx.x;
+  for (int64_t base = sb; base < se; base += 64) {
+    __syncthreads();
+    const int64_t j = base + lane;
+    if (j < se) {
+      const double4 s = src4[j];
+      L.s4[lane] = make_float4((float)(s.x - cx), (float)(s.y - cy), (float)(s.z - cz), (float)s.w);
+      L.se2[lane] = (float)(srce2[j] + e2t);
+    } else {
+      L.s4[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+      L.se2[lane] = 1.0f;
+    }
+    __syncthreads();
+    float ax[8], ay[8], az[8], ap[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
+    grid_tile_f32<FL>(L.s4, L.se2, tx, ty, tz, ax, ay, az, ap);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      two_sum(hx[k], lx[k], ax[k]);
+      two_sum(hy[k], ly[k], ay[k]);
+      two_sum(hz[k], lz[k], az[k]);
+      if (!(FL & F_NOPOT)) two_sum(hp[k], lp[k], ap[k]);
+    }
+  }
+}
+
+struct GridF64Run {
+  template <int FL>
+  static __device__ __forceinline__ void run(LdsGrid<64>& L, const double4* src4, const double* srce2,
+                                             int64_t sb, int64_t se, double e2t, double tx,
+                                             double ty, const double (&tz)[8], double (&ax)[8],
+                                             double (&ay)[8], double (&az)[8], double (&ap)[8]) {
+    grid_sources_f64<FL | F_UNIF>(L, src4, srce2, sb, se, e2t, tx, ty, tz, ax, ay, az, ap);
+  }
+};
+
+struct GridF32Run {
+  template <int FL>
+  static __device__ __forceinline__ void run(LdsGridF<64>& L, const double4* src4, const double* srce2,
+                                             int64_t sb, int64_t se, double e2t, double cx,
+                                             double cy, double cz, float tx, float ty,
+                                             const float (&tz)[8], float (&hx)[8], float (&hy)[8],
+                                             float (&hz)[8], float (&hp)[8], float (&lx)[8],
+                                             float (&ly)[8], float (&lz)[8], float (&lp)[8]) {
+    grid_sources_f32<FL | F_UNIF>(L, src4, srce2, sb, se, e2t, cx, cy, cz, tx, ty, tz, hx, hy, hz,
+                                  hp, lx, ly, lz, lp);
+  }
+};
+
+#define TG_DISPATCH_GRID(FN, fl, ...)                                         \
+  switch (fl & (F_GUARD | F_NOPOT | F_SEED32)) {                              \
+    case 0: FN<0>(__VA_ARGS__); break;                                        \
+    case F_GUARD: FN<F_GUARD>(__VA_ARGS__); break;                            \
+    case F_NOPOT: FN<F_NOPOT>(__VA_ARGS__); break;                            \
+    case F_NOPOT | F_GUARD: FN<F_NOPOT | F_GUARD>(__VA_ARGS__); break;        \
+    case F_SEED32: FN<F_SEED32>(__VA_ARGS__); break;                          \
+    default: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                      \
+  }
+
+// blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
+// workgroup of 64 lanes.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restrict__ src4,
+                                                  const double* __restrict__ srce2, int64_t ns,
+                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
+  __shared__ Lds L;
+  const int lane = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  const int i = lane >> 3, jj = lane & 7;
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  const double d = T.bdx[b];
+  const double e = T.beps ? T.beps[b] : 0.0;
+  const double e2t = e * e;
+  double x, y, z[8];
+  {
+#pragma clang fp contract(off)
+    x = T.ox[b] + ((double)i + 0.5) * d;
+    y = T.oy[b] + ((double)jj + 0.5) * d;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) z[k] = T.oz[b] + ((double)k + 0.5) * d;
+  }
+  const int fl = tile_flags(*e2s_min_p, e2t, e2t, O, MIXED);
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  const int64_t t0 = b * 512 + (int64_t)lane * 8;
+  if constexpr (MIXED) {
+    // anchor: the block centre (the bounding box centre of its cell centres)
+    const double cx = T.ox[b] + 4.0 * d, cy = T.oy[b] + 4.0 * d, cz = T.oz[b] + 4.0 * d;
+    const float tx = (float)(x - cx), ty = (float)(y - cy);
+    float tz[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) tz[k] = (float)(z[k] - cz);
+    float hx[8], hy[8], hz[8], hp[8], lx[8], ly[8], lz[8], lp[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = lx[k] = ly[k] = lz[k] = lp[k] = 0.f;
+    TG_DISPATCH_GRID(GridF32Run::run, fl & ~F_SEED32, L, src4, srce2, sb, se, e2t, cx, cy, cz, tx,
+                     ty, tz, hx, hy, hz, hp, lx, ly, lz, lp)
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      store_result(o, t0 + k, (double)hx[k] + (double)lx[k], (double)hy[k] + (double)ly[k],
+                   (double)hz[k] + (double)lz[k], (double)hp[k] + (double)lp[k]);
+  } else {
+    double ax[8], ay[8], az[8], ap[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+    TG_DISPATCH_GRID(GridF64Run::run, fl, L, src4, srce2, sb, se, e2t, x, y, z, ax, ay, az, ap)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
+  }
+}
+
+// ---------------------------------------------------------------- block sources ----
+template <bool MIXED>
+struct LdsGS {
+  typedef typename std::conditional<MIXED, float, double>::type T;
+  T m[512];                    // cell masses of the staged block, C order
+  T cx[8], cy[8], cz[8];       // cell-centre coordinates (mixed: relative to the anchor)
+  T e2;                        // block eps^2 (+ the tile's uniform target eps^2)
+  double red[2 * 4 * 4];
+};
+
+template <int K, bool MIXED, int FL>
+__device__ __forceinline__ void gs_block(const LdsGS<MIXED>& L,
+                                         const typename LdsGS<MIXED>::T (&tx)[K],
+                                         const typename LdsGS<MIXED>::T (&ty)[K],
+                                         const typename LdsGS<MIXED>::T (&tz)[K],
+                                         const typename LdsGS<MIXED>::T (&te2)[K],
+                                         typename LdsGS<MIXED>::T (&ax)[K],
+                                         typename LdsGS<MIXED>::T (&ay)[K],
+                                         typename LdsGS<MIXED>::T (&az)[K],
+                                         typename LdsGS<MIXED>::T (&ap)[K]) {
+  typedef typename LdsGS<MIXED>::T T;
+  const T e2 = L.e2;
+  for (int col = 0; col < 64; ++col) {
+    const T xs = L.cx[col >> 3], ys = L.cy[col & 7];
+    T dx[K], dy[K], base[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      dx[k] = xs - tx[k];
+      dy[k] = ys - ty[k];
+      base[k] = fma(dx[k], dx[k], fma(dy[k], dy[k], (FL & F_UNIF) ? e2 : e2 + te2[k]));
+    }
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const T zs = L.cz[kk];
+      const T m = L.m[col * 8 + kk];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const T dz = zs - tz[k];
+        const T r2 = fma(dz, dz, base[k]);
+        T y;
+        if constexpr (MIXED) {
+          y = __builtin_amdgcn_rsqf(r2);
+          if (FL & F_GUARD) y = r2 > 0.0f ? y : 0.0f;
+        } else {
+          y = rsqrt_f64<FL>(r2);
+        }
+        const T mr = m * y;
+        if (!(FL & F_NOPOT)) ap[k] += mr;
+        const T w = (mr * y) * y;
+        ax[k] = fma(w, dx[k], ax[k]);
+        ay[k] = fma(w, dy[k], ay[k]);
+        az[k] = fma(w, dz, az[k]);
+      }
+    }
+  }
+}
+
+// Stages one source block (masses and cell-centre coordinates) into LDS.
+template <bool MIXED>
+__device__ __forceinline__ void gs_stage(LdsGS<MIXED>& L, const double4* __restrict__ bd,
+                                         const double* __restrict__ be2,
+                                         const double4* __restrict__ s4, int64_t blk, double e2u,
+                                         double ax_, double ay_, double az_) {
+  typedef typename LdsGS<MIXED>::T T;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < 512; q += blockDim.x) L.m[q] = (T)s4[blk * 512 + q].w;
+  if (tid < 24) {
+    const double4 b = bd[blk];   // {ox, oy, oz, dx}
+    const int axis = tid >> 3, idx = tid & 7;
+    double c;
+    {
+#pragma clang fp contract(off)
+      c = (axis == 0 ? b.x : axis == 1 ? b.y : b.z) + ((double)idx + 0.5) * b.w;
+    }
+    const double a = axis == 0 ? ax_ : axis == 1 ? ay_ : az_;
+    T v = MIXED ? (T)(c - a) : (T)c;
+    if (axis == 0) L.cx[idx] = v;
+    else if (axis == 1) L.cy[idx] = v;
+    else L.cz[idx] = v;
+  }
+  if (tid == 0) L.e2 = (T)(be2[blk] + e2u);
+}
+
+template <int K, int NT, bool MIXED>
+struct GsRun {
+  typedef typename LdsGS<MIXED>::T T;
+  template <int FL>
+  static __device__ __forceinline__ void run(LdsGS<MIXED>& L, const double4* bd, const double* be2,
+                                             const double4* bm, int64_t b0, int64_t b1, double e2u,
+                                             double ax_, double ay_, double az_, const T (&tx)[K],
+                                             const T (&ty)[K], const T (&tz)[K], const T (&te2)[K],
+                                             double (&hx)[K], double (&hy)[K], double (&hz)[K],
+                                             double (&hp)[K]) {
+    for (int64_t blk = b0; blk < b1; ++blk) {
+      __syncthreads();
+      gs_stage<MIXED>(L, bd, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
+      __syncthreads();
+      T ax[K], ay[K], az[K], ap[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = (T)0;
+      gs_block<K, MIXED, FL>(L, tx, ty, tz, te2, ax, ay, az, ap);
+      // per-block partials folded into FP64 accumulators (mixed: one FP32 sum per 512 cells)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        hx[k] += (double)ax[k];
+        hy[k] += (double)ay[k];
+        hz[k] += (double)az[k];
+        hp[k] += (double)ap[k];
+      }
+    }
+  }
+};
+
+// Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
+// blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
+template <int K, int NT, bool MIXED>
+__global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double4* __restrict__ bd,
+                                                     const double* __restrict__ be2,
+                                                     const double4* __restrict__ bm, int64_t nblk,
+                                                     int64_t chunk, const double* e2s_min_p, Out O) {
+  typedef typename LdsGS<MIXED>::T TT;
+  __shared__ LdsGS<MIXED> L;
+  const int tid = threadIdx.x;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t b0 = (int64_t)blockIdx.y * chunk;
+  const int64_t b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
+  double px[K], py[K], pz[K], pe2[K];
+  double mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  double mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    load_target(T, t < T.n ? t : T.n - 1, px[k], py[k], pz[k], pe2[k]);
+    if (t < T.n) {
+      mn[0] = fmin(mn[0], px[k]); mx[0] = fmax(mx[0], px[k]);
+      mn[1] = fmin(mn[1], py[k]); mx[1] = fmax(mx[1], py[k]);
+      mn[2] = fmin(mn[2], pz[k]); mx[2] = fmax(mx[2], pz[k]);
+      mn[3] = fmin(mn[3], pe2[k]); mx[3] = fmax(mx[3], pe2[k]);
+    }
+  }
+  block_minmax<NT, 4>(mn, mx, L.red);
+  double ax_ = 0.0, ay_ = 0.0, az_ = 0.0;
+  if (MIXED) {
+    ax_ = 0.5 * (mn[0] + mx[0]);
+    ay_ = 0.5 * (mn[1] + mx[1]);
+    az_ = 0.5 * (mn[2] + mx[2]);
+  }
+  const int fl = tile_flags(*e2s_min_p, mn[3], mx[3], O, MIXED);
+  TT tx[K], ty[K], tz[K], te2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    tx[k] = (TT)(px[k] - ax_);
+    ty[k] = (TT)(py[k] - ay_);
+    tz[k] = (TT)(pz[k] - az_);
+    te2[k] = (TT)pe2[k];
+  }
+  double hx[K], hy[K], hz[K], hp[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
+  typedef GsRun<K, NT, MIXED> R;
+  if constexpr (MIXED) {
+    TG_DISPATCH8(R::template run, fl, L, bd, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
+                 hx, hy, hz, hp)
+  } else {
+    TG_DISPATCH(R::template run, fl, L, bd, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
+                hx, hy, hz, hp)
+  }
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    if (t < T.n) store_result(o, t, hx[k], hy[k], hz[k], hp[k]);
+  }
+}
+
+// Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them) and
+// folds the smallest block eps^2 into stat[0]; one lane per cell.
+__global__ void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox,
+                                     const double* __restrict__ oy, const double* __restrict__ oz,
+                                     const double* __restrict__ dx, const double* __restrict__ m,
+                                     const double* __restrict__ eps, double4* __restrict__ s4,
+                                     double* __restrict__ se2, double4* __restrict__ bd,
+                                     double* __restrict__ be2, unsigned long long* stat) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nblk * 512) return;
+  const int64_t b = i >> 9;
+  const int c = (int)(i & 511);
+  const double d = dx[b];
+  double x, y, z;
+  {
+#pragma clang fp contract(off)
+    x = ox[b] + ((double)(c >> 6) + 0.5) * d;
+    y = oy[b] + ((double)((c >> 3) & 7) + 0.5) * d;
+    z = oz[b] + ((double)(c & 7) + 0.5) * d;
+  }
+  const double e = eps ? eps[b] : 0.0;
+  s4[i] = make_double4(x, y, z, m[i]);
+  se2[i] = e * e;
+  if (c == 0) {
+    bd[b] = make_double4(ox[b], oy[b], oz[b], d);
+    be2[b] = e * e;
+    atomicMin(&stat[0], (unsigned long long)__double_as_longlong(e * e));
+  }
+}
+
+}  // namespace tg

@@ -19,6 +19,7 @@
 #include "kernels.hpp"
 #include "batch.hpp"
 #include "bidir.hpp"
+#include "grid.hpp"
 
 #define TGRAV_VERSION 1
 
@@ -162,6 +163,7 @@ struct tgrav_ctx {
   int validate = 1;
   int variant[2] = {kDefaultFp64, kDefaultMixed};
   int nsplit[2] = {0, 0};
+  int block_kernel = 1;      // 8^3 block targets use the column kernel (k_eval_grid)
   int bvariant[2] = {1, 1};  // fused bidirectional kernel: variant and particle chunks
   int bnsplit[2] = {0, 0};
   double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
@@ -169,6 +171,9 @@ struct tgrav_ctx {
   // packed sources; stat[0] = bits of min eps_s^2 (read by the kernels), stat[1] = NaN flag
   int64_t ns = -1;
   DevBuf s4, se2, stat;
+  // sources given as 8^3 blocks: descriptors {ox, oy, oz, dx} and eps^2 per block
+  int64_t src_nblk = 0;      // > 0: the current source set is block-structured
+  DevBuf sbd, sbe2;
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
   DevBuf in[12], outb[8], vel[3], part, flag;
   DevBuf bp4, bpe2, bstat, bppart;   // fused bidirectional pass: packed particles, partials
@@ -296,6 +301,100 @@ static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t
   if (*nsplit < 1) *nsplit = 1;
 }
 
+// 8^3 block targets: one 64-lane workgroup per leaf block (k_eval_grid), sources split over
+// blockIdx.y until there are about 16 waves per CU.
+static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
+  const int64_t nt = T.n, ns = c->ns, nblk = nt / 512;
+  if (nblk > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many blocks");
+  int want = c->nsplit[mode];
+  if (want <= 0) {
+    const int64_t target = 16LL * c->num_cu;
+    want = nblk >= target / 2 ? 1
+                              : (int)std::min<int64_t>((target + nblk - 1) / nblk,
+                                                       std::max<int64_t>(1, ns / 256));
+  }
+  if (want > 65535) want = 65535;
+  int64_t chunk = (ns + want - 1) / want;
+  chunk = (chunk + 63) / 64 * 64;
+  const int nsplit = (int)((ns + chunk - 1) / chunk);
+  tg::Out o = O;
+  if (nsplit > 1) {
+    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+    if (rc0) return rc0;
+    o.part = c->part.as<double>();
+    o.pstride = nt;
+  } else {
+    o.part = nullptr;
+  }
+  int rc = timed_launch(c, nt, ns, [&]() -> int {
+    if (mode == TGRAV_MIXED)
+      tg::k_eval_grid<true><<<dim3((unsigned)nblk, (unsigned)nsplit), 64, 0, c->stream>>>(
+          T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
+    else
+      tg::k_eval_grid<false><<<dim3((unsigned)nblk, (unsigned)nsplit), 64, 0, c->stream>>>(
+          T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
+    LAUNCHCHECK("k_eval_grid");
+    return 0;
+  });
+  if (rc) return rc;
+  if (nsplit > 1) {
+    tg::Out fin = O;
+    fin.part = nullptr;
+    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
+                                                                nt, fin);
+    LAUNCHCHECK("k_reduce_chunks");
+  }
+  return 0;
+}
+
+// Explicit targets against 8^3 source blocks: K = 4 targets per lane, 256 lanes, source
+// blocks split over blockIdx.y until there are about 4 workgroups per CU.
+static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
+  const int K = 4, NT = 256;
+  const int64_t nt = T.n, nblk = c->src_nblk;
+  const int64_t ntiles = (nt + K * NT - 1) / (K * NT);
+  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
+  int want = c->nsplit[mode];
+  if (want <= 0) {
+    const int64_t target = 4LL * c->num_cu;
+    want = ntiles >= target / 2 ? 1 : (int)std::min<int64_t>((target + ntiles - 1) / ntiles, nblk);
+  }
+  if (want > 65535) want = 65535;
+  const int64_t chunk = (nblk + want - 1) / want;
+  const int nsplit = (int)((nblk + chunk - 1) / chunk);
+  tg::Out o = O;
+  if (nsplit > 1) {
+    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+    if (rc0) return rc0;
+    o.part = c->part.as<double>();
+    o.pstride = nt;
+  } else {
+    o.part = nullptr;
+  }
+  int rc = timed_launch(c, nt, c->ns, [&]() -> int {
+    const dim3 g((unsigned)ntiles, (unsigned)nsplit);
+    if (mode == TGRAV_MIXED)
+      tg::k_eval_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
+          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
+          c->stat.as<const double>(), o);
+    else
+      tg::k_eval_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
+          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
+          c->stat.as<const double>(), o);
+    LAUNCHCHECK("k_eval_gridsrc");
+    return 0;
+  });
+  if (rc) return rc;
+  if (nsplit > 1) {
+    tg::Out fin = O;
+    fin.part = nullptr;
+    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
+                                                                nt, fin);
+    LAUNCHCHECK("k_reduce_chunks");
+  }
+  return 0;
+}
+
 // Core launch: targets T (device), outputs O (device), on c->stream.
 static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
   const int64_t nt = T.n, ns = c->ns;
@@ -311,6 +410,8 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
     LAUNCHCHECK("k_fill");
     return 0;
   }
+  if (T.blocks && T.nb == 8 && c->block_kernel) return launch_grid(c, mode, T, O);
+  if (!T.blocks && c->src_nblk > 0 && c->block_kernel) return launch_gridsrc(c, mode, T, O);
   const Variant* v = variant_of(mode, c->variant[mode]);
   int64_t ntiles, chunk;
   int nsplit;
@@ -389,6 +490,8 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->part.release();
   c->stat.release();
   c->flag.release();
+  c->sbd.release();
+  c->sbe2.release();
   c->bp4.release();
   c->bpe2.release();
   c->bstat.release();
@@ -473,6 +576,7 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
     return fail(TGRAV_EINVAL, "source x, y, z and m are required");
   if ((rc = set_device(c))) return rc;
   c->ns = -1;
+  c->src_nblk = 0;
   if (n == 0) {
     c->ns = 0;
     return 0;
@@ -520,6 +624,54 @@ static void bind_targets(tg::Tgt& T, const double* const* tin, int ntin) {
     T.z = tin[2];
     T.eps = tin[3];
   }
+}
+
+int tgrav_set_source_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox,
+                            const double* oy, const double* oz, const double* dx, const double* m,
+                            const double* eps, int kind) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc = check_kind(kind);
+  if (rc) return rc;
+  if (nblk < 0) return fail(TGRAV_EINVAL, "number of blocks must be >= 0");
+  if (nb != 8) return fail(TGRAV_EINVAL, "source blocks must have 8^3 cells (got nb = %d)", nb);
+  if (nblk > 0 && (!ox || !oy || !oz || !dx || !m))
+    return fail(TGRAV_EINVAL, "block origins, widths and cell masses are required");
+  if ((rc = set_device(c))) return rc;
+  c->ns = -1;
+  c->src_nblk = 0;
+  if (nblk == 0) {
+    c->ns = 0;
+    return 0;
+  }
+  const int64_t n = nblk * 512;
+  const double* a[6] = {ox, oy, oz, dx, eps, m};
+  if (kind == TGRAV_HOST) {
+    if (c->validate) {
+      for (int q = 0; q < 5; ++q)
+        if (!host_finite(nblk, a[q])) return fail(TGRAV_ENONFINITE, "non-finite block value");
+      if (!host_finite(n, m)) return fail(TGRAV_ENONFINITE, "non-finite cell mass");
+    }
+    if ((rc = stage_in(c, 0, nblk, a, 5)) || (rc = stage_in(c, 5, n, a + 5, 1))) return rc;
+  } else if (c->validate) {
+    bool bad = false;
+    if ((rc = device_check_finite(c, nblk, a, 5, &bad))) return rc;
+    if (!bad && (rc = device_check_finite(c, n, a + 5, 1, &bad))) return rc;
+    if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in a source block array");
+  }
+  if ((rc = c->s4.ensure((size_t)n * sizeof(double4))) || (rc = c->se2.ensure((size_t)n * 8)) ||
+      (rc = c->stat.ensure(2 * sizeof(unsigned long long))) ||
+      (rc = c->sbd.ensure((size_t)nblk * sizeof(double4))) || (rc = c->sbe2.ensure((size_t)nblk * 8)))
+    return rc;
+  tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->stat.as<unsigned long long>());
+  LAUNCHCHECK("k_reset_stats");
+  tg::k_pack_source_blocks<<<grid1(n, 256), 256, 0, c->stream>>>(
+      nblk, a[0], a[1], a[2], a[3], a[5], a[4], c->s4.as<double4>(), c->se2.as<double>(),
+      c->sbd.as<double4>(), c->sbe2.as<double>(), c->stat.as<unsigned long long>());
+  LAUNCHCHECK("k_pack_source_blocks");
+  if (kind == TGRAV_HOST) HIPCHECK(hipStreamSynchronize(c->stream));
+  c->ns = n;
+  c->src_nblk = nblk;
+  return 0;
 }
 
 // Shared body of tgrav_eval / tgrav_eval_kick / tgrav_eval_blocks.
@@ -803,6 +955,12 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     }
     HIPCHECK(hipStreamSynchronize(c->stream));
   }
+  return 0;
+}
+
+int tgrav_set_block_kernel(tgrav_ctx* c, int on) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  c->block_kernel = on ? 1 : 0;
   return 0;
 }
 

@@ -255,6 +255,10 @@ class GravityField:
         _lib.check(rc, "tgrav_bidirectional")
         return tuple(co), tuple(po)
 
+    def set_block_kernel(self, on=True):
+        """Use (default) or bypass the 8^3 column kernel for block-structured targets."""
+        _lib.check(self.lib.tgrav_set_block_kernel(self._ctx, int(bool(on))), "tgrav_set_block_kernel")
+
     def set_bidir_config(self, variant=-1, nsplit=-1):
         _lib.check(self.lib.tgrav_set_bidir_config(self._ctx, self.mode, int(variant), int(nsplit)),
                    "tgrav_set_bidir_config")
@@ -327,12 +331,22 @@ class GasGravityField(GravityField):
         mass = rho * dx * dx * dx
         self.set_sources(x, y, z, mass, dx if eps is None else eps)
 
-    def set_blocks(self, origin, dx, rho, nb=8, eps=None):
-        """Leaf blocks (lower corners origin[n,3], cell width dx[n]) with densities
-        rho[n*nb^3] in C order; softening eps per cell (default: the cell width)."""
-        from .blocks import block_centres
-        x, y, z, dxc = block_centres(origin, dx, nb)
-        self.set_cells(x, y, z, rho, dxc, eps)
+    def set_blocks(self, origin, dx, mass, nb=8, eps=None):
+        """Leaf blocks as the source set: lower corners origin[n,3], cell width dx[n], cell
+        masses mass[n*512] (rho dV) in C order, softening eps[n] per block (default: the cell
+        width).  Evaluations use the column kernel for 8^3 blocks."""
+        nblk = int(origin.shape[0])
+        dev = _torch_device(origin, dx, mass, eps)
+        A, B = _Arrays(nblk, dev), _Arrays(nblk * nb ** 3, dev)
+        o = [A.inp(origin[:, q], "origin") for q in range(3)]
+        d = A.inp(dx, "dx")
+        e = A.inp(dx if eps is None else eps, "eps")
+        m = B.inp(mass, "mass")
+        self._bind_stream(A)
+        rc = self.lib.tgrav_set_source_blocks(self._ctx, nblk, int(nb), *[A.ptr(q) for q in o],
+                                              A.ptr(d), B.ptr(m), A.ptr(e), A.kind)
+        _lib.check(rc, "tgrav_set_source_blocks")
+        self.n_sources = nblk * nb ** 3
 
     def get_accel_gas_on_particles(self, eps, x, y, z):
         ax, ay, az, _ = self.evaluate(eps, x, y, z, potential=False)
