@@ -78,7 +78,11 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
     from torch_amd import GasGravityField, StellarGravityField, autotune
     fwd = StellarGravityField(mode, validate=False)
     rev = GasGravityField(mode, validate=False)
-    cfg = (None, None) if blocks else (autotune.apply_eval(fwd, ns, nc), autotune.apply_eval(rev, nc, ns))
+    if blocks:
+        nblk = nc // 512
+        cfg = (autotune.apply_blocks(fwd, "blockfwd", ns, nblk), autotune.apply_blocks(rev, "blocksrc", ns, nblk))
+    else:
+        cfg = (autotune.apply_eval(fwd, ns, nc), autotune.apply_eval(rev, nc, ns))
     log(f"{mode}: {'block' if blocks else 'explicit'} format, tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]

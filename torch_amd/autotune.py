@@ -163,6 +163,51 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
     return best, table
 
 
+def blocks_key(mode, direction, ns, nblk):
+    return f"{device_key()}|{direction}|{mode}|s{_bucket(ns)}|b{_bucket(nblk)}"
+
+
+def tune_blocks(mode, workload="c2", reps=5, log=print):
+    """Source split of the 8^3 column kernels on a workload's own leaf blocks:
+    forward (stars -> block targets) and reverse (block sources -> stars)."""
+    import torch
+    from . import workloads as W
+    w = {"c2": W.c2, "c5": W.c5}[workload]()
+    st, ce = w["stars"], w["cells"]
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    S = [d(st[k]) for k in ("x", "y", "z", "m", "eps")]
+    BO, BD, CM = d(ce["block_origin"]), d(ce["block_size"] / 8), d(ce["m"])
+    ns, nblk = len(st["x"]), len(ce["block_size"])
+    out = {}
+    with GravityField(mode, validate=False) as f:
+        f.set_sources(*S)
+        table = []
+        for sp in (0, 1, 2, 3, 4, 6, 8, 12, 16):
+            f.set_config(nsplit=sp)
+            ms = _time(lambda: f.evaluate_blocks(BO, BD), reps)
+            table.append(dict(nsplit=sp, ms=ms))
+            log(f"  {mode} block targets nsplit={sp}: {ms:.3f} ms")
+        out[blocks_key(mode, "blockfwd", ns, nblk)] = min(table, key=lambda r: r["ms"])
+        f.set_blocks(BO, BD, CM)
+        table = []
+        for sp in (0, 16, 32, 64, 100, 128, 200, 256, 400):
+            f.set_config(nsplit=sp)
+            ms = _time(lambda: f.evaluate(None, S[0], S[1], S[2]), reps)
+            table.append(dict(nsplit=sp, ms=ms))
+            log(f"  {mode} block sources nsplit={sp}: {ms:.3f} ms")
+        out[blocks_key(mode, "blocksrc", ns, nblk)] = min(table, key=lambda r: r["ms"])
+    return out
+
+
+def apply_blocks(field, direction, ns, nblk, cache=None):
+    cache = load_cache() if cache is None else cache
+    mode = {v: k for k, v in MODES.items() if k != "plain_fp64"}[field.mode]
+    e = cache.get(blocks_key(mode, direction, ns, nblk))
+    if e:
+        field.set_config(nsplit=e["nsplit"])
+    return e
+
+
 def apply_eval(field, ns, nt, cache=None):
     """Configure `field` from the cache for an evaluation of this size; returns the entry."""
     cache = load_cache() if cache is None else cache
@@ -189,6 +234,7 @@ def main():
     ap.add_argument("--eval", nargs=2, type=int, action="append", default=[],
                     metavar=("NS", "NT"))
     ap.add_argument("--batch", action="append", default=[], choices=["c4"])
+    ap.add_argument("--blocks", action="append", default=[], choices=["c2", "c5"])
     ap.add_argument("--modes", default="mixed,fp64")
     ap.add_argument("--cache", default=CACHE)
     ap.add_argument("--reps", type=int, default=5)
@@ -201,6 +247,11 @@ def main():
             cache[eval_key(mode, ns, nt)] = dict(variant=best["variant"], nsplit=best["nsplit"],
                                                  ms=best["ms"], k=best["k"], threads=best["threads"])
             print(f"best {mode} eval ns={ns} nt={nt}: {cache[eval_key(mode, ns, nt)]}", flush=True)
+            save_cache(cache, args.cache)
+        for name in args.blocks:
+            res = tune_blocks(mode, name, reps=args.reps)
+            cache.update(res)
+            print(f"best {mode} blocks {name}: {res}", flush=True)
             save_cache(cache, args.cache)
         for name in args.batch:
             from . import workloads as W

@@ -3,7 +3,8 @@
     python -m torch_amd.profile_driver --workload c2 --runs mixed:fwd fp64:fwd:1 --reps 3 --copy-mb 512
 
 Each run spec is MODE:DIRECTION[:VARIANT[:NSPLIT]] (direction fwd = stars -> cells,
-rev = cells -> stars; variant/nsplit default to the tuned shape).  After the evaluations
+rev = cells -> stars, fwdb / revb = the same with the cells as 8^3 leaf blocks (column
+kernels; VARIANT is ignored there); variant/nsplit default to the tuned shape).  After the evaluations
 it performs one plain device-to-device copy of --copy-mb MiB (torch copy_), whose
 FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters against a known byte count.
 Everything runs on torch's current stream; the program prints one line per step.
@@ -32,18 +33,26 @@ def main():
     S = [d(st[k]) for k in ("x", "y", "z", "m", "eps")]
     C = [d(ce[k]) for k in ("x", "y", "z", "m", "eps")]
     torch.cuda.synchronize()
+    BO = d(ce["block_origin"])
+    BD = d(ce["block_size"] / 8)
     for spec in args.runs:
         parts = spec.split(":")
         mode, direction = parts[0], parts[1]
-        src, tgt = (S, C) if direction == "fwd" else (C, S)
+        src, tgt = (S, C) if direction.startswith("fwd") else (C, S)
         with GravityField(mode, validate=False) as f:
             e = autotune.apply_eval(f, src[0].numel(), tgt[0].numel())
             if len(parts) > 2:
                 f.set_config(variant=int(parts[2]), nsplit=int(parts[3]) if len(parts) > 3 else 0)
-            f.set_sources(*src)
+            if direction == "revb":
+                f.set_blocks(BO, BD, C[3])
+            else:
+                f.set_sources(*src)
             f.set_timing(True)
             for r in range(args.reps):
-                f.evaluate(None, tgt[0], tgt[1], tgt[2])
+                if direction == "fwdb":
+                    f.evaluate_blocks(BO, BD)
+                else:
+                    f.evaluate(None, tgt[0], tgt[1], tgt[2])
             torch.cuda.synchronize()
             ks = f.kernel_times()
             print(f"{spec}: tuned={e} kernel ms={[round(k[0], 3) for k in ks]}", flush=True)
