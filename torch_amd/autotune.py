@@ -188,7 +188,7 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
             table.append(dict(nsplit=sp, ms=ms))
             log(f"  {mode} block targets nsplit={sp}: {ms:.3f} ms")
         out[blocks_key(mode, "blockfwd", ns, nblk)] = min(table, key=lambda r: r["ms"])
-        f.set_blocks(BO, BD, CM)
+        f.set_source_blocks(BO, BD, CM)
         table = []
         for sp in (0, 16, 32, 64, 100, 128, 200, 256, 400):
             f.set_config(nsplit=sp)

@@ -178,6 +178,23 @@ class GravityField:
         _lib.check(rc, "tgrav_set_sources")
         self.n_sources = n
 
+    def set_source_blocks(self, origin, dx, mass, nb=8, eps=None):
+        """Leaf blocks as the source set: lower corners origin[n,3], cell width dx[n], cell
+        masses mass[n*512] (rho dV) in C order, softening eps[n] per block (default: the cell
+        width).  Evaluations use the column kernel for 8^3 blocks."""
+        nblk = int(origin.shape[0])
+        dev = _torch_device(origin, dx, mass, eps)
+        A, B = _Arrays(nblk, dev), _Arrays(nblk * nb ** 3, dev)
+        o = [A.inp(origin[:, q], "origin") for q in range(3)]
+        d = A.inp(dx, "dx")
+        e = A.inp(dx if eps is None else eps, "eps")
+        m = B.inp(mass, "mass")
+        self._bind_stream(A)
+        rc = self.lib.tgrav_set_source_blocks(self._ctx, nblk, int(nb), *[A.ptr(q) for q in o],
+                                              A.ptr(d), B.ptr(m), A.ptr(e), A.kind)
+        _lib.check(rc, "tgrav_set_source_blocks")
+        self.n_sources = nblk * nb ** 3
+
     # -- evaluation ---------------------------------------------------------------
     def evaluate(self, eps, x, y, z, acceleration=True, potential=True):
         """(ax, ay, az, phi) at the points; unrequested outputs are None."""
@@ -332,21 +349,9 @@ class GasGravityField(GravityField):
         self.set_sources(x, y, z, mass, dx if eps is None else eps)
 
     def set_blocks(self, origin, dx, mass, nb=8, eps=None):
-        """Leaf blocks as the source set: lower corners origin[n,3], cell width dx[n], cell
-        masses mass[n*512] (rho dV) in C order, softening eps[n] per block (default: the cell
-        width).  Evaluations use the column kernel for 8^3 blocks."""
-        nblk = int(origin.shape[0])
-        dev = _torch_device(origin, dx, mass, eps)
-        A, B = _Arrays(nblk, dev), _Arrays(nblk * nb ** 3, dev)
-        o = [A.inp(origin[:, q], "origin") for q in range(3)]
-        d = A.inp(dx, "dx")
-        e = A.inp(dx if eps is None else eps, "eps")
-        m = B.inp(mass, "mass")
-        self._bind_stream(A)
-        rc = self.lib.tgrav_set_source_blocks(self._ctx, nblk, int(nb), *[A.ptr(q) for q in o],
-                                              A.ptr(d), B.ptr(m), A.ptr(e), A.kind)
-        _lib.check(rc, "tgrav_set_source_blocks")
-        self.n_sources = nblk * nb ** 3
+        """Leaf blocks as the source set (see GravityField.set_source_blocks); softening
+        defaults to the cell width, as for set_cells."""
+        self.set_source_blocks(origin, dx, mass, nb, eps)
 
     def get_accel_gas_on_particles(self, eps, x, y, z):
         ax, ay, az, _ = self.evaluate(eps, x, y, z, potential=False)

@@ -44,7 +44,7 @@ def main():
             if len(parts) > 2:
                 f.set_config(variant=int(parts[2]), nsplit=int(parts[3]) if len(parts) > 3 else 0)
             if direction == "revb":
-                f.set_blocks(BO, BD, C[3])
+                f.set_source_blocks(BO, BD, C[3])
             else:
                 f.set_sources(*src)
             f.set_timing(True)
