@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Pairwise interactions/s of the direct-sum gravity field on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|c5|c2fused] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|c5|c2fused|c2flash|c1periodic] [--no-cpu]
 
 Default workload c2 (seed 1002): 10,000 Plummer/Kroupa stars (20% hard-binary members) and
 1,024,000 AMR cell centres (2,000 leaf blocks of 8^3).  One step is one bridge field
@@ -13,6 +13,12 @@ loading its source set.  2 x 1.024e10 pair interactions per step.
       one replay, acc + pot of all problems).
   c2fused: c2 with symmetric softening (eps2 = eps_cell^2 + eps_star^2) evaluated by the fused
       bidirectional pass (one pass gives both directions); two directed passes timed beside it.
+  c2flash: the reverse direction of c2 (1,024,000 cells as 8^3 blocks -> 10,000 stars) with
+      FLASH's own spline softening (global radius 0.078125 pc = 2.5 finest cells, the reference's
+      potential), FP64.
+  c1periodic: c1 reverse (4,096 cells -> 1,024 stars, 4 pc box) with FLASH's spline softening
+      and its periodic image sum (nrep = 6: images within 24 pc), FP64; the rate counts pairs,
+      `image_interactions_per_step` the images summed.
 Inputs are resident in HBM (device tensors) before timing.
 
 The headline `value` is the mixed-precision rate; the plain-FP64 kernel is timed the same
@@ -199,18 +205,81 @@ def run_fused(mode, dev, ns, nc, steps, warmup):
                 directed_res=res["directed"]["res"], cfg=None)
 
 
+# ------------------------------------------------------- c2flash / c1periodic ----
+def run_flash(dev, steps, warmup, rsoft, box=None, nrep=6):
+    """Cells -> stars with FLASH's spline kernel (reference potential), FP64; cells as 8^3
+    leaf blocks when the workload has them, else as explicit arrays."""
+    import torch
+
+    from torch_amd import GasGravityField
+    rev = GasGravityField("fp64", validate=False)
+    rev.set_softening("spline", rsoft, quirks=True)
+    if box is not None:
+        rev.set_periodic(box, nrep)
+    S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
+    C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
+
+    def step():
+        if "bo" in dev:
+            rev.set_blocks(dev["bo"], dev["bdx"], C[3])
+        else:
+            rev.set_sources(*C)
+        return rev.evaluate(None, S[0], S[1], S[2])
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    rev.set_timing(True)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        out = step()
+    t1.record()
+    torch.cuda.synchronize()
+    kr = [r[0] for r in rev.kernel_times()]
+    res = tuple(a.cpu().numpy() for a in out)
+    rev.close()
+    return dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.mean(kr)), res=res)
+
+
+def mean_images_in_cutoff(S, T, box, nrep, npairs=2000, seed=99):
+    """Mean number of images per (cell, point) pair inside nrep * min(box), over seeded pairs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    i = rng.integers(0, len(T["x"]), npairs)
+    j = rng.integers(0, len(S["x"]), npairs)
+    d = np.stack([T[k][i] - S[k][j] for k in ("x", "y", "z")], 1)
+    r = np.arange(-nrep, nrep + 1)
+    n = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3) * np.asarray(box)
+    cnt = 0
+    for q in range(0, npairs, 100):
+        rr = np.sqrt(((d[q:q + 100, None, :] + n[None, :, :]) ** 2).sum(-1))
+        cnt += int((rr < nrep * min(box)).sum())
+    return cnt / npairs
+
+
 # ----------------------------------------------------------- checker legs ----
-def cpu_baseline(S, T, G, budget_s, label):
+def cpu_baseline(S, T, G, budget_s, label, flash=None):
     """The oracle (C port of the reference loop nest, -O2) on seeded target subsamples:
-    one thread, then all host cores (<= 16)."""
+    one thread, then all host cores (<= 16).  flash = dict(rsoft, box, nrep) times the FLASH
+    kernel instead of the Plummer one."""
     import oracle
     ns = len(S["x"])
     rng = np.random.Generator(np.random.PCG64(4242))
-    nprobe = min(len(T["x"]), max(8, int(2e6 / ns)))
+
+    def run(idx, threads):
+        if flash is None:
+            oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][idx], T["y"][idx],
+                                T["z"][idx], None, G, mode=oracle.SEQ, threads=threads)
+        else:
+            oracle.eval_flash(S["x"], S["y"], S["z"], S["m"], T["x"][idx], T["y"][idx], T["z"][idx], G,
+                              flash["rsoft"], quirks=True, threads=threads, box=flash.get("box"),
+                              nrep=flash.get("nrep", 6))
+
+    work = 1.0 if flash is None or flash.get("box") is None else flash["images"]
+    nprobe = min(len(T["x"]), max(8, int(2e6 / (ns * work))))
     probe = np.sort(rng.choice(len(T["x"]), nprobe, replace=False))
     t = time.perf_counter()
-    oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][probe], T["y"][probe],
-                        T["z"][probe], None, G, mode=oracle.SEQ, threads=1)
+    run(probe, 1)
     rate1 = nprobe * ns / (time.perf_counter() - t)
     out = {}
     cores = max(1, min(16, os.cpu_count() or 1))
@@ -218,15 +287,16 @@ def cpu_baseline(S, T, G, budget_s, label):
         n = int(min(len(T["x"]), max(16, budget_s * share * rate1 * (0.8 * threads) / ns)))
         idx = np.sort(rng.choice(len(T["x"]), n, replace=False))
         t = time.perf_counter()
-        oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][idx], T["y"][idx],
-                            T["z"][idx], None, G, mode=oracle.SEQ, threads=threads)
+        run(idx, threads)
         dt = time.perf_counter() - t
         out[threads] = dict(value=n * ns / dt, seconds=dt, targets=n)
         log(f"cpu oracle {threads} thread(s): {n} targets x {ns} sources in {dt:.2f} s "
             f"= {n * ns / dt:.3e} interactions/s")
     cb = {"value": out[cores]["value"], "unit": "interactions/s", "cores": cores, "kind": "port",
           "sample": f"{label}: {out[cores]['targets']} seeded targets x {ns} sources "
-                    f"(acc + pot, FP64, OpenMP over targets); one thread: "
+                    f"(acc + pot, FP64, {'Plummer' if flash is None else 'FLASH spline'}"
+                    f"{'' if flash is None or flash.get('box') is None else ', periodic images'}, "
+                    f"OpenMP over targets); one thread: "
                     f"{out[1]['value']:.3e} interactions/s on {out[1]['targets']} targets",
           "scalar_value": out[1]["value"]}
     return cb
@@ -256,7 +326,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "c2fused"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "c2fused", "c2flash", "c1periodic"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--modes", default="mixed,fp64")
@@ -279,6 +349,56 @@ def main():
             "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32/f64" if head == "mixed" else "f64", "data": f"synthetic (seeded {args.workload})"}
     results = {}
+    if args.workload in ("c2flash", "c1periodic"):
+        periodic = args.workload == "c1periodic"
+        w = W.c1(1001) if periodic else W.c2(1002)
+        st, ce = w["stars"], w["cells"]
+        ns, nc = len(st["x"]), len(ce["x"])
+        rsoft, box, nrep = (0.25, (4.0, 4.0, 4.0), 6) if periodic else (2.5 * 0.03125, None, 6)
+        dev = dict(sx=d(st["x"]), sy=d(st["y"]), sz=d(st["z"]), sm=d(st["m"]), se=d(st["eps"]),
+                   cx=d(ce["x"]), cy=d(ce["y"]), cz=d(ce["z"]), cm=d(ce["m"]), ce=d(ce["eps"]))
+        if not periodic:
+            dev.update(bo=d(ce["block_origin"]), bdx=d(ce["block_size"] / 8))
+        torch.cuda.synchronize()
+        r = run_flash(dev, args.steps, args.warmup, rsoft, box, nrep)
+        inter_step = float(ns) * nc
+        images = mean_images_in_cutoff(ce, st, box, nrep) if periodic else 1.0
+        log(f"{args.workload}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} pair interactions/s), "
+            f"kernel {r['kernel_ms']:.3f} ms, {images:.1f} images per pair")
+        line["value"] = inter_step / (r["ms"] * 1e-3)
+        line["ms_per_step"] = r["ms"]
+        line["dtype"] = "f64"
+        line["config"] = {"workload": (f"c1periodic: 4,096 cells (16^3, 4 pc box) -> 1,024 stars, FLASH spline "
+                                       f"softening (radius {rsoft} pc), periodic images n in [-6, 6]^3 within "
+                                       f"24 pc, acc + pot" if periodic else
+                                       f"c2flash: 1,024,000 cells (2,000 leaf blocks of 8^3) -> 10,000 stars, "
+                                       f"FLASH spline softening (radius {rsoft} pc), isolated, acc + pot"),
+                          "seed": w["seed"], "mode": "fp64", "n_stars": ns, "n_cells": nc,
+                          "interactions_per_step": inter_step,
+                          "image_interactions_per_step": inter_step * images}
+        kern_inter = inter_step * images
+        achieved = FLOP_PER_INTERACTION * kern_inter / (r["kernel_ms"] * 1e-3) / 1e12
+        line["roofline"] = {"bound": "valu", "kernel": f"k_eval_flash<{'1,256,periodic' if periodic else '2,256,isolated'}>",
+                            "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel_ms": r["kernel_ms"],
+                            "flop_per_interaction": FLOP_PER_INTERACTION,
+                            "interactions_per_launch": kern_inter}
+        if not args.no_cpu:
+            import oracle
+            S = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
+            T = dict(x=st["x"], y=st["y"], z=st["z"])
+            rng = np.random.Generator(np.random.PCG64(78))
+            idx = np.sort(rng.choice(ns, 24 if periodic else 64, replace=False))
+            ref = oracle.eval_flash(S["x"], S["y"], S["z"], S["m"], T["x"][idx], T["y"][idx], T["z"][idx], G,
+                                    rsoft, quirks=True, threads=16, box=box, nrep=nrep)
+            ea, ep = rel_err(tuple(a[idx] for a in r["res"]), ref)
+            line["fp64_vs_oracle"] = dict(targets_checked=len(idx), max_rel_acc=float(ea.max()),
+                                          max_rel_pot=float(ep.max()))
+            log("fp64 vs oracle:", line["fp64_vs_oracle"])
+            line["cpu_baseline"] = cpu_baseline(S, T, G, args.cpu_seconds, f"{args.workload} reverse direction",
+                                                flash=dict(rsoft=rsoft, box=box, nrep=nrep, images=images))
+        print(json.dumps(line), flush=True)
+        return
     if args.workload in ("c2", "c5", "c2fused"):
         w = W.c5(1005) if args.workload == "c5" else W.c2(1002)
         st, ce = w["stars"], w["cells"]

@@ -38,6 +38,11 @@ extern "C" {
 #define TGRAV_HOST 0
 #define TGRAV_DEVICE 1
 
+#define TGRAV_SOFT_PLUMMER 0       /* eps2_ij = eps_s^2 + eps_t^2 (the default) */
+#define TGRAV_SOFT_FLASH_SPLINE 1  /* FLASH sink_softening_type_gas = "spline" */
+#define TGRAV_SOFT_FLASH_LINEAR 2  /* FLASH sink_softening_type_gas = "linear" */
+#define TGRAV_FLASH_QUIRKS 1       /* potential exactly as the reference computes it */
+
 #define TGRAV_OK 0
 #define TGRAV_EINVAL -1      /* bad argument: n < 0, NULL required pointer, bad mode */
 #define TGRAV_ENOSOURCES -2  /* evaluation before tgrav_set_sources */
@@ -150,6 +155,27 @@ int tgrav_bidirectional(tgrav_ctx* ctx, int64_t nc, const double* cx, const doub
                         const double* pm, const double* pe, double* cax, double* cay,
                         double* caz, double* cphi, double* pax, double* pay, double* paz,
                         double* pphi, int mode, int ptr_kind);
+/* Pair kernel of later tgrav_eval / tgrav_eval_kick / tgrav_eval_blocks calls.
+ * TGRAV_SOFT_PLUMMER (default): the Plummer kernel above, per-point softening.
+ * TGRAV_SOFT_FLASH_SPLINE / _LINEAR: FLASH's gas-gravity kernels with one global radius
+ * rsoft (sink_softening_radius; Gravity_getAccelAtPoint.F90:88-100,140-145), per-point
+ * softening ignored:  a_i = -G sum_j m_j k(r) (x_i - x_j),  phi_i = -G sum_j m_j p(r),
+ * k = 1/r^3 and p = 1/r for r >= rsoft, the cubic spline (h = rsoft/2) or the constant
+ * 1/rsoft^3 inside (:212-241).  flags = TGRAV_FLASH_QUIRKS takes the potential inside
+ * rsoft as the reference computes it (Gravity_getPotentialAtPoint.F90:197-222: the spline
+ * branch with the opposite sign, `pgpot = slope` in the linear one); without it the
+ * potential is the one whose gradient is the force.  FP64 only (TGRAV_PLAIN_FP64); the
+ * fused pass and batch plans stay Plummer. */
+int tgrav_set_softening(tgrav_ctx* ctx, int kernel, double rsoft, int flags);
+/* Periodic boundaries of the FLASH kernels (grav_boundary_type = "periodic",
+ * Gravity_getAccelAtPoint.F90:101-131, 250-312): every cell also acts through its images
+ * x_c - (nx lx, ny ly, nz lz), n in [-nrep, nrep]^3, that lie closer than
+ * nrep * min(lx, ly, lz) to the point (the reference fixes nrep_pbc = 6, :60).  With
+ * TGRAV_FLASH_QUIRKS the potential of a cell is that of its last image in the reference's
+ * loop order (it assigns, not adds, per image, Gravity_getPotentialAtPoint.F90:262-274).
+ * on = 0 restores isolated boundaries.  nrep in [1, 64]. */
+int tgrav_set_periodic(tgrav_ctx* ctx, int on, double lx, double ly, double lz, int nrep);
+
 /* Block targets with nb == 8 use a column kernel (each lane owns one (i, j) column of 8
  * cells and shares dx, dy, dx^2 + dy^2 + eps^2 between them); on = 0 falls back to the
  * generic kernel (same results to rounding; for tests and comparisons).  Default on. */

@@ -11,7 +11,9 @@ function cites the reference file:line it restates.  ``oracle.brute`` holds a
 Parity status of each function (what pins it is in tests/test_oracle.py):
   eval_plummer    pinned: decimal brute force, closed forms, invariants
   eval_flash      pinned: closed forms (r >= rsoft Newtonian, continuity at rsoft,
-                  r -> 0 limit, a = -grad phi in clean mode)
+                  r -> 0 limit, a = -grad phi in clean mode); periodic branch: equals the
+                  isolated kernel over explicitly replicated, cutoff-filtered images; zero
+                  field at lattice symmetry points
   bidirectional   pinned: equals two eval_plummer calls; Newton's third law
   kick            pinned: exact arithmetic on representable inputs
   block_centres   pinned: numpy meshgrid('ij').ravel('C') of the same centres
@@ -43,7 +45,8 @@ def lib():
         L = ctypes.CDLL(path)
         i64, dbl, i32 = ctypes.c_int64, ctypes.c_double, ctypes.c_int
         L.oracle_eval_plummer.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 4 + [dbl] + [_D] * 4 + [i32, i32]
-        L.oracle_eval_flash.argtypes = [i64] + [_D] * 4 + [i64] + [_D] * 3 + [dbl, dbl, i32, i32] + [_D] * 4 + [i32]
+        L.oracle_eval_flash.argtypes = ([i64] + [_D] * 4 + [i64] + [_D] * 3 + [dbl, dbl, i32, i32, i32, dbl, dbl, dbl]
+                                        + [_D] * 4 + [i32])
         L.oracle_bidirectional.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 5 + [dbl] + [_D] * 8
         L.oracle_kick.argtypes = [i64, dbl] + [_D] * 6
         L.oracle_block_centres.argtypes = [i64, i32] + [_D] * 7
@@ -80,13 +83,18 @@ def eval_plummer(sx, sy, sz, sm, se, tx, ty, tz, te, G, mode=SEQ, threads=1):
     return tuple(o[0] for o in outs)
 
 
-def eval_flash(sx, sy, sz, sm, tx, ty, tz, G, rsoft, kernel="spline", quirks=False, threads=1):
+def eval_flash(sx, sy, sz, sm, tx, ty, tz, G, rsoft, kernel="spline", quirks=False, threads=1,
+               box=None, nrep=6):
+    """(ax, ay, az, phi) with FLASH's softening kernels; box=(Lx, Ly, Lz) selects the periodic
+    image sum over n in [-nrep, nrep]^3 within nrep * min(box)."""
     ns, nt = len(sx), len(tx)
+    L = (0.0, 0.0, 0.0) if box is None else tuple(float(v) for v in box)
+    nr = 0 if box is None else int(nrep)
     keep = [_in(a, ns) for a in (sx, sy, sz, sm)] + [_in(a, nt) for a in (tx, ty, tz)]
     outs = [_out(nt) for _ in range(4)]
     typ = {"spline": 1, "linear": 2}[kernel]
     rc = lib().oracle_eval_flash(ns, *[k[1] for k in keep[:4]], nt, *[k[1] for k in keep[4:]],
-                                 float(G), float(rsoft), typ, int(bool(quirks)),
+                                 float(G), float(rsoft), typ, int(bool(quirks)), nr, *L,
                                  *[o[1] for o in outs], int(threads))
     if rc != 0:
         raise RuntimeError(f"oracle_eval_flash failed ({rc})")

@@ -23,7 +23,8 @@
  *   oracle_eval_flash     -- the FLASH spline / linear softening kernels with a global
  *       softening radius (Gravity_getAccelAtPoint.F90:140-145, 206-241;
  *       Gravity_getPotentialAtPoint.F90:190-222), with or without the reference's
- *       potential quirks (wrong sign inside the radius, `pgpot = slope`).
+ *       potential quirks (wrong sign inside the radius, `pgpot = slope`), isolated or
+ *       with the periodic image sum (Gravity_getAccelAtPoint.F90:250-312).
  *   oracle_bidirectional  -- the fused gas<->particle pass of
  *       src/flash/source/Particles/ParticlesMain/active/Sink/
  *           Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:204-316
@@ -58,7 +59,7 @@ static inline void nadd(nsum* a, double v) {
 
 static inline double sq(double v) { return v * v; }
 
-int oracle_version(void) { return 3; }
+int oracle_version(void) { return 4; }
 
 /* Plummer direct sum.  Any of se/te may be NULL (= 0).  Any of ax/ay/az/phi may be NULL. */
 int oracle_eval_plummer(int64_t ns, const double* sx, const double* sy, const double* sz,
@@ -127,59 +128,108 @@ int oracle_eval_plummer(int64_t ns, const double* sx, const double* sy, const do
   return 0;
 }
 
-/* FLASH softening kernels with the global radius `rsoft` (Gravity_getAccelAtPoint.F90:140-145).
- * type 1 = spline, 2 = linear.  quirks != 0 reproduces the reference potential as written:
- * the spline branch's pot_kernel (with its -q^4/10 term) multiplied by prefactor = -G m
+/* FLASH softening kernel at distance r from a cell (Gravity_getAccelAtPoint.F90:212-241,
+ * Gravity_getPotentialAtPoint.F90:197-222).  type 1 = spline, 2 = linear.  Returns the force
+ * kernel kv (a pair adds prefactor * kv * d, d = x_t - x_c) and the potential kernel pk (a pair
+ * adds prefactor * pk).  quirks != 0 reproduces the reference potential as written: the spline
+ * branch's pot_kernel (with its -q^4/10 term) multiplied by prefactor = -G m
  * (Gravity_getPotentialAtPoint.F90:203-211,225) and `pgpot = slope` in the linear branch
  * (:215-217).  quirks == 0 uses the standard M4-spline potential (Price & Monaghan 2007,
  * eq. A2 with h = rsoft/2) and the potential of the linear force, (r^2 - 3 rsoft^2)/(2 rsoft^3),
  * both continuous with -1/r at r = rsoft.  The reference leaves the spline force kernel
  * unassigned for q <= 1e-5 (a stale value from an earlier pair); both modes use its r -> 0
- * limit there, h^-3 (4/3 - 6/5 q^2 + 1/2 q^3), times d (so a coincident pair adds 0).
- * Sum order and sign follow Gravity_getAccelAtPoint.F90:212-246: d = x_t - x_c,
- * g += prefactor * k * d with prefactor = -G m. */
+ * limit there, h^-3 (4/3 - 6/5 q^2 + 1/2 q^3), times d (so a coincident pair adds 0). */
+static void flash_kernel(double r, double rsoft, double hinv, double h2inv, double slope,
+                         int type, int quirks, double* kvp, double* pkp) {
+  double kv, pk;
+  if (r < rsoft) {
+    if (type == 1) {
+      const double q = r * hinv;
+      if (q < 1.0) {
+        if (q > 1.0e-5) kv = h2inv * (4.0 / 3.0 * q - 1.2 * q * q * q + 0.5 * q * q * q * q) / r;
+        else kv = h2inv * hinv * (4.0 / 3.0 - 1.2 * q * q + 0.5 * q * q * q);
+        if (quirks)
+          pk = hinv * (2.0 / 3.0 * q * q - 0.1 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);
+        else
+          pk = -hinv * (2.0 / 3.0 * q * q - 0.3 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);
+      } else {
+        kv = h2inv * (8.0 / 3.0 * q - 3.0 * q * q + 1.2 * q * q * q - 1.0 / 6.0 * pow(q, 4) -
+                      1.0 / (15.0 * q * q)) / r;
+        pk = hinv * (4.0 / 3.0 * q * q - q * q * q + 0.3 * pow(q, 4) - 1.0 / 30.0 * pow(q, 5) -
+                     8.0 / 5.0 + 1.0 / (15.0 * q));
+        if (!quirks) pk = -pk;
+      }
+    } else {
+      kv = slope;
+      pk = quirks ? slope : -(r * r - 3.0 * rsoft * rsoft) / (2.0 * rsoft * rsoft * rsoft);
+    }
+  } else {
+    kv = 1.0 / (r * r * r);
+    pk = 1.0 / r;
+  }
+  *kvp = kv;
+  *pkp = pk;
+}
+
+/* FLASH direct sum over cells with the global softening radius `rsoft`
+ * (Gravity_getAccelAtPoint.F90:140-145, 168-248; Gravity_getPotentialAtPoint.F90:137-227).
+ * Sum order and sign follow the reference: d = x_t - x_c, g += prefactor * k * d with
+ * prefactor = -G m; radius = sqrt(dx*dx + dy*dy + dz*dz) summed left to right.
+ *
+ * nrep > 0 selects the periodic branch (Gravity_getAccelAtPoint.F90:250-312,
+ * Gravity_getPotentialAtPoint.F90:229-283; the reference fixes nrep_pbc = 6, :60): for every
+ * cell the images d + (nx Lx, ny Ly, nz Lz), n in [-nrep, nrep]^3 (nz fastest), whose radius is
+ * below maxradius_pbc = nrep * min(Lx, Ly, Lz) (:125) are summed with the same kernel,
+ * dx = dx_inside + nx*Lx and radius = sqrt(x2 + y2 + z2) as written there.  The periodic
+ * potential of the reference *assigns* pgpot per image (:262-274), so with quirks != 0 a cell
+ * contributes the kernel of its last image inside the cutoff; with quirks == 0 the images'
+ * kernels are summed. */
 int oracle_eval_flash(int64_t ns, const double* sx, const double* sy, const double* sz,
                       const double* sm, int64_t nt, const double* tx, const double* ty,
-                      const double* tz, double G, double rsoft, int type, int quirks,
+                      const double* tz, double G, double rsoft, int type, int quirks, int nrep,
+                      double Lx, double Ly, double Lz,
                       double* ax, double* ay, double* az, double* phi, int nthreads) {
-  if (ns < 0 || nt < 0 || !(rsoft > 0) || (type != 1 && type != 2)) return -1;
+  if (ns < 0 || nt < 0 || !(rsoft > 0) || (type != 1 && type != 2) || nrep < 0) return -1;
+  if (nrep > 0 && !(Lx > 0 && Ly > 0 && Lz > 0)) return -1;
   if (nthreads < 1) nthreads = 1;
   const double hinv = 2.0 / rsoft, h2inv = hinv * hinv, slope = 1.0 / (rsoft * rsoft * rsoft);
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  double lmin = Lx < Ly ? Lx : Ly;
+  lmin = lmin < Lz ? lmin : Lz;
+  const double rmax = (double)nrep * lmin;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
   for (int64_t i = 0; i < nt; ++i) {
     double gx = 0, gy = 0, gz = 0, gp = 0;
     for (int64_t j = 0; j < ns; ++j) {
       const double pref = -G * sm[j];
-      const double dx = tx[i] - sx[j], dy = ty[i] - sy[j], dz = tz[i] - sz[j];
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
       double kv, pk;
-      if (r < rsoft) {
-        if (type == 1) {
-          const double q = r * hinv;
-          if (q < 1.0) {
-            if (q > 1.0e-5) kv = h2inv * (4.0 / 3.0 * q - 1.2 * q * q * q + 0.5 * q * q * q * q) / r;
-            else kv = h2inv * hinv * (4.0 / 3.0 - 1.2 * q * q + 0.5 * q * q * q);
-            if (quirks)
-              pk = hinv * (2.0 / 3.0 * q * q - 0.1 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);
-            else
-              pk = -hinv * (2.0 / 3.0 * q * q - 0.3 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);
-          } else {
-            kv = h2inv * (8.0 / 3.0 * q - 3.0 * q * q + 1.2 * q * q * q - 1.0 / 6.0 * pow(q, 4) -
-                          1.0 / (15.0 * q * q)) / r;
-            pk = hinv * (4.0 / 3.0 * q * q - q * q * q + 0.3 * pow(q, 4) - 1.0 / 30.0 * pow(q, 5) -
-                         8.0 / 5.0 + 1.0 / (15.0 * q));
-            if (!quirks) pk = -pk;
-          }
-        } else {
-          kv = slope;
-          pk = quirks ? slope : -(r * r - 3.0 * rsoft * rsoft) / (2.0 * rsoft * rsoft * rsoft);
-        }
+      if (nrep == 0) {
+        const double dx = tx[i] - sx[j], dy = ty[i] - sy[j], dz = tz[i] - sz[j];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        flash_kernel(r, rsoft, hinv, h2inv, slope, type, quirks, &kv, &pk);
+        gx += pref * (kv * dx); gy += pref * (kv * dy); gz += pref * (kv * dz);
+        gp += pref * pk;
       } else {
-        kv = 1.0 / (r * r * r);
-        pk = 1.0 / r;
+        const double dxi = tx[i] - sx[j], dyi = ty[i] - sy[j], dzi = tz[i] - sz[j];
+        double px = 0, py = 0, pz = 0, pp = 0;
+        for (int nx = -nrep; nx <= nrep; ++nx) {
+          const double dx = dxi + nx * Lx, x2 = dx * dx;
+          for (int ny = -nrep; ny <= nrep; ++ny) {
+            const double dy = dyi + ny * Ly, y2 = dy * dy;
+            for (int nz = -nrep; nz <= nrep; ++nz) {
+              const double dz = dzi + nz * Lz, z2 = dz * dz;
+              const double r = sqrt(x2 + y2 + z2);
+              if (r < rmax) {
+                flash_kernel(r, rsoft, hinv, h2inv, slope, type, quirks, &kv, &pk);
+                px += kv * dx; py += kv * dy; pz += kv * dz;
+                if (quirks) pp = pk;
+                else pp += pk;
+              }
+            }
+          }
+        }
+        gx += pref * px; gy += pref * py; gz += pref * pz;
+        gp += pref * pp;
       }
-      gx += pref * (kv * dx); gy += pref * (kv * dy); gz += pref * (kv * dz);
-      gp += pref * pk;
     }
     if (ax) ax[i] = gx;
     if (ay) ay[i] = gy;

@@ -287,6 +287,73 @@ def test_flash_spline_matches_closed_form_table(G):
         np.testing.assert_allclose(ax[0], -f, rtol=1e-14)
 
 
+def _images_in_cutoff(d_inside, box, nrep):
+    """Image offsets n*L (n in [-nrep, nrep]^3, nz fastest) of one pair whose radius is below
+    nrep * min(box), and the margin of the closest one to the cutoff."""
+    r = np.arange(-nrep, nrep + 1)
+    n = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    d = d_inside[None, :] + n * np.asarray(box)[None, :]
+    rad = np.sqrt((d ** 2).sum(1))
+    rmax = nrep * min(box)
+    return n[rad < rmax], np.min(np.abs(rad - rmax)) / rmax
+
+
+@pytest.mark.parametrize("kernel", ["spline", "linear"])
+@pytest.mark.parametrize("quirks", [False, True])
+def test_flash_periodic_equals_isolated_over_replicated_images(G, kernel, quirks):
+    """The periodic branch = the (closed-form-pinned) isolated kernel applied to every image
+    of every cell inside the cutoff, built independently with numpy; the quirk potential
+    keeps only the last such image in loop order."""
+    rng = W.rng_for(31)
+    box, nrep, rs = (1.0, 1.3, 0.9), 3, 0.35
+    ns, nt = 5, 4
+    s = rng.uniform(0, 1, (ns, 3)) * box
+    t = rng.uniform(0, 1, (nt, 3)) * box
+    t[0] = s[0] + np.array([0.05, -0.02, 0.03])          # one pair inside the softening radius
+    m = rng.uniform(0.5, 2.0, ns)
+    got = oracle.eval_flash(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], G, rs,
+                            kernel=kernel, quirks=quirks, box=box, nrep=nrep)
+    for i in range(nt):
+        img_x, img_m, last_x, last_m = [], [], [], []
+        for j in range(ns):
+            n, margin = _images_in_cutoff(t[i] - s[j], box, nrep)
+            assert margin > 1e-9
+            # an image at offset n*L is the cell moved to x_s - n*L
+            pos = s[j][None, :] - n * np.asarray(box)[None, :]
+            img_x.append(pos)
+            img_m.append(np.full(len(n), m[j]))
+            last_x.append(pos[-1:])
+            last_m.append(m[j:j + 1])
+        X, M = np.concatenate(img_x), np.concatenate(img_m)
+        ref = oracle.eval_flash(X[:, 0], X[:, 1], X[:, 2], M, t[i:i + 1, 0], t[i:i + 1, 1], t[i:i + 1, 2],
+                                G, rs, kernel=kernel, quirks=quirks)
+        for c in range(3):
+            np.testing.assert_allclose(got[c][i], ref[c][0], rtol=1e-11, atol=1e-14 * abs(ref[c][0]) + 1e-300)
+        if quirks:
+            LX, LM = np.concatenate(last_x), np.concatenate(last_m)
+            ref_p = oracle.eval_flash(LX[:, 0], LX[:, 1], LX[:, 2], LM, t[i:i + 1, 0], t[i:i + 1, 1],
+                                      t[i:i + 1, 2], G, rs, kernel=kernel, quirks=True)[3]
+            np.testing.assert_allclose(got[3][i], ref_p[0], rtol=1e-12)
+        else:
+            np.testing.assert_allclose(got[3][i], ref[3][0], rtol=1e-11)
+
+
+def test_flash_periodic_symmetry_points(G):
+    """Half a box length from a lone cell the image lattice (cut by a sphere about the target)
+    is mirror-symmetric: the field vanishes; a huge box reduces to the isolated field."""
+    ax, ay, az, _ = oracle.eval_flash(np.zeros(1), np.zeros(1), np.zeros(1), np.ones(1),
+                                      np.array([0.5, 0.5]), np.array([0.0, 0.5]), np.array([0.0, 0.5]),
+                                      G, 0.1, box=(1.0, 1.0, 1.0), nrep=4)
+    assert np.max(np.abs([ax, ay, az])) < 1e-13 * G
+    s = np.array([0.2]), np.array([0.1]), np.array([-0.3])
+    t = np.array([0.25, 1.0]), np.array([0.1, -0.4]), np.array([-0.3, 0.7])
+    iso = oracle.eval_flash(*s, np.array([3.0]), *t, G, 0.1)
+    per = oracle.eval_flash(*s, np.array([3.0]), *t, G, 0.1, box=(1e7, 1e7, 1e7), nrep=2)
+    scale = max(np.max(np.abs(iso[c])) for c in range(3))
+    for c in range(3):
+        np.testing.assert_allclose(per[c], iso[c], rtol=1e-9, atol=1e-12 * scale)
+
+
 def test_bidirectional_equals_two_directed_sums_and_third_law(G):
     rng = W.rng_for(18)
     cells = W.gas_cells(rng, box=4.0, base_n=2, n_leaves=2, sigma=1.0, ngrid=16, rho_mean=10.0)

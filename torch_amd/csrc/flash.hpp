@@ -1,0 +1,202 @@
+// FLASH's own gas-gravity kernels, FP64: the spline / linear softening inside a global
+// radius (Gravity_getAccelAtPoint.F90:140-145, 206-241; Gravity_getPotentialAtPoint.F90:
+// 190-222) and the periodic image sum (Gravity_getAccelAtPoint.F90:250-312,
+// Gravity_getPotentialAtPoint.F90:229-283).
+//
+// Execution model: as the Plummer FP64 kernel (k_eval_fp64) -- K targets per lane in VGPRs,
+// sources {x, y, z, m} staged through LDS NT at a time and read as broadcasts, source
+// chunks over blockIdx.y with fixed-order chunk reduction.  Two things differ:
+//   * branch decisions follow the reference bit for bit: radius = sqrt((dx*dx + dy*dy) +
+//     dz*dz) with every product and sum rounded separately, and the correctly rounded
+//     sqrt is taken only where r2 lies within 1e-14 of a threshold (rsoft^2 or the image
+//     cutoff^2); elsewhere r2 decides alone.  The kernels are discontinuous there (the
+//     image cutoff; the reference's potential inside rsoft), so a decision that differed
+//     from the reference's would change a result by a whole term;
+//   * Newtonian pairs (r >= rsoft, nearly all of them) use the refined 1/sqrt of the
+//     Plummer kernel: kv = y^3, pk = y; the softened branch recomputes both from r.
+#pragma once
+#include "kernels.hpp"
+
+namespace tg {
+
+struct Flash {
+  double rsoft, hinv, h2inv, slope;
+  double soft_hi2;                // r2 >= soft_hi2 guarantees fl(sqrt(r2)) >= rsoft
+  double lx, ly, lz, ilx, ily, ilz;  // periodic box and reciprocals
+  double rmax, cut_lo2, cut_hi2;  // image cutoff nrep*min(L): r2 < cut_lo2 in, >= cut_hi2 out
+  int type;                       // 1 spline, 2 linear
+  int quirks;                     // potential as the reference writes it
+  int nrep;                       // periodic: images n in [-nrep, nrep]^3
+};
+
+// Kernel values for r < rsoft.  Force: the reference's spline pieces and linear slope;
+// q <= 1e-5, where the reference leaves the force kernel unassigned, takes the r -> 0 limit
+// h^-3 (4/3 - 6/5 q^2 + 1/2 q^3).  Potential: with quirks the reference's expressions
+// (spline pot_kernel with its -q^4/10 term, `pgpot = slope`), otherwise the M4-spline
+// potential and that of the linear force, both continuous with 1/r at rsoft.
+__device__ __forceinline__ void flash_soft(const Flash& P, double r, double& kv, double& pk) {
+  if (P.type == 1) {
+    const double q = r * P.hinv, q2 = q * q, q3 = q2 * q, q4 = q2 * q2, q5 = q4 * q;
+    if (q < 1.0) {
+      kv = q > 1.0e-5 ? P.h2inv * (4.0 / 3.0 * q - 1.2 * q3 + 0.5 * q4) / r
+                      : P.h2inv * P.hinv * (4.0 / 3.0 - 1.2 * q2 + 0.5 * q3);
+      pk = P.quirks ? P.hinv * (2.0 / 3.0 * q2 - 0.1 * q4 + 0.1 * q5 - 1.4)
+                    : -P.hinv * (2.0 / 3.0 * q2 - 0.3 * q4 + 0.1 * q5 - 1.4);
+    } else {
+      kv = P.h2inv * (8.0 / 3.0 * q - 3.0 * q2 + 1.2 * q3 - q4 / 6.0 - 1.0 / (15.0 * q2)) / r;
+      const double p =
+          P.hinv * (4.0 / 3.0 * q2 - q3 + 0.3 * q4 - q5 / 30.0 - 1.6 + 1.0 / (15.0 * q));
+      pk = P.quirks ? p : -p;
+    }
+  } else {
+    kv = P.slope;
+    pk = P.quirks ? P.slope : (3.0 * P.rsoft * P.rsoft - r * r) * (0.5 * P.slope);
+  }
+}
+
+// Force kernel kv (the pair adds -m kv d, d = x_t - x_s) and potential kernel pk of one pair.
+template <bool S32>
+__device__ __forceinline__ void flash_pair(const Flash& P, double r2, double& kv, double& pk) {
+  const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(r2);
+  kv = (y * y) * y;
+  pk = y;
+  if (r2 < P.soft_hi2) {   // possibly inside the softening radius (divergent, rare)
+    const double r = sqrt(r2);
+    if (r < P.rsoft) flash_soft(P, r, kv, pk);
+  }
+}
+
+// Range [lo, hi] of image indices n in [-N, N] that can satisfy (d + n L)^2 < rem.  A
+// superset: the exact test is applied per image, so the bounds only need to miss nothing
+// (float sqrt: relative error 2^-23; 1e-4 of slack in units of n covers it for N <= 64).
+__device__ __forceinline__ void image_range(double d, double iL, double rem, int N, int& lo,
+                                            int& hi) {
+  if (!(rem > 0.0)) {
+    lo = 1;
+    hi = 0;
+    return;
+  }
+  const double s = (double)sqrtf((float)rem);
+  double a = (-s - d) * iL - 1e-4, b = (s - d) * iL + 1e-4;
+  a = fmax(a, -(double)N - 1.0);   // also keeps NaN / inf out of the int conversion
+  b = fmin(b, (double)N + 1.0);
+  lo = max(-N, (int)floor(a));
+  hi = min(N, (int)ceil(b));
+}
+
+// One (target, cell) pair of the periodic branch: the images of the cell inside the cutoff,
+// in the reference's loop order (nx, ny, nz ascending, nz fastest).  Returns the image sums
+// of kv * d (d = x_t - x_c + n L) and of pk -- or, with quirks, the pk of the last image
+// inside the cutoff, as the reference assigns pgpot per image (Gravity_getPotentialAtPoint.F90:
+// 262-274).
+template <bool S32>
+__device__ __forceinline__ void flash_images(const Flash& P, double dxi, double dyi, double dzi,
+                                             double& px, double& py, double& pz, double& pp) {
+  px = py = pz = pp = 0.0;
+  const int N = P.nrep;
+  int x0, x1;
+  image_range(dxi, P.ilx, P.cut_hi2, N, x0, x1);
+  for (int nx = x0; nx <= x1; ++nx) {
+    double dx, x2;
+    {
+#pragma clang fp contract(off)
+      dx = dxi + (double)nx * P.lx;
+      x2 = dx * dx;
+    }
+    int y0, y1;
+    image_range(dyi, P.ily, P.cut_hi2 - x2, N, y0, y1);
+    for (int ny = y0; ny <= y1; ++ny) {
+      double dy, xy;
+      {
+#pragma clang fp contract(off)
+        dy = dyi + (double)ny * P.ly;
+        xy = x2 + dy * dy;
+      }
+      int z0, z1;
+      image_range(dzi, P.ilz, P.cut_hi2 - xy, N, z0, z1);
+      for (int nz = z0; nz <= z1; ++nz) {
+        double dz, r2;
+        {
+#pragma clang fp contract(off)
+          dz = dzi + (double)nz * P.lz;
+          r2 = xy + dz * dz;
+        }
+        bool in = r2 < P.cut_lo2;
+        if (!in && r2 < P.cut_hi2) in = sqrt(r2) < P.rmax;
+        if (in) {
+          double kv, pk;
+          flash_pair<S32>(P, r2, kv, pk);
+          px = fma(kv, dx, px);
+          py = fma(kv, dy, py);
+          pz = fma(kv, dz, pz);
+          pp = P.quirks ? pk : pp + pk;
+        }
+      }
+    }
+  }
+}
+
+// blockIdx.x = target tile of K*NT targets, blockIdx.y = source chunk.  Accumulators follow
+// store_result's convention (a = G s, phi = -G s_p): s -= m kv d, s_p += m pk.
+template <int K, int NT, bool PER, bool S32>
+__global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restrict__ src4,
+                                                   int64_t ns, int64_t chunk, Flash P, Out O) {
+  __shared__ double4 ls[NT];
+  const int tid = threadIdx.x;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  double tx[K], ty[K], tz[K], ax[K], ay[K], az[K], ap[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    double e2;   // the FLASH kernels use the global radius; per-point softening is ignored
+    load_target(T, t < T.n ? t : T.n - 1, tx[k], ty[k], tz[k], e2);
+    ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+  }
+  for (int64_t base = sb; base < se; base += NT) {
+    __syncthreads();
+    if (base + tid < se) ls[tid] = src4[base + tid];
+    __syncthreads();
+    const int cnt = (int)(se - base < NT ? se - base : NT);
+    for (int j = 0; j < cnt; ++j) {
+      const double4 s = ls[j];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (PER) {
+          double px, py, pz, pp;
+          flash_images<S32>(P, tx[k] - s.x, ty[k] - s.y, tz[k] - s.z, px, py, pz, pp);
+          ax[k] = fma(-s.w, px, ax[k]);
+          ay[k] = fma(-s.w, py, ay[k]);
+          az[k] = fma(-s.w, pz, az[k]);
+          ap[k] = fma(s.w, pp, ap[k]);
+        } else {
+          double dx, dy, dz, r2;
+          {
+#pragma clang fp contract(off)
+            dx = tx[k] - s.x;
+            dy = ty[k] - s.y;
+            dz = tz[k] - s.z;
+            r2 = (dx * dx + dy * dy) + dz * dz;
+          }
+          double kv, pk;
+          flash_pair<S32>(P, r2, kv, pk);
+          const double w = -s.w * kv;
+          ax[k] = fma(w, dx, ax[k]);
+          ay[k] = fma(w, dy, ay[k]);
+          az[k] = fma(w, dz, az[k]);
+          ap[k] = fma(s.w, pk, ap[k]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    if (t < T.n) store_result(o, t, ax[k], ay[k], az[k], ap[k]);
+  }
+}
+
+}  // namespace tg

@@ -20,6 +20,7 @@
 #include "batch.hpp"
 #include "bidir.hpp"
 #include "grid.hpp"
+#include "flash.hpp"
 
 #define TGRAV_VERSION 1
 
@@ -166,6 +167,14 @@ struct tgrav_ctx {
   int block_kernel = 1;      // 8^3 block targets use the column kernel (k_eval_grid)
   int bvariant[2] = {1, 1};  // fused bidirectional kernel: variant and particle chunks
   int bnsplit[2] = {0, 0};
+  // softening kernel: TGRAV_SOFT_* (FLASH kernels use the global radius rsoft), and the
+  // periodic image sum of the FLASH kernels
+  int soft = TGRAV_SOFT_PLUMMER;
+  double rsoft = 0.0;
+  int soft_quirks = 0;
+  int periodic = 0;
+  double box[3] = {0.0, 0.0, 0.0};
+  int nrep = 6;
   double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
   int batch_wg_per_cu = 0;   // persistent workgroups per CU (0 = occupancy limit)
   // packed sources; stat[0] = bits of min eps_s^2 (read by the kernels), stat[1] = NaN flag
@@ -395,6 +404,80 @@ static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
   return 0;
 }
 
+// FLASH softening kernels (FP64 only).  Isolated: K = 2 targets per lane; periodic: K = 1
+// (hundreds of images per pair), sources split until there are about 8 workgroups per CU.
+static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
+  if (mode != TGRAV_PLAIN_FP64)
+    return fail(TGRAV_EINVAL, "the FLASH softening kernels run in TGRAV_PLAIN_FP64 only");
+  const int64_t nt = T.n, ns = c->ns;
+  const bool per = c->periodic != 0;
+  const int NT = 256, K = per ? 1 : 2;
+  tg::Flash P;
+  P.rsoft = c->rsoft;
+  P.hinv = 2.0 / c->rsoft;
+  P.h2inv = P.hinv * P.hinv;
+  P.slope = 1.0 / (c->rsoft * c->rsoft * c->rsoft);
+  P.soft_hi2 = c->rsoft * c->rsoft * (1.0 + 1e-14);
+  P.lx = c->box[0];
+  P.ly = c->box[1];
+  P.lz = c->box[2];
+  P.ilx = per ? 1.0 / P.lx : 0.0;
+  P.ily = per ? 1.0 / P.ly : 0.0;
+  P.ilz = per ? 1.0 / P.lz : 0.0;
+  P.rmax = per ? (double)c->nrep * std::min(P.lx, std::min(P.ly, P.lz)) : 0.0;
+  P.cut_lo2 = P.rmax * P.rmax * (1.0 - 1e-14);
+  P.cut_hi2 = P.rmax * P.rmax * (1.0 + 1e-14);
+  P.type = c->soft;
+  P.quirks = c->soft_quirks;
+  P.nrep = per ? c->nrep : 0;
+  // Pairs reaching the refined 1/sqrt have r2 >= rsoft^2 (the rest take the softened
+  // branch); the FP32 seed is valid from 1e-30 up.
+  const bool s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
+  const int64_t ntiles = (nt + (int64_t)K * NT - 1) / ((int64_t)K * NT);
+  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
+  const int64_t target_wg = (per ? 8LL : 4LL) * c->num_cu;
+  int64_t want = 1;
+  if (ntiles < target_wg / 2) {
+    want = (target_wg + ntiles - 1) / ntiles;
+    want = std::min<int64_t>(want, per ? ns : (ns + 63) / 64);
+    want = std::min<int64_t>(want, (int64_t)(((size_t)1 << 31) / ((size_t)32 * nt)));
+    want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
+  }
+  const int64_t chunk = (ns + want - 1) / want;
+  const int nsplit = (int)((ns + chunk - 1) / chunk);
+  tg::Out o = O;
+  if (nsplit > 1) {
+    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+    if (rc0) return rc0;
+    o.part = c->part.as<double>();
+    o.pstride = nt;
+  } else {
+    o.part = nullptr;
+  }
+  int rc = timed_launch(c, nt, ns, [&]() -> int {
+    const dim3 g((unsigned)ntiles, (unsigned)nsplit);
+    const double4* s4 = c->s4.as<double4>();
+    if (per) {
+      if (s32) tg::k_eval_flash<1, 256, true, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      else tg::k_eval_flash<1, 256, true, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+    } else {
+      if (s32) tg::k_eval_flash<2, 256, false, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      else tg::k_eval_flash<2, 256, false, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+    }
+    LAUNCHCHECK("k_eval_flash");
+    return 0;
+  });
+  if (rc) return rc;
+  if (nsplit > 1) {
+    tg::Out fin = O;
+    fin.part = nullptr;
+    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
+                                                                nt, fin);
+    LAUNCHCHECK("k_reduce_chunks");
+  }
+  return 0;
+}
+
 // Core launch: targets T (device), outputs O (device), on c->stream.
 static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
   const int64_t nt = T.n, ns = c->ns;
@@ -405,11 +488,16 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   if (O.vx && (!O.vy || !O.vz)) return fail(TGRAV_EINVAL, "internal: partial velocity set");
   if (ns > 0 && (!c->s4.p || !c->se2.p || !c->stat.p))
     return fail(TGRAV_EINVAL, "internal: source buffers missing");
+  if (c->soft == TGRAV_SOFT_PLUMMER && c->periodic)
+    return fail(TGRAV_EINVAL, "the periodic image sum needs a FLASH softening kernel (tgrav_set_softening)");
+  if (c->soft != TGRAV_SOFT_PLUMMER && mode != TGRAV_PLAIN_FP64)
+    return fail(TGRAV_EINVAL, "the FLASH softening kernels run in TGRAV_PLAIN_FP64 only");
   if (ns == 0) {   // no sources: zero field, kick adds nothing
     tg::k_fill<<<grid1(nt, 256), 256, 0, c->stream>>>(nt, 0.0, O.ax, O.ay, O.az, O.phi);
     LAUNCHCHECK("k_fill");
     return 0;
   }
+  if (c->soft != TGRAV_SOFT_PLUMMER) return launch_flash(c, mode, T, O);
   if (T.blocks && T.nb == 8 && c->block_kernel) return launch_grid(c, mode, T, O);
   if (!T.blocks && c->src_nblk > 0 && c->block_kernel) return launch_gridsrc(c, mode, T, O);
   const Variant* v = variant_of(mode, c->variant[mode]);
@@ -846,6 +934,8 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
   if (!c) return fail(TGRAV_EINVAL, "context is NULL");
   int rc;
   if ((rc = check_mode(mode)) || (rc = check_kind(kind))) return rc;
+  if (c->soft != TGRAV_SOFT_PLUMMER || c->periodic)
+    return fail(TGRAV_EINVAL, "the fused pass uses the Plummer kernel (tgrav_set_softening(ctx, 0, 0, 0))");
   if (nc < 0 || np < 0) return fail(TGRAV_EINVAL, "sizes must be >= 0");
   if (nc > 0 && (!cx || !cy || !cz || !cm)) return fail(TGRAV_EINVAL, "cell x, y, z and m are required");
   if (np > 0 && (!px || !py || !pz || !pm))
@@ -955,6 +1045,37 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     }
     HIPCHECK(hipStreamSynchronize(c->stream));
   }
+  return 0;
+}
+
+int tgrav_set_softening(tgrav_ctx* c, int kernel, double rsoft, int flags) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  if (kernel != TGRAV_SOFT_PLUMMER && kernel != TGRAV_SOFT_FLASH_SPLINE &&
+      kernel != TGRAV_SOFT_FLASH_LINEAR)
+    return fail(TGRAV_EINVAL, "unknown softening kernel %d", kernel);
+  if (flags & ~TGRAV_FLASH_QUIRKS) return fail(TGRAV_EINVAL, "unknown softening flags 0x%x", flags);
+  if (kernel != TGRAV_SOFT_PLUMMER && !(rsoft > 0.0 && rsoft < 1e150))
+    return fail(TGRAV_EINVAL, "the softening radius must be positive and finite");
+  c->soft = kernel;
+  c->rsoft = kernel == TGRAV_SOFT_PLUMMER ? 0.0 : rsoft;
+  c->soft_quirks = (flags & TGRAV_FLASH_QUIRKS) ? 1 : 0;
+  return 0;
+}
+
+int tgrav_set_periodic(tgrav_ctx* c, int on, double lx, double ly, double lz, int nrep) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  if (!on) {
+    c->periodic = 0;
+    return 0;
+  }
+  if (!(lx > 0.0 && ly > 0.0 && lz > 0.0 && lx < 1e150 && ly < 1e150 && lz < 1e150))
+    return fail(TGRAV_EINVAL, "box lengths must be positive and finite");
+  if (nrep < 1 || nrep > 64) return fail(TGRAV_EINVAL, "nrep must be in [1, 64], got %d", nrep);
+  c->periodic = 1;
+  c->box[0] = lx;
+  c->box[1] = ly;
+  c->box[2] = lz;
+  c->nrep = nrep;
   return 0;
 }
 
@@ -1071,6 +1192,8 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   if (!c || !out) return fail(TGRAV_EINVAL, "context and out are required");
   int rc = check_mode(mode);
   if (rc) return rc;
+  if (c->soft != TGRAV_SOFT_PLUMMER || c->periodic)
+    return fail(TGRAV_EINVAL, "batch plans use the Plummer kernel (tgrav_set_softening(ctx, 0, 0, 0))");
   if (n < 0 || (n > 0 && !pr)) return fail(TGRAV_EINVAL, "bad problem list");
   if (!isfinite(dt)) return fail(TGRAV_EINVAL, "dt must be finite");
   const bool kick = dt != 0.0;

@@ -21,7 +21,9 @@ tensors are returned).  Scalars are broadcast.  A non-zero library return code r
 
 Softening: eps2_ij = eps_source_j^2 + eps_i^2 where eps_i is the per-point ``eps``
 argument (in the reference's FLASH exports ``eps`` is accepted but unused and a global
-radius applies, Gravity_getAccelAtPoint.F90:89-145; here it is honoured).
+radius applies, Gravity_getAccelAtPoint.F90:89-145; here it is honoured).  FLASH's own
+spline / linear kernels with that global radius, and its periodic image sum, are
+available too (:meth:`GravityField.set_softening`, :meth:`GravityField.set_periodic`).
 Units are those of the inputs; the default G = 4.300917270e-3 pc (km/s)^2 / Msun.
 """
 import ctypes
@@ -271,6 +273,27 @@ class GravityField:
                                           *[B.ptr(o) for o in po], self.mode, A.kind)
         _lib.check(rc, "tgrav_bidirectional")
         return tuple(co), tuple(po)
+
+    SOFTENING = {"plummer": 0, "spline": 1, "linear": 2}
+
+    def set_softening(self, kernel="plummer", rsoft=0.0, quirks=False):
+        """Pair kernel of later evaluations: "plummer" (default; per-point softening) or
+        FLASH's "spline" / "linear" kernels with the global radius `rsoft`
+        (sink_softening_radius, Gravity_getAccelAtPoint.F90:88-100,140-145), FP64 mode only.
+        quirks=True reproduces the reference's potential inside rsoft as written."""
+        if kernel not in self.SOFTENING:
+            raise ValueError(f"kernel must be one of {sorted(self.SOFTENING)}")
+        _lib.check(self.lib.tgrav_set_softening(self._ctx, self.SOFTENING[kernel], float(rsoft),
+                                                int(bool(quirks))), "tgrav_set_softening")
+
+    def set_periodic(self, box=None, nrep=6):
+        """Periodic image sum of the FLASH kernels over n in [-nrep, nrep]^3 within
+        nrep * min(box) (Gravity_getAccelAtPoint.F90:250-312); box=None: isolated."""
+        if box is None:
+            _lib.check(self.lib.tgrav_set_periodic(self._ctx, 0, 0.0, 0.0, 0.0, 0), "tgrav_set_periodic")
+            return
+        lx, ly, lz = (float(b) for b in box)
+        _lib.check(self.lib.tgrav_set_periodic(self._ctx, 1, lx, ly, lz, int(nrep)), "tgrav_set_periodic")
 
     def set_block_kernel(self, on=True):
         """Use (default) or bypass the 8^3 column kernel for block-structured targets."""
