@@ -184,6 +184,43 @@ int tgrav_set_block_kernel(tgrav_ctx* ctx, int on);
 int tgrav_set_bidir_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
 int tgrav_bidir_variant_info(int mode, int variant, int* targets_per_lane, int* threads);
 
+/* ---- particle-mesh pieces beside the kick (FLASH leaf blocks) ----------- */
+/* Block arrays hold one variable of nblk blocks of nbg^3 cells, nbg = nb + 2 ng (nb interior
+ * cells per axis, ng >= 1 guard layers); block b starts at b * nbg^3 and cell (i, j, k) is
+ * at (k * nbg + j) * nbg + i, x fastest -- solnData(VAR, i, j, k) of one variable.  Block b
+ * has interior lower corner (ox, oy, oz)[b] and physical size bsize[b] (Grid_getBlkPhysicalSize,
+ * cubic); cells are bsize / nb wide and guarded cell i has centre o + (i - ng + 1/2) bsize/nb.
+ * blk[p] is particle p's block (BLK_PART_PROP, 0-based).  A particle whose cloud leaves its
+ * guarded block is an error (TGRAV_EINVAL) with validation on, and is skipped (deposit) or
+ * gets NaN (interpolation) with it off. */
+
+/* Cloud-in-cell mass deposit as density, rho = sum m w / (bsize/nb)^3 over the particles
+ * (rho += ... with accumulate != 0): the mapping of Particles_updateGridVar(MASS_PART_PROP,
+ * PDEN_VAR) through FLASH's Grid_mapParticlesToMesh (Gravity_potentialListOfBlocksPDENonly.F90:
+ * 260-266).  Guard-cell contributions stay in the guard cells.  Deterministic: the sums are
+ * formed in exact fixed point, independent of the order of the particles. */
+int tgrav_pm_deposit(tgrav_ctx* ctx, int64_t np, const double* x, const double* y,
+                     const double* z, const double* m, const int64_t* blk, int64_t nblk, int nb,
+                     int ng, const double* ox, const double* oy, const double* oz,
+                     const double* bsize, double* rho, int accumulate, int ptr_kind);
+/* Finite-difference acceleration g = -grad(phi) on every cell of every guarded block, the
+ * row rule of Gravity_accelOneRow (Simulation/SimulationMain/Cube/Gravity_accelOneRow.F90:
+ * 94-121) as FLASH's Gravity_accelListOfBlocks applies it (interface.F90:1767-1769):
+ * g(i) = 0.5 (nb / bsize) (phi(i-1) - phi(i+1)), the two end cells of each row copying their
+ * neighbour.  gx/gy/gz may be NULL.  vx/vy/vz != NULL also apply the grid kick
+ * v += g dt of interface.F90:1777-1830 in the same pass. */
+int tgrav_pm_gradient(tgrav_ctx* ctx, int64_t nblk, int nb, int ng, const double* bsize,
+                      const double* phi, double* gx, double* gy, double* gz, double dt,
+                      double* vx, double* vy, double* vz, int ptr_kind);
+/* Cloud-in-cell interpolation of a block vector field to the particles, a_p = sum w g: the
+ * Grid_mapMeshToParticles step of Particles_longRangeForce (Particles_longRangeForce.F90:
+ * 70-97). */
+int tgrav_pm_interpolate(tgrav_ctx* ctx, int64_t np, const double* x, const double* y,
+                         const double* z, const int64_t* blk, int64_t nblk, int nb, int ng,
+                         const double* ox, const double* oy, const double* oz,
+                         const double* bsize, const double* gx, const double* gy,
+                         const double* gz, double* ax, double* ay, double* az, int ptr_kind);
+
 /* ---- batched uneven evaluations --------------------------------------- */
 /* Many independent problems in one persistent launch (device pointers only).  The plan
  * packs the sources, builds a cost-sorted work list (target tile x source chunk) and

@@ -17,6 +17,10 @@ Parity status of each function (what pins it is in tests/test_oracle.py):
   bidirectional   pinned: equals two eval_plummer calls; Newton's third law
   kick            pinned: exact arithmetic on representable inputs
   block_centres   pinned: numpy meshgrid('ij').ravel('C') of the same centres
+  pm_deposit      pinned: closed forms (a particle on a cell centre, split weights), mass
+                  conservation, linearity; adjoint identity with pm_interpolate
+  pm_gradient     pinned: exact on quadratic potentials (central differences), end rule
+  pm_interpolate  pinned: exact on linear fields, value at cell centres, adjoint identity
 """
 import ctypes
 import os
@@ -50,8 +54,13 @@ def lib():
         L.oracle_bidirectional.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 5 + [dbl] + [_D] * 8
         L.oracle_kick.argtypes = [i64, dbl] + [_D] * 6
         L.oracle_block_centres.argtypes = [i64, i32] + [_D] * 7
+        _I = ctypes.POINTER(ctypes.c_int64)
+        L.oracle_pm_deposit.argtypes = [i64] + [_D] * 4 + [_I, i64, i32, i32] + [_D] * 5
+        L.oracle_pm_gradient.argtypes = [i64, i32, i32] + [_D] * 5
+        L.oracle_pm_interpolate.argtypes = [i64] + [_D] * 3 + [_I, i64, i32, i32] + [_D] * 10
         for f in (L.oracle_eval_plummer, L.oracle_eval_flash, L.oracle_bidirectional,
-                  L.oracle_kick, L.oracle_block_centres):
+                  L.oracle_kick, L.oracle_block_centres, L.oracle_pm_deposit,
+                  L.oracle_pm_gradient, L.oracle_pm_interpolate):
             f.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -132,4 +141,60 @@ def block_centres(ox, oy, oz, dx, nb=8):
     rc = lib().oracle_block_centres(nblk, nb, *[k[1] for k in keep], *[o[1] for o in outs])
     if rc != 0:
         raise RuntimeError("oracle_block_centres failed")
+    return tuple(o[0] for o in outs)
+
+
+def _blocks(blocks):
+    """blocks = dict(origin=(nblk, 3), size=(nblk,), nb=8, ng=...)."""
+    o = np.ascontiguousarray(np.asarray(blocks["origin"], dtype=np.float64).reshape(-1, 3))
+    bs = np.ascontiguousarray(np.asarray(blocks["size"], dtype=np.float64).reshape(-1))
+    ox, oy, oz = (np.ascontiguousarray(o[:, a]) for a in range(3))
+    return len(bs), int(blocks.get("nb", 8)), int(blocks["ng"]), ox, oy, oz, bs
+
+
+def _i64(a):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def pm_deposit(x, y, z, m, blk, blocks, rho=None):
+    """CIC density of the particles on guarded blocks (added to rho if given)."""
+    nblk, nb, ng, ox, oy, oz, bs = _blocks(blocks)
+    nbg = nb + 2 * ng
+    n = len(x)
+    out = np.zeros(nblk * nbg ** 3) if rho is None else np.array(rho, dtype=np.float64, copy=True)
+    keep = [_in(a, n) for a in (x, y, z, m)]
+    kb = _i64(blk)
+    g = [_in(a, nblk) for a in (ox, oy, oz, bs)]
+    rc = lib().oracle_pm_deposit(n, *[k[1] for k in keep], kb[1], nblk, nb, ng, *[q[1] for q in g],
+                                 out.ctypes.data_as(_D))
+    if rc != 0:
+        raise RuntimeError(f"oracle_pm_deposit failed ({rc})")
+    return out
+
+
+def pm_gradient(phi, blocks):
+    nblk, nb, ng, ox, oy, oz, bs = _blocks(blocks)
+    nbg = nb + 2 * ng
+    f = _in(phi, nblk * nbg ** 3)
+    outs = [_out(nblk * nbg ** 3) for _ in range(3)]
+    rc = lib().oracle_pm_gradient(nblk, nb, ng, _in(bs, nblk)[1], f[1], *[o[1] for o in outs])
+    if rc != 0:
+        raise RuntimeError(f"oracle_pm_gradient failed ({rc})")
+    return tuple(o[0] for o in outs)
+
+
+def pm_interpolate(x, y, z, blk, blocks, gx, gy, gz):
+    nblk, nb, ng, ox, oy, oz, bs = _blocks(blocks)
+    nbg = nb + 2 * ng
+    n = len(x)
+    keep = [_in(a, n) for a in (x, y, z)]
+    kb = _i64(blk)
+    g = [_in(a, nblk) for a in (ox, oy, oz, bs)]
+    fields = [_in(a, nblk * nbg ** 3) for a in (gx, gy, gz)]
+    outs = [_out(n) for _ in range(3)]
+    rc = lib().oracle_pm_interpolate(n, *[k[1] for k in keep], kb[1], nblk, nb, ng, *[q[1] for q in g],
+                                     *[f[1] for f in fields], *[o[1] for o in outs])
+    if rc != 0:
+        raise RuntimeError(f"oracle_pm_interpolate failed ({rc})")
     return tuple(o[0] for o in outs)

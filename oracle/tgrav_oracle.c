@@ -34,6 +34,12 @@
  *       src/torch_mainloop.py:287,289 call it with dt/2).
  *   oracle_block_centres  -- implicit cell centres of 8^3 (nb^3) leaf blocks in the
  *       C order of .deprecated/bridge_multiples.py:566-571 (meshgrid 'ij', ravel 'C').
+ *   oracle_pm_deposit / oracle_pm_gradient / oracle_pm_interpolate -- the particle-mesh
+ *       pieces beside the kick on guarded FLASH leaf blocks: CIC mass deposit, the
+ *       finite-difference acceleration row rule (Simulation/SimulationMain/Cube/
+ *       Gravity_accelOneRow.F90:94-121) and CIC interpolation to particles
+ *       (Particles_longRangeForce.F90:70-97).  FLASH's own mappers are not vendored: the
+ *       deposit and interpolation restate the textbook cloud-in-cell scheme.
  *
  * Accumulation modes: ORACLE_SEQ (plain sequential FP64, the reference's order),
  * ORACLE_NEUMAIER (compensated FP64), ORACLE_LONGDOUBLE (x87 80-bit for the
@@ -59,7 +65,7 @@ static inline void nadd(nsum* a, double v) {
 
 static inline double sq(double v) { return v * v; }
 
-int oracle_version(void) { return 4; }
+int oracle_version(void) { return 5; }
 
 /* Plummer direct sum.  Any of se/te may be NULL (= 0).  Any of ax/ay/az/phi may be NULL. */
 int oracle_eval_plummer(int64_t ns, const double* sx, const double* sy, const double* sz,
@@ -309,5 +315,144 @@ int oracle_block_centres(int64_t nblk, int nb, const double* ox, const double* o
           y[t] = oy[b] + (j + 0.5) * dx[b];
           z[t] = oz[b] + (k + 0.5) * dx[b];
         }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * Particle-mesh route on FLASH leaf blocks (the pieces the bridge kick sits beside).
+ *
+ * Block layout: one variable of nblk blocks of nbg^3 cells, nbg = nb + 2 ng (nb interior
+ * cells per axis, ng guard layers), block b at offset b * nbg^3, cell (i, j, k) at
+ * (k * nbg + j) * nbg + i -- i (x) fastest, the order of solnData(VAR, i, j, k) for one VAR.
+ * Block b has interior lower corner (ox, oy, oz)[b] and physical size bs[b] (cubic); the
+ * cell width is del = bs / nb (Grid_getDeltas) and guarded cell i has centre
+ * ox + ((i - ng) + 0.5) del.
+ */
+
+/* Cloud-in-cell weights along one axis: the point's position u in cell-centre units of the
+ * guarded block (u = (x - o) / del - 0.5 + ng), base cell i0 = floor(u), weights (1 - f, f)
+ * on (i0, i0 + 1) with f = u - i0.  Returns 0 if the two cells lie inside [0, nbg). */
+static int cic_axis(double x, double o, double del, int ng, int nbg, int* i0, double* w0,
+                    double* w1) {
+  const double u = (x - o) / del - 0.5 + (double)ng;
+  const double fl = floor(u);
+  if (!(fl >= 0.0 && fl + 1.0 < (double)nbg)) return -1;
+  *i0 = (int)fl;
+  *w1 = u - fl;
+  *w0 = 1.0 - *w1;
+  return 0;
+}
+
+/* CIC deposit of particle masses as density, rho += m w / del^3, onto the guarded block
+ * blk[p] of each particle, particles in index order, corners with dk slowest, di fastest;
+ * w = (wx * wy) * wz.  Restates the mapping that Particles_updateGridVar(MASS_PART_PROP,
+ * PDEN_VAR) performs through FLASH's Grid_mapParticlesToMesh (called at
+ * Gravity_potentialListOfBlocksPDENonly.F90:260-266 and
+ * Particles_updateGridVarMassiveOnly.F90:40-58); that mapper belongs to FLASH 4.6.2, which
+ * is not vendored, so this is the textbook cloud-in-cell assignment (Hockney & Eastwood,
+ * Computer Simulation Using Particles, 1988, sec. 5-2-2).  Guard-cell contributions stay
+ * in the guard cells (folding them into neighbouring blocks is FLASH's Grid communication).
+ * Returns -2 if a particle's cloud leaves its guarded block, -1 on bad arguments. */
+int oracle_pm_deposit(int64_t np, const double* x, const double* y, const double* z,
+                      const double* m, const int64_t* blk, int64_t nblk, int nb, int ng,
+                      const double* ox, const double* oy, const double* oz, const double* bs,
+                      double* rho) {
+  if (np < 0 || nblk < 0 || nb < 1 || ng < 1) return -1;
+  const int nbg = nb + 2 * ng;
+  const int64_t nc = (int64_t)nbg * nbg * nbg;
+  for (int64_t p = 0; p < np; ++p) {
+    const int64_t b = blk[p];
+    if (b < 0 || b >= nblk) return -2;
+    const double del = bs[b] / (double)nb;
+    const double md = m[p] / (del * del * del);
+    int i0, j0, k0;
+    double wx[2], wy[2], wz[2];
+    if (cic_axis(x[p], ox[b], del, ng, nbg, &i0, &wx[0], &wx[1]) ||
+        cic_axis(y[p], oy[b], del, ng, nbg, &j0, &wy[0], &wy[1]) ||
+        cic_axis(z[p], oz[b], del, ng, nbg, &k0, &wz[0], &wz[1]))
+      return -2;
+    for (int dk = 0; dk < 2; ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int64_t idx = b * nc + ((int64_t)(k0 + dk) * nbg + (j0 + dj)) * nbg + (i0 + di);
+          rho[idx] += md * ((wx[di] * wy[dj]) * wz[dk]);
+        }
+  }
+  return 0;
+}
+
+/* Finite-difference acceleration g = -grad(phi) on every row of every guarded block, the
+ * row rule of Gravity_accelOneRow (src/flash/source/Simulation/SimulationMain/Cube/
+ * Gravity_accelOneRow.F90:94-121, FLASH's Gravity_accelListOfBlocks applies it to each row):
+ * delxinv = 0.5 * (nb / bs), g(i) = delxinv * (phi(i-1) - phi(i+1)) for the cells between
+ * the two ends of the row, and each end copies its neighbour ("invalid data - must not be
+ * used").  Any of gx/gy/gz may be NULL. */
+int oracle_pm_gradient(int64_t nblk, int nb, int ng, const double* bs, const double* phi,
+                       double* gx, double* gy, double* gz) {
+  if (nblk < 0 || nb < 1 || ng < 1) return -1;
+  const int n = nb + 2 * ng;
+  const int64_t nc = (int64_t)n * n * n;
+  for (int64_t b = 0; b < nblk; ++b) {
+    double delxinv = (double)nb / bs[b];
+    delxinv = 0.5 * delxinv;
+    const double* f = phi + b * nc;
+    double* g[3] = {gx ? gx + b * nc : NULL, gy ? gy + b * nc : NULL, gz ? gz + b * nc : NULL};
+    const int64_t stride[3] = {1, n, (int64_t)n * n};
+    for (int a = 0; a < 3; ++a) {
+      if (!g[a]) continue;
+      for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < n; ++i) {
+            const int c[3] = {i, j, k};
+            const int64_t idx = ((int64_t)k * n + j) * n + i;
+            int ii = c[a];
+            if (ii == 0) ii = 1;
+            if (ii == n - 1) ii = n - 2;
+            const int64_t ctr = idx + (int64_t)(ii - c[a]) * stride[a];
+            g[a][idx] = delxinv * (f[ctr - stride[a]] - f[ctr + stride[a]]);
+          }
+    }
+  }
+  return 0;
+}
+
+/* CIC interpolation of a block field to particle positions, a_p = sum_corners w g(corner)
+ * with the weights and corner order of oracle_pm_deposit: the mapping Grid_mapMeshToParticles
+ * performs in Particles_longRangeForce (src/flash/source/Particles/ParticlesForces/longRange/
+ * gravity/ParticleMesh/Particles_longRangeForce.F90:70-97) -- FLASH 4.6.2's mapper, not
+ * vendored, restated as the textbook CIC interpolation (the adjoint of the deposit).
+ * Returns -2 if a particle's stencil leaves its guarded block. */
+int oracle_pm_interpolate(int64_t np, const double* x, const double* y, const double* z,
+                          const int64_t* blk, int64_t nblk, int nb, int ng, const double* ox,
+                          const double* oy, const double* oz, const double* bs,
+                          const double* gx, const double* gy, const double* gz, double* ax,
+                          double* ay, double* az) {
+  if (np < 0 || nblk < 0 || nb < 1 || ng < 1) return -1;
+  const int nbg = nb + 2 * ng;
+  const int64_t nc = (int64_t)nbg * nbg * nbg;
+  for (int64_t p = 0; p < np; ++p) {
+    const int64_t b = blk[p];
+    if (b < 0 || b >= nblk) return -2;
+    const double del = bs[b] / (double)nb;
+    int i0, j0, k0;
+    double wx[2], wy[2], wz[2];
+    if (cic_axis(x[p], ox[b], del, ng, nbg, &i0, &wx[0], &wx[1]) ||
+        cic_axis(y[p], oy[b], del, ng, nbg, &j0, &wy[0], &wy[1]) ||
+        cic_axis(z[p], oz[b], del, ng, nbg, &k0, &wz[0], &wz[1]))
+      return -2;
+    double sx = 0, sy = 0, sz = 0;
+    for (int dk = 0; dk < 2; ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int64_t idx = b * nc + ((int64_t)(k0 + dk) * nbg + (j0 + dj)) * nbg + (i0 + di);
+          const double w = (wx[di] * wy[dj]) * wz[dk];
+          sx += w * gx[idx];
+          sy += w * gy[idx];
+          sz += w * gz[idx];
+        }
+    ax[p] = sx;
+    ay[p] = sy;
+    az[p] = sz;
+  }
   return 0;
 }

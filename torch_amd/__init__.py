@@ -12,6 +12,7 @@ There is no CPU fallback: without the built library every call raises.
 """
 from ._lib import TgravError, MIXED, PLAIN_FP64
 from .field import G_PC_MSUN_KMS, GasGravityField, GravityField, StellarGravityField
+from .pm import LeafBlocks, ParticleMesh
 
-__all__ = ["GravityField", "StellarGravityField", "GasGravityField", "TgravError",
-           "G_PC_MSUN_KMS", "MIXED", "PLAIN_FP64"]
+__all__ = ["GravityField", "StellarGravityField", "GasGravityField", "ParticleMesh", "LeafBlocks",
+           "TgravError", "G_PC_MSUN_KMS", "MIXED", "PLAIN_FP64"]

@@ -59,6 +59,11 @@ SIGNATURES = [
     ("tgrav_set_block_kernel", _I32, [_P, _I32]),
     ("tgrav_set_softening", _I32, [_P, _I32, _D, _I32]),
     ("tgrav_set_periodic", _I32, [_P, _I32, _D, _D, _D, _I32]),
+    ("tgrav_pm_deposit", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P, _P,
+                                _I32, _I32]),
+    ("tgrav_pm_gradient", _I32, [_P, _I64, _I32, _I32, _P, _P, _P, _P, _P, _D, _P, _P, _P, _I32]),
+    ("tgrav_pm_interpolate", _I32, [_P, _I64, _P, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P, _P,
+                                    _P, _P, _P, _P, _P, _I32]),
     ("tgrav_set_bidir_config", _I32, [_P, _I32, _I32, _I32]),
     ("tgrav_bidir_variant_info", _I32, [_I32, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     ("tgrav_batch_create", _I32, [_P, ctypes.c_int32, ctypes.POINTER(Problem), _I32, _D,

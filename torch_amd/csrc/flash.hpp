@@ -14,6 +14,11 @@
 //     from the reference's would change a result by a whole term;
 //   * Newtonian pairs (r >= rsoft, nearly all of them) use the refined 1/sqrt of the
 //     Plummer kernel: kv = y^3, pk = y; the softened branch recomputes both from r.
+//     Isolated boundaries: the inner loop is branch-free -- every pair is summed as
+//     Newtonian with r2 clamped to at least rsoft^2 (1 + 1e-12), and pairs below that clamp
+//     are flagged (one wave-uniform bit per source of a 64-source slice); for each flagged
+//     source the wave then takes that Newtonian term back out of the flagged pairs and adds
+//     the exact one (flash_fix).
 #pragma once
 #include "kernels.hpp"
 
@@ -22,6 +27,7 @@ namespace tg {
 struct Flash {
   double rsoft, hinv, h2inv, slope;
   double soft_hi2;                // r2 >= soft_hi2 guarantees fl(sqrt(r2)) >= rsoft
+  double near2;                   // rsoft^2 (1 + 1e-12): FMA-formed r2 below it may be softened
   double lx, ly, lz, ilx, ily, ilz;  // periodic box and reciprocals
   double rmax, cut_lo2, cut_hi2;  // image cutoff nrep*min(L): r2 < cut_lo2 in, >= cut_hi2 out
   int type;                       // 1 spline, 2 linear
@@ -136,6 +142,42 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
   }
 }
 
+// Correction for one source whose pairs the branch-free loop flagged (FMA-formed r2 below
+// near2): remove the clamped Newtonian term that loop added and add the exact pair term.
+// Any pair the reference puts inside rsoft is flagged: its r2 is below soft_hi2, and the
+// FMA-formed r2 differs from the reference's by a few ulps.
+template <int K, bool S32>
+__device__ __forceinline__ void flash_fix(const Flash& P, const double4 s, const double (&tx)[K],
+                                          const double (&ty)[K], const double (&tz)[K],
+                                          double (&ax)[K], double (&ay)[K], double (&az)[K],
+                                          double (&ap)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double fx = tx[k] - s.x, fy = ty[k] - s.y, fz = tz[k] - s.z;
+    const double r2f = fma(fx, fx, fma(fy, fy, fz * fz));
+    if (r2f < P.near2) {
+      const double y0 = rsqrt_f64<S32 ? F_SEED32 : 0>(P.near2);
+      const double mr0 = s.w * y0;
+      const double w0 = mr0 * (y0 * y0);
+      double dx, dy, dz, r2;
+      {
+#pragma clang fp contract(off)
+        dx = tx[k] - s.x;
+        dy = ty[k] - s.y;
+        dz = tz[k] - s.z;
+        r2 = (dx * dx + dy * dy) + dz * dz;
+      }
+      double kv, pk;
+      flash_pair<S32>(P, r2, kv, pk);
+      const double w = s.w * kv;
+      ax[k] += w0 * fx - w * dx;
+      ay[k] += w0 * fy - w * dy;
+      az[k] += w0 * fz - w * dz;
+      ap[k] += s.w * pk - mr0;
+    }
+  }
+}
+
 // blockIdx.x = target tile of K*NT targets, blockIdx.y = source chunk.  Accumulators follow
 // store_result's convention (a = G s, phi = -G s_p): s -= m kv d, s_p += m pk.
 template <int K, int NT, bool PER, bool S32>
@@ -161,33 +203,47 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
     if (base + tid < se) ls[tid] = src4[base + tid];
     __syncthreads();
     const int cnt = (int)(se - base < NT ? se - base : NT);
-    for (int j = 0; j < cnt; ++j) {
-      const double4 s = ls[j];
+    if (PER) {
+      for (int j = 0; j < cnt; ++j) {
+        const double4 s = ls[j];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if (PER) {
+        for (int k = 0; k < K; ++k) {
           double px, py, pz, pp;
           flash_images<S32>(P, tx[k] - s.x, ty[k] - s.y, tz[k] - s.z, px, py, pz, pp);
           ax[k] = fma(-s.w, px, ax[k]);
           ay[k] = fma(-s.w, py, ay[k]);
           az[k] = fma(-s.w, pz, az[k]);
           ap[k] = fma(s.w, pp, ap[k]);
-        } else {
-          double dx, dy, dz, r2;
-          {
-#pragma clang fp contract(off)
-            dx = tx[k] - s.x;
-            dy = ty[k] - s.y;
-            dz = tz[k] - s.z;
-            r2 = (dx * dx + dy * dy) + dz * dz;
+        }
+      }
+    } else {
+      // 64 sources at a time; fm (wave-uniform) marks the sources that came closer than the
+      // clamp to any of the wave's targets, which flash_fix then revisits one by one.
+      for (int j0 = 0; j0 < cnt; j0 += 64) {
+        const int j1 = j0 + 64 < cnt ? j0 + 64 : cnt;
+        unsigned long long fm = 0ULL;
+        for (int j = j0; j < j1; ++j) {
+          const double4 s = ls[j];
+          bool nj = false;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const double dx = tx[k] - s.x, dy = ty[k] - s.y, dz = tz[k] - s.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            nj = nj || r2 < P.near2;
+            const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(fmax(r2, P.near2));
+            const double mr = s.w * y;
+            ap[k] += mr;
+            const double w = mr * (y * y);
+            ax[k] = fma(-w, dx, ax[k]);
+            ay[k] = fma(-w, dy, ay[k]);
+            az[k] = fma(-w, dz, az[k]);
           }
-          double kv, pk;
-          flash_pair<S32>(P, r2, kv, pk);
-          const double w = -s.w * kv;
-          ax[k] = fma(w, dx, ax[k]);
-          ay[k] = fma(w, dy, ay[k]);
-          az[k] = fma(w, dz, az[k]);
-          ap[k] = fma(s.w, pk, ap[k]);
+          if (__builtin_amdgcn_ballot_w64(nj)) fm |= 1ULL << (j - j0);
+        }
+        while (fm) {
+          const int j = j0 + __builtin_ctzll(fm);
+          fm &= fm - 1ULL;
+          flash_fix<K, S32>(P, ls[j], tx, ty, tz, ax, ay, az, ap);
         }
       }
     }

@@ -21,6 +21,7 @@
 #include "bidir.hpp"
 #include "grid.hpp"
 #include "flash.hpp"
+#include "pm.hpp"
 
 #define TGRAV_VERSION 1
 
@@ -186,6 +187,7 @@ struct tgrav_ctx {
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
   DevBuf in[12], outb[8], vel[3], part, flag;
   DevBuf bp4, bpe2, bstat, bppart;   // fused bidirectional pass: packed particles, partials
+  DevBuf pm[16], pm_acc[3], pm_stat;  // particle-mesh pieces: staging, fixed-point sums, stat
   unsigned long long* host_flag = nullptr;   // pinned
   // instrumentation: HIP events around each evaluation kernel (tgrav_set_timing)
   int timing = 0;
@@ -404,8 +406,9 @@ static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
   return 0;
 }
 
-// FLASH softening kernels (FP64 only).  Isolated: K = 2 targets per lane; periodic: K = 1
-// (hundreds of images per pair), sources split until there are about 8 workgroups per CU.
+// FLASH softening kernels (FP64 only).  Isolated: K = 2 targets per lane, sources split until
+// there are about 16 workgroups per CU (4 waves each, so the last round of workgroups is a
+// small share of the launch); periodic: K = 1 (hundreds of images per pair), about 8.
 static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
   if (mode != TGRAV_PLAIN_FP64)
     return fail(TGRAV_EINVAL, "the FLASH softening kernels run in TGRAV_PLAIN_FP64 only");
@@ -418,6 +421,7 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   P.h2inv = P.hinv * P.hinv;
   P.slope = 1.0 / (c->rsoft * c->rsoft * c->rsoft);
   P.soft_hi2 = c->rsoft * c->rsoft * (1.0 + 1e-14);
+  P.near2 = c->rsoft * c->rsoft * (1.0 + 1e-12);
   P.lx = c->box[0];
   P.ly = c->box[1];
   P.lz = c->box[2];
@@ -435,7 +439,7 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   const bool s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
   const int64_t ntiles = (nt + (int64_t)K * NT - 1) / ((int64_t)K * NT);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
-  const int64_t target_wg = (per ? 8LL : 4LL) * c->num_cu;
+  const int64_t target_wg = (per ? 8LL : 16LL) * c->num_cu;
   int64_t want = 1;
   if (ntiles < target_wg / 2) {
     want = (target_wg + ntiles - 1) / ntiles;
@@ -584,6 +588,9 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->bpe2.release();
   c->bstat.release();
   c->bppart.release();
+  for (auto& b : c->pm) b.release();
+  for (auto& b : c->pm_acc) b.release();
+  c->pm_stat.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
@@ -1076,6 +1083,219 @@ int tgrav_set_periodic(tgrav_ctx* c, int on, double lx, double ly, double lz, in
   c->box[1] = ly;
   c->box[2] = lz;
   c->nrep = nrep;
+  return 0;
+}
+
+// ------------------------------------------------------------ particle mesh ------
+// Host pointers are staged into c->pm[slot]; device pointers pass through.
+static int pm_in(tgrav_ctx* c, int kind, int slot, const void* h, size_t bytes, const void** out) {
+  *out = h;
+  if (kind == TGRAV_DEVICE || !h || bytes == 0) return 0;
+  int rc = c->pm[slot].ensure(bytes);
+  if (rc) return rc;
+  HIPCHECK(hipMemcpyAsync(c->pm[slot].p, h, bytes, hipMemcpyHostToDevice, c->stream));
+  *out = c->pm[slot].p;
+  return 0;
+}
+
+static int pm_out(tgrav_ctx* c, int kind, int slot, void* h, size_t bytes, bool upload,
+                  void** out) {
+  *out = h;
+  if (kind == TGRAV_DEVICE || !h || bytes == 0) return 0;
+  int rc = c->pm[slot].ensure(bytes);
+  if (rc) return rc;
+  if (upload) HIPCHECK(hipMemcpyAsync(c->pm[slot].p, h, bytes, hipMemcpyHostToDevice, c->stream));
+  *out = c->pm[slot].p;
+  return 0;
+}
+
+static int pm_back(tgrav_ctx* c, int kind, void* h, const void* d, size_t bytes) {
+  if (kind == TGRAV_DEVICE || !h || bytes == 0) return 0;
+  HIPCHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+static int pm_geometry(int64_t nblk, int nb, int ng, int64_t* ncell) {
+  if (nblk < 0) return fail(TGRAV_EINVAL, "nblk must be >= 0");
+  if (nb < 1 || ng < 1 || nb + 2 * ng > 256)
+    return fail(TGRAV_EINVAL, "need nb >= 1, ng >= 1 and nb + 2 ng <= 256 (got nb %d, ng %d)", nb, ng);
+  const int64_t nbg = nb + 2 * ng;
+  if (nblk > ((int64_t)1 << 40) / (nbg * nbg * nbg)) return fail(TGRAV_ELIMIT, "too many cells");
+  *ncell = nblk * nbg * nbg * nbg;
+  return 0;
+}
+
+static int pm_read_err(tgrav_ctx* c, const char* what) {
+  bool bad = false;
+  int rc = read_flag(c, c->pm_stat.as<unsigned long long>() + 1, &bad);
+  if (rc) return rc;
+  if (bad)
+    return fail(TGRAV_EINVAL, "%s: a particle's block index is out of range or its cloud leaves "
+                "its guarded block", what);
+  return 0;
+}
+
+int tgrav_pm_deposit(tgrav_ctx* c, int64_t np, const double* x, const double* y, const double* z,
+                     const double* m, const int64_t* blk, int64_t nblk, int nb, int ng,
+                     const double* ox, const double* oy, const double* oz, const double* bsize,
+                     double* rho, int accumulate, int kind) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc;
+  int64_t ncell;
+  if ((rc = check_kind(kind)) || (rc = pm_geometry(nblk, nb, ng, &ncell))) return rc;
+  if (np < 0) return fail(TGRAV_EINVAL, "np must be >= 0");
+  if (np > ((int64_t)1 << 28)) return fail(TGRAV_ELIMIT, "at most 2^28 particles per deposit");
+  if (np > 0 && (!x || !y || !z || !m || !blk)) return fail(TGRAV_EINVAL, "x, y, z, m and blk are required");
+  if (nblk > 0 && (!ox || !oy || !oz || !bsize || !rho))
+    return fail(TGRAV_EINVAL, "ox, oy, oz, bsize and rho are required");
+  if (np > 0 && nblk == 0) return fail(TGRAV_EINVAL, "particles need blocks");
+  if ((rc = set_device(c))) return rc;
+  if (kind == TGRAV_HOST && c->validate) {
+    const double* arr[8] = {x, y, z, m, ox, oy, oz, bsize};
+    for (int q = 0; q < 8; ++q)
+      if (!host_finite(q < 4 ? np : nblk, arr[q])) return fail(TGRAV_ENONFINITE, "non-finite input %d", q);
+  }
+  const void *dx_, *dy_, *dz_, *dm_, *db_, *dox, *doy, *doz, *dbs;
+  void* drho;
+  if ((rc = pm_in(c, kind, 0, x, np * 8, &dx_)) || (rc = pm_in(c, kind, 1, y, np * 8, &dy_)) ||
+      (rc = pm_in(c, kind, 2, z, np * 8, &dz_)) || (rc = pm_in(c, kind, 3, m, np * 8, &dm_)) ||
+      (rc = pm_in(c, kind, 4, blk, np * 8, &db_)) || (rc = pm_in(c, kind, 5, ox, nblk * 8, &dox)) ||
+      (rc = pm_in(c, kind, 6, oy, nblk * 8, &doy)) || (rc = pm_in(c, kind, 7, oz, nblk * 8, &doz)) ||
+      (rc = pm_in(c, kind, 8, bsize, nblk * 8, &dbs)) ||
+      (rc = pm_out(c, kind, 9, rho, ncell * 8, accumulate != 0, &drho)))
+    return rc;
+  if (ncell == 0) return 0;
+  if ((rc = c->pm_acc[0].ensure(ncell * 8)) || (rc = c->pm_acc[1].ensure(ncell * 8)) ||
+      (rc = c->pm_acc[2].ensure(ncell * 8)) || (rc = c->pm_stat.ensure(16)))
+    return rc;
+  tg::PmBlocks B{(const double*)dox, (const double*)doy, (const double*)doz, (const double*)dbs,
+                 nblk, nb, ng, nb + 2 * ng, (int64_t)(nb + 2 * ng) * (nb + 2 * ng) * (nb + 2 * ng)};
+  unsigned long long* H = c->pm_acc[0].as<unsigned long long>();
+  unsigned long long* L = c->pm_acc[1].as<unsigned long long>();
+  unsigned long long* M = c->pm_acc[2].as<unsigned long long>();
+  unsigned long long* st = c->pm_stat.as<unsigned long long>();
+  tg::k_zero_u64<<<grid1(ncell, 256), 256, 0, c->stream>>>(ncell, H, L, M);
+  tg::k_zero_u64<<<1, 64, 0, c->stream>>>(2, st, nullptr, nullptr);
+  LAUNCHCHECK("k_zero_u64");
+  if (np > 0) {
+    tg::k_pm_cell_max<<<grid1(np, 256), 256, 0, c->stream>>>(
+        np, (const double*)dx_, (const double*)dy_, (const double*)dz_, (const double*)dm_,
+        (const int64_t*)db_, B, M, st + 1);
+    LAUNCHCHECK("k_pm_cell_max");
+    if (c->validate && (rc = pm_read_err(c, "tgrav_pm_deposit"))) return rc;
+    rc = timed_launch(c, np, ncell, [&]() -> int {
+      tg::k_pm_deposit<<<grid1(np, 256), 256, 0, c->stream>>>(
+          np, (const double*)dx_, (const double*)dy_, (const double*)dz_, (const double*)dm_,
+          (const int64_t*)db_, B, M, H, L);
+      LAUNCHCHECK("k_pm_deposit");
+      return 0;
+    });
+    if (rc) return rc;
+  }
+  tg::k_pm_finalize<<<grid1(ncell, 256), 256, 0, c->stream>>>(ncell, H, L, M, np, (double*)drho,
+                                                               accumulate != 0);
+  LAUNCHCHECK("k_pm_finalize");
+  if ((rc = pm_back(c, kind, rho, drho, ncell * 8))) return rc;
+  if (kind == TGRAV_HOST) HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tgrav_pm_gradient(tgrav_ctx* c, int64_t nblk, int nb, int ng, const double* bsize,
+                      const double* phi, double* gx, double* gy, double* gz, double dt,
+                      double* vx, double* vy, double* vz, int kind) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc;
+  int64_t ncell;
+  if ((rc = check_kind(kind)) || (rc = pm_geometry(nblk, nb, ng, &ncell))) return rc;
+  if (nblk > 0 && (!bsize || !phi)) return fail(TGRAV_EINVAL, "bsize and phi are required");
+  if ((vx || vy || vz) && !(vx && vy && vz)) return fail(TGRAV_EINVAL, "vx, vy, vz go together");
+  if (!isfinite(dt)) return fail(TGRAV_EINVAL, "dt must be finite");
+  if ((rc = set_device(c))) return rc;
+  if (kind == TGRAV_HOST && c->validate &&
+      (!host_finite(nblk, bsize) || !host_finite(ncell, phi)))
+    return fail(TGRAV_ENONFINITE, "non-finite block size or potential");
+  const void *dbs, *dphi;
+  void *dg[3], *dv[3];
+  if ((rc = pm_in(c, kind, 0, bsize, nblk * 8, &dbs)) || (rc = pm_in(c, kind, 1, phi, ncell * 8, &dphi)) ||
+      (rc = pm_out(c, kind, 2, gx, ncell * 8, false, &dg[0])) ||
+      (rc = pm_out(c, kind, 3, gy, ncell * 8, false, &dg[1])) ||
+      (rc = pm_out(c, kind, 4, gz, ncell * 8, false, &dg[2])) ||
+      (rc = pm_out(c, kind, 5, vx, ncell * 8, true, &dv[0])) ||
+      (rc = pm_out(c, kind, 6, vy, ncell * 8, true, &dv[1])) ||
+      (rc = pm_out(c, kind, 7, vz, ncell * 8, true, &dv[2])))
+    return rc;
+  if (ncell == 0) return 0;
+  tg::PmBlocks B{nullptr, nullptr, nullptr, (const double*)dbs, nblk, nb, ng, nb + 2 * ng,
+                 (int64_t)(nb + 2 * ng) * (nb + 2 * ng) * (nb + 2 * ng)};
+  rc = timed_launch(c, ncell, 0, [&]() -> int {
+    tg::k_pm_gradient<<<grid1(ncell, 256), 256, 0, c->stream>>>(
+        B, (const double*)dphi, (double*)dg[0], (double*)dg[1], (double*)dg[2], dt,
+        (double*)dv[0], (double*)dv[1], (double*)dv[2]);
+    LAUNCHCHECK("k_pm_gradient");
+    return 0;
+  });
+  if (rc) return rc;
+  double* hs[6] = {gx, gy, gz, vx, vy, vz};
+  void* ds[6] = {dg[0], dg[1], dg[2], dv[0], dv[1], dv[2]};
+  for (int q = 0; q < 6; ++q)
+    if ((rc = pm_back(c, kind, hs[q], ds[q], ncell * 8))) return rc;
+  if (kind == TGRAV_HOST) HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tgrav_pm_interpolate(tgrav_ctx* c, int64_t np, const double* x, const double* y,
+                         const double* z, const int64_t* blk, int64_t nblk, int nb, int ng,
+                         const double* ox, const double* oy, const double* oz,
+                         const double* bsize, const double* gx, const double* gy,
+                         const double* gz, double* ax, double* ay, double* az, int kind) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc;
+  int64_t ncell;
+  if ((rc = check_kind(kind)) || (rc = pm_geometry(nblk, nb, ng, &ncell))) return rc;
+  if (np < 0) return fail(TGRAV_EINVAL, "np must be >= 0");
+  if (np > 0 && (!x || !y || !z || !blk || !ax || !ay || !az))
+    return fail(TGRAV_EINVAL, "x, y, z, blk and ax, ay, az are required");
+  if (np > 0 && (nblk == 0 || !ox || !oy || !oz || !bsize || !gx || !gy || !gz))
+    return fail(TGRAV_EINVAL, "blocks (ox, oy, oz, bsize) and fields gx, gy, gz are required");
+  if ((rc = set_device(c))) return rc;
+  if (np == 0) return 0;
+  if (kind == TGRAV_HOST && c->validate) {
+    const double* arr[7] = {x, y, z, ox, oy, oz, bsize};
+    for (int q = 0; q < 7; ++q)
+      if (!host_finite(q < 3 ? np : nblk, arr[q])) return fail(TGRAV_ENONFINITE, "non-finite input %d", q);
+  }
+  const void *dx_, *dy_, *dz_, *db_, *dox, *doy, *doz, *dbs, *dgx, *dgy, *dgz;
+  void *dax, *day, *daz;
+  if ((rc = pm_in(c, kind, 0, x, np * 8, &dx_)) || (rc = pm_in(c, kind, 1, y, np * 8, &dy_)) ||
+      (rc = pm_in(c, kind, 2, z, np * 8, &dz_)) || (rc = pm_in(c, kind, 3, blk, np * 8, &db_)) ||
+      (rc = pm_in(c, kind, 4, ox, nblk * 8, &dox)) || (rc = pm_in(c, kind, 5, oy, nblk * 8, &doy)) ||
+      (rc = pm_in(c, kind, 6, oz, nblk * 8, &doz)) || (rc = pm_in(c, kind, 7, bsize, nblk * 8, &dbs)) ||
+      (rc = pm_in(c, kind, 8, gx, ncell * 8, &dgx)) || (rc = pm_in(c, kind, 9, gy, ncell * 8, &dgy)) ||
+      (rc = pm_in(c, kind, 10, gz, ncell * 8, &dgz)) ||
+      (rc = pm_out(c, kind, 11, ax, np * 8, false, &dax)) ||
+      (rc = pm_out(c, kind, 12, ay, np * 8, false, &day)) ||
+      (rc = pm_out(c, kind, 13, az, np * 8, false, &daz)))
+    return rc;
+  if ((rc = c->pm_stat.ensure(16))) return rc;
+  unsigned long long* st = c->pm_stat.as<unsigned long long>();
+  tg::k_zero_u64<<<1, 64, 0, c->stream>>>(2, st, nullptr, nullptr);
+  LAUNCHCHECK("k_zero_u64");
+  tg::PmBlocks B{(const double*)dox, (const double*)doy, (const double*)doz, (const double*)dbs,
+                 nblk, nb, ng, nb + 2 * ng, (int64_t)(nb + 2 * ng) * (nb + 2 * ng) * (nb + 2 * ng)};
+  rc = timed_launch(c, np, ncell, [&]() -> int {
+    tg::k_pm_interpolate<<<grid1(np, 256), 256, 0, c->stream>>>(
+        np, (const double*)dx_, (const double*)dy_, (const double*)dz_, (const int64_t*)db_, B,
+        (const double*)dgx, (const double*)dgy, (const double*)dgz, (double*)dax, (double*)day,
+        (double*)daz, st + 1);
+    LAUNCHCHECK("k_pm_interpolate");
+    return 0;
+  });
+  if (rc) return rc;
+  if (c->validate && (rc = pm_read_err(c, "tgrav_pm_interpolate"))) return rc;
+  if ((rc = pm_back(c, kind, ax, dax, np * 8)) || (rc = pm_back(c, kind, ay, day, np * 8)) ||
+      (rc = pm_back(c, kind, az, daz, np * 8)))
+    return rc;
+  if (kind == TGRAV_HOST) HIPCHECK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

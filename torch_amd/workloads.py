@@ -313,3 +313,17 @@ def c4(seed=1004, n_problems=64):
         tx, ty, tz, _ = block_cell_centres(o, s)
         probs.append(dict(sources=src, targets=dict(x=tx, y=ty, z=tz, eps=np.zeros(nt))))
     return probs
+
+
+def containing_block(x, y, z, origin, size):
+    """Index of the leaf block (lower corner origin[b], edge size[b]) holding each point, -1
+    outside all blocks (blocks are disjoint leaves of an octree)."""
+    out = np.full(len(x), -1, dtype=np.int64)
+    p = np.stack([x, y, z], 1)
+    for lo in range(0, len(size), 256):
+        o = origin[lo:lo + 256]
+        s = size[lo:lo + 256]
+        inside = np.all((p[:, None, :] >= o[None, :, :]) & (p[:, None, :] < o[None, :, :] + s[None, :, None]), axis=2)
+        hit = inside.any(1)
+        out[hit] = lo + inside[hit].argmax(1)
+    return out
