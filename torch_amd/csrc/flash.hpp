@@ -6,12 +6,13 @@
 // Execution model: as the Plummer FP64 kernel (k_eval_fp64) -- K targets per lane in VGPRs,
 // sources {x, y, z, m} staged through LDS NT at a time and read as broadcasts, source
 // chunks over blockIdx.y with fixed-order chunk reduction.  Two things differ:
-//   * branch decisions follow the reference bit for bit: radius = sqrt((dx*dx + dy*dy) +
-//     dz*dz) with every product and sum rounded separately, and the correctly rounded
-//     sqrt is taken only where r2 lies within 1e-14 of a threshold (rsoft^2 or the image
-//     cutoff^2); elsewhere r2 decides alone.  The kernels are discontinuous there (the
-//     image cutoff; the reference's potential inside rsoft), so a decision that differed
-//     from the reference's would change a result by a whole term;
+//   * branch decisions follow the reference bit for bit: it tests sqrt(r2) < R with
+//     r2 = (dx*dx + dy*dy) + dz*dz, every product and sum rounded separately.  The
+//     correctly rounded sqrt is monotone, so that test equals r2 <= T(R) for the largest
+//     double T(R) whose sqrt rounds below R (computed on the host); the kernels form r2 the
+//     reference's way wherever it decides a branch and compare it with T.  The kernels are
+//     discontinuous there (the image cutoff; the reference's potential inside rsoft), so a
+//     decision that differed from the reference's would change a result by a whole term;
 //   * Newtonian pairs (r >= rsoft, nearly all of them) use the refined 1/sqrt of the
 //     Plummer kernel: kv = y^3, pk = y; the softened branch recomputes both from r.
 //     Isolated boundaries: the inner loop is branch-free -- every pair is summed as
@@ -26,10 +27,10 @@ namespace tg {
 
 struct Flash {
   double rsoft, hinv, h2inv, slope;
-  double soft_hi2;                // r2 >= soft_hi2 guarantees fl(sqrt(r2)) >= rsoft
-  double near2;                   // rsoft^2 (1 + 1e-12): FMA-formed r2 below it may be softened
+  double soft_thr;                // r2 <= soft_thr  <=>  fl(sqrt(r2)) < rsoft
+  double near2;                   // soft_thr (1 + 1e-12): FMA-formed r2 below it may be softened
   double lx, ly, lz, ilx, ily, ilz;  // periodic box and reciprocals
-  double rmax, cut_lo2, cut_hi2;  // image cutoff nrep*min(L): r2 < cut_lo2 in, >= cut_hi2 out
+  double cut_thr;                 // r2 <= cut_thr  <=>  fl(sqrt(r2)) < nrep * min(L)
   int type;                       // 1 spline, 2 linear
   int quirks;                     // potential as the reference writes it
   int nrep;                       // periodic: images n in [-nrep, nrep]^3
@@ -66,10 +67,7 @@ __device__ __forceinline__ void flash_pair(const Flash& P, double r2, double& kv
   const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(r2);
   kv = (y * y) * y;
   pk = y;
-  if (r2 < P.soft_hi2) {   // possibly inside the softening radius (divergent, rare)
-    const double r = sqrt(r2);
-    if (r < P.rsoft) flash_soft(P, r, kv, pk);
-  }
+  if (r2 <= P.soft_thr) flash_soft(P, sqrt(r2), kv, pk);   // inside rsoft (divergent, rare)
 }
 
 // Range [lo, hi] of image indices n in [-N, N] that can satisfy (d + n L)^2 < rem.  A
@@ -101,7 +99,7 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
   px = py = pz = pp = 0.0;
   const int N = P.nrep;
   int x0, x1;
-  image_range(dxi, P.ilx, P.cut_hi2, N, x0, x1);
+  image_range(dxi, P.ilx, P.cut_thr, N, x0, x1);
   for (int nx = x0; nx <= x1; ++nx) {
     double dx, x2;
     {
@@ -110,7 +108,7 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
       x2 = dx * dx;
     }
     int y0, y1;
-    image_range(dyi, P.ily, P.cut_hi2 - x2, N, y0, y1);
+    image_range(dyi, P.ily, P.cut_thr - x2, N, y0, y1);
     for (int ny = y0; ny <= y1; ++ny) {
       double dy, xy;
       {
@@ -119,7 +117,7 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
         xy = x2 + dy * dy;
       }
       int z0, z1;
-      image_range(dzi, P.ilz, P.cut_hi2 - xy, N, z0, z1);
+      image_range(dzi, P.ilz, P.cut_thr - xy, N, z0, z1);
       for (int nz = z0; nz <= z1; ++nz) {
         double dz, r2;
         {
@@ -127,9 +125,7 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
           dz = dzi + (double)nz * P.lz;
           r2 = xy + dz * dz;
         }
-        bool in = r2 < P.cut_lo2;
-        if (!in && r2 < P.cut_hi2) in = sqrt(r2) < P.rmax;
-        if (in) {
+        if (r2 <= P.cut_thr) {
           double kv, pk;
           flash_pair<S32>(P, r2, kv, pk);
           px = fma(kv, dx, px);
@@ -144,7 +140,7 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
 
 // Correction for one source whose pairs the branch-free loop flagged (FMA-formed r2 below
 // near2): remove the clamped Newtonian term that loop added and add the exact pair term.
-// Any pair the reference puts inside rsoft is flagged: its r2 is below soft_hi2, and the
+// Any pair the reference puts inside rsoft is flagged: its r2 is at most soft_thr, and the
 // FMA-formed r2 differs from the reference's by a few ulps.
 template <int K, bool S32>
 __device__ __forceinline__ void flash_fix(const Flash& P, const double4 s, const double (&tx)[K],

@@ -406,6 +406,16 @@ static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
   return 0;
 }
 
+// Largest double x with fl(sqrt(x)) < r, so that for any double r2, sqrt(r2) < r <=> r2 <= x
+// (the correctly rounded sqrt is monotone); -1 if no x >= 0 qualifies (r <= 0).
+static double sqrt_lt_threshold(double r) {
+  if (!(r > 0.0)) return -1.0;
+  double x = r * r;
+  while (x > 0.0 && !(sqrt(x) < r)) x = nextafter(x, 0.0);
+  while (sqrt(nextafter(x, INFINITY)) < r) x = nextafter(x, INFINITY);
+  return x;
+}
+
 // FLASH softening kernels (FP64 only).  Isolated: K = 2 targets per lane, sources split until
 // there are about 16 workgroups per CU (4 waves each, so the last round of workgroups is a
 // small share of the launch); periodic: K = 1 (hundreds of images per pair), about 8.
@@ -420,17 +430,15 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   P.hinv = 2.0 / c->rsoft;
   P.h2inv = P.hinv * P.hinv;
   P.slope = 1.0 / (c->rsoft * c->rsoft * c->rsoft);
-  P.soft_hi2 = c->rsoft * c->rsoft * (1.0 + 1e-14);
-  P.near2 = c->rsoft * c->rsoft * (1.0 + 1e-12);
+  P.soft_thr = sqrt_lt_threshold(c->rsoft);
+  P.near2 = P.soft_thr * (1.0 + 1e-12);
   P.lx = c->box[0];
   P.ly = c->box[1];
   P.lz = c->box[2];
   P.ilx = per ? 1.0 / P.lx : 0.0;
   P.ily = per ? 1.0 / P.ly : 0.0;
   P.ilz = per ? 1.0 / P.lz : 0.0;
-  P.rmax = per ? (double)c->nrep * std::min(P.lx, std::min(P.ly, P.lz)) : 0.0;
-  P.cut_lo2 = P.rmax * P.rmax * (1.0 - 1e-14);
-  P.cut_hi2 = P.rmax * P.rmax * (1.0 + 1e-14);
+  P.cut_thr = per ? sqrt_lt_threshold((double)c->nrep * std::min(P.lx, std::min(P.ly, P.lz))) : -1.0;
   P.type = c->soft;
   P.quirks = c->soft_quirks;
   P.nrep = per ? c->nrep : 0;
