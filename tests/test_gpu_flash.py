@@ -100,18 +100,34 @@ def test_flash_isolated_forward_blocks_equal_centres(c1, kernel):
     check(a, ref, sa, sp)
 
 
-def test_flash_block_sources_equal_point_sources(field):
-    """Cells handed over as 8^3 leaf blocks give the same sums as the same cells as points."""
+@pytest.mark.parametrize("kernel", ["spline", "linear"])
+def test_flash_block_sources(field, kernel):
+    """Cells handed over as 8^3 leaf blocks: the generic kernel on them gives the same sums as
+    the same cells as points, bit for bit; the column kernel (the default for block sources)
+    matches the oracle, including stars placed within the softening radius of cells."""
     rng = W.rng_for(42)
     cells = W.gas_cells(rng, box=4.0, base_n=2, n_leaves=3, sigma=1.0, ngrid=16, rho_mean=10.0)
     st = W.star_cluster(rng, 300)
-    field.set_softening("spline", 0.2)
+    pick = rng.choice(len(cells["x"]), 40, replace=False)          # 40 stars next to cells
+    for q, c in zip("xyz", ("x", "y", "z")):
+        st[q][:40] = cells[c][pick] + rng.normal(0, 0.02, 40)
+    st["x"][40] = cells["x"][pick[0]]                              # one exactly on a cell centre
+    st["y"][40], st["z"][40] = cells["y"][pick[0]], cells["z"][pick[0]]
+    rs = 0.2
+    field.set_softening(kernel, rs, quirks=True)
     field.set_sources(cells["x"], cells["y"], cells["z"], cells["m"], cells["eps"])
     a = field.evaluate(None, st["x"], st["y"], st["z"])
     field.set_source_blocks(cells["block_origin"], cells["block_size"] / 8, cells["m"])
+    field.set_block_kernel(False)
     b = field.evaluate(None, st["x"], st["y"], st["z"])
     for c in range(4):
         np.testing.assert_array_equal(a[c], b[c])
+    field.set_block_kernel(True)
+    col = field.evaluate(None, st["x"], st["y"], st["z"])
+    ref = oracle.eval_flash(cells["x"], cells["y"], cells["z"], cells["m"], st["x"], st["y"], st["z"], G, rs,
+                            kernel=kernel, quirks=True, threads=16)
+    sa, sp = term_scales(cells["x"], cells["y"], cells["z"], cells["m"], st["x"], st["y"], st["z"], rs)
+    check(col, ref, sa, sp)
 
 
 def test_flash_decisions_at_rsoft(field):

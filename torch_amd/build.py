@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "tgrav.hip")
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.hpp", "batch.hpp", "bidir.hpp")] + \
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.hpp", "batch.hpp", "bidir.hpp", "grid.hpp", "flash.hpp", "pm.hpp")] + \
        [os.path.join(ROOT, "include", "tgrav.h")]
 OUT = os.path.join(HERE, "libtgrav.so")
 ARCH = os.environ.get("TGRAV_ARCH", "gfx950")

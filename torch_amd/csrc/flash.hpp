@@ -22,6 +22,7 @@
 //     the exact one (flash_fix).
 #pragma once
 #include "kernels.hpp"
+#include "grid.hpp"
 
 namespace tg {
 
@@ -244,6 +245,115 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
       }
     }
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    if (t < T.n) store_result(o, t, ax[k], ay[k], az[k], ap[k]);
+  }
+}
+
+// ------------------------------------------------------- 8^3 block sources ----
+// Isolated FLASH kernel against cells handed over as 8^3 leaf blocks (tgrav_set_source_blocks):
+// the column walk of k_eval_gridsrc (dx, dy and dx^2 + dy^2 formed once per column and
+// target) with the branch-free Newtonian loop; a wave-uniform bit per cell of the column marks
+// cells that came closer than the clamp to any of the wave's targets, and those are redone
+// exactly (reference-rounded r2, exact threshold) as flash_fix does.
+template <int K, bool S32>
+__device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false>& L,
+                                               const double (&tx)[K], const double (&ty)[K],
+                                               const double (&tz)[K], double (&ax)[K],
+                                               double (&ay)[K], double (&az)[K], double (&ap)[K]) {
+  for (int col = 0; col < 64; ++col) {
+    const double xs = L.cx[col >> 3], ys = L.cy[col & 7];
+    double dx[K], dy[K], base[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      dx[k] = tx[k] - xs;
+      dy[k] = ty[k] - ys;
+      base[k] = fma(dx[k], dx[k], dy[k] * dy[k]);
+    }
+    unsigned int nm = 0u;   // wave-uniform: bit kk = cell kk of the column was flagged
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const double zs = L.cz[kk], m = L.m[col * 8 + kk];
+      bool nj = false;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double dz = tz[k] - zs;
+        const double r2 = fma(dz, dz, base[k]);
+        nj = nj || r2 < P.near2;
+        const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(fmax(r2, P.near2));
+        const double mr = m * y;
+        ap[k] += mr;
+        const double w = mr * (y * y);
+        ax[k] = fma(-w, dx[k], ax[k]);
+        ay[k] = fma(-w, dy[k], ay[k]);
+        az[k] = fma(-w, dz, az[k]);
+      }
+      if (__builtin_amdgcn_ballot_w64(nj)) nm |= 1u << kk;
+    }
+    while (nm) {
+      const int kk = __builtin_ctz(nm);
+      nm &= nm - 1u;
+      const double zs = L.cz[kk], m = L.m[col * 8 + kk];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double dz = tz[k] - zs;
+        const double r2f = fma(dz, dz, base[k]);
+        if (r2f < P.near2) {
+          const double y0 = rsqrt_f64<S32 ? F_SEED32 : 0>(P.near2);
+          const double mr0 = m * y0;
+          const double w0 = mr0 * (y0 * y0);
+          double ex, ey, ez, r2;
+          {
+#pragma clang fp contract(off)
+            ex = tx[k] - xs;
+            ey = ty[k] - ys;
+            ez = tz[k] - zs;
+            r2 = (ex * ex + ey * ey) + ez * ez;
+          }
+          double kv, pk;
+          flash_pair<S32>(P, r2, kv, pk);
+          const double wv = m * kv;
+          ax[k] += w0 * dx[k] - wv * ex;
+          ay[k] += w0 * dy[k] - wv * ey;
+          az[k] += w0 * dz - wv * ez;
+          ap[k] += m * pk - mr0;
+        }
+      }
+    }
+  }
+}
+
+// Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
+// blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
+template <int K, int NT, bool S32>
+__global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double4* __restrict__ bd,
+                                                           const double* __restrict__ be2,
+                                                           const double4* __restrict__ bm,
+                                                           int64_t nblk, int64_t chunk, Flash P,
+                                                           Out O) {
+  __shared__ LdsGS<false> L;
+  const int tid = threadIdx.x;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t b0 = (int64_t)blockIdx.y * chunk;
+  const int64_t b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
+  double tx[K], ty[K], tz[K], ax[K], ay[K], az[K], ap[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    double e2;
+    load_target(T, t < T.n ? t : T.n - 1, tx[k], ty[k], tz[k], e2);
+    ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+  }
+  for (int64_t blk = b0; blk < b1; ++blk) {
+    __syncthreads();
+    gs_stage<false>(L, bd, be2, bm, blk, 0.0, 0.0, 0.0, 0.0);
+    __syncthreads();
+    flash_gs_block<K, S32>(P, L, tx, ty, tz, ax, ay, az, ap);
+  }
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t t = tile0 + (int64_t)k * NT + tid;

@@ -14,6 +14,16 @@
 #pragma once
 #include "kernels.hpp"
 
+// Mixed-precision column kernel: 64-source LDS tiles whose FP32 partial sums are added to FP64
+// accumulators every TG_GRID_FOLD_TILES tiles; TG_GRID_RCP = 1 forms m / r^3 as (m y) rcp(r^2)
+// (a transcendental op in place of a multiply).
+#ifndef TG_GRID_FOLD_TILES
+#define TG_GRID_FOLD_TILES 1
+#endif
+#ifndef TG_GRID_RCP
+#define TG_GRID_RCP 0
+#endif
+
 namespace tg {
 
 template <int NT>
@@ -70,7 +80,13 @@ __device__ __forceinline__ void grid_tile_f32(const float4* __restrict__ s4, con
       if (FL & F_GUARD) y = r2 > 0.0f ? y : 0.0f;
       const float mr = s.w * y;
       if (!(FL & F_NOPOT)) ap[k] += mr;
+#if TG_GRID_RCP
+      float ri = __builtin_amdgcn_rcpf(r2);
+      if (FL & F_GUARD) ri = r2 > 0.0f ? ri : 0.0f;
+      const float w = mr * ri;
+#else
       const float w = (mr * y) * y;
+#endif
       ax[k] = fmaf(w, dx, ax[k]);
       ay[k] = fmaf(w, dy, ay[k]);
       az[k] = fmaf(w, dz, az[k]);
@@ -105,10 +121,13 @@ __device__ __forceinline__ void grid_sources_f32(LdsGridF<64>& L, const double4*
                                                  const double* __restrict__ srce2, int64_t sb,
                                                  int64_t se, double e2t, double cx, double cy,
                                                  double cz, float tx, float ty, const float (&tz)[8],
-                                                 float (&hx)[8], float (&hy)[8], float (&hz)[8],
-                                                 float (&hp)[8], float (&lx)[8], float (&ly)[8],
-                                                 float (&lz)[8], float (&lp)[8]) {
+                                                 double (&hx)[8], double (&hy)[8], double (&hz)[8],
+                                                 double (&hp)[8]) {
   const int lane = threadIdx.x;
+  float ax[8], ay[8], az[8], ap[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
+  int tiles = 0;
   for (int64_t base = sb; base < se; base += 64) {
     __syncthreads();
     const int64_t j = base + lane;
@@ -121,17 +140,25 @@ __device__ __forceinline__ void grid_sources_f32(LdsGridF<64>& L, const double4*
       L.se2[lane] = 1.0f;
     }
     __syncthreads();
-    float ax[8], ay[8], az[8], ap[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
     grid_tile_f32<FL>(L.s4, L.se2, tx, ty, tz, ax, ay, az, ap);
+    if (++tiles == TG_GRID_FOLD_TILES) {   // uniform: every lane counts the same tiles
+      tiles = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      two_sum(hx[k], lx[k], ax[k]);
-      two_sum(hy[k], ly[k], ay[k]);
-      two_sum(hz[k], lz[k], az[k]);
-      if (!(FL & F_NOPOT)) two_sum(hp[k], lp[k], ap[k]);
+      for (int k = 0; k < 8; ++k) {
+        hx[k] += (double)ax[k];
+        hy[k] += (double)ay[k];
+        hz[k] += (double)az[k];
+        if (!(FL & F_NOPOT)) hp[k] += (double)ap[k];
+        ax[k] = ay[k] = az[k] = ap[k] = 0.f;
+      }
     }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    hx[k] += (double)ax[k];
+    hy[k] += (double)ay[k];
+    hz[k] += (double)az[k];
+    if (!(FL & F_NOPOT)) hp[k] += (double)ap[k];
   }
 }
 
@@ -150,11 +177,10 @@ struct GridF32Run {
   static __device__ __forceinline__ void run(LdsGridF<64>& L, const double4* src4, const double* srce2,
                                              int64_t sb, int64_t se, double e2t, double cx,
                                              double cy, double cz, float tx, float ty,
-                                             const float (&tz)[8], float (&hx)[8], float (&hy)[8],
-                                             float (&hz)[8], float (&hp)[8], float (&lx)[8],
-                                             float (&ly)[8], float (&lz)[8], float (&lp)[8]) {
+                                             const float (&tz)[8], double (&hx)[8], double (&hy)[8],
+                                             double (&hz)[8], double (&hp)[8]) {
     grid_sources_f32<FL | F_UNIF>(L, src4, srce2, sb, se, e2t, cx, cy, cz, tx, ty, tz, hx, hy, hz,
-                                  hp, lx, ly, lz, lp);
+                                  hp);
   }
 };
 
@@ -203,15 +229,13 @@ __global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restri
     float tz[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) tz[k] = (float)(z[k] - cz);
-    float hx[8], hy[8], hz[8], hp[8], lx[8], ly[8], lz[8], lp[8];
+    double hx[8], hy[8], hz[8], hp[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = lx[k] = ly[k] = lz[k] = lp[k] = 0.f;
+    for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
     TG_DISPATCH_GRID(GridF32Run::run, fl & ~F_SEED32, L, src4, srce2, sb, se, e2t, cx, cy, cz, tx,
-                     ty, tz, hx, hy, hz, hp, lx, ly, lz, lp)
+                     ty, tz, hx, hy, hz, hp)
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      store_result(o, t0 + k, (double)hx[k] + (double)lx[k], (double)hy[k] + (double)ly[k],
-                   (double)hz[k] + (double)lz[k], (double)hp[k] + (double)lp[k]);
+    for (int k = 0; k < 8; ++k) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
   } else {
     double ax[8], ay[8], az[8], ap[8];
 #pragma unroll

@@ -416,6 +416,54 @@ static double sqrt_lt_threshold(double r) {
   return x;
 }
 
+// FLASH kernel, isolated, explicit targets against 8^3 source blocks: K targets per lane
+// (TG_FLASH_GS_K), 256 lanes, source blocks split over blockIdx.y until there are about 8
+// workgroups per CU.
+#ifndef TG_FLASH_GS_K
+#define TG_FLASH_GS_K 2
+#endif
+static int launch_flash_gridsrc(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O,
+                                const tg::Flash& P, bool s32) {
+  const int K = TG_FLASH_GS_K, NT = 256;
+  const int64_t nt = T.n, nblk = c->src_nblk;
+  const int64_t ntiles = (nt + K * NT - 1) / (K * NT);
+  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
+  const int64_t target = 8LL * c->num_cu;
+  int64_t want = ntiles >= target / 2 ? 1 : std::min<int64_t>((target + ntiles - 1) / ntiles, nblk);
+  want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
+  const int64_t chunk = (nblk + want - 1) / want;
+  const int nsplit = (int)((nblk + chunk - 1) / chunk);
+  tg::Out o = O;
+  if (nsplit > 1) {
+    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+    if (rc0) return rc0;
+    o.part = c->part.as<double>();
+    o.pstride = nt;
+  } else {
+    o.part = nullptr;
+  }
+  int rc = timed_launch(c, nt, c->ns, [&]() -> int {
+    const dim3 g((unsigned)ntiles, (unsigned)nsplit);
+    if (s32)
+      tg::k_eval_flash_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
+          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+    else
+      tg::k_eval_flash_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
+          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+    LAUNCHCHECK("k_eval_flash_gridsrc");
+    return 0;
+  });
+  if (rc) return rc;
+  if (nsplit > 1) {
+    tg::Out fin = O;
+    fin.part = nullptr;
+    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
+                                                                nt, fin);
+    LAUNCHCHECK("k_reduce_chunks");
+  }
+  return 0;
+}
+
 // FLASH softening kernels (FP64 only).  Isolated: K = 2 targets per lane, sources split until
 // there are about 16 workgroups per CU (4 waves each, so the last round of workgroups is a
 // small share of the launch); periodic: K = 1 (hundreds of images per pair), about 8.
@@ -445,6 +493,7 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   // Pairs reaching the refined 1/sqrt have r2 >= rsoft^2 (the rest take the softened
   // branch); the FP32 seed is valid from 1e-30 up.
   const bool s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
+  if (!per && !T.blocks && c->src_nblk > 0 && c->block_kernel) return launch_flash_gridsrc(c, T, O, P, s32);
   const int64_t ntiles = (nt + (int64_t)K * NT - 1) / ((int64_t)K * NT);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
   const int64_t target_wg = (per ? 8LL : 16LL) * c->num_cu;
