@@ -73,7 +73,7 @@ __device__ __forceinline__ void flash_pair(const Flash& P, double r2, double& kv
 
 // Range [lo, hi] of image indices n in [-N, N] that can satisfy (d + n L)^2 < rem.  A
 // superset: the exact test is applied per image, so the bounds only need to miss nothing
-// (float sqrt: relative error 2^-23; 1e-4 of slack in units of n covers it for N <= 64).
+// (float sqrt: relative error ~2^-22; 1e-4 of slack in units of n covers it for N <= 64).
 __device__ __forceinline__ void image_range(double d, double iL, double rem, int N, int& lo,
                                             int& hi) {
   if (!(rem > 0.0)) {
@@ -81,7 +81,7 @@ __device__ __forceinline__ void image_range(double d, double iL, double rem, int
     hi = 0;
     return;
   }
-  const double s = (double)sqrtf((float)rem);
+  const double s = (double)__builtin_amdgcn_sqrtf((float)rem);   // raw v_sqrt_f32, ~1 ulp
   double a = (-s - d) * iL - 1e-4, b = (s - d) * iL + 1e-4;
   a = fmax(a, -(double)N - 1.0);   // also keeps NaN / inf out of the int conversion
   b = fmin(b, (double)N + 1.0);
@@ -94,36 +94,38 @@ __device__ __forceinline__ void image_range(double d, double iL, double rem, int
 // of kv * d (d = x_t - x_c + n L) and of pk -- or, with quirks, the pk of the last image
 // inside the cutoff, as the reference assigns pgpot per image (Gravity_getPotentialAtPoint.F90:
 // 262-274).
-template <bool S32>
+// Image indices run as doubles (exact for |n| <= 64), so n * L is formed without a conversion.
+template <bool S32, bool QUIRKS>
 __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double dyi, double dzi,
                                              double& px, double& py, double& pz, double& pp) {
   px = py = pz = pp = 0.0;
   const int N = P.nrep;
   int x0, x1;
   image_range(dxi, P.ilx, P.cut_thr, N, x0, x1);
-  for (int nx = x0; nx <= x1; ++nx) {
+  for (double fx = (double)x0; fx <= (double)x1; fx += 1.0) {
     double dx, x2;
     {
 #pragma clang fp contract(off)
-      dx = dxi + (double)nx * P.lx;
+      dx = dxi + fx * P.lx;
       x2 = dx * dx;
     }
     int y0, y1;
     image_range(dyi, P.ily, P.cut_thr - x2, N, y0, y1);
-    for (int ny = y0; ny <= y1; ++ny) {
+    for (double fy = (double)y0; fy <= (double)y1; fy += 1.0) {
       double dy, xy;
       {
 #pragma clang fp contract(off)
-        dy = dyi + (double)ny * P.ly;
+        dy = dyi + fy * P.ly;
         xy = x2 + dy * dy;
       }
       int z0, z1;
       image_range(dzi, P.ilz, P.cut_thr - xy, N, z0, z1);
-      for (int nz = z0; nz <= z1; ++nz) {
+      const double fz1 = (double)z1;
+      for (double fz = (double)z0; fz <= fz1; fz += 1.0) {
         double dz, r2;
         {
 #pragma clang fp contract(off)
-          dz = dzi + (double)nz * P.lz;
+          dz = dzi + fz * P.lz;
           r2 = xy + dz * dz;
         }
         if (r2 <= P.cut_thr) {
@@ -132,7 +134,8 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
           px = fma(kv, dx, px);
           py = fma(kv, dy, py);
           pz = fma(kv, dz, pz);
-          pp = P.quirks ? pk : pp + pk;
+          if (QUIRKS) pp = pk;
+          else pp += pk;
         }
       }
     }
@@ -177,7 +180,7 @@ __device__ __forceinline__ void flash_fix(const Flash& P, const double4 s, const
 
 // blockIdx.x = target tile of K*NT targets, blockIdx.y = source chunk.  Accumulators follow
 // store_result's convention (a = G s, phi = -G s_p): s -= m kv d, s_p += m pk.
-template <int K, int NT, bool PER, bool S32>
+template <int K, int NT, bool PER, bool S32, bool QUIRKS = false>
 __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restrict__ src4,
                                                    int64_t ns, int64_t chunk, Flash P, Out O) {
   __shared__ double4 ls[NT];
@@ -206,7 +209,7 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           double px, py, pz, pp;
-          flash_images<S32>(P, tx[k] - s.x, ty[k] - s.y, tz[k] - s.z, px, py, pz, pp);
+          flash_images<S32, QUIRKS>(P, tx[k] - s.x, ty[k] - s.y, tz[k] - s.z, px, py, pz, pp);
           ax[k] = fma(-s.w, px, ax[k]);
           ay[k] = fma(-s.w, py, ay[k]);
           az[k] = fma(-s.w, pz, az[k]);

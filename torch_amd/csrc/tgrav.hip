@@ -519,8 +519,14 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
     const dim3 g((unsigned)ntiles, (unsigned)nsplit);
     const double4* s4 = c->s4.as<double4>();
     if (per) {
-      if (s32) tg::k_eval_flash<1, 256, true, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
-      else tg::k_eval_flash<1, 256, true, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      if (s32 && P.quirks)
+        tg::k_eval_flash<1, 256, true, true, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      else if (s32)
+        tg::k_eval_flash<1, 256, true, true, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      else if (P.quirks)
+        tg::k_eval_flash<1, 256, true, false, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      else
+        tg::k_eval_flash<1, 256, true, false, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
     } else {
       if (s32) tg::k_eval_flash<2, 256, false, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
       else tg::k_eval_flash<2, 256, false, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
