@@ -185,7 +185,7 @@ int tgrav_set_bidir_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
 int tgrav_bidir_variant_info(int mode, int variant, int* targets_per_lane, int* threads);
 
 /* ---- particle-mesh pieces beside the kick (FLASH leaf blocks) ----------- */
-/* Block arrays hold one variable of nblk blocks of nbg^3 cells, nbg = nb + 2 ng (nb interior
+/* Block arrays hold one variable of nblk blocks of nbg^3 cells, nbg = nb + 2 ng <= 128 (nb interior
  * cells per axis, ng >= 1 guard layers); block b starts at b * nbg^3 and cell (i, j, k) is
  * at (k * nbg + j) * nbg + i, x fastest -- solnData(VAR, i, j, k) of one variable.  Block b
  * has interior lower corner (ox, oy, oz)[b] and physical size bsize[b] (Grid_getBlkPhysicalSize,

@@ -156,16 +156,26 @@ __global__ void k_zero_u64(int64_t n, unsigned long long* a, unsigned long long*
 // g = -grad(phi) by the row rule delxinv (phi(i-1) - phi(i+1)), delxinv = 0.5 (nb / bs), the
 // two ends of each row copying their neighbour; any of gx/gy/gz may be null.  With vx != null
 // also the grid kick v += g dt (product and sum rounded separately).
-__global__ void k_pm_gradient(PmBlocks B, const double* __restrict__ phi, double* __restrict__ gx,
-                              double* __restrict__ gy, double* __restrict__ gz, double dt,
-                              double* __restrict__ vx, double* __restrict__ vy,
-                              double* __restrict__ vz) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B.nblk * B.nc) return;
+// Grid: x = slice * 8 + (b mod 8), y = b / 8 - grp0, where a slice is 256 consecutive cells of
+// a block.  Workgroups are dealt to the 8 XCDs round-robin by linear id (x fastest, gridDim.x a
+// multiple of 8), so all slices of block b run on XCD (b mod 8) and a block's z-neighbour reads
+// hit the L2 that its other slices filled.  Cell (i, j, k) comes from the slice offset r by two
+// divisions by nbg done as multiply-high with m = ceil(2^32 / nbg), exact for r < 2^32 / nbg
+// (nbg <= 128 and r < 2^21 here).
+__global__ __launch_bounds__(256) void k_pm_gradient(PmBlocks B, uint32_t magic, int64_t grp0,
+                                                     const double* __restrict__ phi,
+                                                     double* __restrict__ gx, double* __restrict__ gy,
+                                                     double* __restrict__ gz, double dt,
+                                                     double* __restrict__ vx, double* __restrict__ vy,
+                                                     double* __restrict__ vz) {
+  const int64_t b = (grp0 + blockIdx.y) * 8 + (blockIdx.x & 7u);
+  if (b >= B.nblk) return;
+  const uint32_t r = (blockIdx.x >> 3) * 256u + threadIdx.x;
+  if (r >= (uint32_t)B.nc) return;
   const int n = B.nbg;
-  const int64_t b = t / B.nc;
-  const int r = (int)(t - b * B.nc);
-  const int c[3] = {r % n, (r / n) % n, r / (n * n)};
+  const uint32_t q = __umulhi(r, magic);        // r / n
+  const uint32_t k = __umulhi(q, magic);        // r / n^2
+  const int c[3] = {(int)(r - q * (uint32_t)n), (int)(q - k * (uint32_t)n), (int)k};
   const int stride[3] = {1, n, n * n};
   double delxinv = (double)B.nb / B.bs[b];
   delxinv = 0.5 * delxinv;
@@ -174,9 +184,10 @@ __global__ void k_pm_gradient(PmBlocks B, const double* __restrict__ phi, double
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int ii = c[a] < 1 ? 1 : (c[a] > n - 2 ? n - 2 : c[a]);
-    const int ctr = r + (ii - c[a]) * stride[a];
+    const int ctr = (int)r + (ii - c[a]) * stride[a];
     g[a] = delxinv * (f[ctr - stride[a]] - f[ctr + stride[a]]);
   }
+  const int64_t t = b * B.nc + r;
   if (gx) gx[t] = g[0];
   if (gy) gy[t] = g[1];
   if (gz) gz[t] = g[2];

@@ -1180,8 +1180,8 @@ static int pm_back(tgrav_ctx* c, int kind, void* h, const void* d, size_t bytes)
 
 static int pm_geometry(int64_t nblk, int nb, int ng, int64_t* ncell) {
   if (nblk < 0) return fail(TGRAV_EINVAL, "nblk must be >= 0");
-  if (nb < 1 || ng < 1 || nb + 2 * ng > 256)
-    return fail(TGRAV_EINVAL, "need nb >= 1, ng >= 1 and nb + 2 ng <= 256 (got nb %d, ng %d)", nb, ng);
+  if (nb < 1 || ng < 1 || nb + 2 * ng > 128)
+    return fail(TGRAV_EINVAL, "need nb >= 1, ng >= 1 and nb + 2 ng <= 128 (got nb %d, ng %d)", nb, ng);
   const int64_t nbg = nb + 2 * ng;
   if (nblk > ((int64_t)1 << 40) / (nbg * nbg * nbg)) return fail(TGRAV_ELIMIT, "too many cells");
   *ncell = nblk * nbg * nbg * nbg;
@@ -1290,10 +1290,17 @@ int tgrav_pm_gradient(tgrav_ctx* c, int64_t nblk, int nb, int ng, const double* 
   if (ncell == 0) return 0;
   tg::PmBlocks B{nullptr, nullptr, nullptr, (const double*)dbs, nblk, nb, ng, nb + 2 * ng,
                  (int64_t)(nb + 2 * ng) * (nb + 2 * ng) * (nb + 2 * ng)};
+  const uint32_t nbg = (uint32_t)(nb + 2 * ng);
+  const uint32_t magic = (uint32_t)((((uint64_t)1 << 32) + nbg - 1) / nbg);   // ceil(2^32 / nbg)
+  const unsigned S = (unsigned)((B.nc + 255) / 256);                       // slices per block
+  const int64_t ngrp = (nblk + 7) / 8;                                     // groups of 8 blocks
   rc = timed_launch(c, ncell, 0, [&]() -> int {
-    tg::k_pm_gradient<<<grid1(ncell, 256), 256, 0, c->stream>>>(
-        B, (const double*)dphi, (double*)dg[0], (double*)dg[1], (double*)dg[2], dt,
-        (double*)dv[0], (double*)dv[1], (double*)dv[2]);
+    for (int64_t g0 = 0; g0 < ngrp; g0 += 65535) {
+      const dim3 g(8 * S, (unsigned)std::min<int64_t>(65535, ngrp - g0));
+      tg::k_pm_gradient<<<g, 256, 0, c->stream>>>(
+          B, magic, g0, (const double*)dphi, (double*)dg[0], (double*)dg[1], (double*)dg[2], dt,
+          (double*)dv[0], (double*)dv[1], (double*)dv[2]);
+    }
     LAUNCHCHECK("k_pm_gradient");
     return 0;
   });
