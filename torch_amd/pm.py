@@ -21,6 +21,7 @@ A block variable is an array of nblk * (nb + 2 ng)^3 values, block-major, x fast
 Inputs may be numpy arrays (synchronous) or ROCm torch tensors (torch's current stream).
 """
 import ctypes
+from collections import namedtuple
 
 import numpy as np
 
@@ -54,6 +55,9 @@ class LeafBlocks:
             c = o[:, a][:, None, None, None] + idx.reshape(shape)[None] * d[:, None, None, None]
             out.append(np.broadcast_to(c, (self.nblk, self.nbg, self.nbg, self.nbg)).ravel())
         return tuple(out)
+
+
+_Where = namedtuple("_Where", "kind dev")   # what GravityField._bind_stream reads
 
 
 def _host(a):
@@ -105,12 +109,11 @@ class ParticleMesh(GravityField):
         return np.empty(n, dtype=np.float64)
 
     def _stream(self, dev):
-        class _K:
-            pass
-        k = _K()
-        k.kind, k.dev = (DEVICE, dev) if dev is not None else (HOST, None)
-        self._bind_stream(k)
-        return k.kind
+        """Bind the context to torch's current stream (device arrays) or its own stream (host
+        arrays); returns the pointer kind for the call."""
+        where = _Where(DEVICE if dev is not None else HOST, dev)
+        self._bind_stream(where)
+        return where.kind
 
     def _geom(self, blocks, dev):
         o = blocks.origin
