@@ -12,7 +12,8 @@ loading its source set.  2 x 1.024e10 pair interactions per step.
   c4: 64 uneven independent problems in one graph-replayed persistent launch (one step =
       one replay, acc + pot of all problems).
   c2fused: c2 with symmetric softening (eps2 = eps_cell^2 + eps_star^2) evaluated by the fused
-      bidirectional pass (one pass gives both directions); two directed passes timed beside it.
+      bidirectional pass (one pass gives both directions) on the cells as 8^3 blocks; the fused
+      pass on explicit arrays and two directed passes in each format are timed beside it.
   c2flash: the reverse direction of c2 (1,024,000 cells as 8^3 blocks -> 10,000 stars) with
       FLASH's own spline softening (global radius 0.078125 pc = 2.5 finest cells, the reference's
       potential), FP64.
@@ -169,17 +170,29 @@ def run_batch(mode, probs_dev, steps, warmup):
 
 # ---------------------------------------------------------------- c2fused ----
 def run_fused(mode, dev, ns, nc, steps, warmup):
-    """One fused pass per step (cells <-> stars, eps2 = eps_cell^2 + eps_star^2); the same
-    fields from two directed evaluations are timed beside it."""
+    """One fused pass per step (cells <-> stars, eps2 = eps_cell^2 + eps_star^2), with the cells
+    as 8^3 leaf blocks (the headline) and as explicit arrays; the same fields from two directed
+    evaluations per format are timed beside them."""
     import torch
 
     from torch_amd import GravityField
     f = GravityField(mode, validate=False)
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
+    BO, BD = dev["bo"], dev["bdx"]
+
+    def fused_blocks():
+        return f.bidirectional_blocks(BO, BD, C[3], BD, *S)
 
     def fused():
         return f.bidirectional(*C, *S)
+
+    def directed_blocks():
+        f.set_sources(*S)
+        a = f.evaluate_blocks(BO, BD, eps=BD)
+        f.set_source_blocks(BO, BD, C[3], eps=BD)
+        b = f.evaluate(S[4], S[0], S[1], S[2])
+        return a, b
 
     def directed():
         f.set_sources(*S)
@@ -189,7 +202,8 @@ def run_fused(mode, dev, ns, nc, steps, warmup):
         return a, b
 
     res = {}
-    for name, fn in (("fused", fused), ("directed", directed)):
+    for name, fn in (("fused_blocks", fused_blocks), ("fused", fused), ("directed_blocks", directed_blocks),
+                     ("directed", directed)):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -205,9 +219,10 @@ def run_fused(mode, dev, ns, nc, steps, warmup):
         res[name] = dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.mean([k[0] for k in ks])),
                          res=tuple(tuple(a.cpu().numpy() for a in o) for o in out))
     f.close()
-    r = res["fused"]
-    return dict(ms=r["ms"], fwd_kernel_ms=r["kernel_ms"], res=r["res"], directed_ms=res["directed"]["ms"],
-                directed_res=res["directed"]["res"], cfg=None)
+    r = res["fused_blocks"]
+    return dict(ms=r["ms"], fwd_kernel_ms=r["kernel_ms"], res=r["res"],
+                others={k: res[k]["ms"] for k in ("fused", "directed_blocks", "directed")},
+                explicit_res=res["fused"]["res"], cfg=None)
 
 
 # ------------------------------------------------------- c2flash / c1periodic ----
@@ -502,8 +517,9 @@ def main():
         for mode in modes:
             if args.workload == "c2fused":
                 r = run_fused(mode, dev, ns, nc, args.steps, args.warmup)
-                log(f"{mode}: fused {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
-                    f"kernel {r['fwd_kernel_ms']:.3f} ms; two directed passes {r['directed_ms']:.3f} ms")
+                log(f"{mode}: fused (8^3 blocks) {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} "
+                    f"interactions/s), kernel {r['fwd_kernel_ms']:.3f} ms; fused explicit, directed blocks, "
+                    f"directed explicit: {r['others']}")
             else:
                 r = run_pair(mode, dev, ns, nc, args.steps, args.warmup, blocks=True)
                 log(f"{mode}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s); "
@@ -540,8 +556,8 @@ def main():
                                            "launch_shapes": results[m]["explicit"]["cfg"]}
                                        for m in results}
         if args.workload == "c2fused":
-            kernel_name = f"k_bidir<{'mixed' if head == 'mixed' else 'fp64'}> (both directions)"
-            line["directed_two_pass_ms"] = {m: results[m]["directed_ms"] for m in results}
+            kernel_name = f"k_bidir_grid<{'mixed' if head == 'mixed' else 'fp64'}> (both directions, 8^3 cell blocks)"
+            line["other_forms_ms_per_step"] = {m: results[m]["others"] for m in results}
     else:
         probs = W.c4(1004)
         pdev = []

@@ -176,6 +176,18 @@ int tgrav_set_softening(tgrav_ctx* ctx, int kernel, double rsoft, int flags);
  * on = 0 restores isolated boundaries.  nrep in [1, 64]. */
 int tgrav_set_periodic(tgrav_ctx* ctx, int on, double lx, double ly, double lz, int nrep);
 
+/* The fused pass with the gas cells handed over as 8^3 leaf blocks (nb must be 8): block b has
+ * lower corner (ox, oy, oz)[b], cell width dx[b], softening ceps[b] (NULL = 0) and cell masses
+ * cm[b*512 + (i*8 + j)*8 + k]; the cell outputs use the same C order.  Same pair set and
+ * results as tgrav_bidirectional on the blocks' cell centres; each lane walks a column of 8
+ * cells, so the pair arithmetic shared along the column is done once. */
+int tgrav_bidirectional_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const double* ox,
+                               const double* oy, const double* oz, const double* dx,
+                               const double* cm, const double* ceps, int64_t np,
+                               const double* px, const double* py, const double* pz,
+                               const double* pm, const double* pe, double* cax, double* cay,
+                               double* caz, double* cphi, double* pax, double* pay, double* paz,
+                               double* pphi, int mode, int ptr_kind);
 /* Block targets with nb == 8 use a column kernel (each lane owns one (i, j) column of 8
  * cells and shares dx, dy, dx^2 + dy^2 + eps^2 between them); on = 0 falls back to the
  * generic kernel (same results to rounding; for tests and comparisons).  Default on. */

@@ -66,6 +66,63 @@ def test_every_variant_and_chunking_agree(mode):
         assert lib.tgrav_bidir_variant_info(f.mode, 4, None, None) == _lib.EINVAL
 
 
+def blocks_and_particles(seed, nblk, npart):
+    rng = W.rng_for(seed)
+    o = rng.uniform(-1.5, 1.0, (nblk, 3))
+    size = rng.choice([0.125, 0.25, 0.5], nblk)
+    x, y, z, dx = W.block_cell_centres(o, size)
+    m = 50.0 * np.exp(rng.normal(size=x.size)) * dx ** 3
+    st = W.star_cluster(rng, npart, binary_frac=0.2)
+    pick = rng.choice(x.size, min(20, npart), replace=False)     # some particles inside cells
+    for q, c in zip(("x", "y", "z"), (x, y, z)):
+        st[q][:len(pick)] = c[pick] + rng.normal(0, 0.01, len(pick))
+    B = dict(o=o, dx=size / 8, m=m, eps=size / 8)
+    C = dict(x=x, y=y, z=z, m=m, eps=dx)
+    P = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
+    return B, C, P
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("nblk,npart", [(1, 1), (1, 65), (3, 300), (7, 1000)])
+def test_blocks_match_explicit_and_oracle(mode, nblk, npart):
+    """Cells as 8^3 blocks (column kernel): the explicit-cell fused pass and the oracle."""
+    B, C, P = blocks_and_particles(nblk * 1000 + npart, nblk, npart)
+    with GravityField(mode) as f:
+        co, po = f.bidirectional_blocks(B["o"], B["dx"], B["m"], B["eps"], P["x"], P["y"], P["z"], P["m"],
+                                        P["eps"])
+        ce, pe = run(f, C, P)
+    (pa, pb, pc, pp), (ca, cb, cc, cp) = oracle.bidirectional(
+        C["x"], C["y"], C["z"], C["m"], C["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"], G)
+    for name, got, ref, alt in (("cells", co, (ca, cb, cc, cp), ce), ("particles", po, (pa, pb, pc, pp), pe)):
+        s = _util.summary(got, ref)
+        print(mode, nblk, npart, name, s, flush=True)
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (name, s)
+            s2 = _util.summary(got, alt)
+            assert s2["acc_max"] <= 1e-12 and s2["pot_max"] <= 1e-12, (name, s2)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
+
+
+def test_blocks_chunking_deterministic_and_errors():
+    B, C, P = blocks_and_particles(9, 4, 3000)
+    lib = _lib.load()
+    with GravityField("fp64") as f:
+        outs = []
+        for ns in (1, 0, 7, 0):
+            f.set_bidir_config(nsplit=ns)
+            outs.append(f.bidirectional_blocks(B["o"], B["dx"], B["m"], B["eps"], P["x"], P["y"], P["z"],
+                                               P["m"], P["eps"]))
+        for side in range(2):
+            for q in range(4):
+                np.testing.assert_array_equal(outs[1][side][q], outs[3][side][q])   # same shape: bitwise
+                for k in (0, 2):
+                    s = _util.summary(outs[k][side], outs[1][side])
+                    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+        assert lib.tgrav_bidirectional_blocks(f.handle, 1, 4, *([None] * 6), 0, *([None] * 13),
+                                              f.mode, 0) == _lib.EINVAL
+
+
 def test_c2_fused_equals_directed_and_third_law():
     w = W.c2(1002)
     st, ce = w["stars"], w["cells"]

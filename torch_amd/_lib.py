@@ -56,6 +56,8 @@ SIGNATURES = [
                                  _P, _P, _P, _I32, _I32]),
     ("tgrav_bidirectional", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P,
                                    _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
+    ("tgrav_bidirectional_blocks", _I32, [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P,
+                                          _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
     ("tgrav_set_block_kernel", _I32, [_P, _I32]),
     ("tgrav_set_softening", _I32, [_P, _I32, _D, _I32]),
     ("tgrav_set_periodic", _I32, [_P, _I32, _D, _D, _D, _I32]),

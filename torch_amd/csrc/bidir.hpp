@@ -215,6 +215,29 @@ __global__ __launch_bounds__(NT) void k_bidir(Cells C, const double4* __restrict
   }
 }
 
+// Particle side, first level (many cell tiles): blockIdx.y = group g of tiles
+// [g*per, min((g+1)*per, ntiles)); writes the group's sums (tile order) to out[(g*4 + q)*np + j].
+__global__ void k_bidir_group(const double* __restrict__ ppart, int ntiles, int per, int64_t np,
+                              double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const int t0 = blockIdx.y * per;
+  const int t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int t = t0; t < t1; ++t) {
+    const double* p = ppart + (int64_t)t * 4 * np;
+    s0 += p[j];
+    s1 += p[np + j];
+    s2 += p[2 * np + j];
+    s3 += p[3 * np + j];
+  }
+  double* o = out + (int64_t)blockIdx.y * 4 * np;
+  o[j] = s0;
+  o[np + j] = s1;
+  o[2 * np + j] = s2;
+  o[3 * np + j] = s3;
+}
+
 // Particle side: sum the ntiles cell-tile partials of each particle (tile order), then the
 // particle's field is (-G u, -G up).
 __global__ void k_bidir_reduce(const double* __restrict__ ppart, int ntiles, int64_t np, Out O) {
@@ -231,6 +254,175 @@ __global__ void k_bidir_reduce(const double* __restrict__ ppart, int ntiles, int
   Out o = O;
   o.part = nullptr;
   store_result(o, j, -s0, -s1, -s2, s3);
+}
+
+// ------------------------------------------------------------- 8^3 cell blocks ----
+// The fused pass with the cells handed over as 8^3 leaf blocks: one 64-lane workgroup per
+// block, lane l owns the cell column (i, j) = (l / 8, l % 8) and its 8 cells along z (as
+// k_eval_grid).  Particles stream through LDS 64 at a time and are walked in the rotated
+// order of k_bidir (lane l takes particle (l + s) mod 64 at step s), so each lane meets one
+// particle per step with its whole column: dx, dy and dx^2 + dy^2 + eps^2 are formed once
+// per (column, particle), the cell side accumulates in registers, and the particle side sums
+// the column's 8 cells in registers before one conflict-free read-add-write of the
+// particle's LDS accumulator (FP32 in mixed mode: one tile's 512 cells, folded to FP64 in the
+// partial sums, like the cell side's per-tile fold).  Cell eps^2 is uniform per block and
+// folded into the staged particle eps^2.
+struct CellBlocks {
+  const double *ox, *oy, *oz, *dx, *m, *eps;   // per block; m per cell (C order)
+  int64_t nblk;
+};
+
+#ifndef TG_BIGRID_UNROLL
+#define TG_BIGRID_UNROLL 1
+#endif
+
+template <bool MIXED>
+struct LdsBiGrid {
+  typedef typename std::conditional<MIXED, float, double>::type T;
+  typename std::conditional<MIXED, float4, double4>::type p4[64];
+  T pe2[64];
+  typename std::conditional<MIXED, float4, double4>::type acc[64];   // particle sums of the tile
+};
+
+template <bool MIXED, bool GUARD>
+__device__ __forceinline__ void bi_grid_tiles(
+    LdsBiGrid<MIXED>& L, const double4* __restrict__ p4, const double* __restrict__ pe2,
+    int64_t pb, int64_t pe, double e2c, double ax_, double ay_, double az_,
+    typename std::conditional<MIXED, float, double>::type qx,
+    typename std::conditional<MIXED, float, double>::type qy,
+    const typename std::conditional<MIXED, float, double>::type (&qz)[8],
+    const typename std::conditional<MIXED, float, double>::type (&qm)[8],
+    double (&hx)[8], double (&hy)[8], double (&hz)[8], double (&hp)[8],
+    double* __restrict__ ppart, int64_t np) {
+  typedef typename std::conditional<MIXED, float, double>::type T;
+  const int lane = threadIdx.x;
+  for (int64_t base = pb; base < pe; base += 64) {
+    __syncthreads();
+    const int64_t j = base + lane;
+    if (j < pe) {
+      const double4 s = p4[j];
+      if constexpr (MIXED)
+        L.p4[lane] = make_float4((float)(s.x - ax_), (float)(s.y - ay_), (float)(s.z - az_), (float)s.w);
+      else
+        L.p4[lane] = s;
+      L.pe2[lane] = (T)(pe2[j] + e2c);
+    } else {                      // padding particle: zero mass, eps^2 = 1
+      if constexpr (MIXED) L.p4[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+      else L.p4[lane] = make_double4(0.0, 0.0, 0.0, 0.0);
+      L.pe2[lane] = (T)1.0;
+    }
+    if constexpr (MIXED) L.acc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    else L.acc[lane] = make_double4(0.0, 0.0, 0.0, 0.0);
+    __syncthreads();
+    T cx[8], cy[8], cz[8], cp[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cx[k] = cy[k] = cz[k] = cp[k] = (T)0;
+#pragma unroll TG_BIGRID_UNROLL
+    for (int s = 0; s < 64; ++s) {
+      const int jj = (lane + s) & 63;
+      const auto P = L.p4[jj];
+      const T dx = P.x - qx, dy = P.y - qy;
+      const T base2 = fma(dx, dx, fma(dy, dy, L.pe2[jj]));
+      T ux = 0, uy = 0, uz = 0, up = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const T dz = P.z - qz[k];
+        const T r2 = fma(dz, dz, base2);
+        T y;
+        if constexpr (MIXED) {
+          y = __builtin_amdgcn_rsqf(r2);
+          if (GUARD) y = r2 > 0.0f ? y : 0.0f;
+        } else {
+          y = rsqrt_f64<GUARD ? F_GUARD : 0>(r2);
+        }
+        const T y3 = (y * y) * y;
+        const T wp = P.w * y3;
+        cp[k] = fma(P.w, y, cp[k]);
+        cx[k] = fma(wp, dx, cx[k]);
+        cy[k] = fma(wp, dy, cy[k]);
+        cz[k] = fma(wp, dz, cz[k]);
+        const T wc = qm[k] * y3;
+        up = fma(qm[k], y, up);
+        ux = fma(wc, dx, ux);
+        uy = fma(wc, dy, uy);
+        uz = fma(wc, dz, uz);
+      }
+      auto a = L.acc[jj];         // the tile's sum for this particle (FP32 in mixed mode)
+      a.x += ux;
+      a.y += uy;
+      a.z += uz;
+      a.w += up;
+      L.acc[jj] = a;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      hx[k] += (double)cx[k];
+      hy[k] += (double)cy[k];
+      hz[k] += (double)cz[k];
+      hp[k] += (double)cp[k];
+    }
+    __syncthreads();
+    if (j < pe) {                 // this block's partial sums for particle j
+      const auto a = L.acc[lane];
+      ppart[j] = (double)a.x;
+      ppart[np + j] = (double)a.y;
+      ppart[2 * np + j] = (double)a.z;
+      ppart[3 * np + j] = (double)a.w;
+    }
+  }
+}
+
+// blockIdx.x = leaf block, blockIdx.y = particle chunk.  Cell results (C order inside the
+// block) go to CO (final, or chunk partials when CO.part is set); particle partials go to
+// ppart[(block * 4 + q) * np + j], summed by k_bidir_reduce in block order.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_bidir_grid(CellBlocks C, const double4* __restrict__ p4,
+                                                   const double* __restrict__ pe2, int64_t np,
+                                                   int64_t chunk, const double* e2p_min_p, Out CO,
+                                                   double* __restrict__ ppart) {
+  typedef typename std::conditional<MIXED, float, double>::type T;
+  __shared__ LdsBiGrid<MIXED> L;
+  const int lane = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  const int i = lane >> 3, jj = lane & 7;
+  const int64_t pb = (int64_t)blockIdx.y * chunk;
+  const int64_t pe = pb + chunk < np ? pb + chunk : np;
+  const double d = C.dx[b];
+  const double e = C.eps ? C.eps[b] : 0.0;
+  const double e2c = e * e;
+  double x, y, z[8];
+  {
+#pragma clang fp contract(off)
+    x = C.ox[b] + ((double)i + 0.5) * d;
+    y = C.oy[b] + ((double)jj + 0.5) * d;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) z[k] = C.oz[b] + ((double)k + 0.5) * d;
+  }
+  double ax_ = 0.0, ay_ = 0.0, az_ = 0.0;
+  if (MIXED) {                    // anchor: the block centre
+    ax_ = C.ox[b] + 4.0 * d;
+    ay_ = C.oy[b] + 4.0 * d;
+    az_ = C.oz[b] + 4.0 * d;
+  }
+  const int64_t t0 = b * 512 + (int64_t)lane * 8;
+  T qx = (T)(x - ax_), qy = (T)(y - ay_), qz[8], qm[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    qz[k] = (T)(z[k] - az_);
+    qm[k] = (T)C.m[t0 + k];
+  }
+  const double lo = *e2p_min_p + e2c;
+  const bool guard = MIXED ? !(lo > TG_F32_TINY_E2) : !(lo > 0.0);
+  double hx[8], hy[8], hz[8], hp[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
+  double* pp = ppart + b * 4 * np;
+  if (guard) bi_grid_tiles<MIXED, true>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+  else bi_grid_tiles<MIXED, false>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+  Out o = CO;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
 }
 
 }  // namespace tg

@@ -186,7 +186,7 @@ struct tgrav_ctx {
   DevBuf sbd, sbe2;
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
   DevBuf in[12], outb[8], vel[3], part, flag;
-  DevBuf bp4, bpe2, bstat, bppart;   // fused bidirectional pass: packed particles, partials
+  DevBuf bp4, bpe2, bstat, bppart, bpgrp;   // fused passes: packed particles, partials
   DevBuf pm[16], pm_acc[3], pm_stat;  // particle-mesh pieces: staging, fixed-point sums, stat
   unsigned long long* host_flag = nullptr;   // pinned
   // instrumentation: HIP events around each evaluation kernel (tgrav_set_timing)
@@ -545,6 +545,30 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   return 0;
 }
 
+// Particle side of the fused passes: sums the ntiles cell-tile partials per particle in a
+// fixed order.  Above 64 tiles the sum runs in two levels (64 groups of consecutive tiles, then
+// the groups in order), so the partial buffer is read by ~64x more workgroups than particles
+// alone would give.
+static int reduce_particle_partials(tgrav_ctx* c, int64_t ntiles, int64_t np, const tg::Out& PO) {
+  const double* src = c->bppart.as<double>();
+  int64_t nt = ntiles;
+  if (ntiles > 64) {
+    const int G = 64;
+    const int per = (int)((ntiles + G - 1) / G);
+    const int groups = (int)((ntiles + per - 1) / per);
+    int rc = c->bpgrp.ensure((size_t)groups * 4 * np * sizeof(double));
+    if (rc) return rc;
+    tg::k_bidir_group<<<dim3(grid1(np, 256), (unsigned)groups), 256, 0, c->stream>>>(
+        src, (int)ntiles, per, np, c->bpgrp.as<double>());
+    LAUNCHCHECK("k_bidir_group");
+    src = c->bpgrp.as<double>();
+    nt = groups;
+  }
+  tg::k_bidir_reduce<<<grid1(np, 256), 256, 0, c->stream>>>(src, (int)nt, np, PO);
+  LAUNCHCHECK("k_bidir_reduce");
+  return 0;
+}
+
 // Core launch: targets T (device), outputs O (device), on c->stream.
 static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
   const int64_t nt = T.n, ns = c->ns;
@@ -651,6 +675,7 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->bpe2.release();
   c->bstat.release();
   c->bppart.release();
+  c->bpgrp.release();
   for (auto& b : c->pm) b.release();
   for (auto& b : c->pm_acc) b.release();
   c->pm_stat.release();
@@ -1102,9 +1127,139 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
                                                                   nc, CO);
       LAUNCHCHECK("k_reduce_chunks");
     }
-    tg::k_bidir_reduce<<<grid1(np, 256), 256, 0, c->stream>>>(c->bppart.as<double>(), (int)ntiles,
-                                                               np, PO);
-    LAUNCHCHECK("k_bidir_reduce");
+    if ((rc = reduce_particle_partials(c, ntiles, np, PO))) return rc;
+  }
+  if (kind == TGRAV_HOST) {
+    for (int q = 0; q < 4; ++q) {
+      if (co[q] && nc > 0)
+        HIPCHECK(hipMemcpyAsync(co[q], dco[q], (size_t)nc * 8, hipMemcpyDeviceToHost, c->stream));
+      if (po[q] && np > 0)
+        HIPCHECK(hipMemcpyAsync(po[q], dpo[q], (size_t)np * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int tgrav_bidirectional_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox,
+                               const double* oy, const double* oz, const double* dx,
+                               const double* cm, const double* ceps, int64_t np,
+                               const double* px, const double* py, const double* pz,
+                               const double* pm, const double* pe, double* cax, double* cay,
+                               double* caz, double* cphi, double* pax, double* pay, double* paz,
+                               double* pphi, int mode, int kind) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc;
+  if ((rc = check_mode(mode)) || (rc = check_kind(kind))) return rc;
+  if (c->soft != TGRAV_SOFT_PLUMMER || c->periodic)
+    return fail(TGRAV_EINVAL, "the fused pass uses the Plummer kernel (tgrav_set_softening(ctx, 0, 0, 0))");
+  if (nb != 8) return fail(TGRAV_EINVAL, "cell blocks must be 8^3 (nb = 8), got %d", nb);
+  if (nblk < 0 || np < 0) return fail(TGRAV_EINVAL, "sizes must be >= 0");
+  if (nblk > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many blocks");
+  const int64_t nc = nblk * 512;
+  if (nblk > 0 && (!ox || !oy || !oz || !dx || !cm))
+    return fail(TGRAV_EINVAL, "block corners, widths and cell masses are required");
+  if (np > 0 && (!px || !py || !pz || !pm))
+    return fail(TGRAV_EINVAL, "particle x, y, z and m are required");
+  if ((rc = set_device(c))) return rc;
+  const double* ba[4] = {ox, oy, oz, dx};
+  const double* mc[1] = {cm};
+  const double* be[1] = {ceps};
+  const double* pa[5] = {px, py, pz, pm, pe};
+  double* co[4] = {cax, cay, caz, cphi};
+  double* po[4] = {pax, pay, paz, pphi};
+  if (kind == TGRAV_HOST && c->validate) {
+    bool ok = host_finite(nc, cm) && host_finite(nblk, ceps);
+    for (int q = 0; q < 4; ++q) ok = ok && host_finite(nblk, ba[q]);
+    for (int q = 0; q < 5; ++q) ok = ok && host_finite(np, pa[q]);
+    if (!ok) return fail(TGRAV_ENONFINITE, "non-finite value in an input array");
+  }
+  if (kind == TGRAV_HOST) {
+    if ((rc = stage_in(c, 0, nblk, ba, 4)) || (rc = stage_in(c, 4, nc, mc, 1)) ||
+        (rc = stage_in(c, 10, nblk, be, 1)) || (rc = stage_in(c, 5, np, pa, 5)))
+      return rc;
+    for (int q = 0; q < 4; ++q) {
+      if (co[q] && (rc = c->outb[q].ensure((size_t)std::max<int64_t>(nc, 1) * 8))) return rc;
+      if (po[q] && (rc = c->outb[4 + q].ensure((size_t)std::max<int64_t>(np, 1) * 8))) return rc;
+    }
+  } else if (c->validate) {
+    bool bad = false;
+    if ((rc = device_check_finite(c, nblk, ba, 4, &bad))) return rc;
+    if (!bad && (rc = device_check_finite(c, nc, mc, 1, &bad))) return rc;
+    if (!bad && (rc = device_check_finite(c, nblk, be, 1, &bad))) return rc;
+    if (!bad && (rc = device_check_finite(c, np, pa, 5, &bad))) return rc;
+    if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in an input array");
+  }
+  double* dco[4];
+  double* dpo[4];
+  for (int q = 0; q < 4; ++q) {
+    dco[q] = kind == TGRAV_HOST ? (co[q] ? c->outb[q].as<double>() : nullptr) : co[q];
+    dpo[q] = kind == TGRAV_HOST ? (po[q] ? c->outb[4 + q].as<double>() : nullptr) : po[q];
+  }
+  tg::Out CO{dco[0], dco[1], dco[2], dco[3], nullptr, nullptr, nullptr, 0.0, nullptr, 0, c->G};
+  tg::Out PO{dpo[0], dpo[1], dpo[2], dpo[3], nullptr, nullptr, nullptr, 0.0, nullptr, 0, c->G};
+  if (nc > 0 && np == 0) {
+    tg::k_fill<<<grid1(nc, 256), 256, 0, c->stream>>>(nc, 0.0, dco[0], dco[1], dco[2], dco[3]);
+    LAUNCHCHECK("k_fill");
+  }
+  if (np > 0 && nc == 0) {
+    tg::k_fill<<<grid1(np, 256), 256, 0, c->stream>>>(np, 0.0, dpo[0], dpo[1], dpo[2], dpo[3]);
+    LAUNCHCHECK("k_fill");
+  }
+  if (nc > 0 && np > 0) {
+    if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
+        (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
+        (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
+      return rc;
+    tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->bstat.as<unsigned long long>());
+    LAUNCHCHECK("k_reset_stats");
+    tg::k_pack_sources<<<grid1(np, 256), 256, 0, c->stream>>>(
+        np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
+        c->bstat.as<unsigned long long>(), 0);
+    LAUNCHCHECK("k_pack_sources");
+    // particle chunks over blockIdx.y until there are about 16 waves per CU
+    int64_t want = c->bnsplit[mode];
+    if (want <= 0) {
+      const int64_t target = 16LL * c->num_cu;
+      want = nblk >= target / 2 ? 1 : std::min<int64_t>((target + nblk - 1) / nblk,
+                                                        std::max<int64_t>(1, np / 256));
+    }
+    want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
+    int64_t chunk = (np + want - 1) / want;
+    chunk = (chunk + 63) / 64 * 64;
+    const int nsplit = (int)((np + chunk - 1) / chunk);
+    const double part_bytes = (double)nblk * 4 * np * 8 + (nsplit > 1 ? (double)nsplit * 4 * nc * 8 : 0.0);
+    if (part_bytes > 64.0 * (1ULL << 30))
+      return fail(TGRAV_ELIMIT, "fused pass would need %.1f GiB of partial sums; evaluate the two "
+                  "directions separately", part_bytes / (1ULL << 30));
+    if ((rc = c->bppart.ensure((size_t)nblk * 4 * np * sizeof(double)))) return rc;
+    tg::Out o = CO;
+    if (nsplit > 1) {
+      if ((rc = c->part.ensure((size_t)nsplit * 4 * nc * sizeof(double)))) return rc;
+      o.part = c->part.as<double>();
+      o.pstride = nc;
+    }
+    tg::CellBlocks C{ba[0], ba[1], ba[2], ba[3], mc[0], be[0], nblk};
+    rc = timed_launch(c, nc, np, [&]() -> int {
+      const dim3 g((unsigned)nblk, (unsigned)nsplit);
+      if (mode == TGRAV_MIXED)
+        tg::k_bidir_grid<true><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
+                                                       chunk, c->bstat.as<const double>(), o,
+                                                       c->bppart.as<double>());
+      else
+        tg::k_bidir_grid<false><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
+                                                        chunk, c->bstat.as<const double>(), o,
+                                                        c->bppart.as<double>());
+      LAUNCHCHECK("k_bidir_grid");
+      return 0;
+    });
+    if (rc) return rc;
+    if (nsplit > 1) {
+      tg::k_reduce_chunks<<<grid1(nc, 256), 256, 0, c->stream>>>(c->part.as<double>(), nc, nsplit,
+                                                                  nc, CO);
+      LAUNCHCHECK("k_reduce_chunks");
+    }
+    if ((rc = reduce_particle_partials(c, nblk, np, PO))) return rc;
   }
   if (kind == TGRAV_HOST) {
     for (int q = 0; q < 4; ++q) {

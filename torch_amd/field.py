@@ -274,6 +274,29 @@ class GravityField:
         _lib.check(rc, "tgrav_bidirectional")
         return tuple(co), tuple(po)
 
+    def bidirectional_blocks(self, origin, dx, cm, ceps, px, py, pz, pm, pe, potential=True):
+        """The fused pass with the gas cells as 8^3 leaf blocks: lower corners origin[n, 3], cell
+        widths dx[n], cell masses cm[n*512] (C order), softening ceps[n] per block (None = 0).
+        Returns the same as :meth:`bidirectional`, the cell outputs in the blocks' C order."""
+        nblk = int(origin.shape[0])
+        np_ = _length(px, py, pz, pm)
+        dev = _torch_device(origin, dx, cm, ceps, px, py, pz, pm, pe)
+        A, M, B = _Arrays(nblk, dev), _Arrays(nblk * 512, dev), _Arrays(np_, dev)
+        ba = [A.inp(origin[:, q], "origin") for q in range(3)] + [A.inp(dx, "dx")]
+        m = M.inp(cm, "cm")
+        e = A.inp(ceps, "ceps", allow_none=True)
+        pa = [B.inp(px, "px"), B.inp(py, "py"), B.inp(pz, "pz"), B.inp(pm, "pm"),
+              B.inp(pe, "pe", allow_none=True)]
+        co = [M.out() for _ in range(3)] + [M.out() if potential else None]
+        po = [B.out() for _ in range(3)] + [B.out() if potential else None]
+        self._bind_stream(A)
+        rc = self.lib.tgrav_bidirectional_blocks(self._ctx, nblk, 8, *[A.ptr(a) for a in ba], M.ptr(m),
+                                                 A.ptr(e), np_, *[B.ptr(a) for a in pa],
+                                                 *[M.ptr(o) for o in co], *[B.ptr(o) for o in po],
+                                                 self.mode, A.kind)
+        _lib.check(rc, "tgrav_bidirectional_blocks")
+        return tuple(co), tuple(po)
+
     SOFTENING = {"plummer": 0, "spline": 1, "linear": 2}
 
     def set_softening(self, kernel="plummer", rsoft=0.0, quirks=False):
