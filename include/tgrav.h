@@ -55,7 +55,11 @@ typedef struct tgrav_ctx tgrav_ctx;
 typedef struct tgrav_batch tgrav_batch;
 
 /* One problem of a batched evaluation (device pointers).  eps arrays may be NULL
- * (= 0); any output may be NULL; v* may be NULL unless the batch kicks. */
+ * (= 0); any output may be NULL; v* may be NULL unless the batch kicks.
+ * n_target_blocks > 0 gives the targets as that many 8^3 leaf blocks instead of tx/ty/tz/te:
+ * lower corners (tbx, tby, tbz), cell widths tbd and softening tbe (NULL = 0) per block, and
+ * n_targets must be 512 * n_target_blocks; outputs and velocities are then in the blocks' C
+ * order (as tgrav_eval_blocks), and the plan evaluates them with the column kernel. */
 typedef struct tgrav_problem {
   int64_t n_sources;
   const double *sx, *sy, *sz, *sm, *se;
@@ -63,6 +67,8 @@ typedef struct tgrav_problem {
   const double *tx, *ty, *tz, *te;
   double *ax, *ay, *az, *phi;
   double *vx, *vy, *vz;
+  int64_t n_target_blocks;
+  const double *tbx, *tby, *tbz, *tbd, *tbe;
 } tgrav_problem;
 
 /* ---- context ---------------------------------------------------------- */

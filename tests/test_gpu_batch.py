@@ -160,3 +160,74 @@ def test_c4_batch_mixed_against_fp64():
                                   T["z"][idx], None, G, mode=oracle.NEUMAIER, threads=16)
         s = _util.summary(tuple(a[idx] for a in res["fp64"][i]), ref)
         assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+
+
+def block_batch(seed=41, n=9):
+    """Problems whose targets are 8^3 leaf blocks (incl. an empty-source one), given both ways."""
+    rng = W.rng_for(seed)
+    probs = []
+    for p in range(n):
+        ns = 0 if p == 4 else int(10 ** rng.uniform(0, 4.0))
+        nblk = int(rng.integers(1, 12))
+        o = rng.uniform(-2.0, 1.5, (nblk, 3))
+        s = 10.0 ** rng.uniform(np.log10(0.03125), np.log10(0.5), nblk)
+        tx, ty, tz, _ = W.block_cell_centres(o, s)
+        st = W.star_cluster(rng, max(ns, 1))
+        src = dict(x=st["x"][:ns], y=st["y"][:ns], z=st["z"][:ns], m=st["m"][:ns],
+                   eps=rng.uniform(0.005, 0.1, ns))
+        probs.append(dict(sources=src, targets=dict(x=tx, y=ty, z=tz, block_origin=o, block_dx=s / 8)))
+    return probs
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_block_target_problems(mode):
+    """Block-target problems (column kernel inside the batch) against the same problems with
+    explicit cell centres, against the oracle, over graph replays, with the fused kick."""
+    import torch
+    probs = block_batch()
+    explicit = [dict(sources=p["sources"], targets={k: p["targets"][k] for k in ("x", "y", "z")})
+                for p in probs]
+    dev_b, dev_e = problems_from_workload(probs), problems_from_workload(explicit)
+    rng = W.rng_for(6)
+    vel0 = []
+    for p in dev_b:
+        nt = p["targets"]["x"].numel()
+        v = [torch.as_tensor(rng.normal(size=nt), device="cuda") for _ in range(3)]
+        vel0.append([a.clone() for a in v])
+        p["velocities"] = v
+    dt = 3e-4
+    with GravityField(mode) as f:
+        pe = BatchPlan(f, dev_e)
+        ref = to_np(pe.run(graph=False))
+        pb = BatchPlan(f, dev_b, dt=dt)
+        assert pb.n_items > 0
+        eager = to_np(pb.run(graph=False))
+        torch.cuda.synchronize()
+        for rep in range(3):
+            again = to_np(pb.run(graph=True))
+            torch.cuda.synchronize()
+            for i in range(len(probs)):
+                for q in range(4):
+                    assert np.array_equal(again[i][q], eager[i][q]), (rep, i, q)
+        pe.close()
+        pb.close()
+    for i, p in enumerate(probs):
+        S, T = p["sources"], p["targets"]
+        s = _util.summary(eager[i], ref[i]) if len(S["x"]) else None
+        if len(S["x"]) == 0:
+            assert all(np.all(a == 0) for a in eager[i])
+            continue
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+            o = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"], T["y"], T["z"], None,
+                                    G, mode=oracle.NEUMAIER, threads=16)
+            s2 = _util.summary(eager[i], o)
+            assert s2["acc_max"] <= 1e-12 and s2["pot_max"] <= 1e-12, (i, s2)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (i, s)
+        # 4 runs (1 eager + 3 replays), each kicking by dt with the same acceleration
+        v = [a.cpu().numpy() for a in vel0[i]]
+        for _ in range(4):
+            v = list(oracle.kick(dt, eager[i][0], eager[i][1], eager[i][2], *v))
+        for q in range(3):
+            assert np.array_equal(dev_b[i]["velocities"][q].cpu().numpy(), v[q]), (i, q)

@@ -28,7 +28,9 @@ class Problem(ctypes.Structure):
     _fields_ = [("n_sources", _I64), ("sx", _P), ("sy", _P), ("sz", _P), ("sm", _P), ("se", _P),
                 ("n_targets", _I64), ("tx", _P), ("ty", _P), ("tz", _P), ("te", _P),
                 ("ax", _P), ("ay", _P), ("az", _P), ("phi", _P),
-                ("vx", _P), ("vy", _P), ("vz", _P)]
+                ("vx", _P), ("vy", _P), ("vz", _P),
+                ("n_target_blocks", _I64), ("tbx", _P), ("tby", _P), ("tbz", _P), ("tbd", _P),
+                ("tbe", _P)]
 
 
 SIGNATURES = [

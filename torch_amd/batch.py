@@ -34,7 +34,9 @@ def _dev(a, device):
 
 class BatchPlan:
     """problems: list of dict(sources=dict(x,y,z,m[,eps]), targets=dict(x,y,z[,eps])
-    [, velocities=(vx,vy,vz)]).  field: a GravityField whose context (device, stream,
+    [, velocities=(vx,vy,vz)]).  Targets with block_origin[n, 3] and block_dx[n] (optional
+    block_eps[n]) are 8^3 leaf blocks: the plan evaluates them with the column kernel, outputs
+    in the blocks' C order; their x, y, z are not read.  field: a GravityField whose context (device, stream,
     G, kernel configuration) the plan uses."""
 
     def __init__(self, field: GravityField, problems, dt=0.0, potential=True):
@@ -48,20 +50,31 @@ class BatchPlan:
         for i, p in enumerate(problems):
             S, T = p["sources"], p["targets"]
             s = [_dev(S[k], self.device) for k in ("x", "y", "z", "m")] + [_dev(S.get("eps"), self.device)]
-            t = [_dev(T[k], self.device) for k in ("x", "y", "z")] + [_dev(T.get("eps"), self.device)]
-            nt = int(t[0].numel())
+            blk = T.get("block_origin")
+            if blk is not None:      # 8^3 leaf blocks: corners, cell widths, optional eps per block
+                o = _dev(blk, self.device).reshape(-1, 3)
+                tb = [o[:, 0].contiguous(), o[:, 1].contiguous(), o[:, 2].contiguous(),
+                      _dev(T["block_dx"], self.device), _dev(T.get("block_eps"), self.device)]
+                t = [None, None, None, None]
+                nt = 512 * int(tb[3].numel())
+            else:
+                tb = [None] * 5
+                t = [_dev(T[k], self.device) for k in ("x", "y", "z")] + [_dev(T.get("eps"), self.device)]
+                nt = int(t[0].numel())
             out = [torch.empty(nt, dtype=torch.float64, device=self.device) for _ in range(3)]
             out.append(torch.empty(nt, dtype=torch.float64, device=self.device) if potential else None)
             v = [None, None, None]
             if dt != 0.0 and p.get("velocities") is not None:
                 v = [_dev(q, self.device) for q in p["velocities"]]
-            self.keep += s + t + out + v
+            self.keep += s + t + tb + out + v
             self.outputs.append(tuple(out))
             P = arr[i]
             P.n_sources = int(s[0].numel())
             P.sx, P.sy, P.sz, P.sm, P.se = [(a.data_ptr() if a is not None else None) for a in s]
             P.n_targets = nt
             P.tx, P.ty, P.tz, P.te = [(a.data_ptr() if a is not None else None) for a in t]
+            P.n_target_blocks = nt // 512 if blk is not None else 0
+            P.tbx, P.tby, P.tbz, P.tbd, P.tbe = [(a.data_ptr() if a is not None else None) for a in tb]
             P.ax, P.ay, P.az, P.phi = [(a.data_ptr() if a is not None else None) for a in out]
             P.vx, P.vy, P.vz = [(a.data_ptr() if a is not None else None) for a in v]
         self.velocities = [p.get("velocities") for p in problems] if dt != 0.0 else None

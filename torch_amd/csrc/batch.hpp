@@ -12,6 +12,7 @@
 #include <type_traits>
 
 #include "kernels.hpp"
+#include "grid.hpp"
 
 namespace tg {
 
@@ -30,14 +31,17 @@ struct BProb {
   Out O;                   // final outputs (+ kick)
 };
 
-struct BItem {             // work item: problem, target tile, source chunk
+struct BItem {             // work item: problem, target tile (or 8^3 target block), source chunk
   int prob, tile, chunk, pad;
 };
 
-// counter = 0, per-problem stats = {+inf bits, 0}
+// counters = 0 (generic and block-target work lists), per-problem stats = {+inf bits, 0}
 __global__ void k_batch_reset(int nprob, unsigned int* counter, unsigned long long* stats) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *counter = 0u;
+  if (i == 0) {
+    counter[0] = 0u;
+    counter[1] = 0u;
+  }
   if (i < nprob) {
     stats[2 * i] = 0x7ff0000000000000ULL;
     stats[2 * i + 1] = 0ULL;
@@ -127,6 +131,41 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
       process_mixed<K, NT>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
     else
       process_fp64<K, NT>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
+  }
+}
+
+// Problems whose targets are 8^3 leaf blocks: the same persistent scheme with 64-lane
+// workgroups and one target block per work item, evaluated by the column code of k_eval_grid.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict__ probs,
+                                                        const BItem* __restrict__ items,
+                                                        int n_items, unsigned int* counter,
+                                                        const double4* __restrict__ s4,
+                                                        const double* __restrict__ se2,
+                                                        const unsigned long long* __restrict__ stats,
+                                                        double* __restrict__ part) {
+  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
+  __shared__ Lds L;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();                    // all lanes hold `it` before lane 0 overwrites it
+    if (it >= n_items) break;           // every lane of the workgroup leaves together
+    const BItem w = items[it];
+    const BProb& P = probs[w.prob];
+    const int64_t sb = P.src_off + (int64_t)w.chunk * P.chunk;
+    const int64_t se = min(sb + P.chunk, P.src_off + P.ns);
+    const double e2s_min = __longlong_as_double((long long)stats[2 * w.prob]);
+    Out o = P.O;
+    if (P.nchunk != 1) {
+      o.part = part + P.part_off + (int64_t)w.chunk * 4 * P.T.n;
+      o.pstride = P.T.n;
+    } else {
+      o.part = nullptr;
+    }
+    grid_block<MIXED>(L, P.T, w.tile, s4, se2, sb, se, e2s_min, o);
   }
 }
 

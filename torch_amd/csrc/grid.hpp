@@ -194,19 +194,16 @@ struct GridF32Run {
     default: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                      \
   }
 
-// blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
-// workgroup of 64 lanes.
+// One leaf block b of block-structured targets against sources [sb, se), by a 64-lane
+// workgroup (lane = cell column).  `o` already points at this source chunk's partials if the
+// sources are split.  Shared by k_eval_grid and the persistent batch kernel.
 template <bool MIXED>
-__global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restrict__ src4,
-                                                  const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* e2s_min_p, Out O) {
-  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
-  __shared__ Lds L;
+__device__ __forceinline__ void grid_block(
+    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t b,
+    const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
+    double e2s_min, const Out& o) {
   const int lane = threadIdx.x;
-  const int64_t b = blockIdx.x;
   const int i = lane >> 3, jj = lane & 7;
-  const int64_t sb = (int64_t)blockIdx.y * chunk;
-  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   const double d = T.bdx[b];
   const double e = T.beps ? T.beps[b] : 0.0;
   const double e2t = e * e;
@@ -218,9 +215,7 @@ __global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restri
 #pragma unroll
     for (int k = 0; k < 8; ++k) z[k] = T.oz[b] + ((double)k + 0.5) * d;
   }
-  const int fl = tile_flags(*e2s_min_p, e2t, e2t, O, MIXED);
-  Out o = O;
-  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  const int fl = tile_flags(e2s_min, e2t, e2t, o, MIXED);
   const int64_t t0 = b * 512 + (int64_t)lane * 8;
   if constexpr (MIXED) {
     // anchor: the block centre (the bounding box centre of its cell centres)
@@ -244,6 +239,21 @@ __global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restri
 #pragma unroll
     for (int k = 0; k < 8; ++k) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
   }
+}
+
+// blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
+// workgroup of 64 lanes.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restrict__ src4,
+                                                  const double* __restrict__ srce2, int64_t ns,
+                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
+  __shared__ Lds L;
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  grid_block<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, *e2s_min_p, o);
 }
 
 // ---------------------------------------------------------------- block sources ----

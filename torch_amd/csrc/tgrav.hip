@@ -1570,10 +1570,11 @@ int tgrav_set_batch_config(tgrav_ctx* c, double item_interactions, int wg_per_cu
 struct tgrav_batch {
   tgrav_ctx* c = nullptr;
   int mode = 0, nprob = 0, n_items = 0, nred = 0, grid = 0;
+  int n_gitems = 0, ggrid = 0;   // block-target work list (column kernel)
   const Variant* v = nullptr;
   int64_t total_src = 0, red_total = 0;
   double inter = 0.0;
-  DevBuf probs, items, srcs, sprefix, s4, se2, stats, part, counter, ids, rprefix;
+  DevBuf probs, items, gitems, srcs, sprefix, s4, se2, stats, part, counter, ids, rprefix;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1584,7 +1585,7 @@ static void batch_free(tgrav_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
-  DevBuf* bufs[] = {&b->probs, &b->items, &b->srcs, &b->sprefix, &b->s4, &b->se2,
+  DevBuf* bufs[] = {&b->probs, &b->items, &b->gitems, &b->srcs, &b->sprefix, &b->s4, &b->se2,
                     &b->stats, &b->part, &b->counter, &b->ids, &b->rprefix};
   for (DevBuf* d : bufs) d->release();
   delete b;
@@ -1620,6 +1621,24 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
     int rc = timed ? timed_launch_on(b->c, s, b->n_items, b->total_src, go) : go();
     if (rc) return rc;
   }
+  if (b->n_gitems > 0) {
+    auto go = [&]() -> int {
+      if (b->mode == TGRAV_MIXED)
+        tg::k_batch_eval_grid<true><<<b->ggrid, 64, 0, s>>>(
+            b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems,
+            b->counter.as<unsigned int>() + 1, b->s4.as<double4>(), b->se2.as<double>(),
+            b->stats.as<unsigned long long>(), b->part.as<double>());
+      else
+        tg::k_batch_eval_grid<false><<<b->ggrid, 64, 0, s>>>(
+            b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems,
+            b->counter.as<unsigned int>() + 1, b->s4.as<double4>(), b->se2.as<double>(),
+            b->stats.as<unsigned long long>(), b->part.as<double>());
+      LAUNCHCHECK("k_batch_eval_grid");
+      return 0;
+    };
+    int rc = timed ? timed_launch_on(b->c, s, b->n_gitems, b->total_src, go) : go();
+    if (rc) return rc;
+  }
   if (b->nred > 0 && b->red_total > 0) {
     tg::k_batch_reduce<<<grid1(b->red_total, 256), 256, 0, s>>>(
         b->nred, b->ids.as<int>(), b->rprefix.as<int64_t>(), b->probs.as<tg::BProb>(),
@@ -1648,8 +1667,16 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       return fail(TGRAV_EINVAL, "problem %d: negative size", p);
     if (q.n_sources > 0 && (!q.sx || !q.sy || !q.sz || !q.sm))
       return fail(TGRAV_EINVAL, "problem %d: source x, y, z and m are required", p);
-    if (q.n_targets > 0 && (!q.tx || !q.ty || !q.tz))
+    if (q.n_target_blocks < 0) return fail(TGRAV_EINVAL, "problem %d: negative block count", p);
+    if (q.n_target_blocks > 0) {
+      if (q.n_targets != 512 * q.n_target_blocks)
+        return fail(TGRAV_EINVAL, "problem %d: n_targets must be 512 * n_target_blocks", p);
+      if (!q.tbx || !q.tby || !q.tbz || !q.tbd)
+        return fail(TGRAV_EINVAL, "problem %d: block corners and widths are required", p);
+      if (q.n_target_blocks > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "problem %d: too many blocks", p);
+    } else if (q.n_targets > 0 && (!q.tx || !q.ty || !q.tz)) {
       return fail(TGRAV_EINVAL, "problem %d: target x, y and z are required", p);
+    }
     if (kick && q.n_targets > 0 && (!q.vx || !q.vy || !q.vz))
       return fail(TGRAV_EINVAL, "problem %d: velocities are required when dt != 0", p);
   }
@@ -1669,8 +1696,8 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   std::vector<tg::BProb> probs(n);
   std::vector<tg::BSrc> srcs(n);
   std::vector<int64_t> sprefix(n + 1, 0);
-  std::vector<tg::BItem> items;
-  std::vector<double> cost;
+  std::vector<tg::BItem> items, gitems;
+  std::vector<double> cost, gcost;
   std::vector<int> ids;
   std::vector<int64_t> rprefix(1, 0);
   int64_t part_total = 0;
@@ -1678,7 +1705,14 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const tgrav_problem& q = pr[p];
     tg::BProb& P = probs[p];
     memset(&P, 0, sizeof P);
-    P.T.x = q.tx; P.T.y = q.ty; P.T.z = q.tz; P.T.eps = q.te; P.T.n = q.n_targets;
+    const bool blk = q.n_target_blocks > 0;
+    if (blk) {
+      P.T.blocks = 1; P.T.nb = 8;
+      P.T.ox = q.tbx; P.T.oy = q.tby; P.T.oz = q.tbz; P.T.bdx = q.tbd; P.T.beps = q.tbe;
+    } else {
+      P.T.x = q.tx; P.T.y = q.ty; P.T.z = q.tz; P.T.eps = q.te;
+    }
+    P.T.n = q.n_targets;
     P.O = tg::Out{q.ax, q.ay, q.az, q.phi, kick ? q.vx : nullptr, kick ? q.vy : nullptr,
                   kick ? q.vz : nullptr, dt, nullptr, 0, c->G, q.phi == nullptr};
     P.src_off = sprefix[p];
@@ -1686,7 +1720,9 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     srcs[p] = tg::BSrc{q.sx, q.sy, q.sz, q.sm, q.se};
     sprefix[p + 1] = sprefix[p] + q.n_sources;
     if (q.n_targets == 0) { P.nchunk = 1; continue; }
-    const int64_t tiles = (q.n_targets + T - 1) / T;
+    const int64_t TT = blk ? 512 : T;            // targets per work item
+    const int64_t SG = blk ? 64 : NT;            // source-chunk granularity (LDS tile)
+    const int64_t tiles = (q.n_targets + TT - 1) / TT;
     if (tiles > 0x7fffffffLL) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
     if (q.n_sources == 0) {                    // zero field: written by the reduction
       P.nchunk = 0;
@@ -1694,10 +1730,10 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       rprefix.push_back(rprefix.back() + q.n_targets);
       continue;
     }
-    const double per_tile = (double)T * q.n_sources;
+    const double per_tile = (double)TT * q.n_sources;
     int64_t nch = per_tile > C ? (int64_t)ceil(per_tile / C) : 1;
     int64_t ch = (q.n_sources + nch - 1) / nch;
-    ch = (ch + NT - 1) / NT * NT;
+    ch = (ch + SG - 1) / SG * SG;
     nch = (q.n_sources + ch - 1) / ch;
     P.chunk = ch;
     P.nchunk = (int)nch;
@@ -1709,19 +1745,27 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     }
     for (int64_t t = 0; t < tiles; ++t)
       for (int64_t k = 0; k < nch; ++k) {
-        items.push_back(tg::BItem{p, (int)t, (int)k, 0});
-        const double nt_t = (double)std::min(T, q.n_targets - t * T);
+        const double nt_t = (double)std::min(TT, q.n_targets - t * TT);
         const double ns_k = (double)std::min(ch, q.n_sources - k * ch);
-        cost.push_back(nt_t * ns_k);
+        (blk ? gitems : items).push_back(tg::BItem{p, (int)t, (int)k, 0});
+        (blk ? gcost : cost).push_back(nt_t * ns_k);
       }
   }
-  if (items.size() > 0x7fffffffULL) { batch_free(b); return fail(TGRAV_ELIMIT, "too many work items"); }
-  std::vector<size_t> order(items.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t z) { return cost[a] > cost[z]; });
-  std::vector<tg::BItem> sorted(items.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  if (items.size() > 0x7fffffffULL || gitems.size() > 0x7fffffffULL) {
+    batch_free(b);
+    return fail(TGRAV_ELIMIT, "too many work items");
+  }
+  auto by_cost = [](const std::vector<tg::BItem>& it, const std::vector<double>& cs) {
+    std::vector<size_t> order(it.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t z) { return cs[a] > cs[z]; });
+    std::vector<tg::BItem> out(it.size());
+    for (size_t i = 0; i < order.size(); ++i) out[i] = it[order[i]];
+    return out;
+  };
+  const std::vector<tg::BItem> sorted = by_cost(items, cost), gsorted = by_cost(gitems, gcost);
   b->n_items = (int)sorted.size();
+  b->n_gitems = (int)gsorted.size();
   b->nred = (int)ids.size();
   b->red_total = rprefix.back();
   b->total_src = sprefix[n];
@@ -1731,15 +1775,21 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     occ = 1;
   const int per_cu = c->batch_wg_per_cu > 0 ? std::min(c->batch_wg_per_cu, occ) : occ;
   b->grid = std::max(1, std::min(b->n_items, per_cu * c->num_cu));
+  int gocc = 0;
+  const void* gk = mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_grid<true>
+                                       : (const void*)tg::k_batch_eval_grid<false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&gocc, gk, 64, 0) != hipSuccess || gocc < 1) gocc = 1;
+  b->ggrid = std::max(1, std::min(b->n_gitems, gocc * c->num_cu));
 
   if ((rc = upload(b->probs, probs)) || (rc = upload(b->items, sorted)) ||
+      (rc = upload(b->gitems, gsorted)) ||
       (rc = upload(b->srcs, srcs)) || (rc = upload(b->sprefix, sprefix)) ||
       (rc = upload(b->ids, ids)) || (rc = upload(b->rprefix, rprefix)) ||
       (rc = b->s4.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double4))) ||
       (rc = b->se2.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double))) ||
       (rc = b->stats.ensure(std::max(n, 1) * 2 * sizeof(unsigned long long))) ||
       (rc = b->part.ensure(std::max<int64_t>(part_total, 1) * sizeof(double))) ||
-      (rc = b->counter.ensure(sizeof(unsigned int)))) {
+      (rc = b->counter.ensure(2 * sizeof(unsigned int)))) {
     batch_free(b);
     return rc;
   }
@@ -1789,7 +1839,7 @@ int tgrav_batch_destroy(tgrav_batch* b) {
 
 int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions) {
   if (!b) return fail(TGRAV_EINVAL, "batch is NULL");
-  if (n_items) *n_items = b->n_items;
+  if (n_items) *n_items = (int64_t)b->n_items + b->n_gitems;
   if (interactions) *interactions = b->inter;
   return 0;
 }

@@ -311,7 +311,8 @@ def c4(seed=1004, n_problems=64):
         o = rng.uniform(-2.0, 1.5, (nblk, 3))
         s = 10.0 ** rng.uniform(np.log10(0.03125), np.log10(0.5), nblk)
         tx, ty, tz, _ = block_cell_centres(o, s)
-        probs.append(dict(sources=src, targets=dict(x=tx, y=ty, z=tz, eps=np.zeros(nt))))
+        probs.append(dict(sources=src, targets=dict(x=tx, y=ty, z=tz, eps=np.zeros(nt), block_origin=o,
+                                                    block_dx=s / 8)))
     return probs
 
 
