@@ -75,8 +75,15 @@ typedef struct tgrav_problem {
 int tgrav_version(void);
 const char* tgrav_last_error(void);
 /* Replaces the worker start-up of the AMUSE plugin (interface.F90 initialize_code) for
- * this path: binds a device, creates a default-priority stream. */
-int tgrav_create(tgrav_ctx** out, int device);
+ * this path: binds a device, creates a default-priority stream.  tune_cache_path (may be
+ * NULL) names the autotuner's JSON cache (torch_amd/tune_cache.json, written by
+ * python -m torch_amd.autotune): evaluations, column-kernel launches and batch plans then
+ * take the tuned launch shape for their device and size bucket, unless tgrav_set_config /
+ * tgrav_set_batch_config set one explicitly. */
+int tgrav_create(tgrav_ctx** out, int device, const char* tune_cache_path);
+/* Number of entries in a tune cache file, or TGRAV_EINVAL if it cannot be read or parsed
+ * (needs no device). */
+int tgrav_tune_cache_entries(const char* path);
 int tgrav_destroy(tgrav_ctx* ctx);
 /* Run on an existing HIP stream of the same HIP runtime (e.g.
  * torch.cuda.current_stream().cuda_stream); NULL is HIP's null stream. */

@@ -7,6 +7,7 @@
   order (interface.py:221-243, 548-559).
 """
 import ctypes
+import json
 import inspect
 import os
 import re
@@ -60,7 +61,7 @@ def test_version_and_variants():
 
 def test_null_context_and_arguments_are_rejected():
     lib = _lib.load()
-    assert lib.tgrav_create(None, 0) == _lib.EINVAL
+    assert lib.tgrav_create(None, 0, None) == _lib.EINVAL
     assert b"out is NULL" in lib.tgrav_last_error()
     assert lib.tgrav_set_sources(None, 1, None, None, None, None, None, 0) == _lib.EINVAL
     assert lib.tgrav_eval(None, 1, None, None, None, None, None, None, None, None, 0, 0) == _lib.EINVAL
@@ -75,7 +76,7 @@ def test_create_without_a_device_fails_cleanly():
     """Here (no GPU) tgrav_create must return an error code, not crash."""
     lib = _lib.load()
     ctx = ctypes.c_void_p()
-    rc = lib.tgrav_create(ctypes.byref(ctx), 0)
+    rc = lib.tgrav_create(ctypes.byref(ctx), 0, None)
     if rc == 0:
         lib.tgrav_destroy(ctx)
         pytest.skip("a GPU is present")
@@ -92,3 +93,22 @@ def test_python_mirror_keeps_reference_method_names():
                       (GasGravityField, "get_potential_at_point")):
         params = list(inspect.signature(getattr(cls, meth)).parameters)
         assert params == ["self", "eps", "x", "y", "z"], (cls, meth, params)
+
+
+def test_tune_cache_parser(tmp_path):
+    """The library's own reader of the autotuner cache (used by tgrav_create) accepts the
+    committed cache and rejects malformed files; no device needed."""
+    lib = _lib.load()
+    cache = os.path.join(os.path.dirname(HEADER), "..", "torch_amd", "tune_cache.json")
+    with open(cache) as fh:
+        n = len(json.load(fh))
+    assert lib.tgrav_tune_cache_entries(os.fsencode(cache)) == n > 0
+    good = tmp_path / "t.json"
+    good.write_text('{"a|eval|mixed|s1|t2": {"variant": 3, "nsplit": 8, "ms": 1.5, "note": "x",'
+                    ' "list": [1, {"y": 2}]}, "b": {}}')
+    assert lib.tgrav_tune_cache_entries(os.fsencode(str(good))) == 2
+    for bad in ('{"a": 3}', '{"a": {"variant": }}', '[1, 2]', '{"a": {"variant": 1}', ''):
+        f = tmp_path / "bad.json"
+        f.write_text(bad)
+        assert lib.tgrav_tune_cache_entries(os.fsencode(str(f))) == _lib.EINVAL, bad
+    assert lib.tgrav_tune_cache_entries(os.fsencode(str(tmp_path / "missing.json"))) == _lib.EINVAL

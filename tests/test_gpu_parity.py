@@ -469,3 +469,29 @@ def test_block_sources_errors():
         assert lib.tgrav_set_source_blocks(g.handle, -1, 8, None, None, None, None, None, None, 0) == _lib.EINVAL
         with pytest.raises(TgravError):
             g.set_blocks(np.zeros((1, 3)), np.array([0.1]), np.full(512, np.nan))
+
+
+def test_library_applies_the_tune_cache(c2):
+    """tgrav_create with the autotuner's cache: the block forward launch takes the cached source
+    split (bit-identical to setting it explicitly), which differs from the built-in heuristic's;
+    an explicit tgrav_set_config still wins over the cache.  FP64, where the split changes the
+    order of the FP64 sums (the mixed kernel's FP32 tile sums add exactly in FP64, so its results
+    do not depend on the split at all)."""
+    import json
+    cache = os.path.join(HERE, "..", "torch_amd", "tune_cache.json")
+    tuned = [v["nsplit"] for k, v in json.load(open(cache)).items() if "|blockfwd|fp64|s13|b11" in k][0]
+    st, ce = c2["stars"], c2["cells"]
+    bo, bd = ce["block_origin"], ce["block_size"] / 8
+    res = []
+    for tune, split in ((cache, None), (None, tuned), (None, None), (cache, 3)):
+        with GravityField("fp64", tune_cache=tune) as f:
+            if split is not None:
+                f.set_config(nsplit=split)
+            f.set_sources(st["x"], st["y"], st["z"], st["m"], st["eps"])
+            res.append(f.evaluate_blocks(bo, bd))
+    for q in range(4):
+        np.testing.assert_array_equal(res[0][q], res[1][q])          # cache == explicit tuned split
+        np.testing.assert_array_equal(res[2][q], res[3][q])          # explicit 3 == heuristic (3)
+    assert any(not np.array_equal(res[0][q], res[2][q]) for q in range(4))
+    with pytest.raises(TgravError):
+        GravityField("fp64", tune_cache=os.path.join(HERE, "conftest.py"))

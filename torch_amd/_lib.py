@@ -36,7 +36,8 @@ class Problem(ctypes.Structure):
 SIGNATURES = [
     ("tgrav_version", _I32, []),
     ("tgrav_last_error", ctypes.c_char_p, []),
-    ("tgrav_create", _I32, [ctypes.POINTER(_P), _I32]),
+    ("tgrav_create", _I32, [ctypes.POINTER(_P), _I32, ctypes.c_char_p]),
+    ("tgrav_tune_cache_entries", _I32, [ctypes.c_char_p]),
     ("tgrav_destroy", _I32, [_P]),
     ("tgrav_set_stream", _I32, [_P, _P]),
     ("tgrav_use_own_stream", _I32, [_P]),

@@ -13,6 +13,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "../../include/tgrav.h"
@@ -154,11 +156,125 @@ const Variant* variant_of(int mode, int v) {
   return nullptr;
 }
 
+// ------------------------------------------------------------- tune cache ------
+// The autotuner's JSON cache (torch_amd/autotune.py writes it): a flat object whose keys are
+// "<arch>/<CUs>cu|<kind>|<mode>|<size buckets>" and whose values are objects of numbers
+// (variant, nsplit, item, wg_per_cu, ...).  Size buckets are round(log2(n)).
+struct TuneEntry {
+  int variant = -1, nsplit = -1, wg_per_cu = -1;
+  double item = -1.0;
+};
+
+struct JsonIn {
+  const char* p;
+  const char* e;
+  void ws() {
+    while (p < e && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) ++p;
+  }
+  bool eat(char ch) {
+    ws();
+    if (p < e && *p == ch) { ++p; return true; }
+    return false;
+  }
+  bool str(std::string& out) {
+    ws();
+    if (p >= e || *p != '"') return false;
+    ++p;
+    out.clear();
+    while (p < e && *p != '"') {
+      if (*p == '\\' && p + 1 < e) ++p;
+      out.push_back(*p++);
+    }
+    if (p >= e) return false;
+    ++p;
+    return true;
+  }
+  bool num(double& v) {
+    ws();
+    char* end = nullptr;
+    v = strtod(p, &end);
+    if (end == p || end > e) return false;
+    p = end;
+    return true;
+  }
+  bool skip_value() {   // a number, string, literal or nested object/array
+    ws();
+    if (p >= e) return false;
+    if (*p == '"') { std::string t; return str(t); }
+    if (*p == '{' || *p == '[') {
+      int depth = 0;
+      bool in_str = false;
+      for (; p < e; ++p) {
+        if (in_str) { if (*p == '\\') ++p; else if (*p == '"') in_str = false; continue; }
+        if (*p == '"') in_str = true;
+        else if (*p == '{' || *p == '[') ++depth;
+        else if (*p == '}' || *p == ']') { if (--depth == 0) { ++p; return true; } }
+      }
+      return false;
+    }
+    const char* start = p;   // true / false / null
+    while (p < e && *p != ',' && *p != '}' && *p != ']' && *p != ' ' && *p != '\n') ++p;
+    return p > start;
+  }
+};
+
+// Parses the cache text; returns the number of entries or -1 on malformed input.
+static int parse_tune_cache(const std::string& text, std::map<std::string, TuneEntry>* out) {
+  JsonIn in{text.data(), text.data() + text.size()};
+  if (!in.eat('{')) return -1;
+  int n = 0;
+  if (in.eat('}')) return 0;
+  do {
+    std::string key;
+    if (!in.str(key) || !in.eat(':') || !in.eat('{')) return -1;
+    TuneEntry t;
+    if (!in.eat('}')) {
+      do {
+        std::string f;
+        if (!in.str(f) || !in.eat(':')) return -1;
+        in.ws();
+        if (in.p < in.e && (*in.p == '-' || (*in.p >= '0' && *in.p <= '9'))) {
+          double v;
+          if (!in.num(v)) return -1;
+          if (f == "variant") t.variant = (int)v;
+          else if (f == "nsplit") t.nsplit = (int)v;
+          else if (f == "wg_per_cu") t.wg_per_cu = (int)v;
+          else if (f == "item") t.item = v;
+        } else if (!in.skip_value()) {
+          return -1;
+        }
+      } while (in.eat(','));
+      if (!in.eat('}')) return -1;
+    }
+    if (out) (*out)[key] = t;
+    ++n;
+  } while (in.eat(','));
+  if (!in.eat('}')) return -1;
+  return n;
+}
+
+static int read_file(const char* path, std::string* text) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return -1;
+  char buf[65536];
+  size_t k;
+  text->clear();
+  while ((k = fread(buf, 1, sizeof buf, f)) > 0) text->append(buf, k);
+  fclose(f);
+  return 0;
+}
+
+static int size_bucket(double n) { return (int)lround(log2(std::max(n, 1.0))); }
+
 }  // namespace
 
 struct tgrav_ctx {
   int device = 0;
   int num_cu = 256;
+  std::string devkey;                          // "<arch>/<CUs>cu", the tune-cache key prefix
+  std::map<std::string, TuneEntry> tune;       // loaded tune cache (may be empty)
+  bool user_cfg[2] = {false, false};           // tgrav_set_config was called: it wins
+  bool user_batch = false;                     // tgrav_set_batch_config was called
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   double G = 4.300917270e-3;
@@ -195,6 +311,24 @@ struct tgrav_ctx {
   std::vector<int64_t> ev_nt, ev_ns;
   int ev_used = 0;
 };
+
+static const TuneEntry* tune_lookup(const tgrav_ctx* c, const std::string& key) {
+  if (c->tune.empty()) return nullptr;
+  auto it = c->tune.find(key);
+  return it == c->tune.end() ? nullptr : &it->second;
+}
+
+static const char* mode_name(int mode) { return mode == TGRAV_MIXED ? "mixed" : "fp64"; }
+
+// Tuned source split of a column kernel ("blockfwd" / "blocksrc"), or -1.
+static int tuned_block_split(const tgrav_ctx* c, int mode, const char* dir, int64_t ns, int64_t nblk) {
+  if (c->user_cfg[mode] || c->tune.empty()) return -1;
+  char key[256];
+  snprintf(key, sizeof key, "%s|%s|%s|s%d|b%d", c->devkey.c_str(), dir, mode_name(mode),
+           size_bucket((double)ns), size_bucket((double)nblk));
+  const TuneEntry* t = tune_lookup(c, key);
+  return t ? t->nsplit : -1;
+}
 
 static const int kTimingSlots = 256;
 
@@ -289,10 +423,10 @@ static int device_check_finite(tgrav_ctx* c, int64_t n, const double* const* chk
 // Launch shape: the variant's target tile and a source split that gives about
 // 4 workgroups per CU when the target set alone cannot fill the chip.
 static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t nt, int64_t ns,
-                         int64_t* ntiles, int* nsplit, int64_t* chunk) {
+                         int64_t* ntiles, int* nsplit, int64_t* chunk, int force = -1) {
   const int64_t tile = (int64_t)v->k * v->nt;
   *ntiles = (nt + tile - 1) / tile;
-  int want = c->nsplit[mode];
+  int want = force >= 0 ? force : c->nsplit[mode];
   if (want <= 0) {
     const int64_t target_wg = 4LL * c->num_cu;
     if (*ntiles >= target_wg / 2) want = 1;
@@ -318,6 +452,8 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   const int64_t nt = T.n, ns = c->ns, nblk = nt / 512;
   if (nblk > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many blocks");
   int want = c->nsplit[mode];
+  const int tuned = tuned_block_split(c, mode, "blockfwd", ns, nblk);
+  if (tuned > 0) want = tuned;
   if (want <= 0) {
     const int64_t target = 16LL * c->num_cu;
     want = nblk >= target / 2 ? 1
@@ -366,6 +502,8 @@ static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
   const int64_t ntiles = (nt + K * NT - 1) / (K * NT);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
   int want = c->nsplit[mode];
+  const int tuned = tuned_block_split(c, mode, "blocksrc", c->ns, nblk);
+  if (tuned > 0) want = tuned;
   if (want <= 0) {
     const int64_t target = 4LL * c->num_cu;
     want = ntiles >= target / 2 ? 1 : (int)std::min<int64_t>((target + ntiles - 1) / ntiles, nblk);
@@ -592,9 +730,20 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   if (T.blocks && T.nb == 8 && c->block_kernel) return launch_grid(c, mode, T, O);
   if (!T.blocks && c->src_nblk > 0 && c->block_kernel) return launch_gridsrc(c, mode, T, O);
   const Variant* v = variant_of(mode, c->variant[mode]);
+  int force_split = -1;
+  if (!c->user_cfg[mode] && !c->tune.empty()) {
+    char key[256];
+    snprintf(key, sizeof key, "%s|eval|%s|s%d|t%d", c->devkey.c_str(), mode_name(mode),
+             size_bucket((double)ns), size_bucket((double)nt));
+    const TuneEntry* t = tune_lookup(c, key);
+    if (t && t->variant >= 0 && variant_of(mode, t->variant)) {
+      v = variant_of(mode, t->variant);
+      force_split = t->nsplit;
+    }
+  }
   int64_t ntiles, chunk;
   int nsplit;
-  choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk);
+  choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk, force_split);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
   tg::Out o = O;
   if (nsplit > 1) {
@@ -629,9 +778,26 @@ int tgrav_version(void) { return TGRAV_VERSION; }
 
 const char* tgrav_last_error(void) { return g_err; }
 
-int tgrav_create(tgrav_ctx** out, int device) {
+int tgrav_tune_cache_entries(const char* path) {
+  if (!path) return fail(TGRAV_EINVAL, "path is NULL");
+  std::string text;
+  if (read_file(path, &text)) return fail(TGRAV_EINVAL, "cannot read %s", path);
+  const int n = parse_tune_cache(text, nullptr);
+  if (n < 0) return fail(TGRAV_EINVAL, "%s is not a tune cache", path);
+  return n;
+}
+
+int tgrav_create(tgrav_ctx** out, int device, const char* tune_cache_path) {
   if (!out) return fail(TGRAV_EINVAL, "tgrav_create: out is NULL");
   *out = nullptr;
+  std::map<std::string, TuneEntry> tune;
+  if (tune_cache_path) {
+    std::string text;
+    if (read_file(tune_cache_path, &text))
+      return fail(TGRAV_EINVAL, "tgrav_create: cannot read tune cache %s", tune_cache_path);
+    if (parse_tune_cache(text, &tune) < 0)
+      return fail(TGRAV_EINVAL, "tgrav_create: %s is not a tune cache", tune_cache_path);
+  }
   int ndev = 0;
   HIPCHECK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev)
@@ -641,8 +807,12 @@ int tgrav_create(tgrav_ctx** out, int device) {
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) {
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
       c->num_cu = prop.multiProcessorCount;
+      std::string arch(prop.gcnArchName);
+      arch = arch.substr(0, arch.find(':'));
+      c->devkey = arch + "/" + std::to_string(c->num_cu) + "cu";
+    }
     e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
   }
   if (e == hipSuccess)
@@ -653,6 +823,7 @@ int tgrav_create(tgrav_ctx** out, int device) {
     return fail(TGRAV_EHIP, "tgrav_create: %s", hipGetErrorString(e));
   }
   c->stream = c->own;
+  c->tune.swap(tune);
   *out = c;
   return 0;
 }
@@ -746,6 +917,7 @@ int tgrav_set_config(tgrav_ctx* c, int mode, int variant, int nsplit) {
     c->variant[mode] = variant;
   }
   if (nsplit >= 0) c->nsplit[mode] = nsplit;
+  c->user_cfg[mode] = true;
   return 0;
 }
 
@@ -1562,6 +1734,7 @@ int tgrav_set_batch_config(tgrav_ctx* c, double item_interactions, int wg_per_cu
     return fail(TGRAV_EINVAL, "item size and workgroups per CU must be >= 0");
   c->batch_item = item_interactions;
   c->batch_wg_per_cu = wg_per_cu;
+  c->user_batch = true;
   return 0;
 }
 
@@ -1686,13 +1859,33 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   b->mode = mode;
   b->nprob = n;
   b->v = variant_of(mode, c->variant[mode]);
-  const int64_t T = (int64_t)b->v->k * b->v->nt, NT = b->v->nt;
-
   double total = 0.0;
   for (int p = 0; p < n; ++p) total += (double)pr[p].n_sources * (double)pr[p].n_targets;
   b->inter = total;
-  double C = c->batch_item > 0 ? c->batch_item
-                               : std::max((double)T * NT * 4.0, total / (c->num_cu * 8.0));
+  double item = c->batch_item;
+  int wg_per_cu = c->batch_wg_per_cu;
+  if (!c->tune.empty() && n > 0) {   // the autotuner's shape for batches like this one
+    std::vector<double> vs(n), vt(n);
+    for (int p = 0; p < n; ++p) { vs[p] = (double)pr[p].n_sources; vt[p] = (double)pr[p].n_targets; }
+    auto median = [](std::vector<double> v) {
+      std::sort(v.begin(), v.end());
+      const size_t m = v.size() / 2;
+      return v.size() % 2 ? v[m] : 0.5 * (v[m - 1] + v[m]);
+    };
+    char key[256];
+    snprintf(key, sizeof key, "%s|batch|%s|p%d|i%d|s%d|t%d", c->devkey.c_str(), mode_name(mode), n,
+             size_bucket(total), size_bucket(median(vs)), size_bucket(median(vt)));
+    const TuneEntry* t = tune_lookup(c, key);
+    if (t) {
+      if (!c->user_cfg[mode] && t->variant >= 0 && variant_of(mode, t->variant)) b->v = variant_of(mode, t->variant);
+      if (!c->user_batch) {
+        if (t->item >= 0) item = t->item;
+        if (t->wg_per_cu >= 0) wg_per_cu = t->wg_per_cu;
+      }
+    }
+  }
+  const int64_t T = (int64_t)b->v->k * b->v->nt, NT = b->v->nt;
+  double C = item > 0 ? item : std::max((double)T * NT * 4.0, total / (c->num_cu * 8.0));
   std::vector<tg::BProb> probs(n);
   std::vector<tg::BSrc> srcs(n);
   std::vector<int64_t> sprefix(n + 1, 0);
@@ -1773,7 +1966,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   int occ = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->v->bkernel(), b->v->nt, 0) != hipSuccess || occ < 1)
     occ = 1;
-  const int per_cu = c->batch_wg_per_cu > 0 ? std::min(c->batch_wg_per_cu, occ) : occ;
+  const int per_cu = wg_per_cu > 0 ? std::min(wg_per_cu, occ) : occ;
   b->grid = std::max(1, std::min(b->n_items, per_cu * c->num_cu));
   int gocc = 0;
   const void* gk = mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_grid<true>

@@ -27,6 +27,7 @@ available too (:meth:`GravityField.set_softening`, :meth:`GravityField.set_perio
 Units are those of the inputs; the default G = 4.300917270e-3 pc (km/s)^2 / Msun.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -117,14 +118,18 @@ class GravityField:
     compensated sums, FP64 result and kick).
     """
 
-    def __init__(self, mode="fp64", device=0, G=G_PC_MSUN_KMS, validate=True):
+    def __init__(self, mode="fp64", device=0, G=G_PC_MSUN_KMS, validate=True, tune_cache=None):
+        """tune_cache: path of an autotuner cache the library applies by itself (launch shapes
+        per device and size bucket); None leaves the built-in heuristics (the Python-side
+        ``autotune.apply_*`` helpers configure a field explicitly instead)."""
         if mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
         self.lib = _lib.load()
         self.mode = MODES[mode]
         self.device = device
         ctx = ctypes.c_void_p()
-        _lib.check(self.lib.tgrav_create(ctypes.byref(ctx), int(device)), "tgrav_create")
+        path = None if tune_cache is None else os.fsencode(tune_cache)
+        _lib.check(self.lib.tgrav_create(ctypes.byref(ctx), int(device), path), "tgrav_create")
         self._ctx = ctx
         self.G = G
         _lib.check(self.lib.tgrav_set_constant(ctx, float(G)), "tgrav_set_constant")
