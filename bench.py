@@ -52,6 +52,13 @@ PEAK_FP32_UNPACKED_TFLOPS = 78.6   # one FP32 FMA per lane per clock (packed FP3
 BYTES_PER_TARGET = 64              # x, y, z, eps in + ax, ay, az, phi out (FP64)
 BYTES_PER_SOURCE = 40              # x, y, z, m, eps (FP64), read once per launch
 
+# HBM bytes per launch of the c2 forward block kernel k_eval_grid, by (mode, source split): rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py --steps 2 --warmup 1 --no-cpu` (medians over
+# the dispatches), FETCH_SIZE divided by 0.5 and WRITE_SIZE by 1.0, the factors those counters read
+# for a plain 512 MiB device copy.  Reported only when the bench runs the launch shape measured.
+C2_FWD_TRAFFIC = {("mixed", 8): (3881 / 0.5 + 300373 / 1.0) * 1024,
+                  ("fp64", 6): (3124 / 0.5 + 284458 / 1.0) * 1024}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -589,8 +596,11 @@ def main():
     line["ms_per_step"] = results[head]["ms"]
     achieved = FLOP_PER_INTERACTION * kern_inter / (kern_ms * 1e-3) / 1e12
     peak = PEAK_FP32_TFLOPS if head == "mixed" else PEAK_FP64_TFLOPS
+    traffic = None
+    if args.workload == "c2" and results[head]["cfg"][0]:
+        traffic = C2_FWD_TRAFFIC.get((head, results[head]["cfg"][0]["nsplit"]))
     line["roofline"] = {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": peak,
-                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                        "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                         "kernel_ms": kern_ms, "flop_per_interaction": FLOP_PER_INTERACTION,
                         "interactions_per_launch": kern_inter,
                         "algorithmic_bytes_per_launch": kern_bytes,

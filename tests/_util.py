@@ -63,3 +63,45 @@ def within_mixed_bound(got, exp, s, m, se, t, te, G):
     err_p = np.abs(gp - ep)
     ratio = max(float(np.max(err_a / (ab + 1e-300))), float(np.max(err_p / (pb + 1e-300))))
     return ratio <= 1.0, ratio
+
+
+def term_scale_bounds(s, m, se2, t, te2, G, mode, anchor, fold=512):
+    """Per-target error bounds, condition-aware (relative to the sum of |terms|, so cancelling
+    fields are judged fairly), for property tests on small inputs.
+
+    s[ns, 3], m[ns], se2[ns] (source eps^2), t[nt, 3], te2[nt]; anchor: per-target (nt) or
+    per-pair (nt, ns) bound on the distance from the target to the FP64 anchor its mixed
+    kernel subtracts (any point of the target tile's bounding box, or of its leaf block).
+    fp64: 1e-12 of sum_j G|m_j| |d_ij| / r_ij^3 (acceleration) and of sum_j G|m_j| / r_ij
+    (potential).  mixed: the first-order bound of _util.mixed_error_bound with the anchor
+    distance given, the FP32 partial sums over `fold` terms, and 1% slack for second order
+    (inputs keep |d| >= 1/64, so the displacement error is below 3e-5 |d|).
+    Returns (acc_bound[nt], pot_bound[nt])."""
+    d = s[None, :, :] - t[:, None, :]
+    dd = (d ** 2).sum(-1)
+    r2 = dd + se2[None, :] + te2[:, None]
+    ok = r2 > 0
+    r = np.sqrt(np.where(ok, r2, 1.0))
+    dn = np.sqrt(dd)
+    gm = G * np.abs(m)[None, :]
+    if mode == "fp64":
+        acc = np.where(ok, gm * dn / r ** 3, 0).sum(1)
+        pot = np.where(ok, gm / r, 0).sum(1)
+        return 1e-12 * acc + 1e-300, 1e-12 * pot + 1e-300
+    u = 2.0 ** -24
+    A = anchor if np.ndim(anchor) == 2 else np.asarray(anchor)[:, None]
+    delta = u * (2.0 * dn + 2.0 * A)
+    acc = np.where(ok, gm * (2.0 * delta / r ** 3 + (8 + fold) * u * dn / r ** 3), 0).sum(1)
+    pot = np.where(ok, gm * (delta / r ** 2 + (8 + fold) * u / r), 0).sum(1)
+    return 1.01 * acc + 1e-300, 1.01 * pot + 1e-300
+
+
+def within_bounds(got, ref, bounds, potential=True):
+    """True if |a - a_ref| <= acc bound and |phi - phi_ref| <= pot bound for every target;
+    also returns the worst ratio."""
+    ab, pb = bounds
+    ea = np.linalg.norm(np.stack(got[:3], 1) - np.stack(ref[:3], 1), axis=1)
+    worst = float(np.max(ea / ab, initial=0.0))
+    if potential:
+        worst = max(worst, float(np.max(np.abs(got[3] - ref[3]) / pb, initial=0.0)))
+    return worst <= 1.0, worst
