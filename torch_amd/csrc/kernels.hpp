@@ -500,6 +500,47 @@ __global__ void k_reduce_chunks(const double* __restrict__ part, int64_t pstride
   store_result(o, t, s0, s1, s2, s3);
 }
 
+// The same reduction for few targets and many chunks (k_reduce_chunks would run a handful of
+// workgroups, each thread walking every chunk): a workgroup of 16 waves takes 64 targets, wave
+// w sums chunks [w*per, (w+1)*per) in order, and wave 0 adds the 16 wave sums in wave order,
+// so the result does not depend on scheduling.  Grid: ceil(n / 64) workgroups of 1024 lanes.
+__global__ __launch_bounds__(1024) void k_reduce_chunks_wide(const double* __restrict__ part,
+                                                             int64_t pstride, int nsplit, int64_t n,
+                                                             Out O) {
+  __shared__ double red[16][4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t t = (int64_t)blockIdx.x * 64 + lane;
+  const int per = (nsplit + 15) / 16;
+  const int c0 = w * per, c1 = min(nsplit, c0 + per);
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (t < n) {
+    for (int c = c0; c < c1; ++c) {
+      const double* p = part + (int64_t)c * 4 * pstride;
+      s0 += p[t];
+      s1 += p[pstride + t];
+      s2 += p[2 * pstride + t];
+      s3 += p[3 * pstride + t];
+    }
+  }
+  red[w][0][lane] = s0;
+  red[w][1][lane] = s1;
+  red[w][2][lane] = s2;
+  red[w][3][lane] = s3;
+  __syncthreads();
+  if (w == 0 && t < n) {
+    double a0 = red[0][0][lane], a1 = red[0][1][lane], a2 = red[0][2][lane], a3 = red[0][3][lane];
+    for (int q = 1; q < 16; ++q) {
+      a0 += red[q][0][lane];
+      a1 += red[q][1][lane];
+      a2 += red[q][2][lane];
+      a3 += red[q][3][lane];
+    }
+    Out o = O;
+    o.part = nullptr;
+    store_result(o, t, a0, a1, a2, a3);
+  }
+}
+
 // Pack SoA FP64 sources into {x,y,z,m} + eps^2 and fold the smallest eps^2 into
 // stat[0] (as the bits of a non-negative double, whose unsigned-integer order is the
 // numeric order, so one vector atomicMin per workgroup suffices).  With `check`,

@@ -397,6 +397,21 @@ static int stage_in(tgrav_ctx* c, int first, int64_t n, const double** arrs, int
 
 static unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Sums nsplit chunk partials (4 x pstride doubles each) into the outputs of n targets.  Few
+// targets with many chunks take the wide kernel (16 waves per 64 targets); otherwise one
+// thread per target walks the chunks.
+static int reduce_chunks(tgrav_ctx* c, const double* part, int64_t pstride, int nsplit, int64_t n,
+                         const tg::Out& fin) {
+  if (n < 256 * 1024 && nsplit >= 16) {
+    tg::k_reduce_chunks_wide<<<grid1(n, 64), 1024, 0, c->stream>>>(part, pstride, nsplit, n, fin);
+    LAUNCHCHECK("k_reduce_chunks_wide");
+  } else {
+    tg::k_reduce_chunks<<<grid1(n, 256), 256, 0, c->stream>>>(part, pstride, nsplit, n, fin);
+    LAUNCHCHECK("k_reduce_chunks");
+  }
+  return 0;
+}
+
 // Reads a device flag word written by k_pack_sources / k_check_finite (synchronous).
 static int read_flag(tgrav_ctx* c, const void* dflag, bool* bad) {
   HIPCHECK(hipMemcpyAsync(c->host_flag, dflag, sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -487,9 +502,8 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   if (nsplit > 1) {
     tg::Out fin = O;
     fin.part = nullptr;
-    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
-                                                                nt, fin);
-    LAUNCHCHECK("k_reduce_chunks");
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
+    if (rc1) return rc1;
   }
   return 0;
 }
@@ -537,9 +551,8 @@ static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
   if (nsplit > 1) {
     tg::Out fin = O;
     fin.part = nullptr;
-    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
-                                                                nt, fin);
-    LAUNCHCHECK("k_reduce_chunks");
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
+    if (rc1) return rc1;
   }
   return 0;
 }
@@ -595,9 +608,8 @@ static int launch_flash_gridsrc(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O
   if (nsplit > 1) {
     tg::Out fin = O;
     fin.part = nullptr;
-    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
-                                                                nt, fin);
-    LAUNCHCHECK("k_reduce_chunks");
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
+    if (rc1) return rc1;
   }
   return 0;
 }
@@ -676,9 +688,8 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   if (nsplit > 1) {
     tg::Out fin = O;
     fin.part = nullptr;
-    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
-                                                                nt, fin);
-    LAUNCHCHECK("k_reduce_chunks");
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
+    if (rc1) return rc1;
   }
   return 0;
 }
@@ -764,9 +775,8 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   if (nsplit > 1) {
     tg::Out fin = O;
     fin.part = nullptr;
-    tg::k_reduce_chunks<<<grid1(nt, 256), 256, 0, c->stream>>>(c->part.as<double>(), nt, nsplit,
-                                                                nt, fin);
-    LAUNCHCHECK("k_reduce_chunks");
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
+    if (rc1) return rc1;
   }
   return 0;
 }
@@ -1295,9 +1305,7 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     });
     if (rc) return rc;
     if (nsplit > 1) {
-      tg::k_reduce_chunks<<<grid1(nc, 256), 256, 0, c->stream>>>(c->part.as<double>(), nc, nsplit,
-                                                                  nc, CO);
-      LAUNCHCHECK("k_reduce_chunks");
+      if ((rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CO))) return rc;
     }
     if ((rc = reduce_particle_partials(c, ntiles, np, PO))) return rc;
   }
@@ -1427,9 +1435,7 @@ int tgrav_bidirectional_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double*
     });
     if (rc) return rc;
     if (nsplit > 1) {
-      tg::k_reduce_chunks<<<grid1(nc, 256), 256, 0, c->stream>>>(c->part.as<double>(), nc, nsplit,
-                                                                  nc, CO);
-      LAUNCHCHECK("k_reduce_chunks");
+      if ((rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CO))) return rc;
     }
     if ((rc = reduce_particle_partials(c, nblk, np, PO))) return rc;
   }
