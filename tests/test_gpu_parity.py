@@ -246,6 +246,7 @@ def test_every_variant_agrees(mode, c2):
                               T["z"][:1024], None, G, mode=oracle.NEUMAIER, threads=THREADS)
     lib = _lib.load()
     with GravityField(mode) as f:
+        f.set_block_kernel(False)      # the cells are in block C order: keep the generic kernels
         for v in range(lib.tgrav_num_variants(_lib.PLAIN_FP64 if mode == "fp64" else _lib.MIXED)):
             f.set_config(variant=v)
             got = gpu_eval(f, st, T)
@@ -307,15 +308,17 @@ def test_fused_kick_equals_eval_then_kick(mode, c2):
 def test_block_targets_equal_explicit_centres(mode, c2):
     """Block-structured targets: with the generic kernel the centres rebuilt on the device
     give the explicit-centre results bit for bit; the 8^3 column kernel agrees with them to
-    the parity bar (its r^2 is summed as (dx^2 + dy^2 + eps^2) + dz^2)."""
+    the parity bar (its r^2 is summed as (dx^2 + dy^2 + eps^2) + dz^2).  Explicit centres in
+    the blocks' C order take the column kernel too (detected on the device) and then equal
+    the block form bit for bit."""
     st, ce = c2["stars"], c2["cells"]
     o, size = ce["block_origin"][:300], ce["block_size"][:300]
     dx = size / 8
     bx, by, bz = oracle.block_centres(o[:, 0], o[:, 1], o[:, 2], dx)
     with StellarGravityField(mode) as f:
         f.set_stars(st["x"], st["y"], st["z"], st["m"], st["eps"])
-        exp = f.evaluate(np.repeat(dx, 512), bx, by, bz)
         f.set_block_kernel(False)
+        exp = f.evaluate(np.repeat(dx, 512), bx, by, bz)
         blk = f.evaluate_blocks(o, dx, eps=dx)
         for p, q in zip(blk, exp):
             assert np.array_equal(p, q)
@@ -327,10 +330,62 @@ def test_block_targets_equal_explicit_centres(mode, c2):
             assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
         else:
             assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+        cols = f.evaluate(np.repeat(dx, 512), bx, by, bz)
+        for p, q in zip(cols, col):
+            assert np.array_equal(p, q)
         v = [np.zeros(len(bx)) for _ in range(3)]
         f.evaluate_blocks(o, dx, eps=dx, dt=0.25, vel=v)
         for p, q in zip(v, col[:3]):
             assert np.array_equal(p, 0.25 * q)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_explicit_column_detection(mode, c2):
+    """Explicit targets take the column kernel exactly when every aligned group of 8 shares x
+    and y and every aligned group of 512 shares eps (checked on the device); otherwise the
+    generic kernel runs, bit-identical to the generic kernel forced on.  A ragged tail (not a
+    multiple of 512) and the fused kick take the same path."""
+    st, ce = c2["stars"], c2["cells"]
+    o, dx = ce["block_origin"][:40], ce["block_size"][:40] / 8
+    bx, by, bz = oracle.block_centres(o[:, 0], o[:, 1], o[:, 2], dx)
+    e = np.repeat(dx, 512)
+    n = 39 * 512 + 203                                       # ragged: last group partial
+    cases = {"structured": (e, bx, by, bz)}
+    px = bx.copy()
+    px[5 * 512 + 8 * 3 + 5] += 1e-9                          # one target leaves its column
+    cases["perturbed_x"] = (e, px, by, bz)
+    pe = e.copy()
+    pe[7 * 512 + 100] *= 1.5                                 # eps varies inside a group of 512
+    cases["perturbed_eps"] = (pe, bx, by, bz)
+    with StellarGravityField(mode) as f:
+        f.set_stars(st["x"], st["y"], st["z"], st["m"], st["eps"])
+        for name, (ee, x, y, z) in cases.items():
+            for m in (len(x), n):
+                args = (ee[:m], x[:m], y[:m], z[:m])
+                got = f.evaluate(*args)
+                f.set_block_kernel(False)
+                gen = f.evaluate(*args)
+                f.set_block_kernel(True)
+                if name != "structured":
+                    for p, q in zip(got, gen):
+                        assert np.array_equal(p, q), name
+                    continue
+                s = _util.summary(got, gen)
+                print(mode, name, m, "columns vs generic", s, flush=True)
+                if mode == "fp64":
+                    assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12
+                else:
+                    assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4
+                blk = f.evaluate_blocks(o, dx, eps=dx)
+                for p, q in zip(got, blk):
+                    if m == len(x) or mode == "fp64":     # the mixed anchor of a partial group differs
+                        assert np.array_equal(p, q[:m])
+                dt = 0.37
+                v = [np.zeros(m) for _ in range(3)]
+                kicked = f.evaluate_and_kick(*args, dt, *v)
+                for p, q, w in zip(kicked[:3], got[:3], v):
+                    assert np.array_equal(p, q)
+                    assert np.array_equal(w, q * dt)
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])

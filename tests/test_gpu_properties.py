@@ -191,10 +191,13 @@ def test_block_targets(fields, mode, src, blk, hit, column, device):
             return host(f.evaluate_blocks(B[0], B[1], eps=B[2]))
 
         got = run()
+        # the same centres as explicit targets (block C order): the same kernel, bit for bit
+        te = None if e is None else np.repeat(e, 512)
+        expl = host(f.evaluate(*as_device([te, t[:, 0], t[:, 1], t[:, 2]], device)))
+        same(expl, got)
         moved = run(shift=1.0, scale=4.0)
     finally:
         f.set_block_kernel(True)
-    te = None if e is None else np.repeat(e, 512)
     ref = oracle.eval_plummer(s[:, 0], s[:, 1], s[:, 2], m, se, t[:, 0], t[:, 1], t[:, 2], te, G,
                               mode=oracle.NEUMAIER)
     check(mode, got, ref, s, m, sq(se, len(s)), t, sq(te, len(t)), np.full(len(t), diag(t) + 2 * dx.max()))

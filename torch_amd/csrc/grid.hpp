@@ -194,32 +194,18 @@ struct GridF32Run {
     default: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                      \
   }
 
-// One leaf block b of block-structured targets against sources [sb, se), by a 64-lane
-// workgroup (lane = cell column).  `o` already points at this source chunk's partials if the
-// sources are split.  Shared by k_eval_grid and the persistent batch kernel.
+// One wave of column targets against sources [sb, se): lane l owns 8 targets t0 + k that share
+// x, y and eps^2 (e2t, uniform over the wave) and differ in z[k]; (cx, cy, cz) is the FP64 anchor
+// of the mixed form (the centre of the wave's bounding box); targets t0 + k with k >= nk are not
+// stored.  `o` already points at this source chunk's partials if the sources are split.
 template <bool MIXED>
-__device__ __forceinline__ void grid_block(
-    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t b,
+__device__ __forceinline__ void grid_columns(
+    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, double x, double y,
+    const double (&z)[8], double e2t, double cx, double cy, double cz, int64_t t0, int nk,
     const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
     double e2s_min, const Out& o) {
-  const int lane = threadIdx.x;
-  const int i = lane >> 3, jj = lane & 7;
-  const double d = T.bdx[b];
-  const double e = T.beps ? T.beps[b] : 0.0;
-  const double e2t = e * e;
-  double x, y, z[8];
-  {
-#pragma clang fp contract(off)
-    x = T.ox[b] + ((double)i + 0.5) * d;
-    y = T.oy[b] + ((double)jj + 0.5) * d;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) z[k] = T.oz[b] + ((double)k + 0.5) * d;
-  }
   const int fl = tile_flags(e2s_min, e2t, e2t, o, MIXED);
-  const int64_t t0 = b * 512 + (int64_t)lane * 8;
   if constexpr (MIXED) {
-    // anchor: the block centre (the bounding box centre of its cell centres)
-    const double cx = T.ox[b] + 4.0 * d, cy = T.oy[b] + 4.0 * d, cz = T.oz[b] + 4.0 * d;
     const float tx = (float)(x - cx), ty = (float)(y - cy);
     float tz[8];
 #pragma unroll
@@ -230,15 +216,47 @@ __device__ __forceinline__ void grid_block(
     TG_DISPATCH_GRID(GridF32Run::run, fl & ~F_SEED32, L, src4, srce2, sb, se, e2t, cx, cy, cz, tx,
                      ty, tz, hx, hy, hz, hp)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
+    for (int k = 0; k < 8; ++k)
+      if (k < nk) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
   } else {
     double ax[8], ay[8], az[8], ap[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
     TG_DISPATCH_GRID(GridF64Run::run, fl, L, src4, srce2, sb, se, e2t, x, y, z, ax, ay, az, ap)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
+    for (int k = 0; k < 8; ++k)
+      if (k < nk) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
   }
+}
+
+// One leaf block b of block-structured targets against sources [sb, se), by a 64-lane
+// workgroup (lane = cell column (i, j) = (l / 8, l % 8)).  Shared by k_eval_grid and the
+// persistent batch kernel.  The mixed anchor is the centre of the bounding box of the block's
+// cell centres, (c_0 + c_7) / 2 per axis, the same value the explicit-target form below
+// computes from the same centres.
+template <bool MIXED>
+__device__ __forceinline__ void grid_block(
+    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t b,
+    const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
+    double e2s_min, const Out& o) {
+  const int lane = threadIdx.x;
+  const int i = lane >> 3, jj = lane & 7;
+  const double d = T.bdx[b];
+  const double e = T.beps ? T.beps[b] : 0.0;
+  double x, y, z[8], cx, cy, cz;
+  {
+#pragma clang fp contract(off)
+    const double ox = T.ox[b], oy = T.oy[b], oz = T.oz[b];
+    x = ox + ((double)i + 0.5) * d;
+    y = oy + ((double)jj + 0.5) * d;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) z[k] = oz + ((double)k + 0.5) * d;
+    cx = 0.5 * ((ox + 0.5 * d) + (ox + 7.5 * d));
+    cy = 0.5 * ((oy + 0.5 * d) + (oy + 7.5 * d));
+    cz = 0.5 * ((oz + 0.5 * d) + (oz + 7.5 * d));
+  }
+  grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, b * 512 + (int64_t)lane * 8, 8, src4, srce2,
+                      sb, se, e2s_min, o);
 }
 
 // blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
@@ -254,6 +272,57 @@ __global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restri
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
   grid_block<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, *e2s_min_p, o);
+}
+
+// Explicit targets with column structure (every aligned group of 8 consecutive targets shares
+// x and y, every aligned group of 512 shares eps: the C order of 8^3 leaf-block meshgrids, as
+// checked by k_check_cols): the column code of k_eval_grid on explicit coordinates.
+// blockIdx.x = group of 512 targets (one wave, lane l owns targets 512 x + 8 l ... + 7),
+// blockIdx.y = source chunk.  Gated (Out::gate) against the generic kernel on the check's flag.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restrict__ src4,
+                                                  const double* __restrict__ srce2, int64_t ns,
+                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
+  __shared__ Lds L;
+  if (gated_off(O)) return;
+  const int lane = threadIdx.x;
+  const int64_t t0 = (int64_t)blockIdx.x * 512 + (int64_t)lane * 8;
+  const int64_t last = T.n - 1;
+  const int64_t tc = t0 < last ? t0 : last;
+  const double x = T.x[tc], y = T.y[tc];
+  double z[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) z[k] = T.z[t0 + k < last ? t0 + k : last];
+  const int64_t te = (int64_t)blockIdx.x * 512 < last ? (int64_t)blockIdx.x * 512 : last;
+  const double e = T.eps ? T.eps[te] : 0.0;
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if constexpr (MIXED) {   // centre of the bounding box of the wave's targets
+    double mn[3] = {x, y, z[0]}, mx[3] = {x, y, z[0]};
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      mn[2] = fmin(mn[2], z[k]);
+      mx[2] = fmax(mx[2], z[k]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        mn[v] = fmin(mn[v], __shfl_xor(mn[v], off));
+        mx[v] = fmax(mx[v], __shfl_xor(mx[v], off));
+      }
+    }
+    cx = 0.5 * (mn[0] + mx[0]);
+    cy = 0.5 * (mn[1] + mx[1]);
+    cz = 0.5 * (mn[2] + mx[2]);
+  }
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  const int64_t left = T.n - t0;
+  grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
+                      sb, se, *e2s_min_p, o);
 }
 
 // ---------------------------------------------------------------- block sources ----

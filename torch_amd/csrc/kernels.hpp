@@ -34,7 +34,12 @@ struct Out {
   int64_t pstride;
   double G;
   int nopot;                    // 1: the potential is not wanted (its accumulator is skipped)
+  const int* gate;              // non-null: the kernel runs only if *gate == gate_on, else it
+  int gate_on;                  //   returns at once (one of two launches picked on the device)
 };
+
+// Uniform across the grid: every workgroup of a gated-off launch returns before any barrier.
+__device__ __forceinline__ bool gated_off(const Out& O) { return O.gate && *O.gate != O.gate_on; }
 
 // Cell centre of block-structured target t: o + (idx + 1/2) dx with the multiply and the
 // add rounded separately (no FMA contraction) so the centres are bit-identical to the
@@ -458,6 +463,7 @@ __global__ __launch_bounds__(NT) void k_eval_fp64(Tgt T, const double4* __restri
                                                   const double* __restrict__ srce2, int64_t ns,
                                                   int64_t chunk, const double* e2s_min_p, Out O) {
   __shared__ LdsF64<NT> L;
+  if (gated_off(O)) return;
   const double e2s_min = *e2s_min_p;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t sb = (int64_t)blockIdx.y * chunk;
@@ -472,6 +478,7 @@ __global__ __launch_bounds__(NT) void k_eval_mixed(Tgt T, const double4* __restr
                                                    const double* __restrict__ srce2, int64_t ns,
                                                    int64_t chunk, const double* e2s_min_p, Out O) {
   __shared__ LdsF32<NT> L;
+  if (gated_off(O)) return;
   const double e2s_min = *e2s_min_p;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t sb = (int64_t)blockIdx.y * chunk;
@@ -485,6 +492,7 @@ __global__ __launch_bounds__(NT) void k_eval_mixed(Tgt T, const double4* __restr
 // the stores and the fused kick.
 __global__ void k_reduce_chunks(const double* __restrict__ part, int64_t pstride, int nsplit,
                                 int64_t n, Out O) {
+  if (gated_off(O)) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -508,6 +516,7 @@ __global__ __launch_bounds__(1024) void k_reduce_chunks_wide(const double* __res
                                                              int64_t pstride, int nsplit, int64_t n,
                                                              Out O) {
   __shared__ double red[16][4][64];
+  if (gated_off(O)) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t t = (int64_t)blockIdx.x * 64 + lane;
   const int per = (nsplit + 15) / 16;
@@ -612,6 +621,28 @@ __global__ void k_kick(int64_t n, double dt, const double* __restrict__ ax,
     vy[i] = vy[i] + dt * ay[i];
     vz[i] = vz[i] + dt * az[i];
   }
+}
+
+// flag[0] = v (one lane; a kernel rather than a memset, so a captured graph replays it).
+__global__ void k_set_flag(int* flag, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = v;
+}
+
+// Column structure of explicit targets, the C order of 8^3 leaf blocks
+// (.deprecated/bridge_multiples.py:566-575 flattens each block's meshgrid this way): every
+// aligned group of 8 consecutive targets shares x and y, and every aligned group of 512 shares
+// eps.  Any violation clears flag[0] (set to 1 before); all writers store the same value.
+__global__ void k_check_cols(Tgt T, int* flag) {
+  const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+  if (t >= T.n) return;
+  const double x0 = T.x[t], y0 = T.y[t];
+  const double e0 = T.eps ? T.eps[t & ~(int64_t)511] : 0.0;
+  bool ok = true;
+  for (int k = 0; k < 8; ++k) {
+    if (t + k >= T.n) break;
+    ok = ok && T.x[t + k] == x0 && T.y[t + k] == y0 && (!T.eps || T.eps[t + k] == e0);
+  }
+  if (!ok) flag[0] = 0;
 }
 
 __global__ void k_fill(int64_t n, double v, double* a, double* b, double* c, double* d) {

@@ -302,6 +302,7 @@ struct tgrav_ctx {
   DevBuf sbd, sbe2;
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
   DevBuf in[12], outb[8], vel[3], part, flag;
+  DevBuf colflag;            // explicit targets: 1 if they have 8^3 column structure
   DevBuf bp4, bpe2, bstat, bppart, bpgrp;   // fused passes: packed particles, partials
   DevBuf pm[16], pm_acc[3], pm_stat;  // particle-mesh pieces: staging, fixed-point sums, stat
   unsigned long long* host_flag = nullptr;   // pinned
@@ -463,9 +464,8 @@ static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t
 
 // 8^3 block targets: one 64-lane workgroup per leaf block (k_eval_grid), sources split over
 // blockIdx.y until there are about 16 waves per CU.
-static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
-  const int64_t nt = T.n, ns = c->ns, nblk = nt / 512;
-  if (nblk > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many blocks");
+static void grid_shape(const tgrav_ctx* c, int mode, int64_t ns, int64_t nblk, int* nsplit,
+                       int64_t* chunk) {
   int want = c->nsplit[mode];
   const int tuned = tuned_block_split(c, mode, "blockfwd", ns, nblk);
   if (tuned > 0) want = tuned;
@@ -476,9 +476,18 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
                                                        std::max<int64_t>(1, ns / 256));
   }
   if (want > 65535) want = 65535;
-  int64_t chunk = (ns + want - 1) / want;
-  chunk = (chunk + 63) / 64 * 64;
-  const int nsplit = (int)((ns + chunk - 1) / chunk);
+  int64_t ch = (ns + want - 1) / want;
+  ch = (ch + 63) / 64 * 64;
+  *chunk = ch;
+  *nsplit = (int)((ns + ch - 1) / ch);
+}
+
+static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
+  const int64_t nt = T.n, ns = c->ns, nblk = nt / 512;
+  if (nblk > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many blocks");
+  int nsplit;
+  int64_t chunk;
+  grid_shape(c, mode, ns, nblk, &nsplit, &chunk);
   tg::Out o = O;
   if (nsplit > 1) {
     int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
@@ -756,24 +765,64 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   int nsplit;
   choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk, force_split);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
-  tg::Out o = O;
-  if (nsplit > 1) {
-    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+  // Explicit targets in the C order of 8^3 leaf-block meshgrids (the reference's call shape,
+  // .deprecated/bridge_multiples.py:566-575) run the column kernel.  Whether they have that
+  // structure is decided on the device (k_check_cols sets a flag); both kernels are launched
+  // and the flag lets exactly one of them (and its chunk reduction) run, so nothing waits for
+  // the host.
+  const bool cols = c->block_kernel && !T.blocks && nt >= 512;
+  const int64_t ngrp = (nt + 511) / 512;
+  int nsplit_c = 1;
+  int64_t chunk_c = 0;
+  if (cols) {
+    if (ngrp > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
+    grid_shape(c, mode, ns, ngrp, &nsplit_c, &chunk_c);
+  }
+  const int nsp = std::max(nsplit, cols ? nsplit_c : 1);
+  if (nsp > 1) {
+    int rc0 = c->part.ensure((size_t)nsp * 4 * nt * sizeof(double));
     if (rc0) return rc0;
-    o.part = c->part.as<double>();
-    o.pstride = nt;
-  } else {
-    o.part = nullptr;
+  }
+  tg::Out o = O, oc = O;
+  o.part = nsplit > 1 ? c->part.as<double>() : nullptr;
+  oc.part = nsplit_c > 1 ? c->part.as<double>() : nullptr;
+  o.pstride = oc.pstride = nt;
+  if (cols) {
+    int rc0 = c->colflag.ensure(sizeof(int));
+    if (rc0) return rc0;
+    int* flag = c->colflag.as<int>();
+    tg::k_set_flag<<<1, 64, 0, c->stream>>>(flag, 1);
+    LAUNCHCHECK("k_set_flag");
+    tg::k_check_cols<<<grid1(ngrp * 64, 256), 256, 0, c->stream>>>(T, flag);
+    LAUNCHCHECK("k_check_cols");
+    o.gate = oc.gate = flag;
+    o.gate_on = 0;
+    oc.gate_on = 1;
   }
   int rc = timed_launch(c, nt, ns, [&]() -> int {
+    if (cols) {
+      if (mode == TGRAV_MIXED)
+        tg::k_eval_cols<true><<<dim3((unsigned)ngrp, (unsigned)nsplit_c), 64, 0, c->stream>>>(
+            T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk_c, c->stat.as<const double>(), oc);
+      else
+        tg::k_eval_cols<false><<<dim3((unsigned)ngrp, (unsigned)nsplit_c), 64, 0, c->stream>>>(
+            T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk_c, c->stat.as<const double>(), oc);
+      LAUNCHCHECK("k_eval_cols");
+    }
     v->fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v->nt), c->stream, T, c->s4.as<double4>(),
           c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
     LAUNCHCHECK(mode == TGRAV_MIXED ? "k_eval_mixed" : "k_eval_fp64");
     return 0;
   });
   if (rc) return rc;
+  if (cols && nsplit_c > 1) {
+    tg::Out fin = oc;
+    fin.part = nullptr;
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit_c, nt, fin);
+    if (rc1) return rc1;
+  }
   if (nsplit > 1) {
-    tg::Out fin = O;
+    tg::Out fin = o;
     fin.part = nullptr;
     int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
     if (rc1) return rc1;
@@ -850,6 +899,7 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->part.release();
   c->stat.release();
   c->flag.release();
+  c->colflag.release();
   c->sbd.release();
   c->sbe2.release();
   c->bp4.release();
