@@ -561,7 +561,11 @@ def main():
                                            "ms_per_step": results[m]["explicit"]["ms"],
                                            "fwd_kernel_ms": results[m]["explicit"]["fwd_kernel_ms"],
                                            "rev_kernel_ms": results[m]["explicit"]["rev_kernel_ms"],
-                                           "launch_shapes": results[m]["explicit"]["cfg"]}
+                                           "generic_launch_shapes": results[m]["explicit"]["cfg"],
+                                           "kernels": "the cells are in the blocks' C order, so the library "
+                                                      "detects the 8^3 structure on the device and both "
+                                                      "directions run the column kernels; the generic "
+                                                      "shapes apply to unstructured points"}
                                        for m in results}
         if args.workload == "c2fused":
             kernel_name = f"k_bidir_grid<{'mixed' if head == 'mixed' else 'fp64'}> (both directions, 8^3 cell blocks)"

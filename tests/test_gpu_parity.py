@@ -476,21 +476,25 @@ def test_reference_method_names_give_the_oracle_field(c1):
 def test_block_sources(mode, c2):
     """Cells given as 8^3 source blocks (tgrav_set_source_blocks): with the generic kernel
     they reproduce the explicit-cell results bit for bit; the column kernel agrees to the
-    parity bar and with the oracle."""
+    parity bar and with the oracle.  Explicit cells in the blocks' C order are detected as
+    blocks and take the same column kernel, bit for bit."""
     st, ce = c2["stars"], c2["cells"]
     nb = 400
     sl = slice(0, nb * 512)
     o, dxb = ce["block_origin"][:nb], ce["block_size"][:nb] / 8
     with GasGravityField(mode) as g:
         g.set_sources(ce["x"][sl], ce["y"][sl], ce["z"][sl], ce["m"][sl], ce["eps"][sl])
+        detected = g.evaluate(None, st["x"], st["y"], st["z"])
+        g.set_block_kernel(False)
         exp = g.evaluate(None, st["x"], st["y"], st["z"])
         g.set_blocks(o, dxb, ce["m"][sl])
-        g.set_block_kernel(False)
         gen = g.evaluate(None, st["x"], st["y"], st["z"])
         for p, q in zip(gen, exp):
             assert np.array_equal(p, q)
         g.set_block_kernel(True)
         col = g.evaluate(None, st["x"], st["y"], st["z"])
+        for p, q in zip(detected, col):
+            assert np.array_equal(p, q)
         for ns in (1, 7):
             g.set_config(nsplit=ns)
             again = g.evaluate(None, st["x"], st["y"], st["z"])
@@ -515,6 +519,42 @@ def test_block_sources(mode, c2):
                                 mode=oracle.NEUMAIER, threads=THREADS)
     s = _util.summary(few, ref37)
     assert s["acc_max"] <= (1e-12 if mode == "fp64" else 1e-4), s
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_explicit_block_source_detection(mode, c2):
+    """Explicit sources are taken as 8^3 blocks exactly when every aligned group of 512 is a
+    C-ordered tensor-product block (x from i only, y from j only, z from k only) with one eps;
+    otherwise the generic kernel runs, bit-identical to the generic kernel forced on.  With
+    explicit block-ordered targets as well (cells onto cells), the block-source kernel wins."""
+    st, ce = c2["stars"], c2["cells"]
+    nb = 64
+    sl = slice(0, nb * 512)
+    o, dxb = ce["block_origin"][:nb], ce["block_size"][:nb] / 8
+    S = [ce[k][sl].copy() for k in ("x", "y", "z", "m", "eps")]
+    T = (None, st["x"], st["y"], st["z"])
+    with GasGravityField(mode) as g:
+        g.set_blocks(o, dxb, S[3])
+        blk = g.evaluate(*T)
+        cells_t = (ce["eps"][:4096], ce["x"][:4096], ce["y"][:4096], ce["z"][:4096])
+        blk_cells = g.evaluate(*cells_t)
+        g.set_sources(*S)
+        for p, q in zip(g.evaluate(*T), blk):
+            assert np.array_equal(p, q)
+        for p, q in zip(g.evaluate(*cells_t), blk_cells):
+            assert np.array_equal(p, q)
+        bad = [a.copy() for a in S]
+        bad[1][17 * 512 + 3 * 64 + 2 * 8 + 5] += 1e-9        # one y leaves its row
+        bad_eps = [a.copy() for a in S]
+        bad_eps[4][9 * 512 + 300] *= 2.0                     # eps varies inside a block
+        for srcs in (bad, bad_eps, [a[:-512 + 3] for a in S]):   # the last: not whole blocks
+            g.set_sources(*srcs)
+            got = g.evaluate(*T)
+            g.set_block_kernel(False)
+            gen = g.evaluate(*T)
+            g.set_block_kernel(True)
+            for p, q in zip(got, gen):
+                assert np.array_equal(p, q)
 
 
 def test_block_sources_errors():

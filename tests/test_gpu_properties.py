@@ -228,8 +228,12 @@ def test_block_sources(fields, mode, blk, tgt, mass, hit, device):
         return host(f.evaluate(*T))
 
     got = run()
-    moved = run(shift=1.0, scale=4.0)
     cel = np.repeat(ce, 512)
+    # the same cells as explicit sources (block C order): detected as blocks, bit for bit
+    f.set_sources(*as_device([c[:, 0], c[:, 1], c[:, 2], cm, cel], device))
+    T0 = as_device([te, t[:, 0], t[:, 1], t[:, 2]], device)
+    same(host(f.evaluate(*T0)), got)
+    moved = run(shift=1.0, scale=4.0)
     ref = oracle.eval_plummer(c[:, 0], c[:, 1], c[:, 2], cm, cel, t[:, 0], t[:, 1], t[:, 2], te, G,
                               mode=oracle.NEUMAIER)
     check(mode, got, ref, c, cm, cel * cel, t, sq(te, len(t)), np.full(len(t), diag(t)))

@@ -331,7 +331,7 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
 // Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
 // blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
 template <int K, int NT, bool S32>
-__global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double4* __restrict__ bd,
+__global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double* __restrict__ sax,
                                                            const double* __restrict__ be2,
                                                            const double4* __restrict__ bm,
                                                            int64_t nblk, int64_t chunk, Flash P,
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double4*
   }
   for (int64_t blk = b0; blk < b1; ++blk) {
     __syncthreads();
-    gs_stage<false>(L, bd, be2, bm, blk, 0.0, 0.0, 0.0, 0.0);
+    gs_stage<false>(L, sax, be2, bm, blk, 0.0, 0.0, 0.0, 0.0);
     __syncthreads();
     flash_gs_block<K, S32>(P, L, tx, ty, tz, ax, ay, az, ap);
   }

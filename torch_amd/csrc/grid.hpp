@@ -383,8 +383,9 @@ __device__ __forceinline__ void gs_block(const LdsGS<MIXED>& L,
 }
 
 // Stages one source block (masses and cell-centre coordinates) into LDS.
+// sax[24 blk + 8 axis + idx] is the block's cell-centre coordinate idx along axis (x, y, z).
 template <bool MIXED>
-__device__ __forceinline__ void gs_stage(LdsGS<MIXED>& L, const double4* __restrict__ bd,
+__device__ __forceinline__ void gs_stage(LdsGS<MIXED>& L, const double* __restrict__ sax,
                                          const double* __restrict__ be2,
                                          const double4* __restrict__ s4, int64_t blk, double e2u,
                                          double ax_, double ay_, double az_) {
@@ -392,13 +393,8 @@ __device__ __forceinline__ void gs_stage(LdsGS<MIXED>& L, const double4* __restr
   const int tid = threadIdx.x;
   for (int q = tid; q < 512; q += blockDim.x) L.m[q] = (T)s4[blk * 512 + q].w;
   if (tid < 24) {
-    const double4 b = bd[blk];   // {ox, oy, oz, dx}
     const int axis = tid >> 3, idx = tid & 7;
-    double c;
-    {
-#pragma clang fp contract(off)
-      c = (axis == 0 ? b.x : axis == 1 ? b.y : b.z) + ((double)idx + 0.5) * b.w;
-    }
+    const double c = sax[blk * 24 + tid];
     const double a = axis == 0 ? ax_ : axis == 1 ? ay_ : az_;
     T v = MIXED ? (T)(c - a) : (T)c;
     if (axis == 0) L.cx[idx] = v;
@@ -412,7 +408,7 @@ template <int K, int NT, bool MIXED>
 struct GsRun {
   typedef typename LdsGS<MIXED>::T T;
   template <int FL>
-  static __device__ __forceinline__ void run(LdsGS<MIXED>& L, const double4* bd, const double* be2,
+  static __device__ __forceinline__ void run(LdsGS<MIXED>& L, const double* sax, const double* be2,
                                              const double4* bm, int64_t b0, int64_t b1, double e2u,
                                              double ax_, double ay_, double az_, const T (&tx)[K],
                                              const T (&ty)[K], const T (&tz)[K], const T (&te2)[K],
@@ -420,7 +416,7 @@ struct GsRun {
                                              double (&hp)[K]) {
     for (int64_t blk = b0; blk < b1; ++blk) {
       __syncthreads();
-      gs_stage<MIXED>(L, bd, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
+      gs_stage<MIXED>(L, sax, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
       __syncthreads();
       T ax[K], ay[K], az[K], ap[K];
 #pragma unroll
@@ -441,7 +437,7 @@ struct GsRun {
 // Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
 // blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
 template <int K, int NT, bool MIXED>
-__global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double4* __restrict__ bd,
+__global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
                                                      const double* __restrict__ be2,
                                                      const double4* __restrict__ bm, int64_t nblk,
                                                      int64_t chunk, const double* e2s_min_p, Out O) {
@@ -486,10 +482,10 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double4* __res
   for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
   typedef GsRun<K, NT, MIXED> R;
   if constexpr (MIXED) {
-    TG_DISPATCH8(R::template run, fl, L, bd, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
+    TG_DISPATCH8(R::template run, fl, L, sax, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
                  hx, hy, hz, hp)
   } else {
-    TG_DISPATCH(R::template run, fl, L, bd, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
+    TG_DISPATCH(R::template run, fl, L, sax, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
                 hx, hy, hz, hp)
   }
   Out o = O;
@@ -501,14 +497,16 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double4* __res
   }
 }
 
-// Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them) and
-// folds the smallest block eps^2 into stat[0]; one lane per cell.
+// Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them),
+// writes each block's eps^2 (be2) and cell-centre axes (sax, staged by the block-source
+// kernels) and folds the smallest block eps^2 into stat[0]; one lane per cell.
 __global__ void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox,
                                      const double* __restrict__ oy, const double* __restrict__ oz,
                                      const double* __restrict__ dx, const double* __restrict__ m,
                                      const double* __restrict__ eps, double4* __restrict__ s4,
-                                     double* __restrict__ se2, double4* __restrict__ bd,
-                                     double* __restrict__ be2, unsigned long long* stat) {
+                                     double* __restrict__ se2, double* __restrict__ be2,
+                                     double* __restrict__ sax,
+                                     unsigned long long* stat) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nblk * 512) return;
   const int64_t b = i >> 9;
@@ -524,11 +522,36 @@ __global__ void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox
   const double e = eps ? eps[b] : 0.0;
   s4[i] = make_double4(x, y, z, m[i]);
   se2[i] = e * e;
+  const int ci = c >> 6, cj = (c >> 3) & 7, ck = c & 7;
+  if (cj == 0 && ck == 0) sax[b * 24 + ci] = x;
+  if (ci == 0 && ck == 0) sax[b * 24 + 8 + cj] = y;
+  if (ci == 0 && cj == 0) sax[b * 24 + 16 + ck] = z;
   if (c == 0) {
-    bd[b] = make_double4(ox[b], oy[b], oz[b], d);
     be2[b] = e * e;
     atomicMin(&stat[0], (unsigned long long)__double_as_longlong(e * e));
   }
+}
+
+// Explicit sources that are 8^3 leaf blocks in C order (cell c = 64 i + 8 j + k of block b at
+// b * 512 + c): x depends on i only, y on j only, z on k only, and eps^2 is the block's.  Reads
+// the packed sources; writes the axis arrays sax (as k_pack_source_blocks does) and the block
+// eps^2, and clears flag[0] (set to 1 before) on any violation.  One thread per source.
+__global__ void k_detect_source_blocks(int64_t nblk, const double4* __restrict__ s4,
+                                       const double* __restrict__ se2, double* __restrict__ sax,
+                                       double* __restrict__ be2, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nblk * 512) return;
+  const int64_t b = i >> 9, base = b * 512;
+  const int c = (int)(i & 511);
+  const int ci = c >> 6, cj = (c >> 3) & 7, ck = c & 7;
+  const double4 s = s4[i];
+  const bool ok = s.x == s4[base + ci * 64].x && s.y == s4[base + cj * 8].y &&
+                  s.z == s4[base + ck].z && se2[i] == se2[base];
+  if (!ok) flag[0] = 0;
+  if (cj == 0 && ck == 0) sax[b * 24 + ci] = s.x;
+  if (ci == 0 && ck == 0) sax[b * 24 + 8 + cj] = s.y;
+  if (ci == 0 && cj == 0) sax[b * 24 + 16 + ck] = s.z;
+  if (c == 0) be2[b] = se2[i];
 }
 
 }  // namespace tg

@@ -628,13 +628,21 @@ __global__ void k_set_flag(int* flag, int v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = v;
 }
 
+// Gate value of an explicit-target evaluation: 2 if the sources are explicit 8^3 blocks
+// (*sflag, set by k_detect_source_blocks; sflag null: they are not), else `tcand` (1: the targets
+// may have column structure, which k_check_cols then tests; 0: they do not).
+__global__ void k_eval_gate(int* gate, const int* sflag, int tcand) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) gate[0] = (sflag && sflag[0] == 1) ? 2 : tcand;
+}
+
 // Column structure of explicit targets, the C order of 8^3 leaf blocks
 // (.deprecated/bridge_multiples.py:566-575 flattens each block's meshgrid this way): every
 // aligned group of 8 consecutive targets shares x and y, and every aligned group of 512 shares
-// eps.  Any violation clears flag[0] (set to 1 before); all writers store the same value.
+// eps.  Any violation clears flag[0] (1 before; 2, the block-source path, is left alone); all
+// writers store the same value.
 __global__ void k_check_cols(Tgt T, int* flag) {
   const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
-  if (t >= T.n) return;
+  if (t >= T.n || flag[0] != 1) return;   // 2: the block-source kernel runs; nothing to test
   const double x0 = T.x[t], y0 = T.y[t];
   const double e0 = T.eps ? T.eps[t & ~(int64_t)511] : 0.0;
   bool ok = true;

@@ -299,7 +299,9 @@ struct tgrav_ctx {
   DevBuf s4, se2, stat;
   // sources given as 8^3 blocks: descriptors {ox, oy, oz, dx} and eps^2 per block
   int64_t src_nblk = 0;      // > 0: the current source set is block-structured
-  DevBuf sbd, sbe2;
+  DevBuf sbe2, sax;          // source blocks: eps^2 and cell-centre axes (24 per block)
+  int64_t src_cols = 0;      // > 0: explicit sources that may be 8^3 blocks (scolflag says)
+  DevBuf scolflag;
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
   DevBuf in[12], outb[8], vel[3], part, flag;
   DevBuf colflag;            // explicit targets: 1 if they have 8^3 column structure
@@ -519,21 +521,44 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
 
 // Explicit targets against 8^3 source blocks: K = 4 targets per lane, 256 lanes, source
 // blocks split over blockIdx.y until there are about 4 workgroups per CU.
-static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
+static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk, int64_t* ntiles,
+                          int* nsplit, int64_t* chunk) {
   const int K = 4, NT = 256;
-  const int64_t nt = T.n, nblk = c->src_nblk;
-  const int64_t ntiles = (nt + K * NT - 1) / (K * NT);
-  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
+  *ntiles = (nt + K * NT - 1) / (K * NT);
   int want = c->nsplit[mode];
   const int tuned = tuned_block_split(c, mode, "blocksrc", c->ns, nblk);
   if (tuned > 0) want = tuned;
   if (want <= 0) {
     const int64_t target = 4LL * c->num_cu;
-    want = ntiles >= target / 2 ? 1 : (int)std::min<int64_t>((target + ntiles - 1) / ntiles, nblk);
+    want = *ntiles >= target / 2 ? 1 : (int)std::min<int64_t>((target + *ntiles - 1) / *ntiles, nblk);
   }
   if (want > 65535) want = 65535;
-  const int64_t chunk = (nblk + want - 1) / want;
-  const int nsplit = (int)((nblk + chunk - 1) / chunk);
+  *chunk = (nblk + want - 1) / want;
+  *nsplit = (int)((nblk + *chunk - 1) / *chunk);
+}
+
+static int gridsrc_kernel(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& o, int64_t ntiles,
+                          int nsplit, int64_t chunk, int64_t nblk) {
+  const int K = 4, NT = 256;
+  const dim3 g((unsigned)ntiles, (unsigned)nsplit);
+  if (mode == TGRAV_MIXED)
+    tg::k_eval_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
+        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
+        c->stat.as<const double>(), o);
+  else
+    tg::k_eval_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
+        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
+        c->stat.as<const double>(), o);
+  LAUNCHCHECK("k_eval_gridsrc");
+  return 0;
+}
+
+static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
+  const int64_t nt = T.n, nblk = c->src_nblk;
+  int64_t ntiles, chunk;
+  int nsplit;
+  gridsrc_shape(c, mode, nt, nblk, &ntiles, &nsplit, &chunk);
+  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
   tg::Out o = O;
   if (nsplit > 1) {
     int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
@@ -544,17 +569,7 @@ static int launch_gridsrc(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
     o.part = nullptr;
   }
   int rc = timed_launch(c, nt, c->ns, [&]() -> int {
-    const dim3 g((unsigned)ntiles, (unsigned)nsplit);
-    if (mode == TGRAV_MIXED)
-      tg::k_eval_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
-          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
-          c->stat.as<const double>(), o);
-    else
-      tg::k_eval_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
-          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
-          c->stat.as<const double>(), o);
-    LAUNCHCHECK("k_eval_gridsrc");
-    return 0;
+    return gridsrc_kernel(c, mode, T, o, ntiles, nsplit, chunk, nblk);
   });
   if (rc) return rc;
   if (nsplit > 1) {
@@ -606,10 +621,10 @@ static int launch_flash_gridsrc(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O
     const dim3 g((unsigned)ntiles, (unsigned)nsplit);
     if (s32)
       tg::k_eval_flash_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
-          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+          T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
     else
       tg::k_eval_flash_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
-          T, c->sbd.as<double4>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+          T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
     LAUNCHCHECK("k_eval_flash_gridsrc");
     return 0;
   });
@@ -770,36 +785,50 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   // structure is decided on the device (k_check_cols sets a flag); both kernels are launched
   // and the flag lets exactly one of them (and its chunk reduction) run, so nothing waits for
   // the host.
+  // Likewise explicit sources that are C-ordered 8^3 blocks (k_detect_source_blocks flagged
+  // them when they were set) run the block-source kernel: gate value 2 wins over the column
+  // test (1) and the generic kernel (0).
   const bool cols = c->block_kernel && !T.blocks && nt >= 512;
+  const bool srcb = c->block_kernel && !T.blocks && c->src_cols > 0;
   const int64_t ngrp = (nt + 511) / 512;
-  int nsplit_c = 1;
-  int64_t chunk_c = 0;
+  int nsplit_c = 1, nsplit_s = 1;
+  int64_t chunk_c = 0, chunk_s = 0, ntiles_s = 0;
   if (cols) {
     if (ngrp > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
     grid_shape(c, mode, ns, ngrp, &nsplit_c, &chunk_c);
   }
-  const int nsp = std::max(nsplit, cols ? nsplit_c : 1);
+  if (srcb) gridsrc_shape(c, mode, nt, c->src_cols, &ntiles_s, &nsplit_s, &chunk_s);
+  const int nsp = std::max(nsplit, std::max(cols ? nsplit_c : 1, srcb ? nsplit_s : 1));
   if (nsp > 1) {
     int rc0 = c->part.ensure((size_t)nsp * 4 * nt * sizeof(double));
     if (rc0) return rc0;
   }
-  tg::Out o = O, oc = O;
+  tg::Out o = O, oc = O, os = O;
   o.part = nsplit > 1 ? c->part.as<double>() : nullptr;
   oc.part = nsplit_c > 1 ? c->part.as<double>() : nullptr;
-  o.pstride = oc.pstride = nt;
-  if (cols) {
+  os.part = nsplit_s > 1 ? c->part.as<double>() : nullptr;
+  o.pstride = oc.pstride = os.pstride = nt;
+  if (cols || srcb) {
     int rc0 = c->colflag.ensure(sizeof(int));
     if (rc0) return rc0;
     int* flag = c->colflag.as<int>();
-    tg::k_set_flag<<<1, 64, 0, c->stream>>>(flag, 1);
-    LAUNCHCHECK("k_set_flag");
-    tg::k_check_cols<<<grid1(ngrp * 64, 256), 256, 0, c->stream>>>(T, flag);
-    LAUNCHCHECK("k_check_cols");
-    o.gate = oc.gate = flag;
+    tg::k_eval_gate<<<1, 64, 0, c->stream>>>(flag, srcb ? c->scolflag.as<const int>() : nullptr,
+                                              cols ? 1 : 0);
+    LAUNCHCHECK("k_eval_gate");
+    if (cols) {
+      tg::k_check_cols<<<grid1(ngrp * 64, 256), 256, 0, c->stream>>>(T, flag);
+      LAUNCHCHECK("k_check_cols");
+    }
+    o.gate = oc.gate = os.gate = flag;
     o.gate_on = 0;
     oc.gate_on = 1;
+    os.gate_on = 2;
   }
   int rc = timed_launch(c, nt, ns, [&]() -> int {
+    if (srcb) {
+      int r = gridsrc_kernel(c, mode, T, os, ntiles_s, nsplit_s, chunk_s, c->src_cols);
+      if (r) return r;
+    }
     if (cols) {
       if (mode == TGRAV_MIXED)
         tg::k_eval_cols<true><<<dim3((unsigned)ngrp, (unsigned)nsplit_c), 64, 0, c->stream>>>(
@@ -815,6 +844,12 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
     return 0;
   });
   if (rc) return rc;
+  if (srcb && nsplit_s > 1) {
+    tg::Out fin = os;
+    fin.part = nullptr;
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit_s, nt, fin);
+    if (rc1) return rc1;
+  }
   if (cols && nsplit_c > 1) {
     tg::Out fin = oc;
     fin.part = nullptr;
@@ -900,8 +935,9 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->stat.release();
   c->flag.release();
   c->colflag.release();
-  c->sbd.release();
   c->sbe2.release();
+  c->sax.release();
+  c->scolflag.release();
   c->bp4.release();
   c->bpe2.release();
   c->bstat.release();
@@ -992,6 +1028,7 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
   if ((rc = set_device(c))) return rc;
   c->ns = -1;
   c->src_nblk = 0;
+  c->src_cols = 0;
   if (n == 0) {
     c->ns = 0;
     return 0;
@@ -1012,6 +1049,18 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
       n, a[0], a[1], a[2], a[3], a[4], c->s4.as<double4>(), c->se2.as<double>(),
       c->stat.as<unsigned long long>(), kind == TGRAV_DEVICE && c->validate);
   LAUNCHCHECK("k_pack_sources");
+  const int64_t nb8 = n / 512;
+  if (n % 512 == 0) {   // C-ordered 8^3 blocks? decided on the device (k_detect_source_blocks)
+    if ((rc = c->sax.ensure((size_t)nb8 * 24 * sizeof(double))) ||
+        (rc = c->sbe2.ensure((size_t)nb8 * sizeof(double))) || (rc = c->scolflag.ensure(sizeof(int))))
+      return rc;
+    tg::k_set_flag<<<1, 64, 0, c->stream>>>(c->scolflag.as<int>(), 1);
+    LAUNCHCHECK("k_set_flag");
+    tg::k_detect_source_blocks<<<grid1(n, 256), 256, 0, c->stream>>>(
+        nb8, c->s4.as<double4>(), c->se2.as<double>(), c->sax.as<double>(), c->sbe2.as<double>(),
+        c->scolflag.as<int>());
+    LAUNCHCHECK("k_detect_source_blocks");
+  }
   if (kind == TGRAV_DEVICE) {
     if (c->validate) {
       bool bad = false;
@@ -1022,6 +1071,7 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
     HIPCHECK(hipStreamSynchronize(c->stream));   // host arrays may be freed on return
   }
   c->ns = n;
+  c->src_cols = n % 512 == 0 ? nb8 : 0;
   return 0;
 }
 
@@ -1054,6 +1104,7 @@ int tgrav_set_source_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox
   if ((rc = set_device(c))) return rc;
   c->ns = -1;
   c->src_nblk = 0;
+  c->src_cols = 0;
   if (nblk == 0) {
     c->ns = 0;
     return 0;
@@ -1075,13 +1126,14 @@ int tgrav_set_source_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox
   }
   if ((rc = c->s4.ensure((size_t)n * sizeof(double4))) || (rc = c->se2.ensure((size_t)n * 8)) ||
       (rc = c->stat.ensure(2 * sizeof(unsigned long long))) ||
-      (rc = c->sbd.ensure((size_t)nblk * sizeof(double4))) || (rc = c->sbe2.ensure((size_t)nblk * 8)))
+      (rc = c->sbe2.ensure((size_t)nblk * 8)) ||
+      (rc = c->sax.ensure((size_t)nblk * 24 * sizeof(double))))
     return rc;
   tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->stat.as<unsigned long long>());
   LAUNCHCHECK("k_reset_stats");
   tg::k_pack_source_blocks<<<grid1(n, 256), 256, 0, c->stream>>>(
       nblk, a[0], a[1], a[2], a[3], a[5], a[4], c->s4.as<double4>(), c->se2.as<double>(),
-      c->sbd.as<double4>(), c->sbe2.as<double>(), c->stat.as<unsigned long long>());
+      c->sbe2.as<double>(), c->sax.as<double>(), c->stat.as<unsigned long long>());
   LAUNCHCHECK("k_pack_source_blocks");
   if (kind == TGRAV_HOST) HIPCHECK(hipStreamSynchronize(c->stream));
   c->ns = n;
