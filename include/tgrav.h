@@ -109,7 +109,9 @@ int tgrav_variant_info(int mode, int variant, int* targets_per_lane, int* thread
 /* Source set: the stellar side (stars, for the forward direction of
  * get_gravity_at_point, src/torch_mainloop.py:128-140) or the gas cells as point
  * masses m = rho dV (for the reverse direction, Gravity_getAccelAtPoint.F90:168-204).
- * eps may be NULL (= 0).  The arrays are copied into the context (packed FP64). */
+ * eps may be NULL (= 0).  The arrays are copied into the context (packed FP64).  Sources
+ * that are 8^3 leaf blocks flattened in C order (each aligned 512 a tensor-product block
+ * with one eps) are recognised on the device and evaluated as source blocks. */
 int tgrav_set_sources(tgrav_ctx* ctx, int64_t n, const double* x, const double* y,
                       const double* z, const double* m, const double* eps, int ptr_kind);
 
@@ -129,7 +131,9 @@ int tgrav_set_source_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const double* 
  * (Gravity_getPotentialAtPoint.F90:33-305), and the stellar-side AMUSE
  * get_gravity_at_point / get_potential_at_point(eps, x, y, z) (interface.py:221-243).
  * eps_t is the per-point softening (may be NULL = 0); any of ax/ay/az/phi may be NULL
- * (phi == NULL skips the potential). */
+ * (phi == NULL skips the potential).  Points that are 8^3 leaf-block meshgrids flattened in
+ * C order (.deprecated/bridge_multiples.py:566-575: each aligned 8 share x and y, each
+ * aligned 512 share eps) are recognised on the device and take the column kernel. */
 int tgrav_eval(tgrav_ctx* ctx, int64_t n, const double* eps_t, const double* x,
                const double* y, const double* z, double* ax, double* ay, double* az,
                double* phi, int mode, int ptr_kind);
@@ -202,8 +206,9 @@ int tgrav_bidirectional_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const doubl
                                double* caz, double* cphi, double* pax, double* pay, double* paz,
                                double* pphi, int mode, int ptr_kind);
 /* Block targets with nb == 8 use a column kernel (each lane owns one (i, j) column of 8
- * cells and shares dx, dy, dx^2 + dy^2 + eps^2 between them); on = 0 falls back to the
- * generic kernel (same results to rounding; for tests and comparisons).  Default on. */
+ * cells and shares dx, dy, dx^2 + dy^2 + eps^2 between them), and so do explicit arrays
+ * recognised as blocks (tgrav_eval, tgrav_set_sources); on = 0 falls back to the generic
+ * kernel (same results to rounding; for tests and comparisons).  Default on. */
 int tgrav_set_block_kernel(tgrav_ctx* ctx, int on);
 /* Fused-pass launch shape (variant, particle chunks; -1 keeps, 0 = automatic chunks). */
 int tgrav_set_bidir_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
