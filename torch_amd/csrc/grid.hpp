@@ -10,7 +10,11 @@
 //
 // Block sources (k_eval_gridsrc): the same sharing with the roles swapped -- the cells of a
 // source block are walked column by column, and every lane forms dx, dy and the partial r^2
-// of its K targets once per column.
+// of its K targets once per column; dz depends only on (cell row, target), so it is formed
+// once per block, and the x, y accelerations take dx, dy times the column's sum of weights:
+//   per interaction  r^2 (1) + rsqrt + m y, y^2, w (3) + phi (1) + sum w (1) + a_z (1)
+//   per column/target  dx, dy (2) + base (2) + a_x, a_y (2), shared by 8 cells
+// (the sum of weights in FP64; the mixed kernel keeps a_x, a_y per cell, see gs_block).
 #pragma once
 #include "kernels.hpp"
 
@@ -356,6 +360,13 @@ __device__ __forceinline__ void gs_block(const LdsGS<MIXED>& L,
       dy[k] = ys - ty[k];
       base[k] = fma(dx[k], dx[k], fma(dy[k], dy[k], (FL & F_UNIF) ? e2 : e2 + te2[k]));
     }
+    // The 8 cells of the column share dx and dy, so sum_cells w dx = dx sum_cells w: one add
+    // per cell instead of two FMAs, then two FMAs per column.  FP64 only: in the mixed kernel the
+    // shorter loop measured slower (7% more cycles at the same occupancy, also with the FP64
+    // accumulators moved to LDS for a fourth wave per SIMD), so it keeps the per-cell FMAs.
+    T sw[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) sw[k] = (T)0;
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
       const T zs = L.cz[kk];
@@ -374,9 +385,20 @@ __device__ __forceinline__ void gs_block(const LdsGS<MIXED>& L,
         const T mr = m * y;
         if (!(FL & F_NOPOT)) ap[k] += mr;
         const T w = (mr * y) * y;
-        ax[k] = fma(w, dx[k], ax[k]);
-        ay[k] = fma(w, dy[k], ay[k]);
+        if constexpr (MIXED) {
+          ax[k] = fma(w, dx[k], ax[k]);
+          ay[k] = fma(w, dy[k], ay[k]);
+        } else {
+          sw[k] += w;
+        }
         az[k] = fma(w, dz, az[k]);
+      }
+    }
+    if constexpr (!MIXED) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        ax[k] = fma(sw[k], dx[k], ax[k]);
+        ay[k] = fma(sw[k], dy[k], ay[k]);
       }
     }
   }
