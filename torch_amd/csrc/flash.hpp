@@ -276,6 +276,9 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
       base[k] = fma(dx[k], dx[k], dy[k] * dy[k]);
     }
     unsigned int nm = 0u;   // wave-uniform: bit kk = cell kk of the column was flagged
+    double sw[K];           // the column's sum of weights: its cells share dx and dy
+#pragma unroll
+    for (int k = 0; k < K; ++k) sw[k] = 0.0;
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
       const double zs = L.cz[kk], m = L.m[col * 8 + kk];
@@ -289,11 +292,15 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
         const double mr = m * y;
         ap[k] += mr;
         const double w = mr * (y * y);
-        ax[k] = fma(-w, dx[k], ax[k]);
-        ay[k] = fma(-w, dy[k], ay[k]);
+        sw[k] += w;
         az[k] = fma(-w, dz, az[k]);
       }
       if (__builtin_amdgcn_ballot_w64(nj)) nm |= 1u << kk;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      ax[k] = fma(-sw[k], dx[k], ax[k]);
+      ay[k] = fma(-sw[k], dy[k], ay[k]);
     }
     while (nm) {
       const int kk = __builtin_ctz(nm);
