@@ -95,13 +95,19 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
     import torch
 
     from torch_amd import GasGravityField, StellarGravityField, autotune
-    fwd = StellarGravityField(mode, validate=False)
-    rev = GasGravityField(mode, validate=False)
     if blocks:
+        fwd = StellarGravityField(mode, validate=False)
+        rev = GasGravityField(mode, validate=False)
         nblk = nc // 512
         cfg = (autotune.apply_blocks(fwd, "blockfwd", ns, nblk), autotune.apply_blocks(rev, "blocksrc", ns, nblk))
     else:
-        cfg = (autotune.apply_eval(fwd, ns, nc), autotune.apply_eval(rev, nc, ns))
+        # The library applies the tuned shapes itself: explicit arrays recognised as 8^3 blocks
+        # take the block kernels' entries, others the generic kernels' (reported here).
+        fwd = StellarGravityField(mode, validate=False, tune_cache=autotune.CACHE)
+        rev = GasGravityField(mode, validate=False, tune_cache=autotune.CACHE)
+        cache = autotune.load_cache()
+        m = "mixed" if mode == "mixed" else "fp64"
+        cfg = (cache.get(autotune.eval_key(m, ns, nc)), cache.get(autotune.eval_key(m, nc, ns)))
     log(f"{mode}: {'block' if blocks else 'explicit'} format, tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]

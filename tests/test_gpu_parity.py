@@ -588,5 +588,17 @@ def test_library_applies_the_tune_cache(c2):
         np.testing.assert_array_equal(res[0][q], res[1][q])          # cache == explicit tuned split
         np.testing.assert_array_equal(res[2][q], res[3][q])          # explicit 3 == heuristic (3)
     assert any(not np.array_equal(res[0][q], res[2][q]) for q in range(4))
+    # block sources: the entry is keyed by the target (star) count and the block count
+    tuned_s = [v["nsplit"] for k, v in json.load(open(cache)).items() if "|blocksrc|fp64|s13|b11" in k][0]
+    res = []
+    for tune, split in ((cache, None), (None, tuned_s), (None, None)):
+        with GasGravityField("fp64", tune_cache=tune) as g:
+            if split is not None:
+                g.set_config(nsplit=split)
+            g.set_blocks(bo, bd, ce["m"])
+            res.append(g.evaluate(None, st["x"], st["y"], st["z"]))
+    for q in range(4):
+        np.testing.assert_array_equal(res[0][q], res[1][q])
+    assert any(not np.array_equal(res[0][q], res[2][q]) for q in range(4))
     with pytest.raises(TgravError):
         GravityField("fp64", tune_cache=os.path.join(HERE, "conftest.py"))

@@ -323,7 +323,9 @@ static const TuneEntry* tune_lookup(const tgrav_ctx* c, const std::string& key) 
 
 static const char* mode_name(int mode) { return mode == TGRAV_MIXED ? "mixed" : "fp64"; }
 
-// Tuned source split of a column kernel ("blockfwd" / "blocksrc"), or -1.
+// Tuned source split of a column kernel ("blockfwd" / "blocksrc"), or -1.  The key's sizes are
+// the explicit point count (the sources of "blockfwd", the targets of "blocksrc") and the
+// number of leaf blocks, as the autotuner writes them.
 static int tuned_block_split(const tgrav_ctx* c, int mode, const char* dir, int64_t ns, int64_t nblk) {
   if (c->user_cfg[mode] || c->tune.empty()) return -1;
   char key[256];
@@ -526,7 +528,7 @@ static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk
   const int K = 4, NT = 256;
   *ntiles = (nt + K * NT - 1) / (K * NT);
   int want = c->nsplit[mode];
-  const int tuned = tuned_block_split(c, mode, "blocksrc", c->ns, nblk);
+  const int tuned = tuned_block_split(c, mode, "blocksrc", nt, nblk);
   if (tuned > 0) want = tuned;
   if (want <= 0) {
     const int64_t target = 4LL * c->num_cu;
