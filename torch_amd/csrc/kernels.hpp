@@ -33,9 +33,9 @@ struct Out {
   double *part;                 // non-null: write unscaled chunk sums to part[q*pstride + t]
   int64_t pstride;
   double G;
-  int nopot;                    // 1: the potential is not wanted (its accumulator is skipped)
-  const int* gate;              // non-null: the kernel runs only if *gate == gate_on, else it
-  int gate_on;                  //   returns at once (one of two launches picked on the device)
+  int nopot = 0;                // 1: the potential is not wanted (its accumulator is skipped)
+  const int* gate = nullptr;    // non-null: the kernel runs only if *gate == gate_on, else it
+  int gate_on = 0;              //   returns at once (one of several launches picked on the device)
 };
 
 // Uniform across the grid: every workgroup of a gated-off launch returns before any barrier.
