@@ -568,17 +568,18 @@ def test_block_sources_errors():
 
 def test_library_applies_the_tune_cache(c2):
     """tgrav_create with the autotuner's cache: the block forward launch takes the cached source
-    split (bit-identical to setting it explicitly), which differs from the built-in heuristic's;
-    an explicit tgrav_set_config still wins over the cache.  FP64, where the split changes the
-    order of the FP64 sums (the mixed kernel's FP32 tile sums add exactly in FP64, so its results
-    do not depend on the split at all)."""
+    split (bit-identical to setting it explicitly); an explicit tgrav_set_config wins over the
+    cache; and another split gives other bits, so the comparisons can tell.  FP64, where the
+    split changes the order of the FP64 sums (the mixed kernel's FP32 tile sums add exactly in
+    FP64, so its results do not depend on the split at all)."""
     import json
     cache = os.path.join(HERE, "..", "torch_amd", "tune_cache.json")
     tuned = [v["nsplit"] for k, v in json.load(open(cache)).items() if "|blockfwd|fp64|s13|b11" in k][0]
+    other = 5 if tuned != 5 else 7
     st, ce = c2["stars"], c2["cells"]
     bo, bd = ce["block_origin"], ce["block_size"] / 8
     res = []
-    for tune, split in ((cache, None), (None, tuned), (None, None), (cache, 3)):
+    for tune, split in ((cache, None), (None, tuned), (None, other), (cache, other)):
         with GravityField("fp64", tune_cache=tune) as f:
             if split is not None:
                 f.set_config(nsplit=split)
@@ -586,12 +587,12 @@ def test_library_applies_the_tune_cache(c2):
             res.append(f.evaluate_blocks(bo, bd))
     for q in range(4):
         np.testing.assert_array_equal(res[0][q], res[1][q])          # cache == explicit tuned split
-        np.testing.assert_array_equal(res[2][q], res[3][q])          # explicit 3 == heuristic (3)
+        np.testing.assert_array_equal(res[2][q], res[3][q])          # explicit split wins over the cache
     assert any(not np.array_equal(res[0][q], res[2][q]) for q in range(4))
     # block sources: the entry is keyed by the target (star) count and the block count
     tuned_s = [v["nsplit"] for k, v in json.load(open(cache)).items() if "|blocksrc|fp64|s13|b11" in k][0]
     res = []
-    for tune, split in ((cache, None), (None, tuned_s), (None, None)):
+    for tune, split in ((cache, None), (None, tuned_s), (None, 64)):
         with GasGravityField("fp64", tune_cache=tune) as g:
             if split is not None:
                 g.set_config(nsplit=split)

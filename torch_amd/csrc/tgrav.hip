@@ -474,10 +474,12 @@ static void grid_shape(const tgrav_ctx* c, int mode, int64_t ns, int64_t nblk, i
   const int tuned = tuned_block_split(c, mode, "blockfwd", ns, nblk);
   if (tuned > 0) want = tuned;
   if (want <= 0) {
-    const int64_t target = 16LL * c->num_cu;
-    want = nblk >= target / 2 ? 1
-                              : (int)std::min<int64_t>((target + nblk - 1) / nblk,
-                                                       std::max<int64_t>(1, ns / 256));
+    // One-wave workgroups, ~3 resident per SIMD: aim at ~8 rounds of them so the last round's
+    // imbalance is small, with at most 12 chunks (each adds 32 B of partial sums per target)
+    // and at least 4 source tiles per chunk.
+    const int64_t target = 8LL * 12 * c->num_cu;
+    want = (int)std::min<int64_t>((target + nblk - 1) / nblk,
+                                  std::min<int64_t>(12, std::max<int64_t>(1, ns / 256)));
   }
   if (want > 65535) want = 65535;
   int64_t ch = (ns + want - 1) / want;
@@ -531,8 +533,10 @@ static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk
   const int tuned = tuned_block_split(c, mode, "blocksrc", nt, nblk);
   if (tuned > 0) want = tuned;
   if (want <= 0) {
-    const int64_t target = 4LL * c->num_cu;
-    want = *ntiles >= target / 2 ? 1 : (int)std::min<int64_t>((target + *ntiles - 1) / *ntiles, nblk);
+    // Four-wave workgroups, ~3 resident per CU: aim at ~16 rounds, with at least 4 source
+    // blocks per chunk.
+    const int64_t target = 16LL * 3 * c->num_cu;
+    want = (int)std::min<int64_t>((target + *ntiles - 1) / *ntiles, std::max<int64_t>(1, nblk / 4));
   }
   if (want > 65535) want = 65535;
   *chunk = (nblk + want - 1) / want;
