@@ -598,8 +598,8 @@ static double sqrt_lt_threshold(double r) {
 }
 
 // FLASH kernel, isolated, explicit targets against 8^3 source blocks: K targets per lane
-// (TG_FLASH_GS_K), 256 lanes, source blocks split over blockIdx.y until there are about 8
-// workgroups per CU.
+// (TG_FLASH_GS_K), 256 lanes, source blocks split over blockIdx.y (about 16 rounds of
+// workgroups, or the chunk count of tgrav_set_config).
 #ifndef TG_FLASH_GS_K
 #define TG_FLASH_GS_K 2
 #endif
@@ -609,8 +609,10 @@ static int launch_flash_gridsrc(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O
   const int64_t nt = T.n, nblk = c->src_nblk;
   const int64_t ntiles = (nt + K * NT - 1) / (K * NT);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
-  const int64_t target = 8LL * c->num_cu;
-  int64_t want = ntiles >= target / 2 ? 1 : std::min<int64_t>((target + ntiles - 1) / ntiles, nblk);
+  // As the Plummer block-source kernel: about 16 rounds of workgroups, at least 4 blocks a chunk.
+  const int64_t target = 16LL * 3 * c->num_cu;
+  int64_t want = std::min<int64_t>((target + ntiles - 1) / ntiles, std::max<int64_t>(1, nblk / 4));
+  if (c->nsplit[TGRAV_PLAIN_FP64] > 0) want = c->nsplit[TGRAV_PLAIN_FP64];   // tgrav_set_config
   want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
   const int64_t chunk = (nblk + want - 1) / want;
   const int nsplit = (int)((nblk + chunk - 1) / chunk);
