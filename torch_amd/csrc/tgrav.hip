@@ -1507,12 +1507,13 @@ int tgrav_bidirectional_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double*
         np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
         c->bstat.as<unsigned long long>(), 0);
     LAUNCHCHECK("k_pack_sources");
-    // particle chunks over blockIdx.y until there are about 16 waves per CU
+    // particle chunks over blockIdx.y as for block targets (grid_shape): about 8 rounds of
+    // one-wave workgroups, at most 12 chunks, at least 4 particle tiles a chunk
     int64_t want = c->bnsplit[mode];
     if (want <= 0) {
-      const int64_t target = 16LL * c->num_cu;
-      want = nblk >= target / 2 ? 1 : std::min<int64_t>((target + nblk - 1) / nblk,
-                                                        std::max<int64_t>(1, np / 256));
+      const int64_t target = 8LL * 12 * c->num_cu;
+      want = std::min<int64_t>((target + nblk - 1) / nblk,
+                               std::min<int64_t>(12, std::max<int64_t>(1, np / 256)));
     }
     want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
     int64_t chunk = (np + want - 1) / want;
