@@ -448,14 +448,14 @@ static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t
   *ntiles = (nt + tile - 1) / tile;
   int want = force >= 0 ? force : c->nsplit[mode];
   if (want <= 0) {
-    const int64_t target_wg = 4LL * c->num_cu;
-    if (*ntiles >= target_wg / 2) want = 1;
-    else {
-      int64_t s = (target_wg + *ntiles - 1) / *ntiles;
-      int64_t maxs = (ns + 4LL * v->ts - 1) / (4LL * v->ts);   // >= 4 source tiles a chunk
-      if (s > maxs) s = maxs;
-      want = (int)(s < 1 ? 1 : s);
-    }
+    // About 8 rounds of workgroups at ~3 resident waves per SIMD, at least 4 source tiles a
+    // chunk, and at most 12 x 2^20 target-chunks of partial sums (32 B each).
+    const int64_t target_wg = 8LL * 12 * c->num_cu / (v->nt / 64);
+    int64_t s = (target_wg + *ntiles - 1) / *ntiles;
+    const int64_t maxs = (ns + 4LL * v->ts - 1) / (4LL * v->ts);
+    const int64_t maxp = std::max<int64_t>(1, (12LL << 20) / std::max<int64_t>(nt, 1));
+    s = std::min<int64_t>(s, std::min<int64_t>(maxs, maxp));
+    want = (int)(s < 1 ? 1 : s);
   }
   if (want > 65535) want = 65535;
   int64_t ch = (ns + want - 1) / want;
