@@ -118,7 +118,7 @@ def tune_eval(mode, ns, nt, reps=5, nsplits=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256
     """Best (variant, nsplit) for one evaluation of nt targets from ns sources."""
     S, T = _synthetic(ns, nt)
     table = []
-    with GravityField(mode, validate=False) as f:
+    with GravityField(mode, validate=False, tune_cache=None) as f:
         f.set_sources(*S)
         for v in variants(mode):
             for sp in nsplits:
@@ -144,7 +144,7 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
     dev = problems_from_workload(problems)
     lib = _lib.load()
     table = []
-    with GravityField(mode, validate=False) as f:
+    with GravityField(mode, validate=False, tune_cache=None) as f:
         for v in variants(mode):
             for it in items:
                 for w in wg_per_cu:
@@ -179,7 +179,7 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
     BO, BD, CM = d(ce["block_origin"]), d(ce["block_size"] / 8), d(ce["m"])
     ns, nblk = len(st["x"]), len(ce["block_size"])
     out = {}
-    with GravityField(mode, validate=False) as f:
+    with GravityField(mode, validate=False, tune_cache=None) as f:
         f.set_sources(*S)
         table = []
         for sp in (0, 1, 2, 3, 4, 6, 8, 12, 16):

@@ -118,16 +118,21 @@ class GravityField:
     compensated sums, FP64 result and kick).
     """
 
-    def __init__(self, mode="fp64", device=0, G=G_PC_MSUN_KMS, validate=True, tune_cache=None):
+    def __init__(self, mode="fp64", device=0, G=G_PC_MSUN_KMS, validate=True, tune_cache="default"):
         """tune_cache: path of an autotuner cache the library applies by itself (launch shapes
-        per device and size bucket); None leaves the built-in heuristics (the Python-side
-        ``autotune.apply_*`` helpers configure a field explicitly instead)."""
+        per device and size bucket; shapes without an entry take the built-in heuristics);
+        "default" is the package's tune_cache.json (or $TGRAV_TUNE_CACHE); None uses the
+        heuristics only.  An explicit set_config / set_bidir_config wins over the cache."""
         if mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
         self.lib = _lib.load()
         self.mode = MODES[mode]
         self.device = device
         ctx = ctypes.c_void_p()
+        if isinstance(tune_cache, str) and tune_cache == "default":
+            tune_cache = os.environ.get("TGRAV_TUNE_CACHE", os.path.join(os.path.dirname(__file__), "tune_cache.json"))
+            if not os.path.exists(tune_cache):
+                tune_cache = None
         path = None if tune_cache is None else os.fsencode(tune_cache)
         _lib.check(self.lib.tgrav_create(ctypes.byref(ctx), int(device), path), "tgrav_create")
         self._ctx = ctx
@@ -324,7 +329,8 @@ class GravityField:
         _lib.check(self.lib.tgrav_set_periodic(self._ctx, 1, lx, ly, lz, int(nrep)), "tgrav_set_periodic")
 
     def set_block_kernel(self, on=True):
-        """Use (default) or bypass the 8^3 column kernel for block-structured targets."""
+        """Use (default) or bypass the 8^3 column kernels: block-structured targets and sources,
+        and explicit arrays recognised as leaf blocks in C order."""
         _lib.check(self.lib.tgrav_set_block_kernel(self._ctx, int(bool(on))), "tgrav_set_block_kernel")
 
     def set_bidir_config(self, variant=-1, nsplit=-1):
