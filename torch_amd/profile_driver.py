@@ -4,7 +4,8 @@
 
 Each run spec is MODE:DIRECTION[:VARIANT[:NSPLIT]] (direction fwd = stars -> cells,
 rev = cells -> stars, fwdb / revb = the same with the cells as 8^3 leaf blocks (column
-kernels; VARIANT is ignored there); variant/nsplit default to the tuned shape).  After the evaluations
+kernels; VARIANT is ignored there); variant/nsplit default to the tuned shape of the run's
+kernel (the block entries for fwdb / revb).  After the evaluations
 it performs one plain device-to-device copy of --copy-mb MiB (torch copy_), whose
 FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters against a known byte count.
 Everything runs on torch's current stream; the program prints one line per step.
@@ -40,7 +41,13 @@ def main():
         mode, direction = parts[0], parts[1]
         src, tgt = (S, C) if direction.startswith("fwd") else (C, S)
         with GravityField(mode, validate=False) as f:
-            e = autotune.apply_eval(f, src[0].numel(), tgt[0].numel())
+            nblk = int(BD.numel())
+            if direction == "fwdb":
+                e = autotune.apply_blocks(f, "blockfwd", S[0].numel(), nblk)
+            elif direction == "revb":
+                e = autotune.apply_blocks(f, "blocksrc", S[0].numel(), nblk)
+            else:
+                e = autotune.apply_eval(f, src[0].numel(), tgt[0].numel())
             if len(parts) > 2:
                 f.set_config(variant=int(parts[2]), nsplit=int(parts[3]) if len(parts) > 3 else 0)
             if direction == "revb":
