@@ -44,6 +44,24 @@
  * Accumulation modes: ORACLE_SEQ (plain sequential FP64, the reference's order),
  * ORACLE_NEUMAIER (compensated FP64), ORACLE_LONGDOUBLE (x87 80-bit for the
  * pair math and the sums; used to measure the FP64 paths' own rounding).
+ *
+ * What pins each function (the reference holds no golden vectors for this path):
+ *   oracle_eval_plummer    60-digit decimal brute-force KATs (tests/golden/plummer_kat.npz),
+ *       closed forms (one source, far field, eps -> 0, symmetric pair, Plummer sphere),
+ *       exact invariants (linearity, lattice translation, superposition, third law,
+ *       a = -grad phi) and planted mistakes that must fail (tests/test_oracle.py,
+ *       tests/test_oracle_properties.py).
+ *   oracle_eval_flash      closed-form kernel limits and table values, continuity at the
+ *       radius; the periodic sum against independently replicated, cutoff-filtered images
+ *       (numpy) and symmetry points; planted mistakes (tests/test_oracle.py).
+ *   oracle_bidirectional   equals the two directed sums of oracle_eval_plummer; the third
+ *       law (tests/test_oracle.py, tests/test_oracle_properties.py).
+ *   oracle_kick            exactness against numpy (tests/test_oracle.py).
+ *   oracle_block_centres   numpy meshgrid('ij') / ravel('C') (tests/test_oracle.py).
+ *   oracle_pm_*            a particle on a cell centre, quarter-cell splits, mass
+ *       conservation, linearity, central differences exact on quadratics, CIC exact on
+ *       linear fields, the deposit/interpolation adjoint identity, planted mistakes
+ *       (tests/test_oracle_pm.py).
  */
 #include <math.h>
 #include <stdint.h>
