@@ -112,3 +112,37 @@ def test_tune_cache_parser(tmp_path):
         f.write_text(bad)
         assert lib.tgrav_tune_cache_entries(os.fsencode(str(f))) == _lib.EINVAL, bad
     assert lib.tgrav_tune_cache_entries(os.fsencode(str(tmp_path / "missing.json"))) == _lib.EINVAL
+
+
+def test_tune_cache_parser_fuzz(tmp_path):
+    """Random, truncated and deeply nested inputs: the reader answers with an entry count or
+    TGRAV_EINVAL and never crashes or reads past the text (hypothesis, derandomized)."""
+    from hypothesis import HealthCheck, given, settings
+    from hypothesis import strategies as st
+
+    lib = _lib.load()
+    valid = open(os.path.join(os.path.dirname(HEADER), "..", "torch_amd", "tune_cache.json")).read()
+    path = tmp_path / "fuzz.json"
+
+    def check(text):
+        path.write_bytes(text if isinstance(text, bytes) else text.encode("utf-8", "surrogatepass"))
+        rc = lib.tgrav_tune_cache_entries(os.fsencode(str(path)))
+        assert rc == _lib.EINVAL or rc >= 0
+
+    json_values = st.recursive(
+        st.none() | st.booleans() | st.integers() | st.floats(allow_nan=False) | st.text(max_size=8),
+        lambda inner: st.lists(inner, max_size=4) | st.dictionaries(st.text(max_size=6), inner, max_size=4),
+        max_leaves=20)
+
+    @settings(max_examples=300, derandomize=True, deadline=None, database=None,
+              suppress_health_check=list(HealthCheck))
+    @given(cut=st.integers(0, len(valid)), junk=st.binary(max_size=64), doc=json_values,
+           depth=st.integers(0, 3000))
+    def run(cut, junk, doc, depth):
+        check(valid[:cut])                     # every truncation of the committed cache
+        check(valid[:cut].encode() + junk)     # with trailing garbage
+        check(json.dumps(doc))                 # arbitrary JSON documents
+        check("[" * depth + "]" * depth)       # deep nesting
+        check('{"a": ' * depth)                # unterminated nesting
+
+    run()
