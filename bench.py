@@ -497,7 +497,8 @@ def main():
                           "image_interactions_per_step": inter_step * images}
         kern_inter = inter_step * images
         achieved = FLOP_PER_INTERACTION * kern_inter / (r["kernel_ms"] * 1e-3) / 1e12
-        line["roofline"] = {"bound": "valu", "kernel": f"k_eval_flash<{'1,256,periodic' if periodic else '2,256,isolated'}>",
+        line["roofline"] = {"bound": "valu", "kernel": ("k_eval_flash<1,256,periodic>" if periodic
+                                                          else "k_eval_flash_gridsrc<2,256>"),
                             "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                             "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel_ms": r["kernel_ms"],
                             "flop_per_interaction": FLOP_PER_INTERACTION,
