@@ -1,14 +1,19 @@
 #!/usr/bin/env python3
 """Pairwise interactions/s of the direct-sum gravity field on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4|c5|c2fused|c2flash|c1periodic|c2pm] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c4|c5|c2fused|c2flash|c1periodic|c2pm] [--no-cpu]
 
 Default workload c2 (seed 1002): 10,000 Plummer/Kroupa stars (20% hard-binary members) and
 1,024,000 AMR cell centres (2,000 leaf blocks of 8^3).  One step is one bridge field
 evaluation in both directions: stars -> cells (acceleration + potential at the cell
 centres) and cells as point masses (m = rho dx^3, eps = dx) -> stars, each preceded by
 loading its source set.  2 x 1.024e10 pair interactions per step.
-  c5: 100,000 stars <-> 4,194,304 cells (2 x 4.19e11 interactions per step).
+  c5: 100,000 stars <-> 4,194,304 cells (2 x 4.19e11 interactions per step); both kernels are
+      checked against the oracle on 65,536 seeded cells (forward) and 65,536 seeded stars plus
+      every binary member (reverse), about a minute of CPU work.
+  c3: the c1 inputs (1,024 stars <-> 4,096 cells of a 16^3 grid, explicit arrays) in both
+      directions; both kernels against the oracle over every target, with the condition number
+      kappa = sum |terms| / |sum| of each target's acceleration.
   c4: 64 uneven independent problems in one graph-replayed persistent launch (one step =
       one replay, acc + pot of all problems).
   c2fused: c2 with symmetric softening (eps2 = eps_cell^2 + eps_star^2) evaluated by the fused
@@ -111,7 +116,7 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
     log(f"{mode}: {'block' if blocks else 'explicit'} format, tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
-    B = (dev["bo"], dev["bdx"])
+    B = (dev.get("bo"), dev.get("bdx"))
 
     def step():
         fwd.set_sources(*S)
@@ -370,6 +375,46 @@ def cpu_baseline(S, T, G, budget_s, label, flash=None):
     return cb
 
 
+def acc_condition(S, T, G):
+    """kappa = sum_j |G m_j d_ij / r_ij^3| / |a_i| per target (numpy; small inputs)."""
+    s = np.stack([S["x"], S["y"], S["z"]], 1)
+    t = np.stack([T["x"], T["y"], T["z"]], 1)
+    d = s[None, :, :] - t[:, None, :]
+    r2 = (d * d).sum(-1) + (S["eps"] ** 2)[None, :] + (T["eps_t"] ** 2 if "eps_t" in T else 0.0)
+    ok = r2 > 0
+    w = np.where(ok, G * S["m"][None, :] / np.where(ok, r2, 1.0) ** 1.5, 0.0)
+    a = np.einsum("ij,ijk->ik", w, d)
+    return (w * np.sqrt((d * d).sum(-1))).sum(1) / np.linalg.norm(a, axis=1)
+
+
+def oracle_errors(results, k, S, T, idx, G, label, extra=None):
+    """Each mode's result (direction k) against the compensated oracle on targets idx;
+    extra = {name: boolean mask over idx} adds stats for subsets."""
+    import oracle
+    t0 = time.perf_counter()
+    et = T.get("eps_t")
+    ref = [np.empty(len(idx)) for _ in range(4)]
+    step = max(1, int(4e10 // max(1, len(S["x"]))))     # ~10 s of oracle work between progress lines
+    for a in range(0, len(idx), step):
+        j = idx[a:a + step]
+        part = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S.get("eps"), T["x"][j], T["y"][j],
+                                   T["z"][j], None if et is None else et[j], G, mode=oracle.NEUMAIER,
+                                   threads=16)
+        for q in range(4):
+            ref[q][a:a + len(j)] = part[q]
+        if len(idx) > step:
+            log(f"oracle {label}: {a + len(j)} / {len(idx)} targets")
+    ref = tuple(ref)
+    log(f"oracle {label}: {len(idx)} targets x {len(S['x'])} sources in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    for mode, r in results.items():
+        ea, ep = rel_err(tuple(a[idx] for a in r["res"][k]), ref)
+        out[mode] = err_stats(ea, ep)
+        for name, mask in (extra or {}).items():
+            out[mode][name] = err_stats(ea[mask], ep[mask])
+    return out
+
+
 def oracle_check(pairs, G):
     """FP64 GPU results vs the compensated oracle on seeded subsamples (the checker).
     pairs: [(S, T, gpu_result, n_sub)]."""
@@ -394,8 +439,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5", "c2fused", "c2flash", "c1periodic",
-                                                          "c2pm"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c2fused", "c2flash",
+                                                          "c1periodic", "c2pm"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--modes", default="mixed,fp64")
@@ -517,6 +562,50 @@ def main():
             log("fp64 vs oracle:", line["fp64_vs_oracle"])
             line["cpu_baseline"] = cpu_baseline(S, T, G, args.cpu_seconds, f"{args.workload} reverse direction",
                                                 flash=dict(rsoft=rsoft, box=box, nrep=nrep, images=images))
+        print(json.dumps(line), flush=True)
+        return
+    if args.workload == "c3":
+        w = W.c1(1001)
+        st, ce = w["stars"], w["cells"]
+        ns, nc = len(st["x"]), len(ce["x"])
+        dev = dict(sx=d(st["x"]), sy=d(st["y"]), sz=d(st["z"]), sm=d(st["m"]), se=d(st["eps"]),
+                   cx=d(ce["x"]), cy=d(ce["y"]), cz=d(ce["z"]), cm=d(ce["m"]), ce=d(ce["eps"]))
+        torch.cuda.synchronize()
+        inter_step = 2.0 * ns * nc
+        for mode in modes:
+            results[mode] = run_pair(mode, dev, ns, nc, args.steps, args.warmup, blocks=False)
+            r = results[mode]
+            log(f"{mode}: {r['ms']:.3f} ms/step ({inter_step / r['ms'] * 1e3:.3e} interactions/s); eval "
+                f"kernels fwd {r['fwd_kernel_ms']:.3f} ms, rev {r['rev_kernel_ms']:.3f} ms")
+        r = results[head]
+        line["value"] = inter_step / (r["ms"] * 1e-3)
+        line["ms_per_step"] = r["ms"]
+        line["config"] = {"workload": "c3: the c1 inputs through both kernels -- 1,024 stars (eps 0.01 pc) -> "
+                                      "4,096 cell centres of a 16^3 grid on [-2, 2]^3 pc, and the cells "
+                                      "(m = rho dx^3, eps = dx) -> the stars; explicit arrays, acc + pot",
+                          "seed": w["seed"], "mode": head, "n_stars": ns, "n_cells": nc,
+                          "interactions_per_step": inter_step}
+        kern_ms = r["fwd_kernel_ms"]
+        achieved = FLOP_PER_INTERACTION * ns * nc / (kern_ms * 1e-3) / 1e12
+        peak = PEAK_FP32_TFLOPS if head == "mixed" else PEAK_FP64_TFLOPS
+        line["roofline"] = {"bound": "valu", "kernel": f"k_eval_{head} (forward, generic targets)",
+                            "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                            "traffic": None, "kernel_ms": kern_ms, "flop_per_interaction": FLOP_PER_INTERACTION,
+                            "interactions_per_launch": float(ns) * nc,
+                            "note": "4.2e6 interactions per launch: launch-latency bound, an accuracy workload"}
+        if not args.no_cpu:
+            S = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
+            C = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
+            acc = {}
+            for k, (src, tgt, nm) in enumerate(((S, C, "fwd"), (C, S, "rev"))):
+                kap = acc_condition(src, tgt, G)
+                acc[nm] = oracle_errors(results, k, src, tgt, np.arange(len(tgt["x"])), G, f"c3 {nm}")
+                acc[nm]["kappa"] = dict(median=float(np.median(kap)), max=float(kap.max()))
+            line["error_vs_oracle"] = acc
+            line["error_gate_met"] = all(acc[nm]["mixed"]["acc_median"] <= 1e-6 and acc[nm]["mixed"]["acc_max"] <= 1e-4
+                                         for nm in acc) if "mixed" in results else None
+            log("vs oracle:", json.dumps(acc))
+            line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, "forward c3 direction")
         print(json.dumps(line), flush=True)
         return
     if args.workload in ("c2", "c5", "c2fused"):
@@ -644,10 +733,28 @@ def main():
             C = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
             if args.workload == "c2fused":      # symmetric softening: targets carry their eps too
                 S["eps_t"], C["eps_t"] = st["eps"], ce["eps"]
-            if "fp64" in results:
+            if "fp64" in results and args.workload == "c5":
+                # 65,536 seeded cells (forward); 65,536 seeded stars and every binary member (reverse)
+                rng = np.random.Generator(np.random.PCG64(55))
+                idx_f = np.sort(rng.choice(nc, 65536, replace=False))
+                pick = np.zeros(ns, bool)
+                pick[rng.choice(ns, 65536, replace=False)] = True
+                pick |= st["binary"] >= 0
+                idx_r = np.nonzero(pick)[0]
+                rep = {"fwd": oracle_errors(results, 0, S, C, idx_f, G, "c5 fwd"),
+                       "rev": oracle_errors(results, 1, S=C, T=S, idx=idx_r, G=G, label="c5 rev",
+                                            extra={"binary_members": st["binary"][idx_r] >= 0})}
+                rep["targets"] = {"fwd": len(idx_f), "rev": len(idx_r),
+                                  "rev_binary_members": int((st["binary"] >= 0).sum())}
+                line["error_vs_oracle"] = rep
+                line["fp64_vs_oracle"] = dict(targets_checked=len(idx_f) + len(idx_r),
+                                              max_rel_acc=max(rep[k]["fp64"]["acc_max"] for k in ("fwd", "rev")),
+                                              max_rel_pot=max(rep[k]["fp64"]["pot_max"] for k in ("fwd", "rev")))
+                log("vs oracle:", json.dumps(rep))
+            elif "fp64" in results:
                 line["fp64_vs_oracle"] = oracle_check(
-                    [(S, C, results["fp64"]["res"][0], 1024 if args.workload == "c2" else 256),
-                     (C, S, results["fp64"]["res"][1], 96 if args.workload == "c2" else 24)], G)
+                    [(S, C, results["fp64"]["res"][0], 1024),
+                     (C, S, results["fp64"]["res"][1], 96)], G)
                 log("fp64 vs oracle:", line["fp64_vs_oracle"])
             line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, f"forward {args.workload} direction")
         else:

@@ -80,3 +80,43 @@ def test_two_contexts_on_two_streams(problem):
         np.testing.assert_array_equal(p.cpu().numpy(), q)
     for p, q in zip(og, ref_g):
         np.testing.assert_array_equal(p.cpu().numpy(), q)
+
+
+def test_contexts_on_two_host_threads(problem):
+    """Two host threads, each with its own context, evaluating at the same time (the library
+    keeps no mutable global state; ctypes releases the GIL during the calls): every result is
+    bit-identical to the same evaluation run alone."""
+    import threading
+    st, t = problem
+    S = [st[k] for k in ("x", "y", "z", "m", "eps")]
+    half = t.shape[1] // 2
+    parts = {"fp64": (t[0][:half], t[1][:half], t[2][:half]),
+             "mixed": (t[0][half:], t[1][half:], t[2][half:])}
+    ref = {}
+    for mode, (x, y, z) in parts.items():
+        with GravityField(mode) as f:
+            f.set_sources(*S)
+            ref[mode] = f.evaluate(None, x, y, z)
+    got, errors = {mode: [] for mode in parts}, []
+
+    def work(mode):
+        try:
+            x, y, z = parts[mode]
+            with GravityField(mode) as f:
+                for _ in range(6):
+                    f.set_sources(*S)
+                    got[mode].append(f.evaluate(None, x, y, z))
+        except Exception as e:          # surfaced below, in the test's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(mode,)) for mode in parts]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not errors, errors
+    for mode in parts:
+        assert len(got[mode]) == 6
+        for res in got[mode]:
+            for p, q in zip(res, ref[mode]):
+                np.testing.assert_array_equal(p, q)
