@@ -331,10 +331,11 @@ def mean_images_in_cutoff(S, T, box, nrep, npairs=2000, seed=99):
 
 
 # ----------------------------------------------------------- checker legs ----
-def cpu_baseline(S, T, G, budget_s, label, flash=None):
+def cpu_baseline(S, T, G, budget_s, label, flash=None, reverse=None):
     """The oracle (C port of the reference loop nest, -O2) on seeded target subsamples:
     one thread, then all host cores (<= 16).  flash = dict(rsoft, box, nrep) times the FLASH
-    kernel instead of the Plummer one."""
+    kernel instead of the Plummer one.  reverse = (S2, T2): the other direction too, on all
+    cores, whole when it fits in half the budget (c2's 1.024e10 interactions do)."""
     import oracle
     ns = len(S["x"])
     rng = np.random.Generator(np.random.PCG64(4242))
@@ -365,13 +366,29 @@ def cpu_baseline(S, T, G, budget_s, label, flash=None):
         out[threads] = dict(value=n * ns / dt, seconds=dt, targets=n)
         log(f"cpu oracle {threads} thread(s): {n} targets x {ns} sources in {dt:.2f} s "
             f"= {n * ns / dt:.3e} interactions/s")
+    rev = None
+    if reverse is not None:
+        S2, T2 = reverse
+        ns2 = len(S2["x"])
+        n = int(min(len(T2["x"]), max(16, budget_s * 0.5 * rate1 * (0.8 * cores) / ns2)))
+        idx = np.sort(rng.choice(len(T2["x"]), n, replace=False))
+        t = time.perf_counter()
+        oracle.eval_plummer(S2["x"], S2["y"], S2["z"], S2["m"], S2["eps"], T2["x"][idx], T2["y"][idx],
+                            T2["z"][idx], None, G, mode=oracle.SEQ, threads=cores)
+        dt = time.perf_counter() - t
+        rev = dict(value=n * ns2 / dt, seconds=dt, targets=n, sources=ns2, whole=n == len(T2["x"]))
+        log(f"cpu oracle reverse, {cores} thread(s): {n} targets x {ns2} sources in {dt:.2f} s "
+            f"= {n * ns2 / dt:.3e} interactions/s")
     cb = {"value": out[cores]["value"], "unit": "interactions/s", "cores": cores, "kind": "port",
           "sample": f"{label}: {out[cores]['targets']} seeded targets x {ns} sources "
                     f"(acc + pot, FP64, {'Plummer' if flash is None else 'FLASH spline'}"
                     f"{'' if flash is None or flash.get('box') is None else ', periodic images'}, "
                     f"OpenMP over targets); one thread: "
                     f"{out[1]['value']:.3e} interactions/s on {out[1]['targets']} targets",
-          "scalar_value": out[1]["value"]}
+          "scalar_value": out[1]["value"], "host_cpus": os.cpu_count(),
+          "whole_direction": out[cores]["targets"] == len(T["x"])}
+    if rev is not None:
+        cb["reverse"] = rev
     return cb
 
 
@@ -605,7 +622,7 @@ def main():
             line["error_gate_met"] = all(acc[nm]["mixed"]["acc_median"] <= 1e-6 and acc[nm]["mixed"]["acc_max"] <= 1e-4
                                          for nm in acc) if "mixed" in results else None
             log("vs oracle:", json.dumps(acc))
-            line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, "forward c3 direction")
+            line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, "forward c3 direction", reverse=(C, S))
         print(json.dumps(line), flush=True)
         return
     if args.workload in ("c2", "c5", "c2fused"):
@@ -756,7 +773,8 @@ def main():
                     [(S, C, results["fp64"]["res"][0], 1024),
                      (C, S, results["fp64"]["res"][1], 96)], G)
                 log("fp64 vs oracle:", line["fp64_vs_oracle"])
-            line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, f"forward {args.workload} direction")
+            line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, f"forward {args.workload} direction",
+                                                reverse=None if args.workload == "c2fused" else (C, S))
         else:
             if "fp64" in results:
                 chk = [(probs[i]["sources"], probs[i]["targets"], results["fp64"]["res"][i], 48)
