@@ -67,7 +67,7 @@ def within_mixed_bound(got, exp, s, m, se, t, te, G):
 
 def term_scale_bounds(s, m, se2, t, te2, G, mode, anchor, fold=512):
     """Per-target error bounds, condition-aware (relative to the sum of |terms|, so cancelling
-    fields are judged fairly), for property tests on small inputs.
+    fields are compared fairly), for property tests on small inputs.
 
     s[ns, 3], m[ns], se2[ns] (source eps^2), t[nt, 3], te2[nt]; anchor: per-target (nt) or
     per-pair (nt, ns) bound on the distance from the target to the FP64 anchor its mixed

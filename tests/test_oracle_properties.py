@@ -3,7 +3,7 @@
 The same lattice inputs as the GPU property tests (spacing 1/64 pc): displacements are exact,
 so translation by a lattice vector and scaling the masses by 4 are exact properties of the
 sequential FP64 sum; superposition, the summation modes and Newton's third law hold to the
-rounding of the sums, judged against the sum of |terms|.
+rounding of the sums, measured against the sum of |terms|.
 """
 import numpy as np
 from hypothesis import HealthCheck, given, settings
