@@ -28,14 +28,14 @@ loading its source set.  2 x 1.024e10 pair interactions per step.
   c2pm: the particle-mesh pieces beside the kick on c2's 2,000 leaf blocks with 4 guard layers
       (8,192,000 cells): CIC deposit of the 9,174 stars inside the box, finite-difference
       acceleration fused with the grid kick, CIC interpolation back to the stars (the Poisson
-      solve between them is not part of this build; a Plummer potential stands in).  FP64;
+      solve between them is not part of this library; a Plummer potential stands in).  FP64;
       the rate is guarded cells per second of the whole step.
 Inputs are resident in HBM (device tensors) before timing.
 
 The headline `value` is the mixed-precision rate; the plain-FP64 kernel is timed the same
 way and reported beside it (`fp64_value`, `mixed_over_fp64`).  Mixed-precision error is
 measured over all targets against the FP64 kernel, which is itself checked against the
-CPU oracle on a seeded subsample (the checker leg).  The roofline uses the forward (c2, c5)
+CPU oracle on a seeded subsample (the CPU leg).  The roofline uses the forward (c2, c5)
 or persistent (c4) evaluation kernel alone, timed with HIP events on its own stream through
 the library's instrumentation hook.  Prints one JSON line on stdout; progress on stderr.
 """
@@ -429,7 +429,7 @@ def oracle_errors(results, k, S, T, idx, G, label, extra=None):
 
 
 def oracle_check(pairs, G):
-    """FP64 GPU results vs the compensated oracle on seeded subsamples (the checker).
+    """FP64 GPU results vs the compensated oracle on seeded subsamples.
     pairs: [(S, T, gpu_result, n_sub)]."""
     import oracle
     rng = np.random.Generator(np.random.PCG64(77))
@@ -460,7 +460,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the field evaluation is built and measured on a "
-              "single MI355X (multi-GPU sharding is a design note, not part of this build).",
+              "single MI355X (multi-GPU sharding is a design note, not implemented here).",
               file=sys.stderr)
         sys.exit(2)
 

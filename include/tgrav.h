@@ -49,7 +49,7 @@ extern "C" {
 #define TGRAV_EHIP -3        /* a HIP runtime call or kernel launch failed */
 #define TGRAV_ENONFINITE -4  /* validation found NaN/Inf in an input */
 #define TGRAV_ENOMEM -5      /* device allocation failed */
-#define TGRAV_ELIMIT -6      /* size beyond what this build supports */
+#define TGRAV_ELIMIT -6      /* size beyond what the library supports */
 
 typedef struct tgrav_ctx tgrav_ctx;
 typedef struct tgrav_batch tgrav_batch;

@@ -1,8 +1,8 @@
 """CPU oracle for the softened direct-sum gravity field -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker (or as the timed
-CPU baseline).  The product (``torch_amd``) never imports it.
+``bench.py`` may import this package, and only to check results against (or as
+the timed CPU baseline).  The product (``torch_amd``) never imports it.
 
 ``liboracle.so`` is built from ``tgrav_oracle.c`` by ``oracle/Makefile``; each C
 function cites the reference file:line it restates.  ``oracle.brute`` holds a

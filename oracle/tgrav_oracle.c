@@ -2,8 +2,8 @@
  * tgrav oracle -- CPU FP64 restatement of Torch's softened direct-sum gravity field.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * (or as the timed CPU baseline).  The product path (torch_amd/libtgrav.so)
+ * cpu_baseline leg of bench.py may load this library, and only to check results
+ * against (or as the timed CPU baseline).  The product path (torch_amd/libtgrav.so)
  * never calls into it.
  *
  * What it restates (all FP64, as FLASH builds promote `real` to double,
