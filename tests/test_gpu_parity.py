@@ -284,6 +284,40 @@ def test_device_pointers_match_host_pointers(fp64, c2):
 
 # ------------------------------------------------------------- kick/blocks ----
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("layout", ["c1 grid", "random cloud", "c2 cells shuffled"])
+def test_kick_with_unblocked_sources_multiple_of_512(mode, layout, c2):
+    """Explicit sources whose count is a multiple of 512 but which are not 8^3 leaf blocks in C
+    order: the block-source kernel is launched as a candidate and must stand down, so the fused
+    kick adds dt * a exactly once."""
+    rng = W.rng_for(57)
+    if layout == "c1 grid":
+        w = W.c1(1001)
+        ce, st = w["cells"], w["stars"]
+        src = [ce[k] for k in ("x", "y", "z", "m", "eps")]
+        tgt = [st[k] for k in ("x", "y", "z")]
+    elif layout == "random cloud":
+        src = list(rng.uniform(-1, 1, (3, 1024))) + [rng.uniform(0.1, 1, 1024), np.full(1024, 0.02)]
+        tgt = list(rng.uniform(-1, 1, (3, 3000)))
+    else:
+        ce, st = c2["cells"], c2["stars"]
+        perm = rng.permutation(len(ce["x"]))
+        src = [ce[k][perm] for k in ("x", "y", "z", "m", "eps")]
+        tgt = [st[k][:2000] for k in ("x", "y", "z")]
+    assert len(src[0]) % 512 == 0
+    dt = 0.01
+    with GasGravityField(mode) as f:
+        f.set_sources(*src)
+        b = f.evaluate(None, *tgt)
+        v0 = [rng.normal(0, 1, len(tgt[0])) for _ in range(3)]
+        v = [q.copy() for q in v0]
+        a = f.evaluate_and_kick(None, *tgt, dt, *v)
+    for p, q in zip(a[:3], b[:3]):
+        assert np.array_equal(p, q)
+    for p, q in zip(v, oracle.kick(dt, b[0], b[1], b[2], *v0)):
+        assert np.array_equal(p, q)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_fused_kick_equals_eval_then_kick(mode, c2):
     st, ce = c2["stars"], c2["cells"]
     dt = 0.5 * 1.3e-3

@@ -465,6 +465,7 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __rest
                                                      int64_t chunk, const double* e2s_min_p, Out O) {
   typedef typename LdsGS<MIXED>::T TT;
   __shared__ LdsGS<MIXED> L;
+  if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
   const int tid = threadIdx.x;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t b0 = (int64_t)blockIdx.y * chunk;
