@@ -449,10 +449,12 @@ static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t
   int want = force >= 0 ? force : c->nsplit[mode];
   if (want <= 0) {
     // About 8 rounds of workgroups at ~3 resident waves per SIMD, at least 4 source tiles a
-    // chunk, and at most 12 x 2^20 target-chunks of partial sums (32 B each).
+    // chunk (1 when that would leave fewer workgroups than CUs: small problems, where the
+    // partial sums are few), and at most 12 x 2^20 target-chunks of partial sums (32 B each).
     const int64_t target_wg = 8LL * 12 * c->num_cu / (v->nt / 64);
     int64_t s = (target_wg + *ntiles - 1) / *ntiles;
-    const int64_t maxs = (ns + 4LL * v->ts - 1) / (4LL * v->ts);
+    int64_t maxs = (ns + 4LL * v->ts - 1) / (4LL * v->ts);
+    if (*ntiles * maxs < c->num_cu) maxs = (ns + v->ts - 1) / v->ts;
     const int64_t maxp = std::max<int64_t>(1, (12LL << 20) / std::max<int64_t>(nt, 1));
     s = std::min<int64_t>(s, std::min<int64_t>(maxs, maxp));
     want = (int)(s < 1 ? 1 : s);
@@ -476,10 +478,13 @@ static void grid_shape(const tgrav_ctx* c, int mode, int64_t ns, int64_t nblk, i
   if (want <= 0) {
     // One-wave workgroups, ~3 resident per SIMD: aim at ~8 rounds of them so the last round's
     // imbalance is small, with at most 12 chunks (each adds 32 B of partial sums per target)
-    // and at least 4 source tiles per chunk.
+    // and at least 4 source tiles per chunk -- unless that leaves fewer workgroups than CUs
+    // (few blocks, few sources): then down to one source tile per chunk.
     const int64_t target = 8LL * 12 * c->num_cu;
-    want = (int)std::min<int64_t>((target + nblk - 1) / nblk,
-                                  std::min<int64_t>(12, std::max<int64_t>(1, ns / 256)));
+    const int64_t per = (target + nblk - 1) / nblk;
+    int64_t w = std::min<int64_t>(per, std::min<int64_t>(12, std::max<int64_t>(1, ns / 256)));
+    if (nblk * w < c->num_cu) w = std::min<int64_t>(per, std::max<int64_t>(1, ns / 64));
+    want = (int)std::min<int64_t>(w, 65535);
   }
   if (want > 65535) want = 65535;
   int64_t ch = (ns + want - 1) / want;
@@ -534,9 +539,12 @@ static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk
   if (tuned > 0) want = tuned;
   if (want <= 0) {
     // Four-wave workgroups, ~3 resident per CU: aim at ~16 rounds, with at least 4 source
-    // blocks per chunk.
+    // blocks per chunk (1 when that leaves fewer workgroups than CUs).
     const int64_t target = 16LL * 3 * c->num_cu;
-    want = (int)std::min<int64_t>((target + *ntiles - 1) / *ntiles, std::max<int64_t>(1, nblk / 4));
+    const int64_t per = (target + *ntiles - 1) / *ntiles;
+    int64_t w = std::min<int64_t>(per, std::max<int64_t>(1, nblk / 4));
+    if (*ntiles * w < c->num_cu) w = std::min<int64_t>(per, nblk);
+    want = (int)std::min<int64_t>(w, 65535);
   }
   if (want > 65535) want = 65535;
   *chunk = (nblk + want - 1) / want;
