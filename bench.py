@@ -545,6 +545,14 @@ def main():
         line["value"] = inter_step / (r["ms"] * 1e-3)
         line["ms_per_step"] = r["ms"]
         line["dtype"] = "f64"
+        if not periodic:   # the same cells as explicit arrays (block C order, detected on the device)
+            rx = run_flash({k: v for k, v in dev.items() if k not in ("bo", "bdx")}, args.steps, args.warmup,
+                           rsoft, box, nrep)
+            same = all(np.array_equal(p, q) for p, q in zip(rx["res"], r["res"]))
+            log(f"c2flash explicit arrays: {rx['ms']:.3f} ms/step, kernel {rx['kernel_ms']:.3f} ms, "
+                f"bit-identical to the block form: {same}")
+            line["explicit_arrays"] = {"value": inter_step / (rx["ms"] * 1e-3), "ms_per_step": rx["ms"],
+                                       "kernel_ms": rx["kernel_ms"], "bit_identical_to_block_form": same}
         line["config"] = {"workload": (f"c1periodic: 4,096 cells (16^3, 4 pc box) -> 1,024 stars, FLASH spline "
                                        f"softening (radius {rsoft} pc), periodic images n in [-6, 6]^3 within "
                                        f"24 pc, acc + pot" if periodic else

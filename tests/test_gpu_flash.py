@@ -104,7 +104,9 @@ def test_flash_isolated_forward_blocks_equal_centres(c1, kernel):
 def test_flash_block_sources(field, kernel):
     """Cells handed over as 8^3 leaf blocks: the generic kernel on them gives the same sums as
     the same cells as points, bit for bit; the column kernel (the default for block sources)
-    matches the oracle, including stars placed within the softening radius of cells."""
+    matches the oracle, including stars placed within the softening radius of cells; the cells
+    as explicit arrays in the blocks' C order are recognised and give the column kernel's bits;
+    shuffled, they take the generic kernel alone (a fused kick adds dt * a once)."""
     rng = W.rng_for(42)
     cells = W.gas_cells(rng, box=4.0, base_n=2, n_leaves=3, sigma=1.0, ngrid=16, rho_mean=10.0)
     st = W.star_cluster(rng, 300)
@@ -115,10 +117,10 @@ def test_flash_block_sources(field, kernel):
     st["y"][40], st["z"][40] = cells["y"][pick[0]], cells["z"][pick[0]]
     rs = 0.2
     field.set_softening(kernel, rs, quirks=True)
+    field.set_block_kernel(False)
     field.set_sources(cells["x"], cells["y"], cells["z"], cells["m"], cells["eps"])
     a = field.evaluate(None, st["x"], st["y"], st["z"])
     field.set_source_blocks(cells["block_origin"], cells["block_size"] / 8, cells["m"])
-    field.set_block_kernel(False)
     b = field.evaluate(None, st["x"], st["y"], st["z"])
     for c in range(4):
         np.testing.assert_array_equal(a[c], b[c])
@@ -128,6 +130,22 @@ def test_flash_block_sources(field, kernel):
                             kernel=kernel, quirks=True, threads=16)
     sa, sp = term_scales(cells["x"], cells["y"], cells["z"], cells["m"], st["x"], st["y"], st["z"], rs)
     check(col, ref, sa, sp)
+    field.set_sources(cells["x"], cells["y"], cells["z"], cells["m"], cells["eps"])
+    ex = field.evaluate(None, st["x"], st["y"], st["z"])
+    for c in range(4):
+        np.testing.assert_array_equal(ex[c], col[c])
+    perm = rng.permutation(len(cells["x"]))
+    assert len(perm) % 512 == 0
+    field.set_sources(*(cells[k][perm] for k in ("x", "y", "z", "m", "eps")))
+    sh = field.evaluate(None, st["x"], st["y"], st["z"])
+    check(sh, ref, sa, sp)
+    v0 = [rng.normal(0, 1, len(st["x"])) for _ in range(3)]
+    v = [q.copy() for q in v0]
+    kicked = field.evaluate_and_kick(None, st["x"], st["y"], st["z"], 0.01, *v)
+    for c in range(3):
+        np.testing.assert_array_equal(kicked[c], sh[c])
+    for p, q in zip(v, oracle.kick(0.01, sh[0], sh[1], sh[2], *v0)):
+        np.testing.assert_array_equal(p, q)
 
 
 def test_flash_decisions_at_rsoft(field):

@@ -188,6 +188,7 @@ template <int K, int NT, bool PER, bool S32, bool QUIRKS = false>
 __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restrict__ src4,
                                                    int64_t ns, int64_t chunk, Flash P, Out O) {
   __shared__ double4 ls[NT];
+  if (gated_off(O)) return;
   const int tid = threadIdx.x;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t sb = (int64_t)blockIdx.y * chunk;
@@ -348,6 +349,7 @@ __global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double* 
                                                            int64_t nblk, int64_t chunk, Flash P,
                                                            Out O) {
   __shared__ LdsGS<false> L;
+  if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
   const int tid = threadIdx.x;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t b0 = (int64_t)blockIdx.y * chunk;

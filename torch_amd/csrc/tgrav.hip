@@ -611,46 +611,33 @@ static double sqrt_lt_threshold(double r) {
 #ifndef TG_FLASH_GS_K
 #define TG_FLASH_GS_K 2
 #endif
-static int launch_flash_gridsrc(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O,
-                                const tg::Flash& P, bool s32) {
+static void flash_gridsrc_shape(const tgrav_ctx* c, int64_t nt, int64_t nblk, int64_t* ntiles,
+                                int* nsplit, int64_t* chunk) {
   const int K = TG_FLASH_GS_K, NT = 256;
-  const int64_t nt = T.n, nblk = c->src_nblk;
-  const int64_t ntiles = (nt + K * NT - 1) / (K * NT);
-  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets");
-  // As the Plummer block-source kernel: about 16 rounds of workgroups, at least 4 blocks a chunk.
+  *ntiles = (nt + K * NT - 1) / (K * NT);
+  // As the Plummer block-source kernel: about 16 rounds of workgroups, at least 4 blocks a chunk
+  // (1 when that leaves fewer workgroups than CUs).
   const int64_t target = 16LL * 3 * c->num_cu;
-  int64_t want = std::min<int64_t>((target + ntiles - 1) / ntiles, std::max<int64_t>(1, nblk / 4));
+  const int64_t per = (target + *ntiles - 1) / *ntiles;
+  int64_t want = std::min<int64_t>(per, std::max<int64_t>(1, nblk / 4));
+  if (*ntiles * want < c->num_cu) want = std::min<int64_t>(per, nblk);
   if (c->nsplit[TGRAV_PLAIN_FP64] > 0) want = c->nsplit[TGRAV_PLAIN_FP64];   // tgrav_set_config
   want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
-  const int64_t chunk = (nblk + want - 1) / want;
-  const int nsplit = (int)((nblk + chunk - 1) / chunk);
-  tg::Out o = O;
-  if (nsplit > 1) {
-    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
-    if (rc0) return rc0;
-    o.part = c->part.as<double>();
-    o.pstride = nt;
-  } else {
-    o.part = nullptr;
-  }
-  int rc = timed_launch(c, nt, c->ns, [&]() -> int {
-    const dim3 g((unsigned)ntiles, (unsigned)nsplit);
-    if (s32)
-      tg::k_eval_flash_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
-          T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
-    else
-      tg::k_eval_flash_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
-          T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
-    LAUNCHCHECK("k_eval_flash_gridsrc");
-    return 0;
-  });
-  if (rc) return rc;
-  if (nsplit > 1) {
-    tg::Out fin = O;
-    fin.part = nullptr;
-    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
-    if (rc1) return rc1;
-  }
+  *chunk = (nblk + want - 1) / want;
+  *nsplit = (int)((nblk + *chunk - 1) / *chunk);
+}
+
+static int flash_gridsrc_kernel(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& o, const tg::Flash& P,
+                                bool s32, int64_t ntiles, int nsplit, int64_t chunk, int64_t nblk) {
+  const int K = TG_FLASH_GS_K, NT = 256;
+  const dim3 g((unsigned)ntiles, (unsigned)nsplit);
+  if (s32)
+    tg::k_eval_flash_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
+        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+  else
+    tg::k_eval_flash_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
+        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+  LAUNCHCHECK("k_eval_flash_gridsrc");
   return 0;
 }
 
@@ -683,7 +670,17 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   // Pairs reaching the refined 1/sqrt have r2 >= rsoft^2 (the rest take the softened
   // branch); the FP32 seed is valid from 1e-30 up.
   const bool s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
-  if (!per && !T.blocks && c->src_nblk > 0 && c->block_kernel) return launch_flash_gridsrc(c, T, O, P, s32);
+  // Sources given as 8^3 blocks take the block-source kernel.  Explicit sources that may be
+  // C-ordered 8^3 blocks (flagged on the device when they were set) launch it too, gated against
+  // the generic kernel on that flag, as in the Plummer path.
+  const bool isolated_blocks = !per && !T.blocks && c->block_kernel;
+  const bool srcb = isolated_blocks && c->src_nblk > 0;
+  const bool cand = isolated_blocks && !srcb && c->src_cols > 0;
+  int64_t ntiles_s = 0, chunk_s = 0;
+  int nsplit_s = 1;
+  const int64_t nblk_s = srcb ? c->src_nblk : c->src_cols;
+  if (srcb || cand) flash_gridsrc_shape(c, nt, nblk_s, &ntiles_s, &nsplit_s, &chunk_s);
+  if (ntiles_s > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
   const int64_t ntiles = (nt + (int64_t)K * NT - 1) / ((int64_t)K * NT);
   if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
   const int64_t target_wg = (per ? 8LL : 16LL) * c->num_cu;
@@ -695,17 +692,32 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
     want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
   }
   const int64_t chunk = (ns + want - 1) / want;
-  const int nsplit = (int)((ns + chunk - 1) / chunk);
-  tg::Out o = O;
-  if (nsplit > 1) {
-    int rc0 = c->part.ensure((size_t)nsplit * 4 * nt * sizeof(double));
+  const int nsplit = srcb ? 1 : (int)((ns + chunk - 1) / chunk);
+  const int nsp = std::max(nsplit, (srcb || cand) ? nsplit_s : 1);
+  if (nsp > 1) {
+    int rc0 = c->part.ensure((size_t)nsp * 4 * nt * sizeof(double));
     if (rc0) return rc0;
-    o.part = c->part.as<double>();
-    o.pstride = nt;
-  } else {
-    o.part = nullptr;
+  }
+  tg::Out o = O, os = O;
+  o.part = nsplit > 1 ? c->part.as<double>() : nullptr;
+  os.part = nsplit_s > 1 ? c->part.as<double>() : nullptr;
+  o.pstride = os.pstride = nt;
+  if (cand) {
+    int rc0 = c->colflag.ensure(sizeof(int));
+    if (rc0) return rc0;
+    int* flag = c->colflag.as<int>();
+    tg::k_eval_gate<<<1, 64, 0, c->stream>>>(flag, c->scolflag.as<const int>(), 0);
+    LAUNCHCHECK("k_eval_gate");
+    o.gate = os.gate = flag;
+    o.gate_on = 0;
+    os.gate_on = 2;
   }
   int rc = timed_launch(c, nt, ns, [&]() -> int {
+    if (srcb || cand) {
+      int r = flash_gridsrc_kernel(c, T, os, P, s32, ntiles_s, nsplit_s, chunk_s, nblk_s);
+      if (r) return r;
+    }
+    if (srcb) return 0;
     const dim3 g((unsigned)ntiles, (unsigned)nsplit);
     const double4* s4 = c->s4.as<double4>();
     if (per) {
@@ -725,8 +737,14 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
     return 0;
   });
   if (rc) return rc;
-  if (nsplit > 1) {
-    tg::Out fin = O;
+  if ((srcb || cand) && nsplit_s > 1) {
+    tg::Out fin = os;
+    fin.part = nullptr;
+    int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit_s, nt, fin);
+    if (rc1) return rc1;
+  }
+  if (!srcb && nsplit > 1) {
+    tg::Out fin = o;
     fin.part = nullptr;
     int rc1 = reduce_chunks(c, c->part.as<double>(), nt, nsplit, nt, fin);
     if (rc1) return rc1;
