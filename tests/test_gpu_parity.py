@@ -197,6 +197,44 @@ def test_empty_inputs(mode):
         assert all(o.shape == (0,) for o in out)
 
 
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("device", [False, True])
+def test_empty_inputs_every_entry_point(mode, device):
+    """Zero targets, zero sources, zero particles or zero cells, host or device arrays, with
+    input validation on: every entry point returns empty or zero fields (an empty launch is a
+    no-op, not an invalid configuration)."""
+    import torch
+    rng = W.rng_for(58)
+    d = (lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device="cuda")) if device else \
+        (lambda a: np.asarray(a, dtype=np.float64))
+    h = (lambda t: t.cpu().numpy()) if device else (lambda t: t)
+    z = d(np.zeros(0))
+    o, dx = d(np.zeros((1, 3))), d([0.1])
+    cm, ce = d(rng.uniform(0.1, 1, 512)), d([0.01])
+    p = [d(rng.uniform(-1, 1, 7)) for _ in range(3)]
+    pm, pe = d(rng.uniform(0.1, 1, 7)), d(np.full(7, 0.02))
+    with GravityField(mode) as f:
+        co, po = f.bidirectional_blocks(o, dx, cm, ce, z, z, z, z, z)     # no particles
+        assert all(np.all(h(a) == 0) for a in co) and all(h(a).shape == (0,) for a in po)
+        co, po = f.bidirectional_blocks(d(np.zeros((0, 3))), d(np.zeros(0)), z, z, *p, pm, pe)   # no cells
+        assert all(h(a).shape == (0,) for a in co) and all(np.all(h(a) == 0) for a in po)
+        cx = [d(rng.uniform(-1, 1, 512)) for _ in range(3)]
+        co, po = f.bidirectional(*cx, cm, d(np.full(512, 0.01)), z, z, z, z, z)
+        assert all(np.all(h(a) == 0) for a in co) and all(h(a).shape == (0,) for a in po)
+        co, po = f.bidirectional(z, z, z, z, z, *p, pm, pe)
+        assert all(h(a).shape == (0,) for a in co) and all(np.all(h(a) == 0) for a in po)
+        f.set_sources(z, z, z, z, None)
+        out = f.evaluate(None, *p)
+        assert all(np.all(h(a) == 0) for a in out)
+        out = f.evaluate_blocks(o, dx)
+        assert all(np.all(h(a) == 0) for a in out)
+        f.set_sources(*p, pm, pe)
+        out = f.evaluate(None, z, z, z)
+        assert all(h(a).shape == (0,) for a in out)
+        out = f.evaluate_blocks(d(np.zeros((0, 3))), d(np.zeros(0)))
+        assert all(h(a).shape == (0,) for a in out)
+
+
 def test_error_codes():
     f = GravityField("fp64")
     with pytest.raises(TgravError) as e:

@@ -11,8 +11,11 @@ Inputs live on a lattice of spacing 1/64 pc (a few lattice units apart up to a f
 Bars: FP64 within 1e-12 of the sum of |terms| per target (condition-aware: random clouds
 can cancel), mixed within its analytic bound; the exact properties bit for bit.  Hypothesis
 draws the sizes and the structure (spread, softening pattern, mass pattern, coincidences,
-host or device arrays); numpy, seeded from a drawn integer, draws the values.
+host or device arrays); numpy, seeded from a drawn integer, draws the values.  25 examples per
+property by default; TGRAV_PROPERTY_EXAMPLES sets another count for a longer sweep.
 """
+import os
+
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings
@@ -28,7 +31,7 @@ G = W.G_PC_MSUN_KMS
 H = 1.0 / 64
 EPS = np.array([0.0, 1.0 / 256, 1.0 / 32, 1.0 / 8])
 SHIFT = np.array([1.5, -2.0, 0.75])            # a lattice vector
-PROP = settings(max_examples=25, derandomize=True, deadline=None, database=None,
+PROP = settings(max_examples=int(os.environ.get("TGRAV_PROPERTY_EXAMPLES", 25)), derandomize=True, deadline=None, database=None,
                 suppress_health_check=list(HealthCheck))
 
 

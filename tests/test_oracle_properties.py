@@ -5,6 +5,8 @@ so translation by a lattice vector and scaling the masses by 4 are exact propert
 sequential FP64 sum; superposition, the summation modes and Newton's third law hold to the
 rounding of the sums, measured against the sum of |terms|.
 """
+import os
+
 import numpy as np
 from hypothesis import HealthCheck, given, settings
 from hypothesis import strategies as st
@@ -13,7 +15,7 @@ import oracle
 from tests import _util
 from tests.test_gpu_properties import SHIFT, cloud, masses, points, softening, sq
 
-PROP = settings(max_examples=40, derandomize=True, deadline=None, database=None,
+PROP = settings(max_examples=int(os.environ.get("TGRAV_PROPERTY_EXAMPLES", 40)), derandomize=True, deadline=None, database=None,
                 suppress_health_check=list(HealthCheck))
 
 

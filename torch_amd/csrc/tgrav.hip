@@ -400,7 +400,9 @@ static int stage_in(tgrav_ctx* c, int first, int64_t n, const double** arrs, int
   return 0;
 }
 
-static unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// Workgroups for n lanes of bs (at least one: every kernel launched this way checks its index
+// against n, so an empty input is a no-op launch rather than an invalid configuration).
+static unsigned grid1(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
 
 // Sums nsplit chunk partials (4 x pstride doubles each) into the outputs of n targets.  Few
 // targets with many chunks take the wide kernel (16 waves per 64 targets); otherwise one
@@ -429,6 +431,8 @@ static int read_flag(tgrav_ctx* c, const void* dflag, bool* bad) {
 // NaN/Inf check of device target arrays (synchronous; only with validation on).
 static int device_check_finite(tgrav_ctx* c, int64_t n, const double* const* chk, int nchk,
                                bool* bad) {
+  *bad = false;
+  if (n <= 0) return 0;
   int rc = c->flag.ensure(sizeof(unsigned long long));
   if (rc) return rc;
   const double* cs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
