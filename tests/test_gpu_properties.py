@@ -277,6 +277,53 @@ def test_bidirectional_blocks(fields, mode, blk, par, mass, hit, device):
     same(mp, tuple(4.0 * g for g in gp))
 
 
+# ------------------------------------------------------------ FLASH kernels ----
+@PROP
+@given(src=cloud(400), tgt=cloud(600), blk=blocks(3), kernel=st.sampled_from(["spline", "linear"]),
+       quirks=st.booleans(), rsoft=st.sampled_from([1.0 / 16, 0.25, 1.0]),
+       form=st.sampled_from(["points", "blocks", "points in block order", "periodic"]),
+       box=st.sampled_from([0.5, 1.0, 2.0]), nrep=st.integers(1, 2), hit=st.booleans(), device=st.booleans())
+def test_flash_kernels(fields, src, tgt, blk, kernel, quirks, rsoft, form, box, nrep, hit, device):
+    """FLASH's spline/linear kernels (FP64) on every source form -- explicit points, 8^3 blocks,
+    explicit points in block order (detected on the device), and the periodic image sum -- within
+    1e-12 of the sum of the terms' magnitudes against the oracle.  Lattice inputs put pairs exactly
+    on r = rsoft and on the image cutoff, where the branch decisions must match the reference."""
+    from tests.test_gpu_flash import check as flash_check
+    from tests.test_gpu_flash import term_scales
+    rng = np.random.Generator(np.random.PCG64(src["seed"]))
+    if form in ("blocks", "points in block order"):
+        o, dx, _ = make_blocks(blk)
+        s = centres(o, dx)
+        m = masses(dict(src, n=len(s)), rng)
+    else:
+        s, m = points(src, rng), masses(src, rng)
+    trng = np.random.Generator(np.random.PCG64(tgt["seed"]))
+    t = points(tgt, trng)
+    if hit:
+        t = coincide(t, s, trng)
+    periodic = form == "periodic"
+    if periodic:                   # keep the image sum small: (2 nrep + 1)^3 images per pair
+        s, m, t = s[:150], m[:150], t[:200]
+    f = fields["fp64"]
+    f.set_softening(kernel, rsoft, quirks=quirks)
+    if periodic:
+        f.set_periodic((box, box, box), nrep=nrep)
+    try:
+        if form == "blocks":
+            f.set_source_blocks(*as_device([o, dx, m], device))
+        else:
+            f.set_sources(*as_device([s[:, 0], s[:, 1], s[:, 2], m], device), None)
+        got = host(f.evaluate(None, *as_device([t[:, 0], t[:, 1], t[:, 2]], device)))
+    finally:
+        f.set_softening("plummer")
+        f.set_periodic(None)
+    B = (box, box, box) if periodic else None
+    ref = oracle.eval_flash(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], G, rsoft,
+                            kernel=kernel, quirks=quirks, threads=16, box=B, nrep=nrep)
+    sa, sp = term_scales(s[:, 0], s[:, 1], s[:, 2], np.abs(m), t[:, 0], t[:, 1], t[:, 2], rsoft, box=B, nrep=nrep)
+    flash_check(got, ref, sa + 1e-300, sp + 1e-300)
+
+
 # -------------------------------------------------------------------- batches ----
 @st.composite
 def batch(draw):
