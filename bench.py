@@ -696,11 +696,22 @@ def main():
         inter_step = float(sum(len(p["sources"]["x"]) * len(p["targets"]["x"]) for p in probs))
         log(f"c4 generated in {time.perf_counter() - t:.1f} s: {len(probs)} problems, "
             f"{inter_step:.3e} interactions")
+        # the same problems with the targets as explicit cell centres (the blocks' C order)
+        pdev_x = [{"sources": p["sources"], "targets": {k: p["targets"][k] for k in ("x", "y", "z", "eps")}}
+                  for p in pdev]
+        explicit = {}
         for mode in modes:
             r = run_batch(mode, pdev, args.steps, args.warmup)
             results[mode] = r
             log(f"{mode}: {r['ms']:.3f} ms/replay ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
                 f"persistent kernel {r['kernel_ms']:.3f} ms, {r['items']} work items")
+            rx = run_batch(mode, pdev_x, args.steps, args.warmup)
+            same = all(np.array_equal(a, b) for o, q in zip(rx["res"], r["res"]) for a, b in zip(o, q))
+            explicit[mode] = {"value": inter_step / (rx["ms"] * 1e-3), "ms_per_step": rx["ms"],
+                              "kernel_ms": rx["kernel_ms"], "bit_identical_to_block_form": same}
+            log(f"{mode} explicit targets: {rx['ms']:.3f} ms/replay ({inter_step / rx['ms'] * 1e3:.3e} "
+                f"interactions/s), bit-identical to the block form: {same}")
+        line["explicit_targets"] = explicit
         r = results[head]
         kern_ms, kern_inter = r["kernel_ms"], inter_step
         kern_bytes = sum(BYTES_PER_TARGET * len(p["targets"]["x"]) + BYTES_PER_SOURCE * len(p["sources"]["x"])

@@ -182,7 +182,9 @@ def block_batch(seed=41, n=9):
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_block_target_problems(mode):
     """Block-target problems (column kernel inside the batch) against the same problems with
-    explicit cell centres, against the oracle, over graph replays, with the fused kick."""
+    explicit cell centres, against the oracle, over graph replays, with the fused kick.  The
+    explicit centres are in the blocks' C order, so the plan finds their column structure and
+    gives them the same kernel: the results are the same bits."""
     import torch
     probs = block_batch()
     explicit = [dict(sources=p["sources"], targets={k: p["targets"][k] for k in ("x", "y", "z")})
@@ -211,6 +213,9 @@ def test_block_target_problems(mode):
                     assert np.array_equal(again[i][q], eager[i][q]), (rep, i, q)
         pe.close()
         pb.close()
+    for i in range(len(probs)):
+        for q in range(4):
+            assert np.array_equal(ref[i][q], eager[i][q]), (i, q)
     for i, p in enumerate(probs):
         S, T = p["sources"], p["targets"]
         s = _util.summary(eager[i], ref[i]) if len(S["x"]) else None
@@ -231,3 +236,33 @@ def test_block_target_problems(mode):
             v = list(oracle.kick(dt, eager[i][0], eager[i][1], eager[i][2], *v))
         for q in range(3):
             assert np.array_equal(dev_b[i]["velocities"][q].cpu().numpy(), v[q]), (i, q)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_explicit_targets_without_column_structure(mode):
+    """Explicit targets whose count is a multiple of 512 but which are not in the blocks' C order
+    (shuffled cell centres) stay on the generic work list: against the oracle (FP64) and the
+    FP64 plan (mixed)."""
+    probs = block_batch(seed=43, n=5)
+    rng = W.rng_for(9)
+    shuffled = []
+    for p in probs:
+        perm = rng.permutation(len(p["targets"]["x"]))
+        shuffled.append(dict(sources=p["sources"], targets={k: p["targets"][k][perm] for k in ("x", "y", "z")}))
+    res = {}
+    for m in ("fp64", mode):
+        with GravityField(m) as f:
+            plan = BatchPlan(f, problems_from_workload(shuffled))
+            res[m] = to_np(plan.run(graph=True))
+            plan.close()
+    for i, p in enumerate(shuffled):
+        S, T = p["sources"], p["targets"]
+        if len(S["x"]) == 0:
+            continue
+        o = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"], T["y"], T["z"], None,
+                                G, mode=oracle.NEUMAIER, threads=16)
+        s = _util.summary(res["fp64"][i], o)
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+        if mode == "mixed":
+            s = _util.summary(res["mixed"][i], res["fp64"][i])
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (i, s)

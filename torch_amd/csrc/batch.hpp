@@ -134,8 +134,9 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
   }
 }
 
-// Problems whose targets are 8^3 leaf blocks: the same persistent scheme with 64-lane
-// workgroups and one target block per work item, evaluated by the column code of k_eval_grid.
+// Problems whose targets are 8^3 leaf blocks, or explicit targets with their column structure
+// (found at plan creation): the same persistent scheme with 64-lane workgroups and one target
+// block (group of 512 targets) per work item, evaluated by the column code of k_eval_grid.
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict__ probs,
                                                         const BItem* __restrict__ items,
@@ -165,7 +166,8 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
     } else {
       o.part = nullptr;
     }
-    grid_block<MIXED>(L, P.T, w.tile, s4, se2, sb, se, e2s_min, o);
+    if (P.T.blocks) grid_block<MIXED>(L, P.T, w.tile, s4, se2, sb, se, e2s_min, o);
+    else cols_group<MIXED>(L, P.T, w.tile, s4, se2, sb, se, e2s_min, o);   // explicit columns
   }
 }
 

@@ -280,25 +280,23 @@ __global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restri
 
 // Explicit targets with column structure (every aligned group of 8 consecutive targets shares
 // x and y, every aligned group of 512 shares eps: the C order of 8^3 leaf-block meshgrids, as
-// checked by k_check_cols): the column code of k_eval_grid on explicit coordinates.
-// blockIdx.x = group of 512 targets (one wave, lane l owns targets 512 x + 8 l ... + 7),
-// blockIdx.y = source chunk.  Gated (Out::gate) against the generic kernel on the check's flag.
+// checked by k_check_cols): the column code of k_eval_grid on explicit coordinates.  Group g =
+// targets 512 g ... 512 g + 511, one wave, lane l owns targets 512 g + 8 l ... + 7.  Shared by
+// k_eval_cols and the persistent batch kernel.
 template <bool MIXED>
-__global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restrict__ src4,
-                                                  const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* e2s_min_p, Out O) {
-  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
-  __shared__ Lds L;
-  if (gated_off(O)) return;
+__device__ __forceinline__ void cols_group(
+    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t g,
+    const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
+    double e2s_min, const Out& o) {
   const int lane = threadIdx.x;
-  const int64_t t0 = (int64_t)blockIdx.x * 512 + (int64_t)lane * 8;
+  const int64_t t0 = g * 512 + (int64_t)lane * 8;
   const int64_t last = T.n - 1;
   const int64_t tc = t0 < last ? t0 : last;
   const double x = T.x[tc], y = T.y[tc];
   double z[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) z[k] = T.z[t0 + k < last ? t0 + k : last];
-  const int64_t te = (int64_t)blockIdx.x * 512 < last ? (int64_t)blockIdx.x * 512 : last;
+  const int64_t te = g * 512 < last ? g * 512 : last;
   const double e = T.eps ? T.eps[te] : 0.0;
   double cx = 0.0, cy = 0.0, cz = 0.0;
   if constexpr (MIXED) {   // centre of the bounding box of the wave's targets
@@ -320,13 +318,25 @@ __global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restri
     cy = 0.5 * (mn[1] + mx[1]);
     cz = 0.5 * (mn[2] + mx[2]);
   }
+  const int64_t left = T.n - t0;
+  grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
+                      sb, se, e2s_min, o);
+}
+
+// blockIdx.x = group of 512 targets, blockIdx.y = source chunk.  Gated (Out::gate) against
+// the generic kernel on the check's flag.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restrict__ src4,
+                                                  const double* __restrict__ srce2, int64_t ns,
+                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
+  __shared__ Lds L;
+  if (gated_off(O)) return;
   const int64_t sb = (int64_t)blockIdx.y * chunk;
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  const int64_t left = T.n - t0;
-  grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
-                      sb, se, *e2s_min_p, o);
+  cols_group<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, *e2s_min_p, o);
 }
 
 // ---------------------------------------------------------------- block sources ----

@@ -2001,6 +2001,32 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       return fail(TGRAV_EINVAL, "problem %d: velocities are required when dt != 0", p);
   }
   if ((rc = set_device(c))) return rc;
+  // Explicit targets in the C order of 8^3 leaf-block meshgrids (each aligned 8 share x and y,
+  // each aligned 512 share eps) join the column work list, as tgrav_eval routes them to
+  // k_eval_cols.  Tested once here, on the device; plan creation may wait for the answer.
+  std::vector<int> colmask(n, 0);
+  if (c->block_kernel) {
+    std::vector<int> cand;
+    for (int p = 0; p < n; ++p)
+      if (pr[p].n_target_blocks == 0 && pr[p].n_targets >= 512 && pr[p].n_targets % 512 == 0) cand.push_back(p);
+    if (!cand.empty()) {
+      if ((rc = c->flag.ensure(cand.size() * sizeof(int)))) return rc;
+      int* fl = c->flag.as<int>();
+      for (size_t i = 0; i < cand.size(); ++i) {
+        const tgrav_problem& q = pr[cand[i]];
+        tg::Tgt T{};
+        T.x = q.tx; T.y = q.ty; T.z = q.tz; T.eps = q.te; T.n = q.n_targets;
+        tg::k_set_flag<<<1, 64, 0, c->stream>>>(fl + i, 1);
+        LAUNCHCHECK("k_set_flag");
+        tg::k_check_cols<<<grid1(q.n_targets / 512 * 64, 256), 256, 0, c->stream>>>(T, fl + i);
+        LAUNCHCHECK("k_check_cols");
+      }
+      std::vector<int> got(cand.size(), 0);
+      HIPCHECK(hipMemcpyAsync(got.data(), fl, cand.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(hipStreamSynchronize(c->stream));
+      for (size_t i = 0; i < cand.size(); ++i) colmask[cand[i]] = got[i] == 1;
+    }
+  }
   tgrav_batch* b = new tgrav_batch();
   b->c = c;
   b->mode = mode;
@@ -2045,8 +2071,8 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const tgrav_problem& q = pr[p];
     tg::BProb& P = probs[p];
     memset(&P, 0, sizeof P);
-    const bool blk = q.n_target_blocks > 0;
-    if (blk) {
+    const bool blk = q.n_target_blocks > 0 || colmask[p];   // column work list
+    if (q.n_target_blocks > 0) {
       P.T.blocks = 1; P.T.nb = 8;
       P.T.ox = q.tbx; P.T.oy = q.tby; P.T.oz = q.tbz; P.T.bdx = q.tbd; P.T.beps = q.tbe;
     } else {
