@@ -32,18 +32,22 @@ def run(f, C, P):
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 @pytest.mark.parametrize("nblk,npart", [(1, 1), (1, 300), (3, 257), (5, 1000)])
 def test_matches_oracle(mode, nblk, npart):
+    """Explicit cells (in the blocks' C order) with the block kernels on -- recognised, the column
+    kernel -- and off -- the generic fused kernel -- against the oracle."""
     C, P = cells_and_particles(nblk * 100 + npart, nblk, npart)
-    with GravityField(mode) as f:
-        co, po = run(f, C, P)
     (pa, pb, pc, pp), (ca, cb, cc, cp) = oracle.bidirectional(
         C["x"], C["y"], C["z"], C["m"], C["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"], G)
-    for name, got, ref in (("cells", co, (ca, cb, cc, cp)), ("particles", po, (pa, pb, pc, pp))):
-        s = _util.summary(got, ref)
-        print(mode, nblk, npart, name, s, flush=True)
-        if mode == "fp64":
-            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (name, s)
-        else:
-            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
+    for block_kernel in (True, False):
+        with GravityField(mode) as f:
+            f.set_block_kernel(block_kernel)
+            co, po = run(f, C, P)
+        for name, got, ref in (("cells", co, (ca, cb, cc, cp)), ("particles", po, (pa, pb, pc, pp))):
+            s = _util.summary(got, ref)
+            print(mode, nblk, npart, block_kernel, name, s, flush=True)
+            if mode == "fp64":
+                assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (name, s)
+            else:
+                assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
@@ -52,6 +56,7 @@ def test_every_variant_and_chunking_agree(mode):
     lib = _lib.load()
     ref = None
     with GravityField(mode) as f:
+        f.set_block_kernel(False)          # the generic fused kernel's variants
         for v in range(4):
             for ns in (1, 0, 5):
                 f.set_bidir_config(variant=v, nsplit=ns)
@@ -85,12 +90,26 @@ def blocks_and_particles(seed, nblk, npart):
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
 @pytest.mark.parametrize("nblk,npart", [(1, 1), (1, 65), (3, 300), (7, 1000)])
 def test_blocks_match_explicit_and_oracle(mode, nblk, npart):
-    """Cells as 8^3 blocks (column kernel): the explicit-cell fused pass and the oracle."""
+    """Cells as 8^3 blocks (column kernel): the explicit-cell fused pass and the oracle.  The
+    explicit cells are in the blocks' C order, so they are recognised and take the same column
+    kernel (the same bits); shuffled, they take the generic fused kernel."""
     B, C, P = blocks_and_particles(nblk * 1000 + npart, nblk, npart)
     with GravityField(mode) as f:
         co, po = f.bidirectional_blocks(B["o"], B["dx"], B["m"], B["eps"], P["x"], P["y"], P["z"], P["m"],
                                         P["eps"])
         ce, pe = run(f, C, P)
+        perm = W.rng_for(nblk).permutation(len(C["x"]))
+        cs, ps = run(f, {k: v[perm] for k, v in C.items()}, P)
+    for q in range(4):
+        np.testing.assert_array_equal(ce[q], co[q])
+        np.testing.assert_array_equal(pe[q], po[q])
+    inv = np.argsort(perm)
+    for got, ref in ((tuple(a[inv] for a in cs), co), (ps, po)):
+        s = _util.summary(got, ref)
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, s
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, s
     (pa, pb, pc, pp), (ca, cb, cc, cp) = oracle.bidirectional(
         C["x"], C["y"], C["z"], C["m"], C["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"], G)
     for name, got, ref, alt in (("cells", co, (ca, cb, cc, cp), ce), ("particles", po, (pa, pb, pc, pp), pe)):

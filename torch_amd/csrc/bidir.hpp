@@ -270,6 +270,9 @@ __global__ void k_bidir_reduce(const double* __restrict__ ppart, int ntiles, int
 struct CellBlocks {
   const double *ox, *oy, *oz, *dx, *m, *eps;   // per block; m per cell (C order)
   int64_t nblk;
+  // Explicit cells in the blocks' C order instead (ox .. dx unused): centres and softening per
+  // cell; each aligned 8 share x and y, each aligned 512 share eps (tested by k_check_cols).
+  const double *cx, *cy, *cz, *ceps;
 };
 
 #ifndef TG_BIGRID_UNROLL
@@ -404,24 +407,52 @@ __global__ __launch_bounds__(64) void k_bidir_grid(CellBlocks C, const double4* 
   const int i = lane >> 3, jj = lane & 7;
   const int64_t pb = (int64_t)blockIdx.y * chunk;
   const int64_t pe = pb + chunk < np ? pb + chunk : np;
-  const double d = C.dx[b];
-  const double e = C.eps ? C.eps[b] : 0.0;
-  const double e2c = e * e;
-  double x, y, z[8];
-  {
-#pragma clang fp contract(off)
-    x = C.ox[b] + ((double)i + 0.5) * d;
-    y = C.oy[b] + ((double)jj + 0.5) * d;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) z[k] = C.oz[b] + ((double)k + 0.5) * d;
-  }
-  double ax_ = 0.0, ay_ = 0.0, az_ = 0.0;
-  if (MIXED) {                    // anchor: the block centre
-    ax_ = C.ox[b] + 4.0 * d;
-    ay_ = C.oy[b] + 4.0 * d;
-    az_ = C.oz[b] + 4.0 * d;
-  }
   const int64_t t0 = b * 512 + (int64_t)lane * 8;
+  double x, y, z[8], e;
+  double ax_ = 0.0, ay_ = 0.0, az_ = 0.0;
+  if (C.cx) {                     // explicit cells (uniform branch)
+    x = C.cx[t0];
+    y = C.cy[t0];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) z[k] = C.cz[t0 + k];
+    e = C.ceps ? C.ceps[b * 512] : 0.0;
+    if (MIXED) {                  // anchor: the centre of the group's bounding box
+      double mn[3] = {x, y, z[0]}, mx[3] = {x, y, z[0]};
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        mn[2] = fmin(mn[2], z[k]);
+        mx[2] = fmax(mx[2], z[k]);
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          mn[v] = fmin(mn[v], __shfl_xor(mn[v], off));
+          mx[v] = fmax(mx[v], __shfl_xor(mx[v], off));
+        }
+      }
+      ax_ = 0.5 * (mn[0] + mx[0]);
+      ay_ = 0.5 * (mn[1] + mx[1]);
+      az_ = 0.5 * (mn[2] + mx[2]);
+    }
+  } else {
+    const double d = C.dx[b];
+    e = C.eps ? C.eps[b] : 0.0;
+    {
+#pragma clang fp contract(off)
+      x = C.ox[b] + ((double)i + 0.5) * d;
+      y = C.oy[b] + ((double)jj + 0.5) * d;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) z[k] = C.oz[b] + ((double)k + 0.5) * d;
+    }
+    if (MIXED) {                  // anchor: the centre of the cell centres' bounding box, as
+#pragma clang fp contract(off)      // the explicit form computes it from the same centres
+      ax_ = 0.5 * ((C.ox[b] + 0.5 * d) + (C.ox[b] + 7.5 * d));
+      ay_ = 0.5 * ((C.oy[b] + 0.5 * d) + (C.oy[b] + 7.5 * d));
+      az_ = 0.5 * ((C.oz[b] + 0.5 * d) + (C.oz[b] + 7.5 * d));
+    }
+  }
+  const double e2c = e * e;
   T qx = (T)(x - ax_), qy = (T)(y - ay_), qz[8], qm[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {

@@ -1343,6 +1343,66 @@ int tgrav_bidir_variant_info(int mode, int variant, int* k, int* threads) {
   return 0;
 }
 
+// The fused pass with the cells in 8^3 block columns (k_bidir_grid): block descriptors, or
+// explicit cells in the blocks' C order (C.cx set).  pa: the particles' x, y, z, m, eps.
+static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int64_t np,
+                         const double* const* pa, const tg::Out& CO, const tg::Out& PO) {
+  int rc;
+  const int64_t nblk = C.nblk, nc = nblk * 512;
+  if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
+      (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
+      (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
+    return rc;
+  tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->bstat.as<unsigned long long>());
+  LAUNCHCHECK("k_reset_stats");
+  tg::k_pack_sources<<<grid1(np, 256), 256, 0, c->stream>>>(
+      np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
+      c->bstat.as<unsigned long long>(), 0);
+  LAUNCHCHECK("k_pack_sources");
+  // particle chunks over blockIdx.y as for block targets (grid_shape): about 8 rounds of
+  // one-wave workgroups, at most 12 chunks, at least 4 particle tiles a chunk
+  int64_t want = c->bnsplit[mode];
+  if (want <= 0) {
+    const int64_t target = 8LL * 12 * c->num_cu;
+    want = std::min<int64_t>((target + nblk - 1) / nblk,
+                             std::min<int64_t>(12, std::max<int64_t>(1, np / 256)));
+  }
+  want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
+  int64_t chunk = (np + want - 1) / want;
+  chunk = (chunk + 63) / 64 * 64;
+  const int nsplit = (int)((np + chunk - 1) / chunk);
+  const double part_bytes = (double)nblk * 4 * np * 8 + (nsplit > 1 ? (double)nsplit * 4 * nc * 8 : 0.0);
+  if (part_bytes > 64.0 * (1ULL << 30))
+    return fail(TGRAV_ELIMIT, "fused pass would need %.1f GiB of partial sums; evaluate the two "
+                "directions separately", part_bytes / (1ULL << 30));
+  if ((rc = c->bppart.ensure((size_t)nblk * 4 * np * sizeof(double)))) return rc;
+  tg::Out o = CO;
+  if (nsplit > 1) {
+    if ((rc = c->part.ensure((size_t)nsplit * 4 * nc * sizeof(double)))) return rc;
+    o.part = c->part.as<double>();
+    o.pstride = nc;
+  }
+  rc = timed_launch(c, nc, np, [&]() -> int {
+    const dim3 g((unsigned)nblk, (unsigned)nsplit);
+    if (mode == TGRAV_MIXED)
+      tg::k_bidir_grid<true><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
+                                                     chunk, c->bstat.as<const double>(), o,
+                                                     c->bppart.as<double>());
+    else
+      tg::k_bidir_grid<false><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
+                                                      chunk, c->bstat.as<const double>(), o,
+                                                      c->bppart.as<double>());
+    LAUNCHCHECK("k_bidir_grid");
+    return 0;
+  });
+  if (rc) return rc;
+  if (nsplit > 1) {
+    if ((rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CO))) return rc;
+  }
+  if ((rc = reduce_particle_partials(c, nblk, np, PO))) return rc;
+  return 0;
+}
+
 int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double* cy,
                         const double* cz, const double* cm, const double* ce, int64_t np,
                         const double* px, const double* py, const double* pz, const double* pm,
@@ -1399,7 +1459,28 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     tg::k_fill<<<grid1(np, 256), 256, 0, c->stream>>>(np, 0.0, dpo[0], dpo[1], dpo[2], dpo[3]);
     LAUNCHCHECK("k_fill");
   }
-  if (nc > 0 && np > 0) {
+  // Cells in the C order of 8^3 leaf-block meshgrids take the block-column kernel.  The test
+  // runs on the device and this call waits for its answer (one flag word).
+  bool done = false;
+  if (nc > 0 && np > 0 && nc % 512 == 0 && c->block_kernel) {
+    if ((rc = c->colflag.ensure(sizeof(int)))) return rc;
+    int* fl = c->colflag.as<int>();
+    tg::Tgt T{};
+    T.x = ca[0]; T.y = ca[1]; T.z = ca[2]; T.eps = ca[4]; T.n = nc;
+    tg::k_set_flag<<<1, 64, 0, c->stream>>>(fl, 1);
+    LAUNCHCHECK("k_set_flag");
+    tg::k_check_cols<<<grid1(nc / 8, 256), 256, 0, c->stream>>>(T, fl);
+    LAUNCHCHECK("k_check_cols");
+    int cols = 0;
+    HIPCHECK(hipMemcpyAsync(&cols, fl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (cols == 1) {
+      tg::CellBlocks C{nullptr, nullptr, nullptr, nullptr, ca[3], nullptr, nc / 512, ca[0], ca[1], ca[2], ca[4]};
+      if ((rc = bidir_grid_pass(c, mode, C, np, pa, CO, PO))) return rc;
+      done = true;
+    }
+  }
+  if (nc > 0 && np > 0 && !done) {
     if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
         (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
         (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
@@ -1527,58 +1608,8 @@ int tgrav_bidirectional_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double*
     LAUNCHCHECK("k_fill");
   }
   if (nc > 0 && np > 0) {
-    if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
-        (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
-        (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
-      return rc;
-    tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->bstat.as<unsigned long long>());
-    LAUNCHCHECK("k_reset_stats");
-    tg::k_pack_sources<<<grid1(np, 256), 256, 0, c->stream>>>(
-        np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
-        c->bstat.as<unsigned long long>(), 0);
-    LAUNCHCHECK("k_pack_sources");
-    // particle chunks over blockIdx.y as for block targets (grid_shape): about 8 rounds of
-    // one-wave workgroups, at most 12 chunks, at least 4 particle tiles a chunk
-    int64_t want = c->bnsplit[mode];
-    if (want <= 0) {
-      const int64_t target = 8LL * 12 * c->num_cu;
-      want = std::min<int64_t>((target + nblk - 1) / nblk,
-                               std::min<int64_t>(12, std::max<int64_t>(1, np / 256)));
-    }
-    want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
-    int64_t chunk = (np + want - 1) / want;
-    chunk = (chunk + 63) / 64 * 64;
-    const int nsplit = (int)((np + chunk - 1) / chunk);
-    const double part_bytes = (double)nblk * 4 * np * 8 + (nsplit > 1 ? (double)nsplit * 4 * nc * 8 : 0.0);
-    if (part_bytes > 64.0 * (1ULL << 30))
-      return fail(TGRAV_ELIMIT, "fused pass would need %.1f GiB of partial sums; evaluate the two "
-                  "directions separately", part_bytes / (1ULL << 30));
-    if ((rc = c->bppart.ensure((size_t)nblk * 4 * np * sizeof(double)))) return rc;
-    tg::Out o = CO;
-    if (nsplit > 1) {
-      if ((rc = c->part.ensure((size_t)nsplit * 4 * nc * sizeof(double)))) return rc;
-      o.part = c->part.as<double>();
-      o.pstride = nc;
-    }
-    tg::CellBlocks C{ba[0], ba[1], ba[2], ba[3], mc[0], be[0], nblk};
-    rc = timed_launch(c, nc, np, [&]() -> int {
-      const dim3 g((unsigned)nblk, (unsigned)nsplit);
-      if (mode == TGRAV_MIXED)
-        tg::k_bidir_grid<true><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
-                                                       chunk, c->bstat.as<const double>(), o,
-                                                       c->bppart.as<double>());
-      else
-        tg::k_bidir_grid<false><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
-                                                        chunk, c->bstat.as<const double>(), o,
-                                                        c->bppart.as<double>());
-      LAUNCHCHECK("k_bidir_grid");
-      return 0;
-    });
-    if (rc) return rc;
-    if (nsplit > 1) {
-      if ((rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CO))) return rc;
-    }
-    if ((rc = reduce_particle_partials(c, nblk, np, PO))) return rc;
+    tg::CellBlocks C{ba[0], ba[1], ba[2], ba[3], mc[0], be[0], nblk, nullptr, nullptr, nullptr, nullptr};
+    if ((rc = bidir_grid_pass(c, mode, C, np, pa, CO, PO))) return rc;
   }
   if (kind == TGRAV_HOST) {
     for (int q = 0; q < 4; ++q) {
