@@ -267,6 +267,11 @@ def test_bidirectional_blocks(fields, mode, blk, par, mass, hit, device):
     gc, gp = run()
     mc, mp = run(shift=1.0, scale=4.0)
     cel = np.zeros(len(c)) if e is None else np.repeat(e, 512)
+    # the same cells as explicit arrays (the blocks' C order): recognised, the same bits
+    A = as_device([c[:, 0], c[:, 1], c[:, 2], cm, cel, p[:, 0], p[:, 1], p[:, 2], pm, pe], device)
+    xc, xp = f.bidirectional(*A)
+    same(host(xc), gc)
+    same(host(xp), gp)
     rp, rc = oracle.bidirectional(c[:, 0], c[:, 1], c[:, 2], cm, cel, p[:, 0], p[:, 1], p[:, 2], pm, pe, G)
     ctr = np.repeat(o + 4.0 * dx[:, None], 512, axis=0)              # block centres, per cell
     half = np.repeat(np.sqrt(3.0) * 3.5 * dx, 512)
