@@ -418,6 +418,13 @@ def test_batch_plans(fields, mode, probs):
             torch.cuda.synchronize()
             for a, b in zip(replay, eager):
                 same(a, b)
+    # block problems handed over as explicit centres in the blocks' C order: recognised when the
+    # plan is created, the same bits
+    PX = [dict(sources=p["sources"], targets=dict(x=r[3][:, 0], y=r[3][:, 1], z=r[3][:, 2], eps=r[4]))
+          if "block_origin" in p["targets"] else p for p, r in zip(P, refs)]
+    with BatchPlan(f, PX) as plan:
+        for a, b in zip([host(o) for o in plan.run(graph=True)], eager):
+            same(a, b)
     for got, (s, m, se, t, te, anchor) in zip(eager, refs):
         ref = oracle.eval_plummer(s[:, 0], s[:, 1], s[:, 2], m, se, t[:, 0], t[:, 1], t[:, 2], te, G,
                                   mode=oracle.NEUMAIER)
