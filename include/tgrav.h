@@ -259,7 +259,10 @@ int tgrav_pm_interpolate(tgrav_ctx* ctx, int64_t np, const double* x, const doub
  * packs the sources, builds a cost-sorted work list (target tile x source chunk) and
  * owns the partial-sum scratch.  dt != 0 fuses the FP64 kick v += dt * a.  Explicit
  * targets in the C order of 8^3 leaf-block meshgrids are recognised here (one device test
- * per problem, waited for) and evaluated like block targets. */
+ * per problem, waited for) and evaluated like block targets; such targets must keep that
+ * structure (x, y shared by each aligned 8, eps by each aligned 512) when they are updated in
+ * place between runs -- cell centres do -- or the plan is created with
+ * tgrav_set_block_kernel(ctx, 0). */
 int tgrav_batch_create(tgrav_ctx* ctx, int32_t n_problems, const tgrav_problem* problems,
                        int mode, double dt, tgrav_batch** out);
 /* Runs the plan.  use_graph != 0 captures the launch sequence (counter reset,
