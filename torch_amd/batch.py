@@ -36,8 +36,11 @@ class BatchPlan:
     """problems: list of dict(sources=dict(x,y,z,m[,eps]), targets=dict(x,y,z[,eps])
     [, velocities=(vx,vy,vz)]).  Targets with block_origin[n, 3] and block_dx[n] (optional
     block_eps[n]) are 8^3 leaf blocks: the plan evaluates them with the column kernel, outputs
-    in the blocks' C order; their x, y, z are not read.  field: a GravityField whose context (device, stream,
-    G, kernel configuration) the plan uses."""
+    in the blocks' C order; their x, y, z are not read.  Explicit targets already in that C order
+    (each aligned 8 share x and y, each aligned 512 share eps) are recognised when the plan is
+    created and take the same kernel; updated in place between runs, they must keep that order
+    (field.set_block_kernel(False) before creating the plan turns the test off).  field: a
+    GravityField whose context (device, stream, G, kernel configuration) the plan uses."""
 
     def __init__(self, field: GravityField, problems, dt=0.0, potential=True):
         import torch
