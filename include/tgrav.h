@@ -167,10 +167,8 @@ int tgrav_eval_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const double* ox,
  * returns the acceleration/potential of the cells from the particles and of the
  * particles from the cells in one pass over the pairs.  Outputs may be NULL.  Cells in
  * the C order of 8^3 leaf-block meshgrids (each aligned 8 share x and y, each aligned 512
- * share eps) are recognised on the device -- the call waits for that test, so with a cell
- * count that is a multiple of 512 it cannot be captured into a graph (tgrav_bidirectional_blocks,
- * or tgrav_set_block_kernel(ctx, 0), never waits) -- and take the block-column kernel of
- * tgrav_bidirectional_blocks. */
+ * share eps) are recognised on the device, without a host wait, and take the block-column
+ * kernel of tgrav_bidirectional_blocks. */
 int tgrav_bidirectional(tgrav_ctx* ctx, int64_t nc, const double* cx, const double* cy,
                         const double* cz, const double* cm, const double* ce, int64_t np,
                         const double* px, const double* py, const double* pz,

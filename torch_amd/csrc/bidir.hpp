@@ -150,6 +150,7 @@ __global__ __launch_bounds__(NT) void k_bidir(Cells C, const double4* __restrict
                                               double* __restrict__ ppart) {
   typedef typename std::conditional<MIXED, float, double>::type T;
   __shared__ LdsBi<NT, MIXED> L;
+  if (gated_off(CO)) return;    // the cells turned out to be 8^3 block columns
   const int tid = threadIdx.x;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t pb = (int64_t)blockIdx.y * chunk;
@@ -218,9 +219,9 @@ __global__ __launch_bounds__(NT) void k_bidir(Cells C, const double4* __restrict
 // Particle side, first level (many cell tiles): blockIdx.y = group g of tiles
 // [g*per, min((g+1)*per, ntiles)); writes the group's sums (tile order) to out[(g*4 + q)*np + j].
 __global__ void k_bidir_group(const double* __restrict__ ppart, int ntiles, int per, int64_t np,
-                              double* __restrict__ out) {
+                              double* __restrict__ out, Out O) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= np) return;
+  if (j >= np || gated_off(O)) return;
   const int t0 = blockIdx.y * per;
   const int t1 = t0 + per < ntiles ? t0 + per : ntiles;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -242,7 +243,7 @@ __global__ void k_bidir_group(const double* __restrict__ ppart, int ntiles, int 
 // particle's field is (-G u, -G up).
 __global__ void k_bidir_reduce(const double* __restrict__ ppart, int ntiles, int64_t np, Out O) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= np) return;
+  if (j >= np || gated_off(O)) return;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (int t = 0; t < ntiles; ++t) {
     const double* p = ppart + (int64_t)t * 4 * np;
@@ -402,6 +403,7 @@ __global__ __launch_bounds__(64) void k_bidir_grid(CellBlocks C, const double4* 
                                                    double* __restrict__ ppart) {
   typedef typename std::conditional<MIXED, float, double>::type T;
   __shared__ LdsBiGrid<MIXED> L;
+  if (gated_off(CO)) return;    // explicit cells that are not 8^3 block columns
   const int lane = threadIdx.x;
   const int64_t b = blockIdx.x;
   const int i = lane >> 3, jj = lane & 7;

@@ -770,7 +770,7 @@ static int reduce_particle_partials(tgrav_ctx* c, int64_t ntiles, int64_t np, co
     int rc = c->bpgrp.ensure((size_t)groups * 4 * np * sizeof(double));
     if (rc) return rc;
     tg::k_bidir_group<<<dim3(grid1(np, 256), (unsigned)groups), 256, 0, c->stream>>>(
-        src, (int)ntiles, per, np, c->bpgrp.as<double>());
+        src, (int)ntiles, per, np, c->bpgrp.as<double>(), PO);
     LAUNCHCHECK("k_bidir_group");
     src = c->bpgrp.as<double>();
     nt = groups;
@@ -1343,12 +1343,10 @@ int tgrav_bidir_variant_info(int mode, int variant, int* k, int* threads) {
   return 0;
 }
 
-// The fused pass with the cells in 8^3 block columns (k_bidir_grid): block descriptors, or
-// explicit cells in the blocks' C order (C.cx set).  pa: the particles' x, y, z, m, eps.
-static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int64_t np,
-                         const double* const* pa, const tg::Out& CO, const tg::Out& PO) {
+// Fused passes: packs the particles (x, y, z, m, eps) into c->bp4 / c->bpe2 and their smallest
+// eps^2 into c->bstat.
+static int bidir_pack_particles(tgrav_ctx* c, int64_t np, const double* const* pa) {
   int rc;
-  const int64_t nblk = C.nblk, nc = nblk * 512;
   if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
       (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
       (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
@@ -1359,8 +1357,13 @@ static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int6
       np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
       c->bstat.as<unsigned long long>(), 0);
   LAUNCHCHECK("k_pack_sources");
-  // particle chunks over blockIdx.y as for block targets (grid_shape): about 8 rounds of
-  // one-wave workgroups, at most 12 chunks, at least 4 particle tiles a chunk
+  return 0;
+}
+
+// Launch shape of the block-column fused kernel: particle chunks over blockIdx.y, about 8 rounds
+// of one-wave workgroups, at most 12 chunks, at least 4 particle tiles a chunk.
+static int bidir_grid_shape(const tgrav_ctx* c, int mode, int64_t nblk, int64_t np, int64_t* chunk,
+                            int* nsplit) {
   int64_t want = c->bnsplit[mode];
   if (want <= 0) {
     const int64_t target = 8LL * 12 * c->num_cu;
@@ -1368,13 +1371,38 @@ static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int6
                              std::min<int64_t>(12, std::max<int64_t>(1, np / 256)));
   }
   want = std::max<int64_t>(1, std::min<int64_t>(want, 65535));
-  int64_t chunk = (np + want - 1) / want;
-  chunk = (chunk + 63) / 64 * 64;
-  const int nsplit = (int)((np + chunk - 1) / chunk);
-  const double part_bytes = (double)nblk * 4 * np * 8 + (nsplit > 1 ? (double)nsplit * 4 * nc * 8 : 0.0);
+  *chunk = (np + want - 1) / want;
+  *chunk = (*chunk + 63) / 64 * 64;
+  *nsplit = (int)((np + *chunk - 1) / *chunk);
+  const double part_bytes = (double)nblk * 4 * np * 8 + (*nsplit > 1 ? (double)*nsplit * 4 * nblk * 512 * 8 : 0.0);
   if (part_bytes > 64.0 * (1ULL << 30))
     return fail(TGRAV_ELIMIT, "fused pass would need %.1f GiB of partial sums; evaluate the two "
                 "directions separately", part_bytes / (1ULL << 30));
+  return 0;
+}
+
+static void bidir_grid_kernel(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int64_t np, int64_t chunk,
+                              int nsplit, const tg::Out& o) {
+  const dim3 g((unsigned)C.nblk, (unsigned)nsplit);
+  if (mode == TGRAV_MIXED)
+    tg::k_bidir_grid<true><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
+                                                   chunk, c->bstat.as<const double>(), o,
+                                                   c->bppart.as<double>());
+  else
+    tg::k_bidir_grid<false><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
+                                                    chunk, c->bstat.as<const double>(), o,
+                                                    c->bppart.as<double>());
+}
+
+// The fused pass with the cells as 8^3 block descriptors (k_bidir_grid).
+static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int64_t np,
+                           const double* const* pa, const tg::Out& CO, const tg::Out& PO) {
+  int rc;
+  const int64_t nblk = C.nblk, nc = nblk * 512;
+  if ((rc = bidir_pack_particles(c, np, pa))) return rc;
+  int64_t chunk;
+  int nsplit;
+  if ((rc = bidir_grid_shape(c, mode, nblk, np, &chunk, &nsplit))) return rc;
   if ((rc = c->bppart.ensure((size_t)nblk * 4 * np * sizeof(double)))) return rc;
   tg::Out o = CO;
   if (nsplit > 1) {
@@ -1383,15 +1411,7 @@ static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int6
     o.pstride = nc;
   }
   rc = timed_launch(c, nc, np, [&]() -> int {
-    const dim3 g((unsigned)nblk, (unsigned)nsplit);
-    if (mode == TGRAV_MIXED)
-      tg::k_bidir_grid<true><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
-                                                     chunk, c->bstat.as<const double>(), o,
-                                                     c->bppart.as<double>());
-    else
-      tg::k_bidir_grid<false><<<g, 64, 0, c->stream>>>(C, c->bp4.as<double4>(), c->bpe2.as<double>(), np,
-                                                      chunk, c->bstat.as<const double>(), o,
-                                                      c->bppart.as<double>());
+    bidir_grid_kernel(c, mode, C, np, chunk, nsplit, o);
     LAUNCHCHECK("k_bidir_grid");
     return 0;
   });
@@ -1399,8 +1419,7 @@ static int bidir_grid_pass(tgrav_ctx* c, int mode, const tg::CellBlocks& C, int6
   if (nsplit > 1) {
     if ((rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CO))) return rc;
   }
-  if ((rc = reduce_particle_partials(c, nblk, np, PO))) return rc;
-  return 0;
+  return reduce_particle_partials(c, nblk, np, PO);
 }
 
 int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double* cy,
@@ -1459,38 +1478,12 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     tg::k_fill<<<grid1(np, 256), 256, 0, c->stream>>>(np, 0.0, dpo[0], dpo[1], dpo[2], dpo[3]);
     LAUNCHCHECK("k_fill");
   }
-  // Cells in the C order of 8^3 leaf-block meshgrids take the block-column kernel.  The test
-  // runs on the device and this call waits for its answer (one flag word).
-  bool done = false;
-  if (nc > 0 && np > 0 && nc % 512 == 0 && c->block_kernel) {
-    if ((rc = c->colflag.ensure(sizeof(int)))) return rc;
-    int* fl = c->colflag.as<int>();
-    tg::Tgt T{};
-    T.x = ca[0]; T.y = ca[1]; T.z = ca[2]; T.eps = ca[4]; T.n = nc;
-    tg::k_set_flag<<<1, 64, 0, c->stream>>>(fl, 1);
-    LAUNCHCHECK("k_set_flag");
-    tg::k_check_cols<<<grid1(nc / 8, 256), 256, 0, c->stream>>>(T, fl);
-    LAUNCHCHECK("k_check_cols");
-    int cols = 0;
-    HIPCHECK(hipMemcpyAsync(&cols, fl, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    if (cols == 1) {
-      tg::CellBlocks C{nullptr, nullptr, nullptr, nullptr, ca[3], nullptr, nc / 512, ca[0], ca[1], ca[2], ca[4]};
-      if ((rc = bidir_grid_pass(c, mode, C, np, pa, CO, PO))) return rc;
-      done = true;
-    }
-  }
-  if (nc > 0 && np > 0 && !done) {
-    if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
-        (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
-        (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
-      return rc;
-    tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->bstat.as<unsigned long long>());
-    LAUNCHCHECK("k_reset_stats");
-    tg::k_pack_sources<<<grid1(np, 256), 256, 0, c->stream>>>(
-        np, pa[0], pa[1], pa[2], pa[3], pa[4], c->bp4.as<double4>(), c->bpe2.as<double>(),
-        c->bstat.as<unsigned long long>(), 0);
-    LAUNCHCHECK("k_pack_sources");
+  if (nc > 0 && np > 0) {
+    if ((rc = bidir_pack_particles(c, np, pa))) return rc;
+    // Cells in the C order of 8^3 leaf-block meshgrids take the block-column kernel.  Whether
+    // they are is tested on the device (k_check_cols writes a gate word: 1 columns, 0 not); both
+    // kernels and their reductions are launched, and the word lets one of them run.
+    const bool cand = nc % 512 == 0 && c->block_kernel;
     const BiVariant& v = kBiVariants[mode][c->bvariant[mode]];
     const int64_t tile = (int64_t)v.k * v.nt;
     const int64_t ntiles = (nc + tile - 1) / tile;
@@ -1510,25 +1503,51 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     if (part_bytes > 64.0 * (1ULL << 30))
       return fail(TGRAV_ELIMIT, "fused pass would need %.1f GiB of partial sums; evaluate the two "
                   "directions separately", part_bytes / (1ULL << 30));
-    if ((rc = c->bppart.ensure((size_t)ntiles * 4 * np * sizeof(double)))) return rc;
-    tg::Out o = CO;
-    if (nsplit > 1) {
-      if ((rc = c->part.ensure((size_t)nsplit * 4 * nc * sizeof(double)))) return rc;
-      o.part = c->part.as<double>();
-      o.pstride = nc;
+    const int64_t nblk = nc / 512;
+    int64_t chunk_g = 0;
+    int nsplit_g = 1;
+    if (cand && (rc = bidir_grid_shape(c, mode, nblk, np, &chunk_g, &nsplit_g))) return rc;
+    const int64_t prows = std::max<int64_t>(ntiles, cand ? nblk : 0);
+    if ((rc = c->bppart.ensure((size_t)prows * 4 * np * sizeof(double)))) return rc;
+    const int nsp = std::max(nsplit, cand ? nsplit_g : 1);
+    if (nsp > 1 && (rc = c->part.ensure((size_t)nsp * 4 * nc * sizeof(double)))) return rc;
+    tg::Out CG = CO, PG = PO, CX = CO, PX = PO;   // column pipeline (G), generic pipeline (X)
+    if (cand) {
+      if ((rc = c->colflag.ensure(sizeof(int)))) return rc;
+      int* fl = c->colflag.as<int>();
+      tg::Tgt T{};
+      T.x = ca[0]; T.y = ca[1]; T.z = ca[2]; T.eps = ca[4]; T.n = nc;
+      tg::k_set_flag<<<1, 64, 0, c->stream>>>(fl, 1);
+      LAUNCHCHECK("k_set_flag");
+      tg::k_check_cols<<<grid1(nc / 8, 256), 256, 0, c->stream>>>(T, fl);
+      LAUNCHCHECK("k_check_cols");
+      CG.gate = PG.gate = CX.gate = PX.gate = fl;
+      CG.gate_on = PG.gate_on = 1;
+      CX.gate_on = PX.gate_on = 0;
     }
+    tg::Out og = CG, ox = CX;
+    og.part = nsplit_g > 1 ? c->part.as<double>() : nullptr;
+    ox.part = nsplit > 1 ? c->part.as<double>() : nullptr;
+    og.pstride = ox.pstride = nc;
+    const tg::CellBlocks CB{nullptr, nullptr, nullptr, nullptr, ca[3], nullptr, nblk, ca[0], ca[1], ca[2], ca[4]};
     tg::Cells C{ca[0], ca[1], ca[2], ca[3], ca[4], nc};
     rc = timed_launch(c, nc, np, [&]() -> int {
+      if (cand) {
+        bidir_grid_kernel(c, mode, CB, np, chunk_g, nsplit_g, og);
+        LAUNCHCHECK("k_bidir_grid");
+      }
       v.fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v.nt), c->stream, C, c->bp4.as<double4>(),
-           c->bpe2.as<double>(), np, chunk, c->bstat.as<const double>(), o, c->bppart.as<double>());
+           c->bpe2.as<double>(), np, chunk, c->bstat.as<const double>(), ox, c->bppart.as<double>());
       LAUNCHCHECK("k_bidir");
       return 0;
     });
     if (rc) return rc;
-    if (nsplit > 1) {
-      if ((rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CO))) return rc;
+    if (cand) {
+      if (nsplit_g > 1 && (rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit_g, nc, CG))) return rc;
+      if ((rc = reduce_particle_partials(c, nblk, np, PG))) return rc;
     }
-    if ((rc = reduce_particle_partials(c, ntiles, np, PO))) return rc;
+    if (nsplit > 1 && (rc = reduce_chunks(c, c->part.as<double>(), nc, nsplit, nc, CX))) return rc;
+    if ((rc = reduce_particle_partials(c, ntiles, np, PX))) return rc;
   }
   if (kind == TGRAV_HOST) {
     for (int q = 0; q < 4; ++q) {
