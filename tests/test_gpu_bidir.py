@@ -123,6 +123,30 @@ def test_blocks_match_explicit_and_oracle(mode, nblk, npart):
             assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
 
 
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_block_order_with_varied_softening_in_a_block(mode):
+    """Cells in the blocks' C order whose softening is not uniform within one 512-cell block fail the
+    structure test (the column kernel reads one eps per block) and must take the generic fused
+    kernel: both fields still match the oracle with the varied eps, and differ from the uniform run."""
+    B, C, P = blocks_and_particles(4242, 3, 300)
+    Cv = dict(C)
+    Cv["eps"] = C["eps"].copy()
+    Cv["eps"][512 + 37] *= 1.5
+    with GravityField(mode) as f:
+        cu, pu = run(f, C, P)
+        cv, pv = run(f, Cv, P)
+    (pa, pb, pc, pp), (ca, cb, cc, cp) = oracle.bidirectional(
+        Cv["x"], Cv["y"], Cv["z"], Cv["m"], Cv["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"], G)
+    for name, got, ref in (("cells", cv, (ca, cb, cc, cp)), ("particles", pv, (pa, pb, pc, pp))):
+        s = _util.summary(got, ref)
+        print(mode, name, s, flush=True)
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (name, s)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
+    assert not np.array_equal(cv[0], cu[0]) and not np.array_equal(pv[0], pu[0])
+
+
 def test_blocks_chunking_deterministic_and_errors():
     B, C, P = blocks_and_particles(9, 4, 3000)
     lib = _lib.load()
