@@ -11,7 +11,9 @@ function cites the reference file:line it restates.  ``oracle.brute`` holds a
 Parity status of each function (what pins it is in tests/test_oracle.py):
   eval_plummer    pinned: decimal brute force, closed forms, invariants
   eval_flash      pinned: closed forms (r >= rsoft Newtonian, continuity at rsoft,
-                  r -> 0 limit, a = -grad phi in clean mode); periodic branch: equals the
+                  r -> 0 limit, a = -grad phi in clean mode); the reference's quirk
+                  potential inside rsoft by quirk + clean = hinv q^4 / 5 (q < 1), 0 (1 <= q < 2),
+                  with planted coefficient mistakes failing it; periodic branch: equals the
                   isolated kernel over explicitly replicated, cutoff-filtered images; zero
                   field at lattice symmetry points
   bidirectional   pinned: equals two eval_plummer calls; Newton's third law
@@ -46,24 +48,30 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(path)
-        i64, dbl, i32 = ctypes.c_int64, ctypes.c_double, ctypes.c_int
-        L.oracle_eval_plummer.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 4 + [dbl] + [_D] * 4 + [i32, i32]
-        L.oracle_eval_flash.argtypes = ([i64] + [_D] * 4 + [i64] + [_D] * 3 + [dbl, dbl, i32, i32, i32, dbl, dbl, dbl]
-                                        + [_D] * 4 + [i32])
-        L.oracle_bidirectional.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 5 + [dbl] + [_D] * 8
-        L.oracle_kick.argtypes = [i64, dbl] + [_D] * 6
-        L.oracle_block_centres.argtypes = [i64, i32] + [_D] * 7
-        _I = ctypes.POINTER(ctypes.c_int64)
-        L.oracle_pm_deposit.argtypes = [i64] + [_D] * 4 + [_I, i64, i32, i32] + [_D] * 5
-        L.oracle_pm_gradient.argtypes = [i64, i32, i32] + [_D] * 5
-        L.oracle_pm_interpolate.argtypes = [i64] + [_D] * 3 + [_I, i64, i32, i32] + [_D] * 10
-        for f in (L.oracle_eval_plummer, L.oracle_eval_flash, L.oracle_bidirectional,
-                  L.oracle_kick, L.oracle_block_centres, L.oracle_pm_deposit,
-                  L.oracle_pm_gradient, L.oracle_pm_interpolate):
-            f.restype = ctypes.c_int
-        _LIB = L
+        _LIB = load(path)
     return _LIB
+
+
+def load(path):
+    """A library built from tgrav_oracle.c at `path`, with its argument types declared (the
+    tests load copies with planted mistakes this way)."""
+    L = ctypes.CDLL(path)
+    i64, dbl, i32 = ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    L.oracle_eval_plummer.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 4 + [dbl] + [_D] * 4 + [i32, i32]
+    L.oracle_eval_flash.argtypes = ([i64] + [_D] * 4 + [i64] + [_D] * 3 + [dbl, dbl, i32, i32, i32, dbl, dbl, dbl]
+                                    + [_D] * 4 + [i32])
+    L.oracle_bidirectional.argtypes = [i64] + [_D] * 5 + [i64] + [_D] * 5 + [dbl] + [_D] * 8
+    L.oracle_kick.argtypes = [i64, dbl] + [_D] * 6
+    L.oracle_block_centres.argtypes = [i64, i32] + [_D] * 7
+    _I = ctypes.POINTER(ctypes.c_int64)
+    L.oracle_pm_deposit.argtypes = [i64] + [_D] * 4 + [_I, i64, i32, i32] + [_D] * 5
+    L.oracle_pm_gradient.argtypes = [i64, i32, i32] + [_D] * 5
+    L.oracle_pm_interpolate.argtypes = [i64] + [_D] * 3 + [_I, i64, i32, i32] + [_D] * 10
+    for f in (L.oracle_eval_plummer, L.oracle_eval_flash, L.oracle_bidirectional,
+              L.oracle_kick, L.oracle_block_centres, L.oracle_pm_deposit,
+              L.oracle_pm_gradient, L.oracle_pm_interpolate):
+        f.restype = ctypes.c_int
+    return L
 
 
 def _in(a, n=None):

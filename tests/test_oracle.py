@@ -272,6 +272,56 @@ def test_flash_kernel_limits(G):
     np.testing.assert_allclose(pl[1:4], -G * 2.0 / rs ** 3, rtol=1e-15)
 
 
+def _quirk_pin_error(eval_flash, G, rs, m=2.0):
+    """Largest violation, in units of the term scale G m hinv, of the identity between the
+    reference's spline potential inside rsoft (Gravity_getPotentialAtPoint.F90:202-211) and the
+    clean one: quirk + clean kernel = hinv q^4 / 5 for q < 1 and 0 for 1 <= q < 2, where the
+    clean potential is pinned independently (a = -grad phi, continuity with -1/r at rsoft)."""
+    h = rs / 2.0
+    hinv = 1.0 / h
+    q = np.array([0.01, 0.1, 0.3, 0.5, 0.8, 0.99, 1.0, 1.2, 1.5, 1.9, 1.99])
+    r = q * h
+    t = np.stack([r, np.zeros_like(r), np.zeros_like(r)], 1)
+    s = np.zeros((1, 3))
+    mm = np.array([m])
+    pq = eval_flash(s[:, 0], s[:, 1], s[:, 2], mm, t[:, 0], t[:, 1], t[:, 2], G, rs, kernel="spline",
+                    quirks=True)[3]
+    pc = eval_flash(s[:, 0], s[:, 1], s[:, 2], mm, t[:, 0], t[:, 1], t[:, 2], G, rs, kernel="spline",
+                    quirks=False)[3]
+    # phi = -G m * kernel for one source, so the kernels' identity reads on -phi / (G m)
+    got = -(pq + pc) / (G * m)
+    want = np.where(q < 1.0, 0.2 * hinv * q ** 4, 0.0)
+    return float(np.max(np.abs(got - want)) / hinv)
+
+
+def test_flash_quirk_potential_pinned(G, tmp_path):
+    """The quirk potential inside rsoft is pinned by its sum with the clean one (not only by its
+    sign); a mistyped coefficient planted in a compiled copy of the oracle must fail the pin."""
+    assert _quirk_pin_error(oracle.eval_flash, G, 0.05) < 1e-14
+    assert _quirk_pin_error(oracle.eval_flash, G, 3.0) < 1e-14
+    src = open(os.path.join(os.path.dirname(oracle.__file__), "tgrav_oracle.c")).read()
+    quirk_line = "pk = hinv * (2.0 / 3.0 * q * q - 0.1 * pow(q, 4) + 0.1 * pow(q, 5) - 7.0 / 5.0);"
+    assert src.count(quirk_line) == 1
+    plants = {"q4 coefficient 0.1 -> 0.3": quirk_line.replace("0.1 * pow(q, 4)", "0.3 * pow(q, 4)"),
+              "q5 coefficient 0.1 -> 0.3": quirk_line.replace("0.1 * pow(q, 5)", "0.3 * pow(q, 5)"),
+              "constant 7/5 -> 6/5": quirk_line.replace("7.0 / 5.0", "6.0 / 5.0")}
+    import subprocess
+    for i, (name, bad) in enumerate(plants.items()):
+        c_path = tmp_path / f"planted{i}.c"
+        so_path = tmp_path / f"libplanted{i}.so"
+        c_path.write_text(src.replace(quirk_line, bad))
+        subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-o", str(so_path),
+                        str(c_path), "-lm"], check=True)
+        planted = oracle.load(str(so_path))
+        saved = oracle._LIB
+        oracle._LIB = planted
+        try:
+            err = _quirk_pin_error(oracle.eval_flash, G, 0.05)
+        finally:
+            oracle._LIB = saved
+        assert err > 1e-3, f"planted mistake '{name}' passed the quirk pin ({err})"
+
+
 def test_flash_spline_matches_closed_form_table(G):
     """Spline force kernel at q = 0.5 and 1.5 written out by hand (Monaghan & Lattanzio)."""
     rs = 0.1
