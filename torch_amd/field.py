@@ -24,7 +24,12 @@ argument (in the reference's FLASH exports ``eps`` is accepted but unused and a 
 radius applies, Gravity_getAccelAtPoint.F90:89-145; here it is honoured).  FLASH's own
 spline / linear kernels with that global radius, and its periodic image sum, are
 available too (:meth:`GravityField.set_softening`, :meth:`GravityField.set_periodic`).
-Units are those of the inputs; the default G = 4.300917270e-3 pc (km/s)^2 / Msun.
+Units are those of the inputs; the default G = 4.300917270e-3 pc (km/s)^2 / Msun.  AMUSE
+quantities (anything with ``value_in``) are accepted too, as the reference's legacy functions
+take them (interface.py:1436-1446, 1616-1620): they are read in the field's unit system
+(``units=``, :mod:`torch_amd.units`; pc / Msun / km/s unless given) and the results of a call
+that received quantities are quantities: accelerations in speed^2 / length, potentials in
+speed^2.
 """
 import ctypes
 import os
@@ -32,9 +37,10 @@ import os
 import numpy as np
 
 from . import _lib
+from . import units as _units
 from ._lib import DEVICE, HOST, MIXED, PLAIN_FP64, TgravError
+from .units import G_PC_MSUN_KMS
 
-G_PC_MSUN_KMS = 4.300917270e-3
 MODES = {"fp64": PLAIN_FP64, "plain_fp64": PLAIN_FP64, "mixed": MIXED}
 
 
@@ -103,6 +109,11 @@ def _length(*arrs):
     return 1
 
 
+def _host(a):
+    """Quantities wrap host arrays: device results are copied back for them."""
+    return a.cpu().numpy() if a is not None and _is_torch(a) else a
+
+
 def _torch_device(*arrs):
     for a in arrs:
         if a is not None and _is_torch(a) and a.is_cuda:
@@ -118,11 +129,13 @@ class GravityField:
     compensated sums, FP64 result and kick).
     """
 
-    def __init__(self, mode="fp64", device=0, G=G_PC_MSUN_KMS, validate=True, tune_cache="default"):
+    def __init__(self, mode="fp64", device=0, G=None, validate=True, tune_cache="default", units=None):
         """tune_cache: path of an autotuner cache the library applies by itself (launch shapes
         per device and size bucket; shapes without an entry take the built-in heuristics);
         "default" is the package's tune_cache.json (or $TGRAV_TUNE_CACHE); None uses the
-        heuristics only.  An explicit set_config / set_bidir_config wins over the cache."""
+        heuristics only.  An explicit set_config / set_bidir_config wins over the cache.
+        units: a torch_amd.units.UnitSystem for AMUSE quantities (e.g. units.cgs() for FLASH's
+        cm / g / s); G defaults to its G, else to G in pc, Msun, km/s."""
         if mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
         self.lib = _lib.load()
@@ -136,6 +149,9 @@ class GravityField:
         path = None if tune_cache is None else os.fsencode(tune_cache)
         _lib.check(self.lib.tgrav_create(ctypes.byref(ctx), int(device), path), "tgrav_create")
         self._ctx = ctx
+        self.units = units
+        if G is None:
+            G = units.G if units is not None else G_PC_MSUN_KMS
         self.G = G
         _lib.check(self.lib.tgrav_set_constant(ctx, float(G)), "tgrav_set_constant")
         _lib.check(self.lib.tgrav_set_validation(ctx, int(bool(validate))), "tgrav_set_validation")
@@ -163,6 +179,29 @@ class GravityField:
     def handle(self):
         return self._ctx
 
+    # -- AMUSE quantities -----------------------------------------------------------
+    def _unit_system(self):
+        if self.units is None:
+            if self.G != G_PC_MSUN_KMS:
+                raise TgravError(_lib.EINVAL, "argument", "quantities with a custom G need units= "
+                                 "(a torch_amd.units.UnitSystem with that G)")
+            self.units = _units.pc_msun_kms()
+        return self.units
+
+    def _plain(self, *pairs):
+        """[(arg, kind)] -> (the args as plain numbers in the field's units, whether any was a
+        quantity)."""
+        if not any(_units.is_quantity(a) for a, _ in pairs):
+            return [a for a, _ in pairs], False
+        us = self._unit_system()
+        return [us.value(a, kind) for a, kind in pairs], True
+
+    def _quantities(self, outs, kinds, wrap):
+        if not wrap:
+            return tuple(outs)
+        us = self._unit_system()
+        return tuple(us.quantity(_host(o), k) for o, k in zip(outs, kinds))
+
     def _bind_stream(self, arrays):
         """Device tensors: run on torch's current stream (0 is the null stream) so work is
         ordered with the caller's.  Host arrays: the context's own stream (calls sync)."""
@@ -181,6 +220,8 @@ class GravityField:
 
     # -- sources ----------------------------------------------------------------
     def set_sources(self, x, y, z, mass, eps=None):
+        (x, y, z, mass, eps), _ = self._plain((x, "length"), (y, "length"), (z, "length"), (mass, "mass"),
+                                              (eps, "length"))
         n = _length(x, y, z, mass)
         A = _Arrays(n, _torch_device(x, y, z, mass, eps))
         xs = [A.inp(x, "x"), A.inp(y, "y"), A.inp(z, "z"), A.inp(mass, "mass"),
@@ -194,6 +235,8 @@ class GravityField:
         """Leaf blocks as the source set: lower corners origin[n,3], cell width dx[n], cell
         masses mass[n*512] (rho dV) in C order, softening eps[n] per block (default: the cell
         width).  Evaluations use the column kernel for 8^3 blocks."""
+        (origin, dx, mass, eps), _ = self._plain((origin, "length"), (dx, "length"), (mass, "mass"),
+                                                 (eps, "length"))
         nblk = int(origin.shape[0])
         dev = _torch_device(origin, dx, mass, eps)
         A, B = _Arrays(nblk, dev), _Arrays(nblk * nb ** 3, dev)
@@ -209,7 +252,9 @@ class GravityField:
 
     # -- evaluation ---------------------------------------------------------------
     def evaluate(self, eps, x, y, z, acceleration=True, potential=True):
-        """(ax, ay, az, phi) at the points; unrequested outputs are None."""
+        """(ax, ay, az, phi) at the points; unrequested outputs are None (quantities when any
+        argument was one)."""
+        (eps, x, y, z), wrap = self._plain((eps, "length"), (x, "length"), (y, "length"), (z, "length"))
         n = _length(x, y, z, eps)
         A = _Arrays(n, _torch_device(x, y, z, eps))
         e = A.inp(eps, "eps", allow_none=True)
@@ -220,11 +265,14 @@ class GravityField:
         rc = self.lib.tgrav_eval(self._ctx, n, A.ptr(e), *[A.ptr(a) for a in xs],
                                  *[A.ptr(o) for o in outs], self.mode, A.kind)
         _lib.check(rc, "tgrav_eval")
-        return tuple(outs)
+        return self._quantities(outs, ("acceleration",) * 3 + ("potential",), wrap)
 
     def evaluate_and_kick(self, eps, x, y, z, dt, vx, vy, vz, potential=False):
         """v += dt * a(x) in FP64, fused with the evaluation; returns (ax, ay, az, phi) and
-        updates vx/vy/vz in place (numpy float64 arrays or float64 device tensors)."""
+        updates vx/vy/vz in place (numpy float64 arrays or float64 device tensors, in the
+        field's speed unit when the other arguments are quantities)."""
+        (eps, x, y, z, dt), wrap = self._plain((eps, "length"), (x, "length"), (y, "length"), (z, "length"),
+                                               (dt, "time"))
         n = _length(x, y, z, eps)
         A = _Arrays(n, _torch_device(x, y, z, eps, vx))
         e = A.inp(eps, "eps", allow_none=True)
@@ -237,12 +285,14 @@ class GravityField:
                                       A.ptr(vx), A.ptr(vy), A.ptr(vz), *[A.ptr(o) for o in outs],
                                       self.mode, A.kind)
         _lib.check(rc, "tgrav_eval_kick")
-        return tuple(outs)
+        return self._quantities(outs, ("acceleration",) * 3 + ("potential",), wrap)
 
     def evaluate_blocks(self, origin, dx, nb=8, eps=None, potential=True, dt=0.0, vel=None):
         """Field at the nb^3 cell centres of leaf blocks (lower corners origin[n,3], cell
         width dx[n]); outputs in C order b*nb^3 + (i*nb + j)*nb + k.  vel=(vx,vy,vz)
         per-cell arrays are kicked by dt in place."""
+        (origin, dx, eps, dt), wrap = self._plain((origin, "length"), (dx, "length"), (eps, "length"),
+                                                  (dt, "time"))
         nblk = int(origin.shape[0])
         ncell = nblk * nb ** 3
         A = _Arrays(nblk, _torch_device(origin, dx, eps))
@@ -261,13 +311,16 @@ class GravityField:
                                         A.ptr(d), A.ptr(e), *[B.ptr(o) for o in outs], float(dt),
                                         *[B.ptr(q) for q in v], self.mode, A.kind)
         _lib.check(rc, "tgrav_eval_blocks")
-        return tuple(outs)
+        return self._quantities(outs, ("acceleration",) * 3 + ("potential",), wrap)
 
     def bidirectional(self, cx, cy, cz, cm, ce, px, py, pz, pm, pe, potential=True):
         """One fused pass over all (cell, particle) pairs with eps2 = ce^2 + pe^2
         (Particles_sinkAccelGasOnSinksAndSinksOnGas.F90:204-316): returns
         ((cax, cay, caz, cphi) of the cells from the particles,
          (pax, pay, paz, pphi) of the particles from the cells)."""
+        L, M = "length", "mass"
+        (cx, cy, cz, cm, ce, px, py, pz, pm, pe), wrap = self._plain(
+            (cx, L), (cy, L), (cz, L), (cm, M), (ce, L), (px, L), (py, L), (pz, L), (pm, M), (pe, L))
         nc, np_ = _length(cx, cy, cz, cm), _length(px, py, pz, pm)
         dev = _torch_device(cx, cy, cz, cm, ce, px, py, pz, pm, pe)
         A, B = _Arrays(nc, dev), _Arrays(np_, dev)
@@ -282,12 +335,16 @@ class GravityField:
                                           *[B.ptr(a) for a in pa], *[A.ptr(o) for o in co],
                                           *[B.ptr(o) for o in po], self.mode, A.kind)
         _lib.check(rc, "tgrav_bidirectional")
-        return tuple(co), tuple(po)
+        kinds = ("acceleration",) * 3 + ("potential",)
+        return self._quantities(co, kinds, wrap), self._quantities(po, kinds, wrap)
 
     def bidirectional_blocks(self, origin, dx, cm, ceps, px, py, pz, pm, pe, potential=True):
         """The fused pass with the gas cells as 8^3 leaf blocks: lower corners origin[n, 3], cell
         widths dx[n], cell masses cm[n*512] (C order), softening ceps[n] per block (None = 0).
         Returns the same as :meth:`bidirectional`, the cell outputs in the blocks' C order."""
+        L, M = "length", "mass"
+        (origin, dx, cm, ceps, px, py, pz, pm, pe), wrap = self._plain(
+            (origin, L), (dx, L), (cm, M), (ceps, L), (px, L), (py, L), (pz, L), (pm, M), (pe, L))
         nblk = int(origin.shape[0])
         np_ = _length(px, py, pz, pm)
         dev = _torch_device(origin, dx, cm, ceps, px, py, pz, pm, pe)
@@ -305,7 +362,8 @@ class GravityField:
                                                  *[M.ptr(o) for o in co], *[B.ptr(o) for o in po],
                                                  self.mode, A.kind)
         _lib.check(rc, "tgrav_bidirectional_blocks")
-        return tuple(co), tuple(po)
+        kinds = ("acceleration",) * 3 + ("potential",)
+        return self._quantities(co, kinds, wrap), self._quantities(po, kinds, wrap)
 
     SOFTENING = {"plummer": 0, "spline": 1, "linear": 2}
 
@@ -316,6 +374,7 @@ class GravityField:
         quirks=True reproduces the reference's potential inside rsoft as written."""
         if kernel not in self.SOFTENING:
             raise ValueError(f"kernel must be one of {sorted(self.SOFTENING)}")
+        (rsoft,), _ = self._plain((rsoft, "length"))
         _lib.check(self.lib.tgrav_set_softening(self._ctx, self.SOFTENING[kernel], float(rsoft),
                                                 int(bool(quirks))), "tgrav_set_softening")
 
@@ -325,7 +384,7 @@ class GravityField:
         if box is None:
             _lib.check(self.lib.tgrav_set_periodic(self._ctx, 0, 0.0, 0.0, 0.0, 0), "tgrav_set_periodic")
             return
-        lx, ly, lz = (float(b) for b in box)
+        lx, ly, lz = (float(self._plain((b, "length"))[0][0]) for b in box)
         _lib.check(self.lib.tgrav_set_periodic(self._ctx, 1, lx, ly, lz, int(nrep)), "tgrav_set_periodic")
 
     def set_block_kernel(self, on=True):
@@ -402,6 +461,8 @@ class GasGravityField(GravityField):
     def set_cells(self, x, y, z, rho, dx, eps=None):
         """Cells with centre (x,y,z), density rho and width dx; softening eps (default:
         the cell width)."""
+        (x, y, z, rho, dx, eps), _ = self._plain((x, "length"), (y, "length"), (z, "length"),
+                                                 (rho, "density"), (dx, "length"), (eps, "length"))
         mass = rho * dx * dx * dx
         self.set_sources(x, y, z, mass, dx if eps is None else eps)
 
