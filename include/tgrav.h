@@ -59,7 +59,12 @@ typedef struct tgrav_batch tgrav_batch;
  * n_target_blocks > 0 gives the targets as that many 8^3 leaf blocks instead of tx/ty/tz/te:
  * lower corners (tbx, tby, tbz), cell widths tbd and softening tbe (NULL = 0) per block, and
  * n_targets must be 512 * n_target_blocks; outputs and velocities are then in the blocks' C
- * order (as tgrav_eval_blocks), and the plan evaluates them with the column kernel. */
+ * order (as tgrav_eval_blocks), and the plan evaluates them with the column kernel.
+ * n_source_blocks > 0 gives the sources as that many 8^3 leaf blocks instead of sx/sy/sz/se:
+ * lower corners (sbx, sby, sbz), cell widths sbd and softening sbe (NULL = 0) per block, and
+ * the cell masses in sm, C order b*512 + (i*8 + j)*8 + k (as tgrav_set_source_blocks);
+ * n_sources must be 512 * n_source_blocks, and the plan walks them block by block and column by
+ * column (the per-block source loop of Gravity_getAccelAtPoint.F90:168-204). */
 typedef struct tgrav_problem {
   int64_t n_sources;
   const double *sx, *sy, *sz, *sm, *se;
@@ -69,6 +74,8 @@ typedef struct tgrav_problem {
   double *vx, *vy, *vz;
   int64_t n_target_blocks;
   const double *tbx, *tby, *tbz, *tbd, *tbe;
+  int64_t n_source_blocks;
+  const double *sbx, *sby, *sbz, *sbd, *sbe;
 } tgrav_problem;
 
 /* ---- context ---------------------------------------------------------- */
@@ -256,13 +263,12 @@ int tgrav_pm_interpolate(tgrav_ctx* ctx, int64_t np, const double* x, const doub
 
 /* ---- batched uneven evaluations --------------------------------------- */
 /* Many independent problems in one persistent launch (device pointers only).  The plan
- * packs the sources, builds a cost-sorted work list (target tile x source chunk) and
+ * packs the sources, builds cost-sorted work lists (target tile x source chunk) and
  * owns the partial-sum scratch.  dt != 0 fuses the FP64 kick v += dt * a.  Explicit
- * targets in the C order of 8^3 leaf-block meshgrids are recognised here (one device test
- * per problem, waited for) and evaluated like block targets; such targets must keep that
- * structure (x, y shared by each aligned 8, eps by each aligned 512) when they are updated in
- * place between runs -- cell centres do -- or the plan is created with
- * tgrav_set_block_kernel(ctx, 0). */
+ * targets in the C order of 8^3 leaf-block meshgrids (x, y shared by each aligned 8, eps by
+ * each aligned 512) are evaluated like block targets: the plan tests that structure on the
+ * device at every run (inside the graph too), so targets updated in place between runs are
+ * always evaluated by a kernel that fits them.  Creation does not wait for the device. */
 int tgrav_batch_create(tgrav_ctx* ctx, int32_t n_problems, const tgrav_problem* problems,
                        int mode, double dt, tgrav_batch** out);
 /* Runs the plan.  use_graph != 0 captures the launch sequence (counter reset,

@@ -266,3 +266,121 @@ def test_explicit_targets_without_column_structure(mode):
         if mode == "mixed":
             s = _util.summary(res["mixed"][i], res["fp64"][i])
             assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (i, s)
+
+
+def cell_blocks(rng, nblk, lo=-2.0, hi=1.5):
+    o = rng.uniform(lo, hi, (nblk, 3))
+    s = 10.0 ** rng.uniform(np.log10(0.03125), np.log10(0.5), nblk)
+    tx, ty, tz, _ = W.block_cell_centres(o, s)
+    m = rng.lognormal(0.0, 1.0, nblk * 512) * np.repeat((s / 8) ** 3, 512)
+    return o, s / 8, (tx, ty, tz), m
+
+
+def set_item(f, item):
+    _lib.check(_lib.load().tgrav_set_batch_config(f.handle, float(item), 0), "tgrav_set_batch_config")
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_block_source_problems(mode):
+    """Problems whose sources are 8^3 leaf blocks (the reverse direction: cells as point masses onto
+    stars), in one plan with a block-target problem (the forward direction) and a generic one.
+    Unsplit, each equals the eager block kernels (k_eval_gridsrc, k_eval_grid) bit for bit; with the
+    default item sizes, FP64 is within 1e-12 of the oracle (binary members included) and mixed meets
+    the gate; graph replays equal the eager run of the plan."""
+    import torch
+    rng = W.rng_for(71)
+    o, dx, (cx, cy, cz), cm = cell_blocks(rng, 14)
+    ce = np.repeat(dx, 512)
+    st = W.star_cluster(rng, 900)
+    st["x"][:2] = [0.1, 0.1 + 5e-6]            # a 1 AU pair
+    st["y"][:2] = [0.2, 0.2]
+    st["z"][:2] = [-0.3, -0.3]
+    seps = np.full(900, 0.0)
+    t2 = rng.uniform(-2, 2, (3, 700))
+    probs = [dict(sources=dict(block_origin=o, block_dx=dx, block_eps=dx, m=cm),
+                  targets=dict(x=st["x"], y=st["y"], z=st["z"], eps=seps)),
+             dict(sources=dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=np.full(900, 0.01)),
+                  targets=dict(block_origin=o, block_dx=dx)),
+             dict(sources=dict(x=st["x"][:300], y=st["y"][:300], z=st["z"][:300], m=st["m"][:300]),
+                  targets=dict(x=t2[0], y=t2[1], z=t2[2]))]
+    dev = problems_from_workload(probs)
+    with GravityField(mode) as f:
+        f.set_config(nsplit=1)
+        f.set_source_blocks(o, dx, cm, eps=dx)
+        e_rev = f.evaluate(seps, st["x"], st["y"], st["z"])
+        f.set_sources(st["x"], st["y"], st["z"], st["m"], np.full(900, 0.01))
+        e_fwd = f.evaluate_blocks(o, dx)
+        set_item(f, 1e30)                       # one chunk per problem, as the eager launches
+        plan = BatchPlan(f, dev)
+        assert plan.n_items > 0
+        got = to_np(plan.run(graph=True))
+        torch.cuda.synchronize()
+        for q in range(4):
+            assert np.array_equal(got[0][q], e_rev[q]), q
+            assert np.array_equal(got[1][q], e_fwd[q]), q
+        plan.close()
+        set_item(f, 0)                          # default item sizes: the sources are split
+        f.set_config(nsplit=0)
+        plan = BatchPlan(f, dev)
+        eager = to_np(plan.run(graph=False))
+        torch.cuda.synchronize()
+        for rep in range(2):
+            again = to_np(plan.run(graph=True))
+            torch.cuda.synchronize()
+            for i in range(3):
+                for q in range(4):
+                    assert np.array_equal(again[i][q], eager[i][q]), (rep, i, q)
+        plan.close()
+    refs = [oracle.eval_plummer(cx, cy, cz, cm, ce, st["x"], st["y"], st["z"], None, G, mode=oracle.NEUMAIER,
+                                threads=16),
+            oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], np.full(900, 0.01), cx, cy, cz, None, G,
+                                mode=oracle.NEUMAIER, threads=16),
+            oracle.eval_plummer(st["x"][:300], st["y"][:300], st["z"][:300], st["m"][:300], None, t2[0], t2[1],
+                                t2[2], None, G, mode=oracle.NEUMAIER, threads=16)]
+    for i in range(3):
+        s = _util.summary(eager[i], refs[i])
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (i, s)
+    s = _util.summary(tuple(a[:2] for a in eager[0]), tuple(a[:2] for a in refs[0]))   # the binary
+    assert s["acc_max"] <= (1e-12 if mode == "fp64" else 1e-4), s
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_column_structure_checked_at_every_run(mode):
+    """A plan created on zero-filled explicit targets (every aligned 8 then share x and y) whose
+    targets are written afterwards: each run tests the column structure again, inside the graph,
+    so shuffled centres take the generic kernel and C-ordered ones the column kernel."""
+    import torch
+    rng = W.rng_for(81)
+    o, dx, (cx, cy, cz), _ = cell_blocks(rng, 6)
+    st = W.star_cluster(rng, 500)
+    S = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=np.full(500, 0.01))
+    n = len(cx)
+    tgt = {k: torch.zeros(n, dtype=torch.float64, device="cuda") for k in ("x", "y", "z")}
+    src = {k: torch.as_tensor(v, device="cuda") for k, v in S.items()}
+    perm = rng.permutation(n)
+    with GravityField(mode) as f:
+        plan = BatchPlan(f, [dict(sources=src, targets=tgt)])
+        blk = BatchPlan(f, [dict(sources=src, targets=dict(block_origin=torch.as_tensor(o, device="cuda"),
+                                                          block_dx=torch.as_tensor(dx, device="cuda")))])
+        ref_blk = to_np(blk.run(graph=False))[0]
+        for order in ("shuffled", "c_order", "shuffled"):
+            p = perm if order == "shuffled" else np.arange(n)
+            for k, v in zip(("x", "y", "z"), (cx, cy, cz)):
+                tgt[k].copy_(torch.as_tensor(v[p], device="cuda"))
+            got = to_np(plan.run(graph=True))[0]
+            torch.cuda.synchronize()
+            ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], cx[p], cy[p], cz[p], None, G,
+                                      mode=oracle.NEUMAIER, threads=16)
+            s = _util.summary(got, ref)
+            if mode == "fp64":
+                assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (order, s)
+            else:
+                assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (order, s)
+            if order == "c_order":              # the column kernel: the block form's bits
+                for q in range(4):
+                    assert np.array_equal(got[q], ref_blk[q]), q
+        plan.close()
+        blk.close()

@@ -30,7 +30,9 @@ class Problem(ctypes.Structure):
                 ("ax", _P), ("ay", _P), ("az", _P), ("phi", _P),
                 ("vx", _P), ("vy", _P), ("vz", _P),
                 ("n_target_blocks", _I64), ("tbx", _P), ("tby", _P), ("tbz", _P), ("tbd", _P),
-                ("tbe", _P)]
+                ("tbe", _P),
+                ("n_source_blocks", _I64), ("sbx", _P), ("sby", _P), ("sbz", _P), ("sbd", _P),
+                ("sbe", _P)]
 
 
 SIGNATURES = [

@@ -36,11 +36,14 @@ class BatchPlan:
     """problems: list of dict(sources=dict(x,y,z,m[,eps]), targets=dict(x,y,z[,eps])
     [, velocities=(vx,vy,vz)]).  Targets with block_origin[n, 3] and block_dx[n] (optional
     block_eps[n]) are 8^3 leaf blocks: the plan evaluates them with the column kernel, outputs
-    in the blocks' C order; their x, y, z are not read.  Explicit targets already in that C order
-    (each aligned 8 share x and y, each aligned 512 share eps) are recognised when the plan is
-    created and take the same kernel; updated in place between runs, they must keep that order
-    (field.set_block_kernel(False) before creating the plan turns the test off).  field: a
-    GravityField whose context (device, stream, G, kernel configuration) the plan uses."""
+    in the blocks' C order; their x, y, z are not read.  Sources with block_origin[n, 3],
+    block_dx[n] (optional block_eps[n]) and m[512 n] (cell masses rho dV in the blocks' C order)
+    are 8^3 leaf blocks, walked block by block and column by column; their x, y, z are not read.
+    Explicit targets in the blocks' C order (each aligned 8 share x and y, each aligned 512 share
+    eps) take the column kernel too: the plan tests that order on the device at every run, so
+    targets updated in place between runs always get a kernel that fits them
+    (field.set_block_kernel(False) before creating the plan turns the column kernels off).
+    field: a GravityField whose context (device, stream, G, kernel configuration) the plan uses."""
 
     def __init__(self, field: GravityField, problems, dt=0.0, potential=True):
         import torch
@@ -52,7 +55,17 @@ class BatchPlan:
         arr = (_lib.Problem * max(len(problems), 1))()
         for i, p in enumerate(problems):
             S, T = p["sources"], p["targets"]
-            s = [_dev(S[k], self.device) for k in ("x", "y", "z", "m")] + [_dev(S.get("eps"), self.device)]
+            sblk = S.get("block_origin")
+            if sblk is not None:     # 8^3 source blocks: corners, cell widths, eps per block, cell masses
+                so = _dev(sblk, self.device).reshape(-1, 3)
+                sb = [so[:, 0].contiguous(), so[:, 1].contiguous(), so[:, 2].contiguous(),
+                      _dev(S["block_dx"], self.device), _dev(S.get("block_eps"), self.device)]
+                s = [None, None, None, _dev(S["m"], self.device), None]
+                ns = 512 * int(sb[3].numel())
+            else:
+                sb = [None] * 5
+                s = [_dev(S[k], self.device) for k in ("x", "y", "z", "m")] + [_dev(S.get("eps"), self.device)]
+                ns = int(s[0].numel())
             blk = T.get("block_origin")
             if blk is not None:      # 8^3 leaf blocks: corners, cell widths, optional eps per block
                 o = _dev(blk, self.device).reshape(-1, 3)
@@ -69,15 +82,17 @@ class BatchPlan:
             v = [None, None, None]
             if dt != 0.0 and p.get("velocities") is not None:
                 v = [_dev(q, self.device) for q in p["velocities"]]
-            self.keep += s + t + tb + out + v
+            self.keep += s + sb + t + tb + out + v
             self.outputs.append(tuple(out))
             P = arr[i]
-            P.n_sources = int(s[0].numel())
+            P.n_sources = ns
             P.sx, P.sy, P.sz, P.sm, P.se = [(a.data_ptr() if a is not None else None) for a in s]
             P.n_targets = nt
             P.tx, P.ty, P.tz, P.te = [(a.data_ptr() if a is not None else None) for a in t]
             P.n_target_blocks = nt // 512 if blk is not None else 0
             P.tbx, P.tby, P.tbz, P.tbd, P.tbe = [(a.data_ptr() if a is not None else None) for a in tb]
+            P.n_source_blocks = ns // 512 if sblk is not None else 0
+            P.sbx, P.sby, P.sbz, P.sbd, P.sbe = [(a.data_ptr() if a is not None else None) for a in sb]
             P.ax, P.ay, P.az, P.phi = [(a.data_ptr() if a is not None else None) for a in out]
             P.vx, P.vy, P.vz = [(a.data_ptr() if a is not None else None) for a in v]
         self.velocities = [p.get("velocities") for p in problems] if dt != 0.0 else None

@@ -6,8 +6,16 @@
 // items from a global counter until the list is empty, so uneven problems balance like a
 // longest-processing-time schedule.  Problems whose sources were chunked write chunk
 // partial sums; k_batch_reduce sums them per target in chunk order (deterministic) and
-// applies G, the stores and the fused FP64 kick.  The sequence reset -> pack -> eval ->
-// reduce uses no memset, so it can be captured in a hipGraph and replayed.
+// applies G, the stores and the fused FP64 kick.  The sequence reset -> pack -> column
+// check -> evals -> reduce uses no memset, so it can be captured in a hipGraph and replayed.
+//
+// Three work lists, one persistent kernel each: generic (explicit targets, explicit sources),
+// columns (targets that are 8^3 leaf blocks, or explicit targets in their C order) and block
+// sources (sources that are 8^3 leaf blocks: the column walk of k_eval_gridsrc).  Explicit
+// targets whose count is a multiple of 512 may or may not be in the blocks' C order; the plan
+// tests that on the device at every run (k_batch_check_cols) and holds items of both forms for
+// them, with the same source chunks (so the partial sums line up): the items of the form the
+// test did not pick return at once.
 #pragma once
 #include <type_traits>
 
@@ -18,16 +26,22 @@ namespace tg {
 
 struct BSrc {              // one problem's caller-owned source arrays
   const double *x, *y, *z, *m, *eps;
+  const double *bx, *by, *bz, *bd;   // bd != null: 8^3 source blocks (corners, cell widths; eps and
+  int64_t blk_off;                   //   m per block / per cell), the plan's blocks blk_off ...
 };
 
+// How a problem's work items are evaluated.
+enum { FORM_GENERIC = 0, FORM_COLS = 1, FORM_CAND = 2, FORM_GRIDSRC = 3 };
+
 struct BProb {
-  Tgt T;                   // targets (explicit)
+  Tgt T;                   // targets (explicit or 8^3 blocks)
   int64_t src_off;         // first packed source of this problem
   int64_t ns;
-  int64_t chunk;           // sources per chunk
+  int64_t chunk;           // sources per chunk (FORM_GRIDSRC: source blocks per chunk)
   int nchunk;              // 1: results written by the eval kernel; >1 or 0: by k_batch_reduce
-  int pad;
+  int form;                // FORM_*; FORM_CAND: columns if colflag[p] == 1 at this run, else generic
   int64_t part_off;        // this problem's chunk partials in the scratch array (doubles)
+  int64_t sblk_off, nsblk; // FORM_GRIDSRC: the plan's source blocks [sblk_off, sblk_off + nsblk)
   Out O;                   // final outputs (+ kick)
 };
 
@@ -35,16 +49,24 @@ struct BItem {             // work item: problem, target tile (or 8^3 target blo
   int prob, tile, chunk, pad;
 };
 
-// counters = 0 (generic and block-target work lists), per-problem stats = {+inf bits, 0}
-__global__ void k_batch_reset(int nprob, unsigned int* counter, unsigned long long* stats) {
+// counters = 0 (generic, column and block-source work lists), per-problem source statistics
+// (ST_WORDS each, see kernels.hpp) reset, column flags = 1 (k_batch_check_cols clears them)
+__global__ void k_batch_reset(int nprob, unsigned int* counter, unsigned long long* stats,
+                              int* colflag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     counter[0] = 0u;
     counter[1] = 0u;
+    counter[2] = 0u;
   }
   if (i < nprob) {
-    stats[2 * i] = 0x7ff0000000000000ULL;
-    stats[2 * i + 1] = 0ULL;
+    colflag[i] = 1;
+    unsigned long long* st = stats + (int64_t)ST_WORDS * i;
+    st[ST_E2MIN] = 0x7ff0000000000000ULL;
+    st[ST_BAD] = 0ULL;
+    st[ST_RMAX] = 0ULL;
+    st[ST_E2MAX] = 0ULL;
+    st[ST_MMAX] = 0ULL;
   }
 }
 
@@ -59,42 +81,91 @@ __device__ __forceinline__ int find_segment(const int64_t* __restrict__ prefix, 
 }
 
 // Packs every problem's sources into one {x,y,z,m} + eps^2 array and folds each problem's
-// smallest eps^2 into stats[2p].  A workgroup inside one problem reduces first and issues
-// one atomic; a workgroup straddling problems falls back to one atomic per lane.
-__global__ void k_batch_pack(int nprob, const BSrc* __restrict__ srcs,
-                             const int64_t* __restrict__ prefix, double4* __restrict__ s4,
-                             double* __restrict__ se2, unsigned long long* stats) {
-  __shared__ unsigned long long smin[256];
+// source statistics (ST_*) into stats[ST_WORDS p ...].  Source blocks are expanded to their cell
+// centres (rounded as k_pack_source_blocks rounds them) and also give their cell-centre axes
+// (sax, 24 per block) and eps^2 (be2) for the block-source kernel.  A workgroup inside one
+// problem reduces first and issues one atomic per word; a workgroup straddling problems falls
+// back to atomics per lane.  Workgroups of 256 lanes.
+__global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __restrict__ srcs,
+                                                    const int64_t* __restrict__ prefix,
+                                                    double4* __restrict__ s4, double* __restrict__ se2,
+                                                    double* __restrict__ sax, double* __restrict__ be2,
+                                                    unsigned long long* stats) {
   __shared__ int sp[2];
   const int64_t total = prefix[nprob];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int p = -1;
-  unsigned long long mine = 0x7ff0000000000000ULL;
+  unsigned long long e2min = 0x7ff0000000000000ULL, rmax = 0ULL, e2max = 0ULL, mmax = 0ULL;
   if (i < total) {
     p = find_segment(prefix, nprob, i);
     const int64_t j = i - prefix[p];
     const BSrc S = srcs[p];
-    s4[i] = make_double4(S.x[j], S.y[j], S.z[j], S.m[j]);
-    const double e = S.eps ? S.eps[j] : 0.0;
+    double4 v;
+    double e;
+    if (S.bd) {
+      const int64_t b = j >> 9;
+      const int c = (int)(j & 511), ci = c >> 6, cj = (c >> 3) & 7, ck = c & 7;
+      const double d = S.bd[b];
+      {
+#pragma clang fp contract(off)
+        v.x = S.bx[b] + ((double)ci + 0.5) * d;
+        v.y = S.by[b] + ((double)cj + 0.5) * d;
+        v.z = S.bz[b] + ((double)ck + 0.5) * d;
+      }
+      v.w = S.m[j];
+      e = S.eps ? S.eps[b] : 0.0;
+      const int64_t gb = S.blk_off + b;
+      if (cj == 0 && ck == 0) sax[gb * 24 + ci] = v.x;
+      if (ci == 0 && ck == 0) sax[gb * 24 + 8 + cj] = v.y;
+      if (ci == 0 && cj == 0) sax[gb * 24 + 16 + ck] = v.z;
+      if (c == 0) be2[gb] = e * e;
+    } else {
+      v = make_double4(S.x[j], S.y[j], S.z[j], S.m[j]);
+      e = S.eps ? S.eps[j] : 0.0;
+    }
+    s4[i] = v;
     se2[i] = e * e;
-    mine = (unsigned long long)__double_as_longlong(e * e);
+    e2min = e2max = dbits(e * e);
+    rmax = dbits(max_abs3(v.x, v.y, v.z));
+    mmax = dbits(fabs(v.w));
   }
   if (threadIdx.x == 0) {
     sp[0] = p;
     const int64_t last = min(total, (int64_t)(blockIdx.x + 1) * blockDim.x) - 1;
     sp[1] = last >= 0 ? find_segment(prefix, nprob, last) : -1;
   }
-  smin[threadIdx.x] = mine;
   __syncthreads();
-  if (sp[0] == sp[1]) {
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0 && sp[0] >= 0) atomicMin(&stats[2 * sp[0]], smin[0]);
+  if (sp[0] == sp[1]) {            // uniform over the workgroup
+    if (sp[0] >= 0) fold_source_stats(e2min, 0, rmax, e2max, mmax, stats + (int64_t)ST_WORDS * sp[0]);
   } else if (p >= 0) {
-    atomicMin(&stats[2 * p], mine);
+    unsigned long long* st = stats + (int64_t)ST_WORDS * p;
+    atomicMin(&st[ST_E2MIN], e2min);
+    atomicMax(&st[ST_RMAX], rmax);
+    atomicMax(&st[ST_E2MAX], e2max);
+    atomicMax(&st[ST_MMAX], mmax);
   }
+}
+
+// The column-structure test of k_check_cols over the plan's candidate problems (ids[q], explicit
+// targets, a multiple of 512): lane i checks one aligned group of 8 targets of candidate q =
+// the segment of i in prefix (groups of 8 per candidate) and clears colflag[ids[q]] on a
+// violation.  Runs after k_batch_reset set the flags to 1.
+__global__ void k_batch_check_cols(int ncand, const int* __restrict__ ids,
+                                   const int64_t* __restrict__ prefix, const BProb* __restrict__ probs,
+                                   int* colflag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= prefix[ncand]) return;
+  const int q = find_segment(prefix, ncand, i);
+  const int p = ids[q];
+  const Tgt& T = probs[p].T;
+  const int64_t t = (i - prefix[q]) * 8;
+  const double x0 = T.x[t], y0 = T.y[t];
+  const double e0 = T.eps ? T.eps[t & ~(int64_t)511] : 0.0;
+  bool ok = true;
+  for (int k = 1; k < 8; ++k)
+    ok = ok && T.x[t + k] == x0 && T.y[t + k] == y0 && (!T.eps || T.eps[t + k] == e0);
+  if (T.eps) ok = ok && T.eps[t] == e0;
+  if (!ok) colflag[p] = 0;
 }
 
 template <int K, int NT, bool MIXED>
@@ -104,6 +175,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
                                                    const double4* __restrict__ s4,
                                                    const double* __restrict__ se2,
                                                    const unsigned long long* __restrict__ stats,
+                                                   const int* __restrict__ colflag,
                                                    double* __restrict__ part) {
   typedef typename std::conditional<MIXED, LdsF32<NT>, LdsF64<NT>>::type Lds;
   __shared__ Lds L;
@@ -116,9 +188,10 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
     if (it >= n_items) break;           // every lane of the workgroup leaves together
     const BItem w = items[it];
     const BProb& P = probs[w.prob];
+    if (P.form == FORM_CAND && colflag[w.prob] != 0) continue;   // the column items run instead
     const int64_t sb = P.src_off + (int64_t)w.chunk * P.chunk;
     const int64_t se = min(sb + P.chunk, P.src_off + P.ns);
-    const double e2s_min = __longlong_as_double((long long)stats[2 * w.prob]);
+    const double* sst = (const double*)(stats + (int64_t)ST_WORDS * w.prob);
     Out o = P.O;
     if (P.nchunk != 1) {
       o.part = part + P.part_off + (int64_t)w.chunk * 4 * P.T.n;
@@ -128,9 +201,9 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
     }
     const int64_t tile0 = (int64_t)w.tile * (K * NT);
     if constexpr (MIXED)
-      process_mixed<K, NT>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
+      process_mixed<K, NT>(L, P.T, tile0, s4, se2, sb, se, sst, o);
     else
-      process_fp64<K, NT>(L, P.T, tile0, s4, se2, sb, se, e2s_min, o);
+      process_fp64<K, NT>(L, P.T, tile0, s4, se2, sb, se, sst, o);
   }
 }
 
@@ -144,6 +217,7 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
                                                         const double4* __restrict__ s4,
                                                         const double* __restrict__ se2,
                                                         const unsigned long long* __restrict__ stats,
+                                                        const int* __restrict__ colflag,
                                                         double* __restrict__ part) {
   typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
   __shared__ Lds L;
@@ -156,9 +230,10 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
     if (it >= n_items) break;           // every lane of the workgroup leaves together
     const BItem w = items[it];
     const BProb& P = probs[w.prob];
+    if (P.form == FORM_CAND && colflag[w.prob] != 1) continue;   // not in C order at this run
     const int64_t sb = P.src_off + (int64_t)w.chunk * P.chunk;
     const int64_t se = min(sb + P.chunk, P.src_off + P.ns);
-    const double e2s_min = __longlong_as_double((long long)stats[2 * w.prob]);
+    const double* sst = (const double*)(stats + (int64_t)ST_WORDS * w.prob);
     Out o = P.O;
     if (P.nchunk != 1) {
       o.part = part + P.part_off + (int64_t)w.chunk * 4 * P.T.n;
@@ -166,8 +241,45 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
     } else {
       o.part = nullptr;
     }
-    if (P.T.blocks) grid_block<MIXED>(L, P.T, w.tile, s4, se2, sb, se, e2s_min, o);
-    else cols_group<MIXED>(L, P.T, w.tile, s4, se2, sb, se, e2s_min, o);   // explicit columns
+    if (P.T.blocks) grid_block<MIXED>(L, P.T, w.tile, s4, se2, sb, se, sst, o);
+    else cols_group<MIXED>(L, P.T, w.tile, s4, se2, sb, se, sst, o);   // explicit columns
+  }
+}
+
+// Problems whose sources are 8^3 leaf blocks: the same persistent scheme over (target tile of
+// K NT, chunk of source blocks) items, evaluated by the column walk of k_eval_gridsrc
+// (the per-block source loop of Gravity_getAccelAtPoint.F90:168-204).
+template <int K, int NT, bool MIXED>
+__global__ __launch_bounds__(NT) void k_batch_eval_gridsrc(const BProb* __restrict__ probs,
+                                                           const BItem* __restrict__ items,
+                                                           int n_items, unsigned int* counter,
+                                                           const double4* __restrict__ s4,
+                                                           const double* __restrict__ sax,
+                                                           const double* __restrict__ be2,
+                                                           const unsigned long long* __restrict__ stats,
+                                                           double* __restrict__ part) {
+  __shared__ LdsGS<MIXED> L;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();                    // all lanes hold `it` before lane 0 overwrites it
+    if (it >= n_items) break;           // every lane of the workgroup leaves together
+    const BItem w = items[it];
+    const BProb& P = probs[w.prob];
+    const int64_t b0 = (int64_t)w.chunk * P.chunk;
+    const int64_t b1 = min(b0 + P.chunk, P.nsblk);
+    const double* sst = (const double*)(stats + (int64_t)ST_WORDS * w.prob);
+    Out o = P.O;
+    if (P.nchunk != 1) {
+      o.part = part + P.part_off + (int64_t)w.chunk * 4 * P.T.n;
+      o.pstride = P.T.n;
+    } else {
+      o.part = nullptr;
+    }
+    gridsrc_tile<K, NT, MIXED>(L, P.T, (int64_t)w.tile * (K * NT), sax + P.sblk_off * 24,
+                               be2 + P.sblk_off, s4 + P.src_off, b0, b1, sst, o);
   }
 }
 

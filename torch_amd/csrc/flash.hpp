@@ -35,7 +35,23 @@ struct Flash {
   int type;                       // 1 spline, 2 linear
   int quirks;                     // potential as the reference writes it
   int nrep;                       // periodic: images n in [-nrep, nrep]^3
+  int s32;                        // rsoft^2 > 1e-30: a tile may seed 1/sqrt in FP32 (r2_bound decides)
+  double reach;                   // periodic: (nrep + 1) max(L), the farthest image offset; else 0
 };
+
+// Largest |coordinate| of the workgroup's targets (NT lanes, K each; t >= n excluded), through
+// `red` (2 NT / 64 doubles of LDS).
+template <int K, int NT>
+__device__ __forceinline__ double tile_rmax(const double (&tx)[K], const double (&ty)[K],
+                                            const double (&tz)[K], int64_t tile0, int64_t n,
+                                            double* red) {
+  double mn[1] = {0.0}, mx[1] = {0.0};
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (tile0 + (int64_t)k * NT + threadIdx.x < n) mx[0] = fmax(mx[0], max_abs3(tx[k], ty[k], tz[k]));
+  block_minmax<NT, 1>(mn, mx, red);
+  return mx[0];
+}
 
 // Kernel values for r < rsoft.  Force: the reference's spline pieces and linear slope;
 // q <= 1e-5, where the reference leaves the force kernel unassigned, takes the r -> 0 limit
@@ -182,27 +198,14 @@ __device__ __forceinline__ void flash_fix(const Flash& P, const double4 s, const
   }
 }
 
-// blockIdx.x = target tile of K*NT targets, blockIdx.y = source chunk.  Accumulators follow
-// store_result's convention (a = G s, phi = -G s_p): s -= m kv d, s_p += m pk.
-template <int K, int NT, bool PER, bool S32, bool QUIRKS = false>
-__global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restrict__ src4,
-                                                   int64_t ns, int64_t chunk, Flash P, Out O) {
-  __shared__ double4 ls[NT];
-  if (gated_off(O)) return;
+// The source loop of k_eval_flash over [sb, se) for one target tile (tx, ty, tz held by the lanes).
+template <int K, int NT, bool PER, bool S32, bool QUIRKS>
+__device__ __forceinline__ void flash_sources(const Flash& P, double4* ls, const double4* __restrict__ src4,
+                                              int64_t sb, int64_t se, const double (&tx)[K],
+                                              const double (&ty)[K], const double (&tz)[K],
+                                              double (&ax)[K], double (&ay)[K], double (&az)[K],
+                                              double (&ap)[K]) {
   const int tid = threadIdx.x;
-  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
-  const int64_t sb = (int64_t)blockIdx.y * chunk;
-  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
-  Out o = O;
-  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  double tx[K], ty[K], tz[K], ax[K], ay[K], az[K], ap[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int64_t t = tile0 + (int64_t)k * NT + tid;
-    double e2;   // the FLASH kernels use the global radius; per-point softening is ignored
-    load_target(T, t < T.n ? t : T.n - 1, tx[k], ty[k], tz[k], e2);
-    ax[k] = ay[k] = az[k] = ap[k] = 0.0;
-  }
   for (int64_t base = sb; base < se; base += NT) {
     __syncthreads();
     if (base + tid < se) ls[tid] = src4[base + tid];
@@ -253,6 +256,38 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
       }
     }
   }
+}
+
+// blockIdx.x = target tile of K*NT targets, blockIdx.y = source chunk.  Accumulators follow
+// store_result's convention (a = G s, phi = -G s_p): s -= m kv d, s_p += m pk.  The FP32 seed of
+// 1/sqrt is taken per tile where the tile's r^2 bound (source statistics sst, images included)
+// keeps r^2 inside the FP32 range.
+template <int K, int NT, bool PER, bool QUIRKS = false>
+__global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restrict__ src4,
+                                                   int64_t ns, int64_t chunk, Flash P,
+                                                   const double* sst, Out O) {
+  __shared__ double4 ls[NT];
+  __shared__ double red[2 * (NT / 64)];
+  if (gated_off(O)) return;
+  const int tid = threadIdx.x;
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t sb = (int64_t)blockIdx.y * chunk;
+  const int64_t se = sb + chunk < ns ? sb + chunk : ns;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  double tx[K], ty[K], tz[K], ax[K], ay[K], az[K], ap[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    double e2;   // the FLASH kernels use the global radius; per-point softening is ignored
+    load_target(T, t < T.n ? t : T.n - 1, tx[k], ty[k], tz[k], e2);
+    ax[k] = ay[k] = az[k] = ap[k] = 0.0;
+  }
+  const double rt = tile_rmax<K, NT>(tx, ty, tz, tile0, T.n, red);
+  if (P.s32 && r2_bound(sst, rt, 0.0, P.reach) < TG_F32_HUGE_R2)   // uniform over the workgroup
+    flash_sources<K, NT, PER, true, QUIRKS>(P, ls, src4, sb, se, tx, ty, tz, ax, ay, az, ap);
+  else
+    flash_sources<K, NT, PER, false, QUIRKS>(P, ls, src4, sb, se, tx, ty, tz, ax, ay, az, ap);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t t = tile0 + (int64_t)k * NT + tid;
@@ -342,12 +377,12 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
 
 // Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
 // blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
-template <int K, int NT, bool S32>
+template <int K, int NT>
 __global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double* __restrict__ sax,
                                                            const double* __restrict__ be2,
                                                            const double4* __restrict__ bm,
                                                            int64_t nblk, int64_t chunk, Flash P,
-                                                           Out O) {
+                                                           const double* sst, Out O) {
   __shared__ LdsGS<false> L;
   if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
   const int tid = threadIdx.x;
@@ -362,11 +397,14 @@ __global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double* 
     load_target(T, t < T.n ? t : T.n - 1, tx[k], ty[k], tz[k], e2);
     ax[k] = ay[k] = az[k] = ap[k] = 0.0;
   }
+  const double rt = tile_rmax<K, NT>(tx, ty, tz, tile0, T.n, L.red);
+  const bool s32 = P.s32 && r2_bound(sst, rt, 0.0) < TG_F32_HUGE_R2;   // uniform over the workgroup
   for (int64_t blk = b0; blk < b1; ++blk) {
     __syncthreads();
     gs_stage<false>(L, sax, be2, bm, blk, 0.0, 0.0, 0.0, 0.0);
     __syncthreads();
-    flash_gs_block<K, S32>(P, L, tx, ty, tz, ax, ay, az, ap);
+    if (s32) flash_gs_block<K, true>(P, L, tx, ty, tz, ax, ay, az, ap);
+    else flash_gs_block<K, false>(P, L, tx, ty, tz, ax, ay, az, ap);
   }
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;

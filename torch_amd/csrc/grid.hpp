@@ -201,14 +201,15 @@ struct GridF32Run {
 // One wave of column targets against sources [sb, se): lane l owns 8 targets t0 + k that share
 // x, y and eps^2 (e2t, uniform over the wave) and differ in z[k]; (cx, cy, cz) is the FP64 anchor
 // of the mixed form (the centre of the wave's bounding box); targets t0 + k with k >= nk are not
-// stored.  `o` already points at this source chunk's partials if the sources are split.
+// stored.  `o` already points at this source chunk's partials if the sources are split.  sst: the
+// source statistics; rt: the largest |coordinate| of the wave's targets (FP64 seed choice).
 template <bool MIXED>
 __device__ __forceinline__ void grid_columns(
     typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, double x, double y,
     const double (&z)[8], double e2t, double cx, double cy, double cz, int64_t t0, int nk,
     const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
-    double e2s_min, const Out& o) {
-  const int fl = tile_flags(e2s_min, e2t, e2t, o, MIXED);
+    const double* sst, double rt, const Out& o) {
+  const int fl = tile_flags(sst[ST_E2MIN], e2t, e2t, o, MIXED, r2_bound(sst, rt, e2t));
   if constexpr (MIXED) {
     const float tx = (float)(x - cx), ty = (float)(y - cy);
     float tz[8];
@@ -242,7 +243,7 @@ template <bool MIXED>
 __device__ __forceinline__ void grid_block(
     typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t b,
     const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
-    double e2s_min, const Out& o) {
+    const double* sst, const Out& o) {
   const int lane = threadIdx.x;
   const int i = lane >> 3, jj = lane & 7;
   const double d = T.bdx[b];
@@ -259,8 +260,11 @@ __device__ __forceinline__ void grid_block(
     cy = 0.5 * ((oy + 0.5 * d) + (oy + 7.5 * d));
     cz = 0.5 * ((oz + 0.5 * d) + (oz + 7.5 * d));
   }
+  // every cell centre lies within [o, o + 8 d] on each axis
+  const double rt = fmax(max_abs3(T.ox[b], T.oy[b], T.oz[b]),
+                         max_abs3(T.ox[b] + 8.0 * d, T.oy[b] + 8.0 * d, T.oz[b] + 8.0 * d));
   grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, b * 512 + (int64_t)lane * 8, 8, src4, srce2,
-                      sb, se, e2s_min, o);
+                      sb, se, sst, rt, o);
 }
 
 // blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
@@ -268,14 +272,14 @@ __device__ __forceinline__ void grid_block(
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restrict__ src4,
                                                   const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+                                                  int64_t chunk, const double* sst, Out O) {
   typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
   __shared__ Lds L;
   const int64_t sb = (int64_t)blockIdx.y * chunk;
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  grid_block<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, *e2s_min_p, o);
+  grid_block<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, sst, o);
 }
 
 // Explicit targets with column structure (every aligned group of 8 consecutive targets shares
@@ -287,7 +291,7 @@ template <bool MIXED>
 __device__ __forceinline__ void cols_group(
     typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t g,
     const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
-    double e2s_min, const Out& o) {
+    const double* sst, const Out& o) {
   const int lane = threadIdx.x;
   const int64_t t0 = g * 512 + (int64_t)lane * 8;
   const int64_t last = T.n - 1;
@@ -318,9 +322,17 @@ __device__ __forceinline__ void cols_group(
     cy = 0.5 * (mn[1] + mx[1]);
     cz = 0.5 * (mn[2] + mx[2]);
   }
+  double rt = 0.0;   // the wave's largest |coordinate| (the FP64 seed choice)
+  if constexpr (!MIXED) {
+    rt = fmax(fabs(x), fabs(y));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) rt = fmax(rt, fabs(z[k]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rt = fmax(rt, __shfl_xor(rt, off));
+  }
   const int64_t left = T.n - t0;
   grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
-                      sb, se, e2s_min, o);
+                      sb, se, sst, rt, o);
 }
 
 // blockIdx.x = group of 512 targets, blockIdx.y = source chunk.  Gated (Out::gate) against
@@ -328,7 +340,7 @@ __device__ __forceinline__ void cols_group(
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restrict__ src4,
                                                   const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+                                                  int64_t chunk, const double* sst, Out O) {
   typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
   __shared__ Lds L;
   if (gated_off(O)) return;
@@ -336,7 +348,7 @@ __global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restri
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  cols_group<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, *e2s_min_p, o);
+  cols_group<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, sst, o);
 }
 
 // ---------------------------------------------------------------- block sources ----
@@ -466,20 +478,18 @@ struct GsRun {
   }
 };
 
-// Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
-// blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
+// One target tile (K per lane, targets tile0 + k NT + lane) against source blocks [b0, b1):
+// sax / be2 / bm are the set's cell-centre axes, block eps^2 and packed cells (mass in .w); o
+// already points at this block chunk's partials if the blocks are split.  Shared by
+// k_eval_gridsrc and the persistent batch kernel.
 template <int K, int NT, bool MIXED>
-__global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
-                                                     const double* __restrict__ be2,
-                                                     const double4* __restrict__ bm, int64_t nblk,
-                                                     int64_t chunk, const double* e2s_min_p, Out O) {
+__device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int64_t tile0,
+                                             const double* __restrict__ sax,
+                                             const double* __restrict__ be2,
+                                             const double4* __restrict__ bm, int64_t b0, int64_t b1,
+                                             const double* sst, const Out& o) {
   typedef typename LdsGS<MIXED>::T TT;
-  __shared__ LdsGS<MIXED> L;
-  if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
   const int tid = threadIdx.x;
-  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
-  const int64_t b0 = (int64_t)blockIdx.y * chunk;
-  const int64_t b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
   double px[K], py[K], pz[K], pe2[K];
   double mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
   double mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
@@ -501,7 +511,8 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __rest
     ay_ = 0.5 * (mn[1] + mx[1]);
     az_ = 0.5 * (mn[2] + mx[2]);
   }
-  const int fl = tile_flags(*e2s_min_p, mn[3], mx[3], O, MIXED);
+  const double rt = fmax(fmax(max_abs3(mn[0], mn[1], mn[2]), max_abs3(mx[0], mx[1], mx[2])), 0.0);
+  const int fl = tile_flags(sst[ST_E2MIN], mn[3], mx[3], o, MIXED, r2_bound(sst, rt, mx[3]));
   TT tx[K], ty[K], tz[K], te2[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -521,8 +532,6 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __rest
     TG_DISPATCH(R::template run, fl, L, sax, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
                 hx, hy, hz, hp)
   }
-  Out o = O;
-  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t t = tile0 + (int64_t)k * NT + tid;
@@ -530,10 +539,28 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __rest
   }
 }
 
+// Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
+// blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
+template <int K, int NT, bool MIXED>
+__global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
+                                                     const double* __restrict__ be2,
+                                                     const double4* __restrict__ bm, int64_t nblk,
+                                                     int64_t chunk, const double* sst, Out O) {
+  __shared__ LdsGS<MIXED> L;
+  if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t b0 = (int64_t)blockIdx.y * chunk;
+  const int64_t b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
+  Out o = O;
+  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
+  gridsrc_tile<K, NT, MIXED>(L, T, tile0, sax, be2, bm, b0, b1, sst, o);
+}
+
 // Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them),
 // writes each block's eps^2 (be2) and cell-centre axes (sax, staged by the block-source
-// kernels) and folds the smallest block eps^2 into stat[0]; one lane per cell.
-__global__ void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox,
+// kernels) and folds the source statistics (ST_*) into stat; one lane per cell, workgroups of
+// 256 lanes.
+__global__ __launch_bounds__(256) void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox,
                                      const double* __restrict__ oy, const double* __restrict__ oz,
                                      const double* __restrict__ dx, const double* __restrict__ m,
                                      const double* __restrict__ eps, double4* __restrict__ s4,
@@ -541,28 +568,32 @@ __global__ void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox
                                      double* __restrict__ sax,
                                      unsigned long long* stat) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nblk * 512) return;
-  const int64_t b = i >> 9;
-  const int c = (int)(i & 511);
-  const double d = dx[b];
-  double x, y, z;
-  {
+  unsigned long long e2min = 0x7ff0000000000000ULL, rmax = 0ULL, e2max = 0ULL, mmax = 0ULL;
+  if (i < nblk * 512) {
+    const int64_t b = i >> 9;
+    const int c = (int)(i & 511);
+    const double d = dx[b];
+    double x, y, z;
+    {
 #pragma clang fp contract(off)
-    x = ox[b] + ((double)(c >> 6) + 0.5) * d;
-    y = oy[b] + ((double)((c >> 3) & 7) + 0.5) * d;
-    z = oz[b] + ((double)(c & 7) + 0.5) * d;
+      x = ox[b] + ((double)(c >> 6) + 0.5) * d;
+      y = oy[b] + ((double)((c >> 3) & 7) + 0.5) * d;
+      z = oz[b] + ((double)(c & 7) + 0.5) * d;
+    }
+    const double e = eps ? eps[b] : 0.0;
+    const double mi = m[i];
+    s4[i] = make_double4(x, y, z, mi);
+    se2[i] = e * e;
+    const int ci = c >> 6, cj = (c >> 3) & 7, ck = c & 7;
+    if (cj == 0 && ck == 0) sax[b * 24 + ci] = x;
+    if (ci == 0 && ck == 0) sax[b * 24 + 8 + cj] = y;
+    if (ci == 0 && cj == 0) sax[b * 24 + 16 + ck] = z;
+    if (c == 0) be2[b] = e * e;
+    e2min = e2max = dbits(e * e);
+    rmax = dbits(max_abs3(x, y, z));
+    mmax = dbits(fabs(mi));
   }
-  const double e = eps ? eps[b] : 0.0;
-  s4[i] = make_double4(x, y, z, m[i]);
-  se2[i] = e * e;
-  const int ci = c >> 6, cj = (c >> 3) & 7, ck = c & 7;
-  if (cj == 0 && ck == 0) sax[b * 24 + ci] = x;
-  if (ci == 0 && ck == 0) sax[b * 24 + 8 + cj] = y;
-  if (ci == 0 && cj == 0) sax[b * 24 + 16 + ck] = z;
-  if (c == 0) {
-    be2[b] = e * e;
-    atomicMin(&stat[0], (unsigned long long)__double_as_longlong(e * e));
-  }
+  fold_source_stats(e2min, 0, rmax, e2max, mmax, stat);
 }
 
 // Explicit sources that are 8^3 leaf blocks in C order (cell c = 64 i + 8 j + k of block b at

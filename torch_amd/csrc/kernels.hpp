@@ -214,6 +214,34 @@ struct LdsF32 {
 // Threshold on the smallest possible eps^2 of a tile below which the FP32 loop guards
 // r2 == 0 (and keeps rsq far from overflow) and the FP64 loop keeps the v_rsq_f64 seed.
 #define TG_F32_TINY_E2 1e-30
+// Bound on a tile's largest r^2 above which the FP64 loop keeps the v_rsq_f64 seed: the FP32
+// seed converts r^2 to float, and FLT_MAX is 3.4e38 (a separation of 1.8e19 in the caller's
+// units; cgs coordinates of a few kpc reach it).
+#define TG_F32_HUGE_R2 1e36
+
+// Statistics of a packed source set, device words written by the pack kernels: doubles stored as
+// their bits (non-negative, so integer atomicMin / atomicMax order them like the values).
+enum {
+  ST_E2MIN = 0,   // smallest eps^2 (read by every kernel)
+  ST_BAD = 1,     // non-zero: a NaN/Inf input (validation)
+  ST_RMAX = 2,    // largest |x|, |y|, |z|
+  ST_E2MAX = 3,   // largest eps^2
+  ST_MMAX = 4,    // largest |m|
+  ST_WORDS = 5
+};
+
+// Upper bound on r^2 = |x_s - x_t|^2 + eps_s^2 + eps_t^2 over the tile's pairs, from the source
+// statistics `sst` and the tile's largest |coordinate| `rt` and eps_t^2; `reach` widens the
+// displacement (periodic images).
+__device__ __forceinline__ double r2_bound(const double* sst, double rt, double e2t_max,
+                                           double reach = 0.0) {
+  const double d = sst[ST_RMAX] + rt + reach;
+  return 3.0 * d * d + sst[ST_E2MAX] + e2t_max;
+}
+
+__device__ __forceinline__ double max_abs3(double x, double y, double z) {
+  return fmax(fabs(x), fmax(fabs(y), fabs(z)));
+}
 
 // v + dt*a with the product and the sum rounded separately (no FMA contraction), as the
 // reference's FP64 kick loops compute it (interface.F90:1793-1795).
@@ -241,9 +269,10 @@ __device__ __forceinline__ void store_result(const Out& O, int64_t t, double sx,
   if (O.vx) kick_one(O, t, gx, gy, gz);
 }
 
-// Code path of a tile from its smallest/largest target eps^2 and the sources' smallest.
+// Code path of a tile from its smallest/largest target eps^2, the sources' smallest and (FP64)
+// a bound on the tile's largest r^2: the FP32 seed needs r^2 inside [1e-30, 1e36].
 __device__ __forceinline__ int tile_flags(double e2s_min, double e2t_min, double e2t_max,
-                                          const Out& O, bool f32) {
+                                          const Out& O, bool f32, double r2max = 0.0) {
   int fl = 0;
   if (e2t_min == e2t_max) fl |= F_UNIF;
   const double lo = e2s_min + e2t_min;
@@ -251,7 +280,7 @@ __device__ __forceinline__ int tile_flags(double e2s_min, double e2t_min, double
     if (!(lo > TG_F32_TINY_E2)) fl |= F_GUARD;
   } else {
     if (!(lo > 0.0)) fl |= F_GUARD;
-    else if (lo > TG_F32_TINY_E2) fl |= F_SEED32;
+    else if (lo > TG_F32_TINY_E2 && r2max < TG_F32_HUGE_R2) fl |= F_SEED32;
   }
   if (O.nopot) fl |= F_NOPOT;
   return fl;
@@ -323,13 +352,14 @@ struct Fp64Run {
   }
 };
 
+// sst: the source statistics (ST_*).
 template <int K, int NT>
 __device__ void process_fp64(LdsF64<NT>& L, const Tgt& T, int64_t tile0,
                              const double4* __restrict__ src4, const double* __restrict__ srce2,
-                             int64_t s_begin, int64_t s_end, double e2s_min, const Out& O) {
+                             int64_t s_begin, int64_t s_end, const double* sst, const Out& O) {
   const int tid = threadIdx.x;
   double tx[K], ty[K], tz[K], te2[K];
-  double mn[1] = {INFINITY}, mx[1] = {-INFINITY};
+  double mn[2] = {INFINITY, INFINITY}, mx[2] = {-INFINITY, 0.0};   // eps^2; |coordinate| (max only)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t t = tile0 + (int64_t)k * NT + tid;
@@ -337,10 +367,11 @@ __device__ void process_fp64(LdsF64<NT>& L, const Tgt& T, int64_t tile0,
     if (t < T.n) {
       mn[0] = fmin(mn[0], te2[k]);
       mx[0] = fmax(mx[0], te2[k]);
+      mx[1] = fmax(mx[1], max_abs3(tx[k], ty[k], tz[k]));
     }
   }
-  block_minmax<NT, 1>(mn, mx, L.red);
-  const int fl = tile_flags(e2s_min, mn[0], mx[0], O, false);
+  block_minmax<NT, 2>(mn, mx, L.red);
+  const int fl = tile_flags(sst[ST_E2MIN], mn[0], mx[0], O, false, r2_bound(sst, mx[1], mx[0]));
   double ax[K], ay[K], az[K], ap[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
@@ -414,7 +445,7 @@ struct MixRun {
 template <int K, int NT>
 __device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
                               const double4* __restrict__ src4, const double* __restrict__ srce2,
-                              int64_t s_begin, int64_t s_end, double e2s_min, const Out& O) {
+                              int64_t s_begin, int64_t s_end, const double* sst, const Out& O) {
   const int tid = threadIdx.x;
   double px[K], py[K], pz[K], pe2[K];
   double mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
@@ -432,7 +463,7 @@ __device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
   }
   block_minmax<NT, 4>(mn, mx, L.red);
   const double cx = 0.5 * (mn[0] + mx[0]), cy = 0.5 * (mn[1] + mx[1]), cz = 0.5 * (mn[2] + mx[2]);
-  const int fl = tile_flags(e2s_min, mn[3], mx[3], O, true);
+  const int fl = tile_flags(sst[ST_E2MIN], mn[3], mx[3], O, true);
   float tx[K], ty[K], tz[K], te2[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -461,31 +492,29 @@ __device__ void process_mixed(LdsF32<NT>& L, const Tgt& T, int64_t tile0,
 template <int K, int NT>
 __global__ __launch_bounds__(NT) void k_eval_fp64(Tgt T, const double4* __restrict__ src4,
                                                   const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* e2s_min_p, Out O) {
+                                                  int64_t chunk, const double* sst, Out O) {
   __shared__ LdsF64<NT> L;
   if (gated_off(O)) return;
-  const double e2s_min = *e2s_min_p;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t sb = (int64_t)blockIdx.y * chunk;
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  process_fp64<K, NT>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
+  process_fp64<K, NT>(L, T, tile0, src4, srce2, sb, se, sst, o);
 }
 
 template <int K, int NT>
 __global__ __launch_bounds__(NT) void k_eval_mixed(Tgt T, const double4* __restrict__ src4,
                                                    const double* __restrict__ srce2, int64_t ns,
-                                                   int64_t chunk, const double* e2s_min_p, Out O) {
+                                                   int64_t chunk, const double* sst, Out O) {
   __shared__ LdsF32<NT> L;
   if (gated_off(O)) return;
-  const double e2s_min = *e2s_min_p;
   const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
   const int64_t sb = (int64_t)blockIdx.y * chunk;
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  process_mixed<K, NT>(L, T, tile0, src4, srce2, sb, se, e2s_min, o);
+  process_mixed<K, NT>(L, T, tile0, src4, srce2, sb, se, sst, o);
 }
 
 // Sum of the nsplit chunk partials of each target in chunk order, then the G scaling,
@@ -550,19 +579,58 @@ __global__ __launch_bounds__(1024) void k_reduce_chunks_wide(const double* __res
   }
 }
 
-// Pack SoA FP64 sources into {x,y,z,m} + eps^2 and fold the smallest eps^2 into
-// stat[0] (as the bits of a non-negative double, whose unsigned-integer order is the
-// numeric order, so one vector atomicMin per workgroup suffices).  With `check`,
-// stat[1] becomes non-zero if any input is NaN/Inf.  stat must be reset first
-// (k_reset_stats).
-__global__ void k_pack_sources(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
-                               const double* __restrict__ z, const double* __restrict__ m,
-                               const double* __restrict__ eps, double4* __restrict__ s4,
-                               double* __restrict__ se2, unsigned long long* stat, int check) {
-  __shared__ unsigned long long smin[256];
+// Bits of a non-negative double: their unsigned-integer order is the numeric order.
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+
+// Folds one lane's source statistics (ST_* words) into `stat`: reduced over the workgroup of
+// 256 lanes through LDS, then one atomic per word.  Every lane of the workgroup must call it.
+__device__ __forceinline__ void fold_source_stats(unsigned long long e2min, int bad, unsigned long long rmax,
+                                                  unsigned long long e2max, unsigned long long mmax,
+                                                  unsigned long long* stat) {
+  __shared__ unsigned long long sm[4][256];
   __shared__ int sbad[256];
+  const int t = threadIdx.x;
+  sm[0][t] = e2min;
+  sm[1][t] = rmax;
+  sm[2][t] = e2max;
+  sm[3][t] = mmax;
+  sbad[t] = bad;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      sm[0][t] = min(sm[0][t], sm[0][t + s]);
+      sm[1][t] = max(sm[1][t], sm[1][t + s]);
+      sm[2][t] = max(sm[2][t], sm[2][t + s]);
+      sm[3][t] = max(sm[3][t], sm[3][t + s]);
+      sbad[t] |= sbad[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    atomicMin(&stat[ST_E2MIN], sm[0][0]);
+    atomicMax(&stat[ST_RMAX], sm[1][0]);
+    atomicMax(&stat[ST_E2MAX], sm[2][0]);
+    atomicMax(&stat[ST_MMAX], sm[3][0]);
+    if (sbad[0]) atomicOr(&stat[ST_BAD], 1ULL);
+  }
+}
+
+// Pack SoA FP64 sources into {x,y,z,m} + eps^2 and fold their statistics into stat (ST_*:
+// smallest and largest eps^2, largest |coordinate| and |m|).  With `check`, stat[ST_BAD]
+// becomes non-zero if any input is NaN/Inf.  stat must be reset first (k_reset_stats).
+// Workgroups of 256 lanes.
+__global__ __launch_bounds__(256) void k_pack_sources(int64_t n, const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ m,
+                                                      const double* __restrict__ eps,
+                                                      double4* __restrict__ s4,
+                                                      double* __restrict__ se2,
+                                                      unsigned long long* stat, int check) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long mine = 0x7ff0000000000000ULL;   // +inf
+  unsigned long long e2min = 0x7ff0000000000000ULL, rmax = 0ULL, e2max = 0ULL, mmax = 0ULL;
   int bad = 0;
   if (i < n) {
     const double4 v = make_double4(x[i], y[i], z[i], m[i]);
@@ -570,30 +638,45 @@ __global__ void k_pack_sources(int64_t n, const double* __restrict__ x, const do
     s4[i] = v;
     const double e2 = e * e;
     se2[i] = e2;
-    mine = (unsigned long long)__double_as_longlong(e2);
+    e2min = e2max = dbits(e2);
+    rmax = dbits(max_abs3(v.x, v.y, v.z));
+    mmax = dbits(fabs(v.w));
     if (check)
       bad = !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w) && isfinite(e));
   }
-  smin[threadIdx.x] = mine;
-  sbad[threadIdx.x] = bad;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
-      sbad[threadIdx.x] |= sbad[threadIdx.x + s];
-    }
-    __syncthreads();
+  fold_source_stats(e2min, bad, rmax, e2max, mmax, stat);
+}
+
+// Range statistics (ST_* words: largest |coordinate|, smallest / largest eps^2, largest |m|) of a
+// point set (x, y, z, eps per point; dx == null) or of a set of 8^3 blocks (x, y, z the lower
+// corners and dx the cell widths, so the centres lie within [x, x + 8 dx]; eps per block), with nm
+// masses m (may be null).  For the mixed mode's range check.  out is reset first (k_reset_stats);
+// workgroups of 256 lanes over max(n, nm).
+__global__ __launch_bounds__(256) void k_range(int64_t n, const double* __restrict__ x,
+                                               const double* __restrict__ y, const double* __restrict__ z,
+                                               const double* __restrict__ dx, const double* __restrict__ e,
+                                               int64_t nm, const double* __restrict__ m,
+                                               unsigned long long* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long e2min = 0x7ff0000000000000ULL, rmax = 0ULL, e2max = 0ULL, mmax = 0ULL;
+  if (i < n) {
+    double r = max_abs3(x[i], y[i], z[i]);
+    if (dx) r = fmax(r, max_abs3(x[i] + 8.0 * dx[i], y[i] + 8.0 * dx[i], z[i] + 8.0 * dx[i]));
+    const double ei = e ? e[i] : 0.0;
+    rmax = dbits(r);
+    e2min = e2max = dbits(ei * ei);
   }
-  if (threadIdx.x == 0) {
-    atomicMin(&stat[0], smin[0]);
-    if (sbad[0]) atomicOr(&stat[1], 1ULL);
-  }
+  if (m && i < nm) mmax = dbits(fabs(m[i]));
+  fold_source_stats(e2min, 0, rmax, e2max, mmax, out);
 }
 
 __global__ void k_reset_stats(unsigned long long* stat) {
   if (threadIdx.x == 0) {
-    stat[0] = 0x7ff0000000000000ULL;
-    stat[1] = 0ULL;
+    stat[ST_E2MIN] = 0x7ff0000000000000ULL;
+    stat[ST_BAD] = 0ULL;
+    stat[ST_RMAX] = 0ULL;
+    stat[ST_E2MAX] = 0ULL;
+    stat[ST_MMAX] = 0ULL;
   }
 }
 

@@ -86,14 +86,15 @@ struct DevBuf {
 typedef void (*launch_fn)(dim3, dim3, hipStream_t, const tg::Tgt&, const double4*, const double*,
                           int64_t, int64_t, const double*, const tg::Out&);
 typedef void (*batch_fn)(int, hipStream_t, const tg::BProb*, const tg::BItem*, int, unsigned int*,
-                         const double4*, const double*, const unsigned long long*, double*);
+                         const double4*, const double*, const unsigned long long*, const int*,
+                         double*);
 typedef const void* (*kernel_ptr_fn)();
 
 template <int K, int NT, bool MIXED>
 void launch_batch(int grid, hipStream_t s, const tg::BProb* p, const tg::BItem* it, int n,
                   unsigned int* ctr, const double4* s4, const double* se2,
-                  const unsigned long long* st, double* part) {
-  tg::k_batch_eval<K, NT, MIXED><<<grid, NT, 0, s>>>(p, it, n, ctr, s4, se2, st, part);
+                  const unsigned long long* st, const int* colflag, double* part) {
+  tg::k_batch_eval<K, NT, MIXED><<<grid, NT, 0, s>>>(p, it, n, ctr, s4, se2, st, colflag, part);
 }
 template <int K, int NT, bool MIXED>
 const void* batch_kernel() {
@@ -294,7 +295,7 @@ struct tgrav_ctx {
   int nrep = 6;
   double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
   int batch_wg_per_cu = 0;   // persistent workgroups per CU (0 = occupancy limit)
-  // packed sources; stat[0] = bits of min eps_s^2 (read by the kernels), stat[1] = NaN flag
+  // packed sources; stat: their statistics (tg::ST_*, read by the kernels; ST_BAD = NaN flag)
   int64_t ns = -1;
   DevBuf s4, se2, stat;
   // sources given as 8^3 blocks: descriptors {ox, oy, oz, dx} and eps^2 per block
@@ -307,7 +308,8 @@ struct tgrav_ctx {
   DevBuf colflag;            // explicit targets: 1 if they have 8^3 column structure
   DevBuf bp4, bpe2, bstat, bppart, bpgrp;   // fused passes: packed particles, partials
   DevBuf pm[16], pm_acc[3], pm_stat;  // particle-mesh pieces: staging, fixed-point sums, stat
-  unsigned long long* host_flag = nullptr;   // pinned
+  DevBuf rstat;              // range statistics of the mixed mode's check (2 x tg::ST_WORDS)
+  unsigned long long* host_flag = nullptr;   // pinned, kHostWords words
   // instrumentation: HIP events around each evaluation kernel (tgrav_set_timing)
   int timing = 0;
   std::vector<hipEvent_t> ev;                // 2 per slot
@@ -336,6 +338,7 @@ static int tuned_block_split(const tgrav_ctx* c, int mode, const char* dir, int6
 }
 
 static const int kTimingSlots = 256;
+static const int kHostWords = 16;   // pinned host words: flags and two sets of range statistics
 
 template <class F>
 static int timed_launch_on(tgrav_ctx* c, hipStream_t s, int64_t nt, int64_t ns, F launch) {
@@ -425,6 +428,69 @@ static int read_flag(tgrav_ctx* c, const void* dflag, bool* bad) {
                           c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   *bad = *c->host_flag != 0ULL;
+  return 0;
+}
+
+// ---- range of the mixed mode (only with validation on) ----
+// The mixed kernels hold displacements, r^2 and m / r^3 in FP32.  With S and T the statistics of
+// the two point sets (tg::ST_*), every r^2 is below 3 (R_S + R_T)^2 + eps_S^2 + eps_T^2 and every
+// weight m y^3 below M / lo^(3/2), lo = the smallest eps_S^2 + eps_T^2 (0: unsoftened sets, whose
+// pairs may come arbitrarily close whatever the precision, are not bounded here).
+#define TG_MIXED_R2_MAX 1e36
+#define TG_MIXED_W_MAX 1e30
+
+static int mixed_range_ok(const double* S, const double* T, const char* what) {
+  const double d = S[tg::ST_RMAX] + T[tg::ST_RMAX];
+  const double r2 = 3.0 * d * d + S[tg::ST_E2MAX] + T[tg::ST_E2MAX];
+  if (!(r2 <= TG_MIXED_R2_MAX))
+    return fail(TGRAV_ELIMIT, "mixed precision: %s span |coordinates| up to %.3g, so r^2 may reach %.3g, "
+                "beyond the FP32 range of the mixed kernels (%.0e); use TGRAV_PLAIN_FP64 or larger units",
+                what, std::max(S[tg::ST_RMAX], T[tg::ST_RMAX]), r2, TG_MIXED_R2_MAX);
+  const double lo = S[tg::ST_E2MIN] + T[tg::ST_E2MIN];
+  if (lo > 0.0 && !(S[tg::ST_MMAX] <= TG_MIXED_W_MAX * lo * sqrt(lo)))
+    return fail(TGRAV_ELIMIT, "mixed precision: %s have |m| up to %.3g with softening^2 down to %.3g, so "
+                "m / r^3 may exceed the FP32 range of the mixed kernels; use TGRAV_PLAIN_FP64 or other units",
+                what, S[tg::ST_MMAX], lo);
+  return 0;
+}
+
+// Statistics (tg::ST_*) of a device point set (dx == null) or block set (x, y, z corners, dx cell
+// widths) with nm masses, into out[slot * ST_WORDS ...] of the pinned host words (synchronous).
+static int range_stats(tgrav_ctx* c, int slot, int64_t n, const double* x, const double* y,
+                       const double* z, const double* dx, const double* e, int64_t nm,
+                       const double* m) {
+  int rc = c->rstat.ensure(2 * tg::ST_WORDS * sizeof(unsigned long long));
+  if (rc) return rc;
+  unsigned long long* d = c->rstat.as<unsigned long long>() + slot * tg::ST_WORDS;
+  tg::k_reset_stats<<<1, 64, 0, c->stream>>>(d);
+  LAUNCHCHECK("k_reset_stats");
+  if (std::max(n, nm) > 0) {
+    tg::k_range<<<grid1(std::max(n, nm), 256), 256, 0, c->stream>>>(n, x, y, z, dx, e, nm, m, d);
+    LAUNCHCHECK("k_range");
+  }
+  HIPCHECK(hipMemcpyAsync(c->host_flag + slot * tg::ST_WORDS, d, tg::ST_WORDS * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static const double* host_stats(const tgrav_ctx* c, int slot) {
+  return reinterpret_cast<const double*>(c->host_flag + slot * tg::ST_WORDS);
+}
+
+// Mixed-mode range check of an evaluation: the current source set (its device statistics) against
+// targets given as points or blocks (device pointers).
+static int eval_range_check(tgrav_ctx* c, const tg::Tgt& T, int64_t nin) {
+  int rc;
+  if (T.blocks) rc = range_stats(c, 1, nin, T.ox, T.oy, T.oz, T.bdx, T.beps, 0, nullptr);
+  else rc = range_stats(c, 1, nin, T.x, T.y, T.z, nullptr, T.eps, 0, nullptr);
+  if (rc) return rc;
+  if (c->ns > 0) {
+    HIPCHECK(hipMemcpyAsync(c->host_flag, c->stat.p, tg::ST_WORDS * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if ((rc = mixed_range_ok(host_stats(c, 0), host_stats(c, 1), "the sources and targets"))) return rc;
+  }
   return 0;
 }
 
@@ -632,15 +698,12 @@ static void flash_gridsrc_shape(const tgrav_ctx* c, int64_t nt, int64_t nblk, in
 }
 
 static int flash_gridsrc_kernel(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& o, const tg::Flash& P,
-                                bool s32, int64_t ntiles, int nsplit, int64_t chunk, int64_t nblk) {
+                                int64_t ntiles, int nsplit, int64_t chunk, int64_t nblk) {
   const int K = TG_FLASH_GS_K, NT = 256;
   const dim3 g((unsigned)ntiles, (unsigned)nsplit);
-  if (s32)
-    tg::k_eval_flash_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
-        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
-  else
-    tg::k_eval_flash_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
-        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P, o);
+  tg::k_eval_flash_gridsrc<K, NT><<<g, NT, 0, c->stream>>>(
+      T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P,
+      c->stat.as<const double>(), o);
   LAUNCHCHECK("k_eval_flash_gridsrc");
   return 0;
 }
@@ -672,8 +735,9 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   P.quirks = c->soft_quirks;
   P.nrep = per ? c->nrep : 0;
   // Pairs reaching the refined 1/sqrt have r2 >= rsoft^2 (the rest take the softened
-  // branch); the FP32 seed is valid from 1e-30 up.
-  const bool s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
+  // branch); the FP32 seed is valid from 1e-30 up, and up to the tile's r^2 bound (in the kernels).
+  P.s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
+  P.reach = per ? (double)(c->nrep + 1) * std::max(P.lx, std::max(P.ly, P.lz)) : 0.0;
   // Sources given as 8^3 blocks take the block-source kernel.  Explicit sources that may be
   // C-ordered 8^3 blocks (flagged on the device when they were set) launch it too, gated against
   // the generic kernel on that flag, as in the Plummer path.
@@ -718,24 +782,20 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   }
   int rc = timed_launch(c, nt, ns, [&]() -> int {
     if (srcb || cand) {
-      int r = flash_gridsrc_kernel(c, T, os, P, s32, ntiles_s, nsplit_s, chunk_s, nblk_s);
+      int r = flash_gridsrc_kernel(c, T, os, P, ntiles_s, nsplit_s, chunk_s, nblk_s);
       if (r) return r;
     }
     if (srcb) return 0;
     const dim3 g((unsigned)ntiles, (unsigned)nsplit);
     const double4* s4 = c->s4.as<double4>();
+    const double* sst = c->stat.as<const double>();
     if (per) {
-      if (s32 && P.quirks)
-        tg::k_eval_flash<1, 256, true, true, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
-      else if (s32)
-        tg::k_eval_flash<1, 256, true, true, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
-      else if (P.quirks)
-        tg::k_eval_flash<1, 256, true, false, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      if (P.quirks)
+        tg::k_eval_flash<1, 256, true, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, sst, o);
       else
-        tg::k_eval_flash<1, 256, true, false, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+        tg::k_eval_flash<1, 256, true, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, sst, o);
     } else {
-      if (s32) tg::k_eval_flash<2, 256, false, true><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
-      else tg::k_eval_flash<2, 256, false, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, o);
+      tg::k_eval_flash<2, 256, false><<<g, NT, 0, c->stream>>>(T, s4, ns, chunk, P, sst, o);
     }
     LAUNCHCHECK("k_eval_flash");
     return 0;
@@ -948,7 +1008,7 @@ int tgrav_create(tgrav_ctx** out, int device, const char* tune_cache_path) {
     e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
   }
   if (e == hipSuccess)
-    e = hipHostMalloc((void**)&c->host_flag, sizeof(unsigned long long), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->host_flag, kHostWords * sizeof(unsigned long long), hipHostMallocDefault);
   if (e != hipSuccess) {
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -971,6 +1031,7 @@ int tgrav_destroy(tgrav_ctx* c) {
   for (auto& b : c->vel) b.release();
   c->part.release();
   c->stat.release();
+  c->rstat.release();
   c->flag.release();
   c->colflag.release();
   c->sbe2.release();
@@ -1073,7 +1134,7 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
   }
   if ((rc = c->s4.ensure((size_t)n * sizeof(double4)))) return rc;
   if ((rc = c->se2.ensure((size_t)n * sizeof(double)))) return rc;
-  if ((rc = c->stat.ensure(2 * sizeof(unsigned long long)))) return rc;
+  if ((rc = c->stat.ensure(tg::ST_WORDS * sizeof(unsigned long long)))) return rc;
   const double* a[5] = {x, y, z, m, eps};
   if (kind == TGRAV_HOST) {
     if (c->validate)
@@ -1102,7 +1163,7 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
   if (kind == TGRAV_DEVICE) {
     if (c->validate) {
       bool bad = false;
-      if ((rc = read_flag(c, c->stat.as<unsigned long long>() + 1, &bad))) return rc;
+      if ((rc = read_flag(c, c->stat.as<unsigned long long>() + tg::ST_BAD, &bad))) return rc;
       if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in a source array");
     }
   } else {
@@ -1163,7 +1224,7 @@ int tgrav_set_source_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox
     if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in a source block array");
   }
   if ((rc = c->s4.ensure((size_t)n * sizeof(double4))) || (rc = c->se2.ensure((size_t)n * 8)) ||
-      (rc = c->stat.ensure(2 * sizeof(unsigned long long))) ||
+      (rc = c->stat.ensure(tg::ST_WORDS * sizeof(unsigned long long))) ||
       (rc = c->sbe2.ensure((size_t)nblk * 8)) ||
       (rc = c->sax.ensure((size_t)nblk * 24 * sizeof(double))))
     return rc;
@@ -1193,6 +1254,7 @@ static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[5], int ntin, 
           return fail(TGRAV_ENONFINITE, "non-finite value in target array %d", q);
     if ((rc = stage_in(c, 0, nin, tin, ntin))) return rc;
     bind_targets(T, tin, ntin);
+    if (mode == TGRAV_MIXED && c->validate && (rc = eval_range_check(c, T, nin))) return rc;
     double* dout[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int q = 0; q < 4; ++q)
       if (outs[q]) {
@@ -1224,6 +1286,7 @@ static int eval_common(tgrav_ctx* c, tg::Tgt T, const double* tin[5], int ntin, 
     if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in a target array");
   }
   bind_targets(T, tin, ntin);
+  if (mode == TGRAV_MIXED && c->validate && (rc = eval_range_check(c, T, nin))) return rc;
   tg::Out O{outs[0], outs[1], outs[2], outs[3], vel[0], vel[1], vel[2], dt, nullptr, 0, c->G,
             outs[3] == nullptr};
   return launch_eval(c, mode, T, O);
@@ -1349,7 +1412,7 @@ static int bidir_pack_particles(tgrav_ctx* c, int64_t np, const double* const* p
   int rc;
   if ((rc = c->bp4.ensure((size_t)np * sizeof(double4))) ||
       (rc = c->bpe2.ensure((size_t)np * sizeof(double))) ||
-      (rc = c->bstat.ensure(2 * sizeof(unsigned long long))))
+      (rc = c->bstat.ensure(tg::ST_WORDS * sizeof(unsigned long long))))
     return rc;
   tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->bstat.as<unsigned long long>());
   LAUNCHCHECK("k_reset_stats");
@@ -1461,6 +1524,13 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     if ((rc = device_check_finite(c, nc, ca, 5, &bad))) return rc;
     if (!bad && (rc = device_check_finite(c, np, pa, 5, &bad))) return rc;
     if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in an input array");
+  }
+  if (mode == TGRAV_MIXED && c->validate && nc > 0 && np > 0) {
+    if ((rc = range_stats(c, 0, nc, ca[0], ca[1], ca[2], nullptr, ca[4], nc, ca[3])) ||
+        (rc = range_stats(c, 1, np, pa[0], pa[1], pa[2], nullptr, pa[4], np, pa[3])) ||
+        (rc = mixed_range_ok(host_stats(c, 0), host_stats(c, 1), "the cells and particles")) ||
+        (rc = mixed_range_ok(host_stats(c, 1), host_stats(c, 0), "the particles and cells")))
+      return rc;
   }
   double* dco[4];
   double* dpo[4];
@@ -1609,6 +1679,13 @@ int tgrav_bidirectional_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double*
     if (!bad && (rc = device_check_finite(c, nblk, be, 1, &bad))) return rc;
     if (!bad && (rc = device_check_finite(c, np, pa, 5, &bad))) return rc;
     if (bad) return fail(TGRAV_ENONFINITE, "non-finite value in an input array");
+  }
+  if (mode == TGRAV_MIXED && c->validate && nc > 0 && np > 0) {
+    if ((rc = range_stats(c, 0, nblk, ba[0], ba[1], ba[2], ba[3], be[0], nc, mc[0])) ||
+        (rc = range_stats(c, 1, np, pa[0], pa[1], pa[2], nullptr, pa[4], np, pa[3])) ||
+        (rc = mixed_range_ok(host_stats(c, 0), host_stats(c, 1), "the cells and particles")) ||
+        (rc = mixed_range_ok(host_stats(c, 1), host_stats(c, 0), "the particles and cells")))
+      return rc;
   }
   double* dco[4];
   double* dpo[4];
@@ -1940,11 +2017,14 @@ int tgrav_set_batch_config(tgrav_ctx* c, double item_interactions, int wg_per_cu
 struct tgrav_batch {
   tgrav_ctx* c = nullptr;
   int mode = 0, nprob = 0, n_items = 0, nred = 0, grid = 0;
-  int n_gitems = 0, ggrid = 0;   // block-target work list (column kernel)
+  int n_gitems = 0, ggrid = 0;   // column work list (block targets, explicit targets in C order)
+  int n_sitems = 0, sgrid = 0;   // block-source work list (k_eval_gridsrc's column walk)
+  int ncand = 0;                 // problems whose column structure is tested at every run
   const Variant* v = nullptr;
-  int64_t total_src = 0, red_total = 0;
+  int64_t total_src = 0, red_total = 0, cand_total = 0, total_sblk = 0;
   double inter = 0.0;
-  DevBuf probs, items, gitems, srcs, sprefix, s4, se2, stats, part, counter, ids, rprefix;
+  DevBuf probs, items, gitems, sitems, srcs, sprefix, s4, se2, sax, be2, stats, part, counter, ids,
+      rprefix, cand, cprefix, colflag;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1955,11 +2035,16 @@ static void batch_free(tgrav_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
-  DevBuf* bufs[] = {&b->probs, &b->items, &b->gitems, &b->srcs, &b->sprefix, &b->s4, &b->se2,
-                    &b->stats, &b->part, &b->counter, &b->ids, &b->rprefix};
+  DevBuf* bufs[] = {&b->probs, &b->items, &b->gitems, &b->sitems, &b->srcs, &b->sprefix, &b->s4,
+                    &b->se2, &b->sax, &b->be2, &b->stats, &b->part, &b->counter, &b->ids,
+                    &b->rprefix, &b->cand, &b->cprefix, &b->colflag};
   for (DevBuf* d : bufs) d->release();
   delete b;
 }
+
+// Block-source work items: K targets per lane, NT lanes (the eager kernel's shape).
+#define TG_BATCH_GS_K 4
+#define TG_BATCH_GS_NT 256
 
 template <class T>
 static int upload(DevBuf& d, const std::vector<T>& v) {
@@ -1969,22 +2054,27 @@ static int upload(DevBuf& d, const std::vector<T>& v) {
   return 0;
 }
 
-// Enqueues reset -> pack -> persistent eval -> chunk reduction on `s`.
+// Enqueues reset -> pack -> column check -> persistent evals -> chunk reduction on `s`.
 static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   tg::k_batch_reset<<<grid1(std::max(b->nprob, 1), 256), 256, 0, s>>>(
-      b->nprob, b->counter.as<unsigned int>(), b->stats.as<unsigned long long>());
+      b->nprob, b->counter.as<unsigned int>(), b->stats.as<unsigned long long>(), b->colflag.as<int>());
   LAUNCHCHECK("k_batch_reset");
   if (b->total_src > 0) {
     tg::k_batch_pack<<<grid1(b->total_src, 256), 256, 0, s>>>(
         b->nprob, b->srcs.as<tg::BSrc>(), b->sprefix.as<int64_t>(), b->s4.as<double4>(),
-        b->se2.as<double>(), b->stats.as<unsigned long long>());
+        b->se2.as<double>(), b->sax.as<double>(), b->be2.as<double>(), b->stats.as<unsigned long long>());
     LAUNCHCHECK("k_batch_pack");
+  }
+  if (b->ncand > 0 && b->cand_total > 0) {
+    tg::k_batch_check_cols<<<grid1(b->cand_total, 256), 256, 0, s>>>(
+        b->ncand, b->cand.as<int>(), b->cprefix.as<int64_t>(), b->probs.as<tg::BProb>(), b->colflag.as<int>());
+    LAUNCHCHECK("k_batch_check_cols");
   }
   if (b->n_items > 0) {
     auto go = [&]() -> int {
       b->v->bfn(b->grid, s, b->probs.as<tg::BProb>(), b->items.as<tg::BItem>(), b->n_items,
                 b->counter.as<unsigned int>(), b->s4.as<double4>(), b->se2.as<double>(),
-                b->stats.as<unsigned long long>(), b->part.as<double>());
+                b->stats.as<unsigned long long>(), b->colflag.as<int>(), b->part.as<double>());
       LAUNCHCHECK("k_batch_eval");
       return 0;
     };
@@ -1997,16 +2087,35 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
         tg::k_batch_eval_grid<true><<<b->ggrid, 64, 0, s>>>(
             b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems,
             b->counter.as<unsigned int>() + 1, b->s4.as<double4>(), b->se2.as<double>(),
-            b->stats.as<unsigned long long>(), b->part.as<double>());
+            b->stats.as<unsigned long long>(), b->colflag.as<int>(), b->part.as<double>());
       else
         tg::k_batch_eval_grid<false><<<b->ggrid, 64, 0, s>>>(
             b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems,
             b->counter.as<unsigned int>() + 1, b->s4.as<double4>(), b->se2.as<double>(),
-            b->stats.as<unsigned long long>(), b->part.as<double>());
+            b->stats.as<unsigned long long>(), b->colflag.as<int>(), b->part.as<double>());
       LAUNCHCHECK("k_batch_eval_grid");
       return 0;
     };
     int rc = timed ? timed_launch_on(b->c, s, b->n_gitems, b->total_src, go) : go();
+    if (rc) return rc;
+  }
+  if (b->n_sitems > 0) {
+    auto go = [&]() -> int {
+      const int K = TG_BATCH_GS_K, NT = TG_BATCH_GS_NT;
+      if (b->mode == TGRAV_MIXED)
+        tg::k_batch_eval_gridsrc<K, NT, true><<<b->sgrid, NT, 0, s>>>(
+            b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems,
+            b->counter.as<unsigned int>() + 2, b->s4.as<double4>(), b->sax.as<double>(),
+            b->be2.as<double>(), b->stats.as<unsigned long long>(), b->part.as<double>());
+      else
+        tg::k_batch_eval_gridsrc<K, NT, false><<<b->sgrid, NT, 0, s>>>(
+            b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems,
+            b->counter.as<unsigned int>() + 2, b->s4.as<double4>(), b->sax.as<double>(),
+            b->be2.as<double>(), b->stats.as<unsigned long long>(), b->part.as<double>());
+      LAUNCHCHECK("k_batch_eval_gridsrc");
+      return 0;
+    };
+    int rc = timed ? timed_launch_on(b->c, s, b->n_sitems, b->total_src, go) : go();
     if (rc) return rc;
   }
   if (b->nred > 0 && b->red_total > 0) {
@@ -2035,8 +2144,16 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const tgrav_problem& q = pr[p];
     if (q.n_sources < 0 || q.n_targets < 0)
       return fail(TGRAV_EINVAL, "problem %d: negative size", p);
-    if (q.n_sources > 0 && (!q.sx || !q.sy || !q.sz || !q.sm))
+    if (q.n_source_blocks < 0) return fail(TGRAV_EINVAL, "problem %d: negative source block count", p);
+    if (q.n_source_blocks > 0) {
+      if (q.n_sources != 512 * q.n_source_blocks)
+        return fail(TGRAV_EINVAL, "problem %d: n_sources must be 512 * n_source_blocks", p);
+      if (!q.sbx || !q.sby || !q.sbz || !q.sbd || !q.sm)
+        return fail(TGRAV_EINVAL, "problem %d: source block corners, widths and cell masses are required", p);
+      if (q.n_source_blocks > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "problem %d: too many source blocks", p);
+    } else if (q.n_sources > 0 && (!q.sx || !q.sy || !q.sz || !q.sm)) {
       return fail(TGRAV_EINVAL, "problem %d: source x, y, z and m are required", p);
+    }
     if (q.n_target_blocks < 0) return fail(TGRAV_EINVAL, "problem %d: negative block count", p);
     if (q.n_target_blocks > 0) {
       if (q.n_targets != 512 * q.n_target_blocks)
@@ -2051,32 +2168,6 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       return fail(TGRAV_EINVAL, "problem %d: velocities are required when dt != 0", p);
   }
   if ((rc = set_device(c))) return rc;
-  // Explicit targets in the C order of 8^3 leaf-block meshgrids (each aligned 8 share x and y,
-  // each aligned 512 share eps) join the column work list, as tgrav_eval routes them to
-  // k_eval_cols.  Tested once here, on the device; plan creation may wait for the answer.
-  std::vector<int> colmask(n, 0);
-  if (c->block_kernel) {
-    std::vector<int> cand;
-    for (int p = 0; p < n; ++p)
-      if (pr[p].n_target_blocks == 0 && pr[p].n_targets >= 512 && pr[p].n_targets % 512 == 0) cand.push_back(p);
-    if (!cand.empty()) {
-      if ((rc = c->flag.ensure(cand.size() * sizeof(int)))) return rc;
-      int* fl = c->flag.as<int>();
-      for (size_t i = 0; i < cand.size(); ++i) {
-        const tgrav_problem& q = pr[cand[i]];
-        tg::Tgt T{};
-        T.x = q.tx; T.y = q.ty; T.z = q.tz; T.eps = q.te; T.n = q.n_targets;
-        tg::k_set_flag<<<1, 64, 0, c->stream>>>(fl + i, 1);
-        LAUNCHCHECK("k_set_flag");
-        tg::k_check_cols<<<grid1(q.n_targets / 512 * 64, 256), 256, 0, c->stream>>>(T, fl + i);
-        LAUNCHCHECK("k_check_cols");
-      }
-      std::vector<int> got(cand.size(), 0);
-      HIPCHECK(hipMemcpyAsync(got.data(), fl, cand.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      HIPCHECK(hipStreamSynchronize(c->stream));
-      for (size_t i = 0; i < cand.size(); ++i) colmask[cand[i]] = got[i] == 1;
-    }
-  }
   tgrav_batch* b = new tgrav_batch();
   b->c = c;
   b->mode = mode;
@@ -2087,6 +2178,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   b->inter = total;
   double item = c->batch_item;
   int wg_per_cu = c->batch_wg_per_cu;
+  bool tuned_item = c->user_batch && item > 0;
   if (!c->tune.empty() && n > 0) {   // the autotuner's shape for batches like this one
     std::vector<double> vs(n), vt(n);
     for (int p = 0; p < n; ++p) { vs[p] = (double)pr[p].n_sources; vt[p] = (double)pr[p].n_targets; }
@@ -2104,24 +2196,59 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       if (!c->user_batch) {
         if (t->item >= 0) item = t->item;
         if (t->wg_per_cu >= 0) wg_per_cu = t->wg_per_cu;
+        tuned_item = item > 0;
       }
     }
   }
+  // Resident workgroups of each persistent kernel.
+  auto occupancy = [&](const void* k, int threads) {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, threads, 0) != hipSuccess || o < 1) o = 1;
+    const int per_cu = wg_per_cu > 0 ? std::min(wg_per_cu, o) : o;
+    return per_cu * c->num_cu;
+  };
+  const int GK = TG_BATCH_GS_K, GNT = TG_BATCH_GS_NT;
+  const int slots = occupancy(b->v->bkernel(), b->v->nt);
+  const int gslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_grid<true>
+                                                   : (const void*)tg::k_batch_eval_grid<false>, 64);
+  const int sslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_gridsrc<GK, GNT, true>
+                                                   : (const void*)tg::k_batch_eval_gridsrc<GK, GNT, false>, GNT);
+  // Form of each problem's work items.
+  std::vector<int> form(n, tg::FORM_GENERIC);
+  double tot_list[3] = {0.0, 0.0, 0.0};   // interactions on the generic, column, block-source lists
+  for (int p = 0; p < n; ++p) {
+    const tgrav_problem& q = pr[p];
+    if (q.n_source_blocks > 0 && c->block_kernel) form[p] = tg::FORM_GRIDSRC;
+    else if (q.n_target_blocks > 0) form[p] = tg::FORM_COLS;
+    else if (c->block_kernel && q.n_targets >= 512 && q.n_targets % 512 == 0) form[p] = tg::FORM_CAND;
+    const double w = (double)q.n_sources * (double)q.n_targets;
+    tot_list[form[p] == tg::FORM_GRIDSRC ? 2 : form[p] == tg::FORM_GENERIC ? 0 : 1] += w;
+  }
   const int64_t T = (int64_t)b->v->k * b->v->nt, NT = b->v->nt;
-  double C = item > 0 ? item : std::max((double)T * NT * 4.0, total / (c->num_cu * 8.0));
+  const double C = item > 0 ? item : std::max((double)T * NT * 4.0, total / (c->num_cu * 8.0));
+  // Without a tuned or set item size, each list is cut into about 8 rounds of its resident
+  // workgroups at least (equal-cost items balance the last round), down to a floor of 4 source
+  // tiles (4 blocks) per item.
+  auto list_cost = [&](int li, double floor_cost) {
+    if (tuned_item) return C;
+    const int sl = li == 0 ? slots : li == 1 ? gslots : sslots;
+    return std::min(C, std::max(floor_cost, tot_list[li] / (8.0 * sl)));
+  };
+  const double C_gen = list_cost(0, (double)T * NT * 4.0), C_col = list_cost(1, 512.0 * 64 * 4),
+               C_gs = list_cost(2, (double)GK * GNT * 512.0 * 4);
   std::vector<tg::BProb> probs(n);
   std::vector<tg::BSrc> srcs(n);
   std::vector<int64_t> sprefix(n + 1, 0);
-  std::vector<tg::BItem> items, gitems;
-  std::vector<double> cost, gcost;
-  std::vector<int> ids;
-  std::vector<int64_t> rprefix(1, 0);
-  int64_t part_total = 0;
+  std::vector<tg::BItem> items, gitems, sitems;
+  std::vector<double> cost, gcost, scost;
+  std::vector<int> ids, cand;
+  std::vector<int64_t> rprefix(1, 0), cprefix(1, 0);
+  int64_t part_total = 0, sblk_total = 0;
   for (int p = 0; p < n; ++p) {
     const tgrav_problem& q = pr[p];
     tg::BProb& P = probs[p];
     memset(&P, 0, sizeof P);
-    const bool blk = q.n_target_blocks > 0 || colmask[p];   // column work list
+    P.form = form[p];
     if (q.n_target_blocks > 0) {
       P.T.blocks = 1; P.T.nb = 8;
       P.T.ox = q.tbx; P.T.oy = q.tby; P.T.oz = q.tbz; P.T.bdx = q.tbd; P.T.beps = q.tbe;
@@ -2133,24 +2260,42 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
                   kick ? q.vz : nullptr, dt, nullptr, 0, c->G, q.phi == nullptr};
     P.src_off = sprefix[p];
     P.ns = q.n_sources;
-    srcs[p] = tg::BSrc{q.sx, q.sy, q.sz, q.sm, q.se};
+    if (q.n_source_blocks > 0) {
+      srcs[p] = tg::BSrc{nullptr, nullptr, nullptr, q.sm, q.sbe, q.sbx, q.sby, q.sbz, q.sbd, sblk_total};
+      P.sblk_off = sblk_total;
+      P.nsblk = q.n_source_blocks;
+      sblk_total += q.n_source_blocks;
+    } else {
+      srcs[p] = tg::BSrc{q.sx, q.sy, q.sz, q.sm, q.se, nullptr, nullptr, nullptr, nullptr, 0};
+    }
     sprefix[p + 1] = sprefix[p] + q.n_sources;
+    if (P.form == tg::FORM_CAND) {     // column test at every run: one lane per aligned 8 targets
+      cand.push_back(p);
+      cprefix.push_back(cprefix.back() + q.n_targets / 8);
+    }
     if (q.n_targets == 0) { P.nchunk = 1; continue; }
-    const int64_t TT = blk ? 512 : T;            // targets per work item
-    const int64_t SG = blk ? 64 : NT;            // source-chunk granularity (LDS tile)
-    const int64_t tiles = (q.n_targets + TT - 1) / TT;
-    if (tiles > 0x7fffffffLL) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
     if (q.n_sources == 0) {                    // zero field: written by the reduction
       P.nchunk = 0;
       ids.push_back(p);
       rprefix.push_back(rprefix.back() + q.n_targets);
       continue;
     }
-    const double per_tile = (double)TT * q.n_sources;
-    int64_t nch = per_tile > C ? (int64_t)ceil(per_tile / C) : 1;
-    int64_t ch = (q.n_sources + nch - 1) / nch;
+    const bool gs = P.form == tg::FORM_GRIDSRC;
+    const bool col = P.form == tg::FORM_COLS || P.form == tg::FORM_CAND;
+    // Chunking: GRIDSRC in source blocks; otherwise in sources, a multiple of the LDS tile of every
+    // kernel that may take the problem's items (64 columns, NT generic; NT is a multiple of 64) --
+    // NT for all, so that targets in the blocks' C order and the same targets as blocks get the
+    // same chunks, hence the same bits.
+    const int64_t TT = gs ? (int64_t)GK * GNT : col ? 512 : T;
+    const int64_t SG = gs ? 1 : NT;
+    const int64_t units = gs ? q.n_source_blocks : q.n_sources;
+    const double per_tile = (double)std::min<int64_t>(TT, q.n_targets) * q.n_sources;
+    const double Cp = gs ? C_gs : col ? C_col : C_gen;
+    int64_t nch = per_tile > Cp ? (int64_t)ceil(per_tile / Cp) : 1;
+    int64_t ch = (units + nch - 1) / nch;
     ch = (ch + SG - 1) / SG * SG;
-    nch = (q.n_sources + ch - 1) / ch;
+    nch = (units + ch - 1) / ch;
+    if (nch > 0x7fffffffLL) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d: too many chunks", p); }
     P.chunk = ch;
     P.nchunk = (int)nch;
     if (nch > 1) {
@@ -2159,15 +2304,26 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       ids.push_back(p);
       rprefix.push_back(rprefix.back() + q.n_targets);
     }
-    for (int64_t t = 0; t < tiles; ++t)
-      for (int64_t k = 0; k < nch; ++k) {
-        const double nt_t = (double)std::min(TT, q.n_targets - t * TT);
-        const double ns_k = (double)std::min(ch, q.n_sources - k * ch);
-        (blk ? gitems : items).push_back(tg::BItem{p, (int)t, (int)k, 0});
-        (blk ? gcost : cost).push_back(nt_t * ns_k);
-      }
+    const double per_unit = gs ? 512.0 : 1.0;
+    auto add = [&](std::vector<tg::BItem>& it, std::vector<double>& cs, int64_t tt) -> bool {
+      const int64_t tiles = (q.n_targets + tt - 1) / tt;
+      if (tiles > 0x7fffffffLL) return false;
+      for (int64_t t = 0; t < tiles; ++t)
+        for (int64_t k = 0; k < nch; ++k) {
+          const double nt_t = (double)std::min(tt, q.n_targets - t * tt);
+          const double ns_k = per_unit * (double)std::min(ch, units - k * ch);
+          it.push_back(tg::BItem{p, (int)t, (int)k, 0});
+          cs.push_back(nt_t * ns_k);
+        }
+      return true;
+    };
+    bool ok = true;
+    if (gs) ok = add(sitems, scost, (int64_t)GK * GNT);
+    if (col) ok = ok && add(gitems, gcost, 512);
+    if (P.form == tg::FORM_GENERIC || P.form == tg::FORM_CAND) ok = ok && add(items, cost, T);
+    if (!ok) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
   }
-  if (items.size() > 0x7fffffffULL || gitems.size() > 0x7fffffffULL) {
+  if (items.size() > 0x7fffffffULL || gitems.size() > 0x7fffffffULL || sitems.size() > 0x7fffffffULL) {
     batch_free(b);
     return fail(TGRAV_ELIMIT, "too many work items");
   }
@@ -2179,33 +2335,34 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     for (size_t i = 0; i < order.size(); ++i) out[i] = it[order[i]];
     return out;
   };
-  const std::vector<tg::BItem> sorted = by_cost(items, cost), gsorted = by_cost(gitems, gcost);
+  const std::vector<tg::BItem> sorted = by_cost(items, cost), gsorted = by_cost(gitems, gcost),
+                               ssorted = by_cost(sitems, scost);
   b->n_items = (int)sorted.size();
   b->n_gitems = (int)gsorted.size();
+  b->n_sitems = (int)ssorted.size();
   b->nred = (int)ids.size();
   b->red_total = rprefix.back();
   b->total_src = sprefix[n];
-
-  int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, b->v->bkernel(), b->v->nt, 0) != hipSuccess || occ < 1)
-    occ = 1;
-  const int per_cu = wg_per_cu > 0 ? std::min(wg_per_cu, occ) : occ;
-  b->grid = std::max(1, std::min(b->n_items, per_cu * c->num_cu));
-  int gocc = 0;
-  const void* gk = mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_grid<true>
-                                       : (const void*)tg::k_batch_eval_grid<false>;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&gocc, gk, 64, 0) != hipSuccess || gocc < 1) gocc = 1;
-  b->ggrid = std::max(1, std::min(b->n_gitems, gocc * c->num_cu));
+  b->total_sblk = sblk_total;
+  b->ncand = (int)cand.size();
+  b->cand_total = cprefix.back();
+  b->grid = std::max(1, std::min(b->n_items, slots));
+  b->ggrid = std::max(1, std::min(b->n_gitems, gslots));
+  b->sgrid = std::max(1, std::min(b->n_sitems, sslots));
 
   if ((rc = upload(b->probs, probs)) || (rc = upload(b->items, sorted)) ||
-      (rc = upload(b->gitems, gsorted)) ||
+      (rc = upload(b->gitems, gsorted)) || (rc = upload(b->sitems, ssorted)) ||
       (rc = upload(b->srcs, srcs)) || (rc = upload(b->sprefix, sprefix)) ||
       (rc = upload(b->ids, ids)) || (rc = upload(b->rprefix, rprefix)) ||
+      (rc = upload(b->cand, cand)) || (rc = upload(b->cprefix, cprefix)) ||
       (rc = b->s4.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double4))) ||
       (rc = b->se2.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double))) ||
-      (rc = b->stats.ensure(std::max(n, 1) * 2 * sizeof(unsigned long long))) ||
+      (rc = b->sax.ensure(std::max<int64_t>(sblk_total, 1) * 24 * sizeof(double))) ||
+      (rc = b->be2.ensure(std::max<int64_t>(sblk_total, 1) * sizeof(double))) ||
+      (rc = b->stats.ensure((size_t)std::max(n, 1) * tg::ST_WORDS * sizeof(unsigned long long))) ||
+      (rc = b->colflag.ensure((size_t)std::max(n, 1) * sizeof(int))) ||
       (rc = b->part.ensure(std::max<int64_t>(part_total, 1) * sizeof(double))) ||
-      (rc = b->counter.ensure(2 * sizeof(unsigned int)))) {
+      (rc = b->counter.ensure(3 * sizeof(unsigned int)))) {
     batch_free(b);
     return rc;
   }
