@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
 """Pairwise interactions/s of the direct-sum gravity field on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c4|c5|c2fused|c2flash|c1periodic|c2pm] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c5|c2|c3|c2fused|c2flash|c1periodic|c2pm]
+                    [--no-cpu] [--no-c5]
 
-Default workload c2 (seed 1002): 10,000 Plummer/Kroupa stars (20% hard-binary members) and
-1,024,000 AMR cell centres (2,000 leaf blocks of 8^3).  One step is one bridge field
-evaluation in both directions: stars -> cells (acceleration + potential at the cell
-centres) and cells as point masses (m = rho dx^3, eps = dx) -> stars, each preceded by
-loading its source set.  2 x 1.024e10 pair interactions per step.
-  c5: 100,000 stars <-> 4,194,304 cells (2 x 4.19e11 interactions per step); both kernels are
-      checked against the oracle on 65,536 seeded cells (forward) and 65,536 seeded stars plus
-      every binary member (reverse), about a minute of CPU work.
+Default workload c4 (seed 1004), the configuration the metric's target is quoted on: 64 independent
+problems (N_s log-uniform in [100, 50,000], Plummer/Kroupa stars or gas cells as sources, per-source
+eps in [0.005, 0.1] pc; N_t log-uniform in [1e3, 1e6] as 8^3 leaf blocks), acceleration + potential of
+all of them in one persistent launch replayed from a hipGraph.  One step is one replay.  The plan
+takes the autotuner's launch shape (torch_amd/tune_cache.json).
+  The same problems with the targets as explicit cell centres are timed beside it, and a c5 sub-record
+  (`c5`) runs C5 as one graph-replayed plan: 100,000 stars (30% binary members) -> 4,194,304 cells
+  (8,192 leaf blocks) and the cells as point masses -> the stars.  The CPU leg checks both kernels
+  against the oracle on 256 seeded targets of every c4 problem, and on 65,536 seeded cells (forward)
+  and 65,536 seeded stars plus every binary member (reverse) of c5.
+  c5: that C5 record as the line.
+  c2: 10,000 stars <-> 1,024,000 cells (2,000 leaf blocks of 8^3), forward + reverse, each preceded by
+      loading its source set (eager calls).
   c3: the c1 inputs (1,024 stars <-> 4,096 cells of a 16^3 grid, explicit arrays) in both
       directions; both kernels against the oracle over every target, with the condition number
       kappa = sum |terms| / |sum| of each target's acceleration.
-  c4: 64 uneven independent problems in one graph-replayed persistent launch (one step =
-      one replay, acc + pot of all problems).
   c2fused: c2 with symmetric softening (eps2 = eps_cell^2 + eps_star^2) evaluated by the fused
       bidirectional pass (one pass gives both directions) on the cells as 8^3 blocks; the fused
       pass on explicit arrays and two directed passes in each format are timed beside it.
@@ -33,11 +37,10 @@ loading its source set.  2 x 1.024e10 pair interactions per step.
 Inputs are resident in HBM (device tensors) before timing.
 
 The headline `value` is the mixed-precision rate; the plain-FP64 kernel is timed the same
-way and reported beside it (`fp64_value`, `mixed_over_fp64`).  Mixed-precision error is
-measured over all targets against the FP64 kernel, which is itself checked against the
-CPU oracle on a seeded subsample (the CPU leg).  The roofline uses the forward (c2, c5)
-or persistent (c4) evaluation kernel alone, timed with HIP events on its own stream through
-the library's instrumentation hook.  Prints one JSON line on stdout; progress on stderr.
+way and reported beside it (`fp64_value`, `mixed_over_fp64`).  The roofline uses the dominant
+evaluation kernel alone, timed with HIP events on its own stream through the library's
+instrumentation hook (eager launches of the same plan).  Prints one JSON line on stdout; progress
+on stderr.
 """
 import argparse
 import json
@@ -148,7 +151,9 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
 
 
 # ---------------------------------------------------------------------- c4 ----
-def run_batch(mode, probs_dev, steps, warmup):
+def run_batch(mode, probs_dev, steps, warmup, eager_timed=3):
+    """Graph replays of one batch plan: ms per replay, and each persistent kernel's average
+    duration over `eager_timed` instrumented eager runs (HIP events on the plan's stream)."""
     import torch
 
     from torch_amd import GravityField, autotune
@@ -161,11 +166,16 @@ def run_batch(mode, probs_dev, steps, warmup):
     for _ in range(warmup):
         plan.run(graph=True)
     torch.cuda.synchronize()
-    # the persistent kernel alone, eager launch, instrumented
+    # the persistent kernels alone, eager launches, instrumented: one record per persistent kernel
+    # (generic, column and block-source work lists, those the plan has) and run
     f.set_timing(True)
-    plan.run(graph=False)
-    torch.cuda.synchronize()
-    kms = [r[0] for r in f.kernel_times()]
+    recs = []
+    for _ in range(eager_timed):
+        plan.run(graph=False)
+        torch.cuda.synchronize()
+        recs.append([r[0] for r in f.kernel_times()])
+    f.set_timing(False)
+    per_kernel = [float(np.mean([r[q] for r in recs])) for q in range(len(recs[0]))] if recs and recs[0] else []
     t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
     t0.record()
@@ -175,12 +185,21 @@ def run_batch(mode, probs_dev, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     res = [tuple(a.cpu().numpy() for a in o) for o in out]
-    # one eager run: one record per persistent kernel (explicit-target and block-target items)
-    info = dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.sum(kms)) if kms else None,
-                wall=wall, res=res, cfg=cfg, items=plan.n_items, interactions=plan.interactions)
+    info = dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.sum(per_kernel)) if per_kernel else None,
+                kernels_ms=per_kernel, wall=wall, res=res, cfg=cfg, items=plan.n_items,
+                interactions=plan.interactions)
     plan.close()
     f.close()
     return info
+
+
+def c5_plan_problems(dev):
+    """C5 as one plan: stars -> the 8,192 leaf blocks (column work list) and the blocks' cells as
+    point masses (m = rho dx^3, eps = dx) -> the stars (block-source work list)."""
+    S = dict(x=dev["sx"], y=dev["sy"], z=dev["sz"], m=dev["sm"], eps=dev["se"])
+    return [dict(sources=S, targets=dict(block_origin=dev["bo"], block_dx=dev["bdx"])),
+            dict(sources=dict(block_origin=dev["bo"], block_dx=dev["bdx"], block_eps=dev["bdx"], m=dev["cm"]),
+                 targets=dict(x=dev["sx"], y=dev["sy"], z=dev["sz"]))]
 
 
 # ---------------------------------------------------------------- c2fused ----
@@ -327,9 +346,33 @@ def mean_images_in_cutoff(S, T, box, nrep, npairs=2000, seed=99):
 
 
 # ----------------------------------------------------------- checker legs ----
+def host_threads():
+    """Threads of the CPU legs: the host's CPU share when OMP_NUM_THREADS sets it, else every core
+    this process may run on."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    if v.isdigit() and int(v) > 0:
+        return int(v)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(S, T, G, budget_s, label, flash=None, reverse=None):
     """The oracle (C port of the reference loop nest, -O2) on seeded target subsamples:
-    one thread, then all host cores (<= 16).  flash = dict(rsoft, box, nrep) times the FLASH
+    one thread, then the host's threads (host_threads()).  flash = dict(rsoft, box, nrep) times the FLASH
     kernel instead of the Plummer one.  reverse = (S2, T2): the other direction too, on all
     cores, whole when it fits in half the budget (c2's 1.024e10 interactions do)."""
     import oracle
@@ -352,7 +395,7 @@ def cpu_baseline(S, T, G, budget_s, label, flash=None, reverse=None):
     run(probe, 1)
     rate1 = nprobe * ns / (time.perf_counter() - t)
     out = {}
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = host_threads()
     for threads, share in ((1, 0.35), (cores, 0.65)):
         n = int(min(len(T["x"]), max(16, budget_s * share * rate1 * (0.8 * threads) / ns)))
         idx = np.sort(rng.choice(len(T["x"]), n, replace=False))
@@ -381,7 +424,7 @@ def cpu_baseline(S, T, G, budget_s, label, flash=None, reverse=None):
                     f"{'' if flash is None or flash.get('box') is None else ', periodic images'}, "
                     f"OpenMP over targets); one thread: "
                     f"{out[1]['value']:.3e} interactions/s on {out[1]['targets']} targets",
-          "scalar_value": out[1]["value"], "host_cpus": os.cpu_count(),
+          "scalar_value": out[1]["value"], "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
           "whole_direction": out[cores]["targets"] == len(T["x"])}
     if rev is not None:
         cb["reverse"] = rev
@@ -412,7 +455,7 @@ def oracle_errors(results, k, S, T, idx, G, label, extra=None):
         j = idx[a:a + step]
         part = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S.get("eps"), T["x"][j], T["y"][j],
                                    T["z"][j], None if et is None else et[j], G, mode=oracle.NEUMAIER,
-                                   threads=16)
+                                   threads=host_threads())
         for q in range(4):
             ref[q][a:a + len(j)] = part[q]
         if len(idx) > step:
@@ -440,10 +483,209 @@ def oracle_check(pairs, G):
         et = T.get("eps_t")
         ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S.get("eps"), T["x"][idx], T["y"][idx],
                                   T["z"][idx], None if et is None else et[idx], G, mode=oracle.NEUMAIER,
-                                  threads=16)
+                                  threads=host_threads())
         ea, ep = rel_err(tuple(a[idx] for a in res), ref)
         worst_a, worst_p, n = max(worst_a, ea.max()), max(worst_p, ep.max()), n + len(idx)
     return dict(targets_checked=n, max_rel_acc=float(worst_a), max_rel_pot=float(worst_p))
+
+
+# ------------------------------------------------------------------ c4 / c5 ----
+# Flops the column kernels execute per interaction (the algorithmic count is 21): per cell dz, the
+# r^2 FMA, rsqrt, m y, the potential add, (m y) y y and three FMAs = 14; per source and column of 8
+# cells dx, dy and two FMAs for dx^2 + dy^2 + eps^2 = 6, i.e. 0.75 per cell.  FP64 adds the
+# refinement of the FP32 seed (r2 y, two FMAs, y e, one FMA = 8).
+EXECUTED_FLOP_COLUMN = {"mixed": 14.75, "fp64": 22.75}
+
+
+def roofline_of(mode, interactions, kernel_ms, kernel, traffic=None, algorithmic_bytes=None):
+    achieved = FLOP_PER_INTERACTION * interactions / (kernel_ms * 1e-3) / 1e12
+    peak = PEAK_FP32_TFLOPS if mode == "mixed" else PEAK_FP64_TFLOPS
+    ex = EXECUTED_FLOP_COLUMN[mode] * interactions / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic, "kernel_ms": kernel_ms,
+            "flop_per_interaction": FLOP_PER_INTERACTION, "interactions_per_launch": interactions,
+            "algorithmic_bytes_per_launch": algorithmic_bytes,
+            "executed": {"flop_per_interaction": EXECUTED_FLOP_COLUMN[mode], "achieved": ex,
+                         "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS if mode == "mixed" else PEAK_FP64_TFLOPS,
+                         "frac_of_unpacked": ex / (PEAK_FP32_UNPACKED_TFLOPS if mode == "mixed"
+                                                   else PEAK_FP64_TFLOPS)}}
+
+
+def c5_record(args, modes, d, steps, warmup):
+    """C5 as one graph-replayed plan (forward: 100,000 stars -> 8,192 leaf blocks; reverse: their
+    4,194,304 cells -> the stars): rates per mode, each direction's persistent kernel, mixed vs FP64,
+    and (CPU leg) both kernels against the oracle on 65,536 seeded cells (forward) and 65,536 seeded
+    stars plus every binary member (reverse)."""
+    import torch
+
+    from torch_amd import workloads as W
+    G = W.G_PC_MSUN_KMS
+    t = time.perf_counter()
+    w = W.c5(1005)
+    st, ce = w["stars"], w["cells"]
+    ns, nc = len(st["x"]), len(ce["x"])
+    dev = dict(sx=d(st["x"]), sy=d(st["y"]), sz=d(st["z"]), sm=d(st["m"]), se=d(st["eps"]),
+               cm=d(ce["m"]), bo=d(ce["block_origin"]), bdx=d(ce["block_size"] / 8))
+    torch.cuda.synchronize()
+    log(f"c5 generated in {time.perf_counter() - t:.1f} s: {ns} stars, {nc} cells")
+    probs = c5_plan_problems(dev)
+    inter = 2.0 * ns * nc
+    results = {}
+    for mode in modes:
+        r = run_batch(mode, probs, steps, warmup, eager_timed=2)
+        results[mode] = r
+        log(f"c5 {mode}: {r['ms']:.2f} ms/replay ({inter / r['ms'] * 1e3:.3e} interactions/s); persistent "
+            f"kernels {['%.2f' % k for k in r['kernels_ms']]} ms, {r['items']} work items")
+    rec = {"workload": "c5: 100,000 stars (30% binary members) <-> 4,194,304 AMR cell centres (8,192 leaf "
+                       "blocks of 8^3, lognormal sigma = 2, 10 pc box); forward + reverse, acc + pot, one "
+                       "graph-replayed plan", "seed": w["seed"], "n_stars": ns, "n_cells": nc,
+           "interactions_per_step": inter, "steps": steps, "warmup": warmup}
+    for mode, r in results.items():
+        kf, kr = (r["kernels_ms"] + [None, None])[:2]
+        peak = PEAK_FP32_TFLOPS if mode == "mixed" else PEAK_FP64_TFLOPS
+        frac = lambda k: None if k is None else FLOP_PER_INTERACTION * ns * nc / (k * 1e-3) / 1e12 / peak  # noqa
+        rec[mode] = {"value": inter / (r["ms"] * 1e-3), "ms_per_step": r["ms"], "fwd_kernel_ms": kf,
+                     "rev_kernel_ms": kr, "fwd_roofline_frac": frac(kf), "rev_roofline_frac": frac(kr),
+                     "peak": peak, "launch_shape": r["cfg"], "work_items": r["items"]}
+    if "mixed" in results and "fp64" in results:
+        rec["mixed_over_fp64"] = rec["mixed"]["value"] / rec["fp64"]["value"]
+        errs = {}
+        for k, nm in ((0, "fwd"), (1, "rev")):
+            ea, ep = rel_err(results["mixed"]["res"][k], results["fp64"]["res"][k])
+            errs[nm] = err_stats(ea, ep)
+            if nm == "rev":
+                errs["rev_binary_members"] = err_stats(ea[st["binary"] >= 0])
+        rec["mixed_error_vs_fp64"] = errs
+    if not args.no_cpu and "fp64" in results:
+        S = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
+        C = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
+        rng = np.random.Generator(np.random.PCG64(55))
+        idx_f = np.sort(rng.choice(nc, 65536, replace=False))
+        pick = np.zeros(ns, bool)
+        pick[rng.choice(ns, 65536, replace=False)] = True
+        pick |= st["binary"] >= 0
+        idx_r = np.nonzero(pick)[0]
+        rep = {"fwd": oracle_errors(results, 0, S, C, idx_f, G, "c5 fwd"),
+               "rev": oracle_errors(results, 1, S=C, T=S, idx=idx_r, G=G, label="c5 rev",
+                                    extra={"binary_members": st["binary"][idx_r] >= 0})}
+        rep["targets"] = {"fwd": len(idx_f), "rev": len(idx_r), "rev_binary_members": int((st["binary"] >= 0).sum())}
+        rec["error_vs_oracle"] = rep
+        rec["fp64_vs_oracle"] = dict(targets_checked=len(idx_f) + len(idx_r),
+                                     max_rel_acc=max(rep[k]["fp64"]["acc_max"] for k in ("fwd", "rev")),
+                                     max_rel_pot=max(rep[k]["fp64"]["pot_max"] for k in ("fwd", "rev")))
+        if "mixed" in results:
+            rec["error_gate_met"] = all(rep[k]["mixed"]["acc_median"] <= 1e-6 and rep[k]["mixed"]["acc_max"] <= 1e-4
+                                        for k in ("fwd", "rev"))
+        log("c5 vs oracle:", json.dumps(rep))
+    return rec
+
+
+def bench_c4_c5(args, line, modes, head, d):
+    """The metric's configuration: C4, 64 uneven problems in one graph-replayed plan (the headline
+    `value`), with C5 as a sub-record; `--workload c5` makes C5 the line."""
+    import torch
+
+    from torch_amd import workloads as W
+    G = W.G_PC_MSUN_KMS
+    steps5 = args.c5_steps if args.c5_steps > 0 else max(2, args.steps // 4)
+    warm5 = max(1, min(args.warmup, 2))
+    if args.workload == "c5":
+        rec = c5_record(args, modes, d, args.steps, args.warmup)
+        r = rec[head]
+        line.update({"value": r["value"], "ms_per_step": r["ms_per_step"],
+                     "config": {k: rec[k] for k in ("workload", "seed", "n_stars", "n_cells", "interactions_per_step")}})
+        line["config"]["mode"] = head
+        line["roofline"] = roofline_of(head, float(rec["n_stars"]) * rec["n_cells"], r["fwd_kernel_ms"],
+                                       f"k_batch_eval_grid<{head}> (forward, stars -> 8^3 cell blocks)")
+        for k in ("fp64", "mixed_over_fp64", "mixed_error_vs_fp64", "error_vs_oracle", "fp64_vs_oracle",
+                  "error_gate_met"):
+            if k in rec:
+                line["fp64_value" if k == "fp64" else k] = rec[k]["value"] if k == "fp64" else rec[k]
+        line["c5_detail"] = rec
+        return
+    t = time.perf_counter()
+    probs = W.c4(1004)
+    pdev = [{side: {k: d(v) for k, v in p[side].items()} for side in ("sources", "targets")} for p in probs]
+    torch.cuda.synchronize()
+    sizes = [(len(p["sources"]["x"]), len(p["targets"]["x"])) for p in probs]
+    inter_step = float(sum(a * b for a, b in sizes))
+    log(f"c4 generated in {time.perf_counter() - t:.1f} s: {len(probs)} problems, {inter_step:.3e} interactions")
+    # the same problems with the targets as explicit cell centres (the blocks' C order)
+    pdev_x = [{"sources": p["sources"], "targets": {k: p["targets"][k] for k in ("x", "y", "z", "eps")}}
+              for p in pdev]
+    results, explicit = {}, {}
+    for mode in modes:
+        r = run_batch(mode, pdev, args.steps, args.warmup)
+        results[mode] = r
+        log(f"c4 {mode}: {r['ms']:.3f} ms/replay ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
+            f"persistent kernels {r['kernel_ms']:.3f} ms, {r['items']} work items")
+        rx = run_batch(mode, pdev_x, args.steps, args.warmup)
+        same = all(np.array_equal(a, b) for o, q in zip(rx["res"], r["res"]) for a, b in zip(o, q))
+        explicit[mode] = {"value": inter_step / (rx["ms"] * 1e-3), "ms_per_step": rx["ms"],
+                          "kernel_ms": rx["kernel_ms"], "bit_identical_to_block_form": same}
+        log(f"c4 {mode} explicit targets: {rx['ms']:.3f} ms/replay, bit-identical to the block form: {same}")
+    r = results[head]
+    line["value"] = inter_step / (r["ms"] * 1e-3)
+    line["ms_per_step"] = r["ms"]
+    line["config"] = {"workload": "c4: 64 uneven problems (N_s log-uniform 100-50,000, per-source eps 0.005-0.1 pc; "
+                                  "N_t log-uniform 1e3-1e6 as 8^3 leaf blocks) in one graph-replayed persistent "
+                                  "launch, acc + pot", "seed": 1004, "mode": head, "problems": len(probs),
+                      "interactions_per_step": inter_step,
+                      "launch_shapes": {m: results[m]["cfg"] for m in results}}
+    kern_bytes = sum(BYTES_PER_TARGET * b + BYTES_PER_SOURCE * a for a, b in sizes)
+    line["roofline"] = roofline_of(head, inter_step, r["kernel_ms"],
+                                   f"k_batch_eval_grid<{head}> (persistent column kernel over the 64 problems' "
+                                   f"leaf blocks)", algorithmic_bytes=kern_bytes)
+    line["explicit_targets"] = explicit
+    if "fp64" in results and head == "mixed":
+        r64 = results["fp64"]
+        line["fp64_value"] = inter_step / (r64["ms"] * 1e-3)
+        line["mixed_over_fp64"] = line["value"] / line["fp64_value"]
+        line["fp64_kernel_ms"] = r64["kernel_ms"]
+        line["fp64_roofline_frac"] = (FLOP_PER_INTERACTION * inter_step / (r64["kernel_ms"] * 1e-3) / 1e12 /
+                                      PEAK_FP64_TFLOPS)
+        pairs = [rel_err(m, q) for m, q in zip(results["mixed"]["res"], r64["res"]) if len(q[0])]
+        line["mixed_error_vs_fp64"] = {"all_problems": err_stats(np.concatenate([p[0] for p in pairs]),
+                                                                 np.concatenate([p[1] for p in pairs]))}
+    if not args.no_cpu:
+        # both kernels against the oracle on a seeded subsample of every problem (256 targets each)
+        import oracle
+        rng = np.random.Generator(np.random.PCG64(44))
+        t0 = time.perf_counter()
+        errs = {m: ([], []) for m in results}
+        worst = {m: [] for m in results}
+        for i, p in enumerate(probs):
+            S, T = p["sources"], p["targets"]
+            nt = len(T["x"])
+            if nt == 0 or len(S["x"]) == 0:
+                continue
+            idx = np.sort(rng.choice(nt, min(nt, 256), replace=False))
+            ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S.get("eps"), T["x"][idx], T["y"][idx],
+                                      T["z"][idx], T["eps"][idx] if "eps" in T else None, G, mode=oracle.NEUMAIER,
+                                      threads=host_threads())
+            for m in results:
+                ea, ep = rel_err(tuple(a[idx] for a in results[m]["res"][i]), ref)
+                errs[m][0].append(ea)
+                errs[m][1].append(ep)
+                worst[m].append(float(ea.max()))
+        rep = {m: err_stats(np.concatenate(errs[m][0]), np.concatenate(errs[m][1])) for m in results}
+        for m in results:
+            rep[m]["worst_problem_acc_max"] = max(worst[m])
+        rep["targets_checked"] = int(sum(len(e) for e in errs[head][0]))
+        rep["problems_checked"] = len(worst[head])
+        line["error_vs_oracle"] = rep
+        if "fp64" in rep:
+            line["fp64_vs_oracle"] = dict(targets_checked=rep["targets_checked"], max_rel_acc=rep["fp64"]["acc_max"],
+                                          max_rel_pot=rep["fp64"]["pot_max"])
+        if "mixed" in rep:
+            line["error_gate_met"] = rep["mixed"]["acc_median"] <= 1e-6 and rep["mixed"]["acc_max"] <= 1e-4
+        log(f"c4 vs oracle ({time.perf_counter() - t0:.1f} s):", json.dumps(rep))
+    if not args.no_c5:
+        line["c5"] = c5_record(args, modes, d, steps5, warm5)
+    if not args.no_cpu:
+        big = max(range(len(probs)), key=lambda i: sizes[i][0] * sizes[i][1])
+        line["cpu_baseline"] = cpu_baseline(probs[big]["sources"], probs[big]["targets"], G, args.cpu_seconds,
+                                            f"c4 problem {big} ({sizes[big][0]} sources x {sizes[big][1]} targets)")
 
 
 # -------------------------------------------------------------------- main ----
@@ -452,8 +694,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c2fused", "c2flash",
+    ap.add_argument("--workload", default="c4", choices=["c4", "c5", "c2", "c3", "c2fused", "c2flash",
                                                           "c1periodic", "c2pm"])
+    ap.add_argument("--no-c5", action="store_true", help="c4: leave out the c5 sub-record")
+    ap.add_argument("--c5-steps", type=int, default=0, help="replays timed for c5 (0: max(2, steps / 4))")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle check")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--modes", default="mixed,fp64")
@@ -576,7 +820,7 @@ def main():
             rng = np.random.Generator(np.random.PCG64(78))
             idx = np.sort(rng.choice(ns, 24 if periodic else 64, replace=False))
             ref = oracle.eval_flash(S["x"], S["y"], S["z"], S["m"], T["x"][idx], T["y"][idx], T["z"][idx], G,
-                                    rsoft, quirks=True, threads=16, box=box, nrep=nrep)
+                                    rsoft, quirks=True, threads=host_threads(), box=box, nrep=nrep)
             ea, ep = rel_err(tuple(a[idx] for a in r["res"]), ref)
             line["fp64_vs_oracle"] = dict(targets_checked=len(idx), max_rel_acc=float(ea.max()),
                                           max_rel_pot=float(ep.max()))
@@ -629,8 +873,12 @@ def main():
             line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, "forward c3 direction", reverse=(C, S))
         print(json.dumps(line), flush=True)
         return
-    if args.workload in ("c2", "c5", "c2fused"):
-        w = W.c5(1005) if args.workload == "c5" else W.c2(1002)
+    if args.workload in ("c4", "c5"):
+        bench_c4_c5(args, line, modes, head, d)
+        print(json.dumps(line), flush=True)
+        return
+    if args.workload in ("c2", "c2fused"):
+        w = W.c2(1002)
         st, ce = w["stars"], w["cells"]
         ns, nc = len(st["x"]), len(ce["x"])
         log(f"{args.workload} generated in {time.perf_counter() - t:.1f} s: {ns} stars, {nc} cells")
@@ -662,8 +910,6 @@ def main():
         if args.workload == "c2fused":
             kern_bytes = (BYTES_PER_SOURCE + 32) * (nc + ns)
         name = ("10,000 stars (20% binary members) <-> 1,024,000 AMR cell centres (2,000 leaf "
-                "blocks of 8^3)" if args.workload != "c5" else
-                "100,000 stars (30% binary members) <-> 4,194,304 AMR cell centres (8,192 leaf "
                 "blocks of 8^3)")
         if args.workload == "c2fused":
             name += ", symmetric softening eps2 = eps_cell^2 + eps_star^2, one fused pass per step"
@@ -688,42 +934,7 @@ def main():
             kernel_name = f"k_bidir_grid<{'mixed' if head == 'mixed' else 'fp64'}> (both directions, 8^3 cell blocks)"
             line["other_forms_ms_per_step"] = {m: results[m]["others"] for m in results}
     else:
-        probs = W.c4(1004)
-        pdev = []
-        for p in probs:
-            pdev.append({side: {k: d(v) for k, v in p[side].items()} for side in ("sources", "targets")})
-        torch.cuda.synchronize()
-        inter_step = float(sum(len(p["sources"]["x"]) * len(p["targets"]["x"]) for p in probs))
-        log(f"c4 generated in {time.perf_counter() - t:.1f} s: {len(probs)} problems, "
-            f"{inter_step:.3e} interactions")
-        # the same problems with the targets as explicit cell centres (the blocks' C order)
-        pdev_x = [{"sources": p["sources"], "targets": {k: p["targets"][k] for k in ("x", "y", "z", "eps")}}
-                  for p in pdev]
-        explicit = {}
-        for mode in modes:
-            r = run_batch(mode, pdev, args.steps, args.warmup)
-            results[mode] = r
-            log(f"{mode}: {r['ms']:.3f} ms/replay ({inter_step / r['ms'] * 1e3:.3e} interactions/s), "
-                f"persistent kernel {r['kernel_ms']:.3f} ms, {r['items']} work items")
-            rx = run_batch(mode, pdev_x, args.steps, args.warmup)
-            same = all(np.array_equal(a, b) for o, q in zip(rx["res"], r["res"]) for a, b in zip(o, q))
-            explicit[mode] = {"value": inter_step / (rx["ms"] * 1e-3), "ms_per_step": rx["ms"],
-                              "kernel_ms": rx["kernel_ms"], "bit_identical_to_block_form": same}
-            log(f"{mode} explicit targets: {rx['ms']:.3f} ms/replay ({inter_step / rx['ms'] * 1e3:.3e} "
-                f"interactions/s), bit-identical to the block form: {same}")
-        line["explicit_targets"] = explicit
-        r = results[head]
-        kern_ms, kern_inter = r["kernel_ms"], inter_step
-        kern_bytes = sum(BYTES_PER_TARGET * len(p["targets"]["x"]) + BYTES_PER_SOURCE * len(p["sources"]["x"])
-                         for p in probs)
-        line["config"] = {"workload": "c4: 64 uneven problems (N_s log-uniform 100-50,000, N_t "
-                                      "log-uniform 1e3-1e6 as 8^3 leaf blocks) in one graph-replayed "
-                                      "persistent launch, acc + pot", "seed": 1004, "mode": head,
-                          "interactions_per_step": inter_step,
-                          "launch_shapes": {m: results[m]["cfg"] for m in results}}
-        kernel_name = (f"k_batch_eval + k_batch_eval_grid<{'mixed' if head == 'mixed' else 'fp64'}> "
-                       f"(persistent; block-target problems on the column kernel)")
-
+        raise SystemExit(f"unknown workload {args.workload}")
     line["value"] = inter_step / (results[head]["ms"] * 1e-3)
     line["ms_per_step"] = results[head]["ms"]
     achieved = FLOP_PER_INTERACTION * kern_inter / (kern_ms * 1e-3) / 1e12
@@ -745,7 +956,7 @@ def main():
         k64 = r64["fwd_kernel_ms"] if "fwd_kernel_ms" in r64 else r64["kernel_ms"]
         line["fp64_kernel_ms"] = k64
         line["fp64_roofline_frac"] = FLOP_PER_INTERACTION * kern_inter / (k64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
-        if args.workload in ("c2", "c5", "c2fused"):
+        if args.workload in ("c2", "c2fused"):
             errs = {}
             for k, nm in ((0, "fwd"), (1, "rev")):
                 ea, ep = rel_err(results["mixed"]["res"][k], r64["res"][k])
@@ -760,45 +971,18 @@ def main():
         line["mixed_error_vs_fp64"] = errs
         line["error_gate_met"] = all(e["acc_median"] <= 1e-6 and e["acc_max"] <= 1e-4 for e in errs.values())
     if not args.no_cpu:
-        if args.workload in ("c2", "c5", "c2fused"):
+        if args.workload in ("c2", "c2fused"):
             S = dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=st["eps"])
             C = dict(x=ce["x"], y=ce["y"], z=ce["z"], m=ce["m"], eps=ce["eps"])
             if args.workload == "c2fused":      # symmetric softening: targets carry their eps too
                 S["eps_t"], C["eps_t"] = st["eps"], ce["eps"]
-            if "fp64" in results and args.workload == "c5":
-                # 65,536 seeded cells (forward); 65,536 seeded stars and every binary member (reverse)
-                rng = np.random.Generator(np.random.PCG64(55))
-                idx_f = np.sort(rng.choice(nc, 65536, replace=False))
-                pick = np.zeros(ns, bool)
-                pick[rng.choice(ns, 65536, replace=False)] = True
-                pick |= st["binary"] >= 0
-                idx_r = np.nonzero(pick)[0]
-                rep = {"fwd": oracle_errors(results, 0, S, C, idx_f, G, "c5 fwd"),
-                       "rev": oracle_errors(results, 1, S=C, T=S, idx=idx_r, G=G, label="c5 rev",
-                                            extra={"binary_members": st["binary"][idx_r] >= 0})}
-                rep["targets"] = {"fwd": len(idx_f), "rev": len(idx_r),
-                                  "rev_binary_members": int((st["binary"] >= 0).sum())}
-                line["error_vs_oracle"] = rep
-                line["fp64_vs_oracle"] = dict(targets_checked=len(idx_f) + len(idx_r),
-                                              max_rel_acc=max(rep[k]["fp64"]["acc_max"] for k in ("fwd", "rev")),
-                                              max_rel_pot=max(rep[k]["fp64"]["pot_max"] for k in ("fwd", "rev")))
-                log("vs oracle:", json.dumps(rep))
-            elif "fp64" in results:
+            if "fp64" in results:
                 line["fp64_vs_oracle"] = oracle_check(
                     [(S, C, results["fp64"]["res"][0], 1024),
                      (C, S, results["fp64"]["res"][1], 96)], G)
                 log("fp64 vs oracle:", line["fp64_vs_oracle"])
             line["cpu_baseline"] = cpu_baseline(S, C, G, args.cpu_seconds, f"forward {args.workload} direction",
                                                 reverse=None if args.workload == "c2fused" else (C, S))
-        else:
-            if "fp64" in results:
-                chk = [(probs[i]["sources"], probs[i]["targets"], results["fp64"]["res"][i], 48)
-                       for i in (0, 9, 21, 40, 63)]
-                line["fp64_vs_oracle"] = oracle_check(chk, G)
-                log("fp64 vs oracle:", line["fp64_vs_oracle"])
-            big = max(range(len(probs)), key=lambda i: len(probs[i]["sources"]["x"]) * len(probs[i]["targets"]["x"]))
-            line["cpu_baseline"] = cpu_baseline(probs[big]["sources"], probs[big]["targets"], G,
-                                                args.cpu_seconds, f"c4 problem {big}")
     print(json.dumps(line), flush=True)
 
 

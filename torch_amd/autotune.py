@@ -56,9 +56,17 @@ def eval_key(mode, ns, nt):
     return f"{device_key()}|eval|{mode}|s{_bucket(ns)}|t{_bucket(nt)}"
 
 
+def problem_sizes(p):
+    """(sources, targets) of a batch problem: explicit arrays, or 8^3 leaf blocks (512 cells each)."""
+    S, T = p["sources"], p["targets"]
+    ns = len(S["m"]) if S.get("block_origin") is not None else len(S["x"])
+    nt = 512 * len(T["block_dx"]) if T.get("block_origin") is not None else len(T["x"])
+    return ns, nt
+
+
 def batch_key(mode, problems):
-    ns = [len(p["sources"]["x"]) for p in problems]
-    nt = [len(p["targets"]["x"]) for p in problems]
+    ns = [problem_sizes(p)[0] for p in problems]
+    nt = [problem_sizes(p)[1] for p in problems]
     inter = sum(a * b for a, b in zip(ns, nt))
     return (f"{device_key()}|batch|{mode}|p{len(problems)}|i{_bucket(inter)}"
             f"|s{_bucket(np.median(ns))}|t{_bucket(np.median(nt))}")
