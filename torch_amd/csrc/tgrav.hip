@@ -2200,19 +2200,20 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       }
     }
   }
-  // Resident workgroups of each persistent kernel.
-  auto occupancy = [&](const void* k, int threads) {
+  // Resident workgroups of each persistent kernel: the occupancy limit, or for the generic kernel
+  // the tuned / set workgroups per CU (the column and block-source kernels fill their CUs).
+  auto occupancy = [&](const void* k, int threads, int limit) {
     int o = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k, threads, 0) != hipSuccess || o < 1) o = 1;
-    const int per_cu = wg_per_cu > 0 ? std::min(wg_per_cu, o) : o;
+    const int per_cu = limit > 0 ? std::min(limit, o) : o;
     return per_cu * c->num_cu;
   };
   const int GK = TG_BATCH_GS_K, GNT = TG_BATCH_GS_NT;
-  const int slots = occupancy(b->v->bkernel(), b->v->nt);
+  const int slots = occupancy(b->v->bkernel(), b->v->nt, wg_per_cu);
   const int gslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_grid<true>
-                                                   : (const void*)tg::k_batch_eval_grid<false>, 64);
+                                                   : (const void*)tg::k_batch_eval_grid<false>, 64, 0);
   const int sslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_gridsrc<GK, GNT, true>
-                                                   : (const void*)tg::k_batch_eval_gridsrc<GK, GNT, false>, GNT);
+                                                   : (const void*)tg::k_batch_eval_gridsrc<GK, GNT, false>, GNT, 0);
   // Form of each problem's work items.
   std::vector<int> form(n, tg::FORM_GENERIC);
   double tot_list[3] = {0.0, 0.0, 0.0};   // interactions on the generic, column, block-source lists
