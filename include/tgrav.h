@@ -153,6 +153,20 @@ int tgrav_eval_kick(tgrav_ctx* ctx, int64_t n, const double* eps_t, const double
                     double* vz, double* ax, double* ay, double* az, double* phi, int mode,
                     int ptr_kind);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  tgrav_eval's direct sum
+ * (generic kernels, same launch shape) with the source set passed in instead of set beforehand,
+ * device pointers only, no validation; it calls no hipMalloc/hipFree.  The packed sources (40 B
+ * each) and the chunk partial sums (32 B per target per source chunk) live in a fixed 64 MiB array
+ * per GPU, held until the process ends: sources beyond what fits (~1.6 million) give TGRAV_ELIMIT,
+ * and the source split is reduced until the partial sums fit.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (several streams or threads) race.  FLASH kernels and
+ * periodic images are not available here (TGRAV_EINVAL). */
+int tgrav_eval_static(tgrav_ctx* ctx, int64_t ns, const double* sx, const double* sy,
+                      const double* sz, const double* sm, const double* se, int64_t nt,
+                      const double* eps_t, const double* x, const double* y, const double* z,
+                      double* ax, double* ay, double* az, double* phi, int mode);
+
 /* Standalone FP64 kick v += dt * a (.deprecated/bridge_multiples.py:612-616). */
 int tgrav_kick(tgrav_ctx* ctx, int64_t n, double dt, const double* ax, const double* ay,
                const double* az, double* vx, double* vy, double* vz, int ptr_kind);

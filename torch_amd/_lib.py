@@ -57,6 +57,8 @@ SIGNATURES = [
     ("tgrav_eval_kick", _I32, [_P, _I64, _P, _P, _P, _P, _D, _P, _P, _P, _P, _P, _P, _P,
                                _I32, _I32]),
     ("tgrav_kick", _I32, [_P, _I64, _D, _P, _P, _P, _P, _P, _P, _I32]),
+    ("tgrav_eval_static", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P,
+                                 _I32]),
     ("tgrav_eval_blocks", _I32, [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D,
                                  _P, _P, _P, _I32, _I32]),
     ("tgrav_bidirectional", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P,

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,6 +52,30 @@ static int fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess)                                                                \
       return fail(TGRAV_EHIP, "launch of %s failed: %s", what, hipGetErrorString(e_));    \
   } while (0)
+
+// ---- fixed scratch of the allocation-free entry point (tgrav_eval_static) ----
+// One array per GPU the library runs on, held until the process ends.  Host code never passes
+// `scratch` itself (on the host it names a stand-in, not device memory): it passes the device
+// address hipGetSymbolAddress gave for the current device, kept per device in g_scratch_ptr.
+#define TG_STATIC_SCRATCH_BYTES (64ull << 20)
+#define TG_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[TG_STATIC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[TG_MAX_DEVICES] = {};
+
+// Once per device, with that device current (called by tgrav_create).
+static int scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= TG_MAX_DEVICES)
+    return fail(TGRAV_ELIMIT, "device %d beyond the %d the library supports", device, TG_MAX_DEVICES);
+  if (g_scratch_ptr[device]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(TGRAV_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
 
 namespace {
 
@@ -841,6 +866,24 @@ static int reduce_particle_partials(tgrav_ctx* c, int64_t ntiles, int64_t np, co
 }
 
 // Core launch: targets T (device), outputs O (device), on c->stream.
+// Variant of the generic kernels for ns sources onto nt targets: the tune cache's entry for this
+// size bucket (with its source split in *force_split), else the context's choice.
+static const Variant* generic_variant(const tgrav_ctx* c, int mode, int64_t ns, int64_t nt, int* force_split) {
+  const Variant* v = variant_of(mode, c->variant[mode]);
+  *force_split = -1;
+  if (!c->user_cfg[mode] && !c->tune.empty()) {
+    char key[256];
+    snprintf(key, sizeof key, "%s|eval|%s|s%d|t%d", c->devkey.c_str(), mode_name(mode),
+             size_bucket((double)ns), size_bucket((double)nt));
+    const TuneEntry* t = tune_lookup(c, key);
+    if (t && t->variant >= 0 && variant_of(mode, t->variant)) {
+      v = variant_of(mode, t->variant);
+      *force_split = t->nsplit;
+    }
+  }
+  return v;
+}
+
 static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
   const int64_t nt = T.n, ns = c->ns;
   if (nt == 0) return 0;
@@ -862,18 +905,8 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   if (c->soft != TGRAV_SOFT_PLUMMER) return launch_flash(c, mode, T, O);
   if (T.blocks && T.nb == 8 && c->block_kernel) return launch_grid(c, mode, T, O);
   if (!T.blocks && c->src_nblk > 0 && c->block_kernel) return launch_gridsrc(c, mode, T, O);
-  const Variant* v = variant_of(mode, c->variant[mode]);
   int force_split = -1;
-  if (!c->user_cfg[mode] && !c->tune.empty()) {
-    char key[256];
-    snprintf(key, sizeof key, "%s|eval|%s|s%d|t%d", c->devkey.c_str(), mode_name(mode),
-             size_bucket((double)ns), size_bucket((double)nt));
-    const TuneEntry* t = tune_lookup(c, key);
-    if (t && t->variant >= 0 && variant_of(mode, t->variant)) {
-      v = variant_of(mode, t->variant);
-      force_split = t->nsplit;
-    }
-  }
+  const Variant* v = generic_variant(c, mode, ns, nt, &force_split);
   int64_t ntiles, chunk;
   int nsplit;
   choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk, force_split);
@@ -1013,6 +1046,13 @@ int tgrav_create(tgrav_ctx** out, int device, const char* tune_cache_path) {
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return fail(TGRAV_EHIP, "tgrav_create: %s", hipGetErrorString(e));
+  }
+  const int rs = scratch_init(device);
+  if (rs) {
+    (void)hipHostFree(c->host_flag);
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return rs;
   }
   c->stream = c->own;
   c->tune.swap(tune);
@@ -1353,6 +1393,71 @@ int tgrav_eval_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox, cons
   double* outs[4] = {ax, ay, az, phi};
   double* vel[3] = {kick ? vx : nullptr, kick ? vy : nullptr, kick ? vz : nullptr};
   return eval_common(c, T, tin, 5, outs, dt, vel, mode, kind);
+}
+
+// The allocation-free variant of tgrav_eval (device pointers, sources passed in): packed sources,
+// their statistics and the chunk partial sums live in the fixed per-device array `scratch`
+// (layout: ST_WORDS statistics words padded to 256 B, then {x,y,z,m} and eps^2 of the sources,
+// then the partials); the kernels and the launch shape are tgrav_eval's generic ones.
+int tgrav_eval_static(tgrav_ctx* c, int64_t ns, const double* sx, const double* sy, const double* sz,
+                      const double* sm, const double* se, int64_t nt, const double* eps_t,
+                      const double* x, const double* y, const double* z, double* ax, double* ay,
+                      double* az, double* phi, int mode) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  int rc = check_mode(mode);
+  if (rc) return rc;
+  if (ns < 0 || nt < 0) return fail(TGRAV_EINVAL, "sizes must be >= 0");
+  if (ns > 0 && (!sx || !sy || !sz || !sm)) return fail(TGRAV_EINVAL, "source x, y, z and m are required");
+  if (nt > 0 && (!x || !y || !z)) return fail(TGRAV_EINVAL, "target x, y and z are required");
+  if (c->soft != TGRAV_SOFT_PLUMMER || c->periodic)
+    return fail(TGRAV_EINVAL, "tgrav_eval_static uses the Plummer kernel (tgrav_set_softening(ctx, 0, 0, 0))");
+  if (c->device < 0 || c->device >= TG_MAX_DEVICES || !g_scratch_ptr[c->device])
+    return fail(TGRAV_EINVAL, "no scratch array for device %d", c->device);
+  const size_t head = 256, src_bytes = ((size_t)ns * (sizeof(double4) + sizeof(double)) + 255) / 256 * 256;
+  if ((double)head + (double)src_bytes > (double)TG_STATIC_SCRATCH_BYTES)
+    return fail(TGRAV_ELIMIT, "tgrav_eval_static: %lld sources need %.0f bytes of scratch, more than its %llu",
+                (long long)ns, (double)head + (double)src_bytes, TG_STATIC_SCRATCH_BYTES);
+  if ((rc = set_device(c))) return rc;
+  if (nt == 0) return 0;
+  if (ns == 0) {   // no sources: zero field
+    tg::k_fill<<<grid1(nt, 256), 256, 0, c->stream>>>(nt, 0.0, ax, ay, az, phi);
+    LAUNCHCHECK("k_fill");
+    return 0;
+  }
+  unsigned char* base = (unsigned char*)g_scratch_ptr[c->device];
+  unsigned long long* stat = (unsigned long long*)base;
+  double4* s4 = (double4*)(base + head);
+  double* se2 = (double*)(base + head + (size_t)ns * sizeof(double4));
+  double* part = (double*)(base + head + src_bytes);
+  const size_t part_cap = TG_STATIC_SCRATCH_BYTES - head - src_bytes;
+  int force_split = -1;
+  const Variant* v = generic_variant(c, mode, ns, nt, &force_split);
+  int64_t ntiles, chunk;
+  int nsplit;
+  choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk, force_split);
+  if (ntiles > 0x7fffffffLL) return fail(TGRAV_ELIMIT, "too many targets (%lld)", (long long)nt);
+  const double per_split = 4.0 * sizeof(double) * (double)nt;
+  if (nsplit > 1 && nsplit * per_split > (double)part_cap) {   // fewer chunks, so the partials fit
+    const int64_t most = (int64_t)((double)part_cap / per_split);
+    choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk, (int)std::max<int64_t>(1, std::min<int64_t>(most, 65535)));
+    if (nsplit > 1 && nsplit * per_split > (double)part_cap)
+      choose_split(c, mode, v, nt, ns, &ntiles, &nsplit, &chunk, 1);
+  }
+  tg::Tgt T{};
+  T.x = x; T.y = y; T.z = z; T.eps = eps_t; T.n = nt;
+  tg::Out O{ax, ay, az, phi, nullptr, nullptr, nullptr, 0.0, nullptr, 0, c->G, phi == nullptr};
+  tg::Out o = O;
+  o.part = nsplit > 1 ? part : nullptr;
+  o.pstride = nt;
+  tg::k_reset_stats<<<1, 64, 0, c->stream>>>(stat);
+  LAUNCHCHECK("k_reset_stats");
+  tg::k_pack_sources<<<grid1(ns, 256), 256, 0, c->stream>>>(ns, sx, sy, sz, sm, se, s4, se2, stat, 0);
+  LAUNCHCHECK("k_pack_sources");
+  v->fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v->nt), c->stream, T, s4, se2, ns, chunk,
+        (const double*)stat, o);
+  LAUNCHCHECK(mode == TGRAV_MIXED ? "k_eval_mixed" : "k_eval_fp64");
+  if (nsplit > 1) return reduce_chunks(c, part, nt, nsplit, nt, O);
+  return 0;
 }
 
 int tgrav_kick(tgrav_ctx* c, int64_t n, double dt, const double* ax, const double* ay,
