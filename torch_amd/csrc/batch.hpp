@@ -147,9 +147,9 @@ __global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __res
 }
 
 // The column-structure test of k_check_cols over the plan's candidate problems (ids[q], explicit
-// targets, a multiple of 512): lane i checks one aligned group of 8 targets of candidate q =
-// the segment of i in prefix (groups of 8 per candidate) and clears colflag[ids[q]] on a
-// violation.  Runs after k_batch_reset set the flags to 1.
+// targets, a multiple of 512): lane i checks target i - prefix[q] of candidate q = the segment of
+// i in prefix (targets per candidate) and clears colflag[ids[q]] on a violation.  Runs after
+// k_batch_reset set the flags to 1.
 __global__ void k_batch_check_cols(int ncand, const int* __restrict__ ids,
                                    const int64_t* __restrict__ prefix, const BProb* __restrict__ probs,
                                    int* colflag) {
@@ -157,15 +157,7 @@ __global__ void k_batch_check_cols(int ncand, const int* __restrict__ ids,
   if (i >= prefix[ncand]) return;
   const int q = find_segment(prefix, ncand, i);
   const int p = ids[q];
-  const Tgt& T = probs[p].T;
-  const int64_t t = (i - prefix[q]) * 8;
-  const double x0 = T.x[t], y0 = T.y[t];
-  const double e0 = T.eps ? T.eps[t & ~(int64_t)511] : 0.0;
-  bool ok = true;
-  for (int k = 1; k < 8; ++k)
-    ok = ok && T.x[t + k] == x0 && T.y[t + k] == y0 && (!T.eps || T.eps[t + k] == e0);
-  if (T.eps) ok = ok && T.eps[t] == e0;
-  if (!ok) colflag[p] = 0;
+  if (!column_ok(probs[p].T, i - prefix[q])) colflag[p] = 0;
 }
 
 template <int K, int NT, bool MIXED>
@@ -181,14 +173,19 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
   __shared__ Lds L;
   __shared__ int s_item;
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {   // items of problems whose targets took the column kernel are skipped here
+      int i;
+      do {
+        i = (int)atomicAdd(counter, 1u);
+      } while (i < n_items && probs[items[i].prob].form == FORM_CAND && colflag[items[i].prob] != 0);
+      s_item = i;
+    }
     __syncthreads();
     const int it = s_item;
     __syncthreads();                    // all lanes hold `it` before lane 0 overwrites it
     if (it >= n_items) break;           // every lane of the workgroup leaves together
     const BItem w = items[it];
     const BProb& P = probs[w.prob];
-    if (P.form == FORM_CAND && colflag[w.prob] != 0) continue;   // the column items run instead
     const int64_t sb = P.src_off + (int64_t)w.chunk * P.chunk;
     const int64_t se = min(sb + P.chunk, P.src_off + P.ns);
     const double* sst = (const double*)(stats + (int64_t)ST_WORDS * w.prob);
@@ -223,14 +220,19 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
   __shared__ Lds L;
   __shared__ int s_item;
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {   // items of problems whose targets are not in C order at this run are skipped
+      int i;
+      do {
+        i = (int)atomicAdd(counter, 1u);
+      } while (i < n_items && probs[items[i].prob].form == FORM_CAND && colflag[items[i].prob] != 1);
+      s_item = i;
+    }
     __syncthreads();
     const int it = s_item;
     __syncthreads();                    // all lanes hold `it` before lane 0 overwrites it
     if (it >= n_items) break;           // every lane of the workgroup leaves together
     const BItem w = items[it];
     const BProb& P = probs[w.prob];
-    if (P.form == FORM_CAND && colflag[w.prob] != 1) continue;   // not in C order at this run
     const int64_t sb = P.src_off + (int64_t)w.chunk * P.chunk;
     const int64_t se = min(sb + P.chunk, P.src_off + P.ns);
     const double* sst = (const double*)(stats + (int64_t)ST_WORDS * w.prob);

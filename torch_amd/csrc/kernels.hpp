@@ -608,12 +608,22 @@ __device__ __forceinline__ void fold_source_stats(unsigned long long e2min, int 
     }
     __syncthreads();
   }
-  if (t == 0) {
-    atomicMin(&stat[ST_E2MIN], sm[0][0]);
-    atomicMax(&stat[ST_RMAX], sm[1][0]);
-    atomicMax(&stat[ST_E2MAX], sm[2][0]);
-    atomicMax(&stat[ST_MMAX], sm[3][0]);
-    if (sbad[0]) atomicOr(&stat[ST_BAD], 1ULL);
+  // One lane per word; the atomic only where the workgroup's value improves on what the word
+  // already holds (a plain read, possibly stale, which only costs an atomic): with thousands of
+  // workgroups on the same five words, most then skip their atomics instead of queueing on them.
+  if (t < 5) {
+    volatile unsigned long long* w = stat + t;
+    if (t == ST_E2MIN) {
+      if (sm[0][0] < *w) atomicMin(&stat[ST_E2MIN], sm[0][0]);
+    } else if (t == ST_RMAX) {
+      if (sm[1][0] > *w) atomicMax(&stat[ST_RMAX], sm[1][0]);
+    } else if (t == ST_E2MAX) {
+      if (sm[2][0] > *w) atomicMax(&stat[ST_E2MAX], sm[2][0]);
+    } else if (t == ST_MMAX) {
+      if (sm[3][0] > *w) atomicMax(&stat[ST_MMAX], sm[3][0]);
+    } else if (sbad[0]) {
+      atomicOr(&stat[ST_BAD], 1ULL);
+    }
   }
 }
 
@@ -721,19 +731,20 @@ __global__ void k_eval_gate(int* gate, const int* sflag, int tcand) {
 // Column structure of explicit targets, the C order of 8^3 leaf blocks
 // (.deprecated/bridge_multiples.py:566-575 flattens each block's meshgrid this way): every
 // aligned group of 8 consecutive targets shares x and y, and every aligned group of 512 shares
-// eps.  Any violation clears flag[0] (1 before; 2, the block-source path, is left alone); all
-// writers store the same value.
+// eps.  One lane per target, compared with the first of its group (coalesced reads); any
+// violation clears flag[0] (1 before; 2, the block-source path, is left alone); all writers store
+// the same value.
+__device__ __forceinline__ bool column_ok(const Tgt& T, int64_t t) {
+  const int64_t g = t & ~(int64_t)7;
+  bool ok = T.x[t] == T.x[g] && T.y[t] == T.y[g];
+  if (T.eps) ok = ok && T.eps[t] == T.eps[t & ~(int64_t)511];
+  return ok;
+}
+
 __global__ void k_check_cols(Tgt T, int* flag) {
-  const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T.n || flag[0] != 1) return;   // 2: the block-source kernel runs; nothing to test
-  const double x0 = T.x[t], y0 = T.y[t];
-  const double e0 = T.eps ? T.eps[t & ~(int64_t)511] : 0.0;
-  bool ok = true;
-  for (int k = 0; k < 8; ++k) {
-    if (t + k >= T.n) break;
-    ok = ok && T.x[t + k] == x0 && T.y[t + k] == y0 && (!T.eps || T.eps[t + k] == e0);
-  }
-  if (!ok) flag[0] = 0;
+  if (!column_ok(T, t)) flag[0] = 0;
 }
 
 __global__ void k_fill(int64_t n, double v, double* a, double* b, double* c, double* d) {

@@ -947,7 +947,7 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
                                               cols ? 1 : 0);
     LAUNCHCHECK("k_eval_gate");
     if (cols) {
-      tg::k_check_cols<<<grid1(ngrp * 64, 256), 256, 0, c->stream>>>(T, flag);
+      tg::k_check_cols<<<grid1(nt, 256), 256, 0, c->stream>>>(T, flag);
       LAUNCHCHECK("k_check_cols");
     }
     o.gate = oc.gate = os.gate = flag;
@@ -1658,7 +1658,7 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     // Cells in the C order of 8^3 leaf-block meshgrids take the block-column kernel.  Whether
     // they are is tested on the device (k_check_cols writes a gate word: 1 columns, 0 not); both
     // kernels and their reductions are launched, and the word lets one of them run.
-    const bool cand = nc % 512 == 0 && c->block_kernel;
+    bool cand = nc % 512 == 0 && c->block_kernel;
     const BiVariant& v = kBiVariants[mode][c->bvariant[mode]];
     const int64_t tile = (int64_t)v.k * v.nt;
     const int64_t ntiles = (nc + tile - 1) / tile;
@@ -1681,7 +1681,13 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
     const int64_t nblk = nc / 512;
     int64_t chunk_g = 0;
     int nsplit_g = 1;
-    if (cand && (rc = bidir_grid_shape(c, mode, nblk, np, &chunk_g, &nsplit_g))) return rc;
+    // The size limit is the generic pipeline's; a column pipeline that would exceed it is simply
+    // not offered (the generic one then runs whatever the structure).
+    if (cand && bidir_grid_shape(c, mode, nblk, np, &chunk_g, &nsplit_g) != 0) cand = false;
+    if (cand && ((double)std::max<int64_t>(ntiles, nblk) * 4 * np * 8 +
+                 (std::max(nsplit, nsplit_g) > 1 ? (double)std::max(nsplit, nsplit_g) * 4 * nc * 8 : 0.0)) >
+                    64.0 * (1ULL << 30))
+      cand = false;
     const int64_t prows = std::max<int64_t>(ntiles, cand ? nblk : 0);
     if ((rc = c->bppart.ensure((size_t)prows * 4 * np * sizeof(double)))) return rc;
     const int nsp = std::max(nsplit, cand ? nsplit_g : 1);
@@ -1694,7 +1700,7 @@ int tgrav_bidirectional(tgrav_ctx* c, int64_t nc, const double* cx, const double
       T.x = ca[0]; T.y = ca[1]; T.z = ca[2]; T.eps = ca[4]; T.n = nc;
       tg::k_set_flag<<<1, 64, 0, c->stream>>>(fl, 1);
       LAUNCHCHECK("k_set_flag");
-      tg::k_check_cols<<<grid1(nc / 8, 256), 256, 0, c->stream>>>(T, fl);
+      tg::k_check_cols<<<grid1(nc, 256), 256, 0, c->stream>>>(T, fl);
       LAUNCHCHECK("k_check_cols");
       CG.gate = PG.gate = CX.gate = PX.gate = fl;
       CG.gate_on = PG.gate_on = 1;
@@ -2375,9 +2381,9 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       srcs[p] = tg::BSrc{q.sx, q.sy, q.sz, q.sm, q.se, nullptr, nullptr, nullptr, nullptr, 0};
     }
     sprefix[p + 1] = sprefix[p] + q.n_sources;
-    if (P.form == tg::FORM_CAND) {     // column test at every run: one lane per aligned 8 targets
+    if (P.form == tg::FORM_CAND) {     // column test at every run: one lane per target
       cand.push_back(p);
-      cprefix.push_back(cprefix.back() + q.n_targets / 8);
+      cprefix.push_back(cprefix.back() + q.n_targets);
     }
     if (q.n_targets == 0) { P.nchunk = 1; continue; }
     if (q.n_sources == 0) {                    // zero field: written by the reduction

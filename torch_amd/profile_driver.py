@@ -1,14 +1,17 @@
 """Short, fixed GPU workload for counter runs (rocprofv3 --pmc / --kernel-trace).
 
     python -m torch_amd.profile_driver --workload c2 --runs mixed:fwd fp64:fwd:1 --reps 3 --copy-mb 512
+    python -m torch_amd.profile_driver --workload c4 --runs mixed:plan fp64:plan --reps 3
 
 Each run spec is MODE:DIRECTION[:VARIANT[:NSPLIT]] (direction fwd = stars -> cells,
 rev = cells -> stars, fwdb / revb = the same with the cells as 8^3 leaf blocks (column
 kernels; VARIANT is ignored there); variant/nsplit default to the tuned shape of the run's
-kernel (the block entries for fwdb / revb).  After the evaluations
-it performs one plain device-to-device copy of --copy-mb MiB (torch copy_), whose
-FETCH_SIZE / WRITE_SIZE readings calibrate the traffic counters against a known byte count.
-Everything runs on torch's current stream; the program prints one line per step.
+kernel (the block entries for fwdb / revb).  Workloads c4 and c5 take MODE:plan: the bench's
+batch plan (c4: the 64 problems with block targets; c5: stars -> blocks and blocks -> stars),
+run eagerly --reps times with the tuned shape.  After the evaluations it performs one plain
+device-to-device copy of --copy-mb MiB (torch copy_), whose FETCH_SIZE / WRITE_SIZE readings
+calibrate the traffic counters against a known byte count.  Everything runs on torch's current
+stream; the program prints one line per step.
 """
 import argparse
 import time
@@ -16,26 +19,60 @@ import time
 import numpy as np
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c5"])
-    ap.add_argument("--runs", nargs="+", default=["mixed:fwd", "fp64:fwd"])
-    ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--copy-mb", type=int, default=512)
-    args = ap.parse_args()
+def _dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
 
+
+def plan_problems(workload):
+    """The bench's batch-plan problems of c4 or c5, as device tensors."""
+    from . import workloads as W
+    if workload == "c4":
+        return [{side: {k: _dev(v) for k, v in p[side].items()} for side in ("sources", "targets")}
+                for p in W.c4(1004)]
+    w5 = W.c5(1005)
+    st, ce = w5["stars"], w5["cells"]
+    S = dict(x=_dev(st["x"]), y=_dev(st["y"]), z=_dev(st["z"]), m=_dev(st["m"]), eps=_dev(st["eps"]))
+    bo, bdx = _dev(ce["block_origin"]), _dev(ce["block_size"] / 8)
+    return [dict(sources=S, targets=dict(block_origin=bo, block_dx=bdx)),
+            dict(sources=dict(block_origin=bo, block_dx=bdx, block_eps=bdx, m=_dev(ce["m"])),
+                 targets=dict(x=S["x"], y=S["y"], z=S["z"]))]
+
+
+def run_plans(args):
+    import torch
+
+    from . import autotune
+    from .batch import BatchPlan
+    from .field import GravityField
+    probs = plan_problems(args.workload)
+    torch.cuda.synchronize()
+    for spec in args.runs:
+        mode = spec.split(":")[0]
+        with GravityField(mode, validate=False) as f:
+            e = autotune.apply_batch(f, probs)
+            plan = BatchPlan(f, probs)
+            f.set_timing(True)
+            for _ in range(args.reps):
+                plan.run(graph=False)
+            torch.cuda.synchronize()
+            ks = f.kernel_times()
+            print(f"{spec}: tuned={e} items={plan.n_items} kernel ms={[round(k[0], 3) for k in ks]}", flush=True)
+            plan.close()
+
+
+def run_points(args):
     import torch
 
     from . import autotune, workloads as W
     from .field import GravityField
     w = {"c1": W.c1, "c2": W.c2, "c5": W.c5}[args.workload]()
     st, ce = w["stars"], w["cells"]
-    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
-    S = [d(st[k]) for k in ("x", "y", "z", "m", "eps")]
-    C = [d(ce[k]) for k in ("x", "y", "z", "m", "eps")]
+    S = [_dev(st[k]) for k in ("x", "y", "z", "m", "eps")]
+    C = [_dev(ce[k]) for k in ("x", "y", "z", "m", "eps")]
     torch.cuda.synchronize()
-    BO = d(ce["block_origin"])
-    BD = d(ce["block_size"] / 8)
+    BO = _dev(ce["block_origin"])
+    BD = _dev(ce["block_size"] / 8)
     for spec in args.runs:
         parts = spec.split(":")
         mode, direction = parts[0], parts[1]
@@ -55,7 +92,7 @@ def main():
             else:
                 f.set_sources(*src)
             f.set_timing(True)
-            for r in range(args.reps):
+            for _ in range(args.reps):
                 if direction == "fwdb":
                     f.evaluate_blocks(BO, BD)
                 else:
@@ -63,6 +100,24 @@ def main():
             torch.cuda.synchronize()
             ks = f.kernel_times()
             print(f"{spec}: tuned={e} kernel ms={[round(k[0], 3) for k in ks]}", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5"])
+    ap.add_argument("--runs", nargs="+", default=["mixed:fwd", "fp64:fwd"])
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-mb", type=int, default=512)
+    args = ap.parse_args()
+
+    import torch
+
+    if all(spec.endswith(":plan") for spec in args.runs):
+        if args.workload not in ("c4", "c5"):
+            raise SystemExit("MODE:plan runs need --workload c4 or c5")
+        run_plans(args)
+    else:
+        run_points(args)
     if args.copy_mb > 0:
         n = args.copy_mb * (1 << 20) // 8
         a = torch.ones(n, dtype=torch.float64, device="cuda")
