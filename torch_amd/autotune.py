@@ -10,7 +10,7 @@ on the stream the kernels use; the fastest is kept.  Results are cached in a JSO
 by device, mode and log2 size buckets, so a tuned shape is reused for every problem of
 similar size.  ``apply_eval`` / ``apply_batch`` look the cache up and configure a field.
 
-    python -m torch_amd.autotune --eval 10000 1024000 --eval 1024000 10000 --batch c4
+    python -m torch_amd.autotune --eval 10000 1024000 --eval 1024000 10000 --batch c4 --batch c5
 """
 import argparse
 import ctypes
@@ -96,13 +96,21 @@ def _flush():
     _FLUSH[1].copy_(_FLUSH[0])
 
 
-def _time(fn, reps):
+def _time(fn, reps, before=None):
+    """Median of `reps` timed calls of fn.  Before each: `before()` when given (the work that
+    precedes the candidate where it is used, e.g. the other direction of a bridge step), else a
+    256 MiB copy.  Two untimed calls first: the GPU clock ramps up over the first milliseconds
+    of work after idling, so the first launches of a sequence run slower."""
     import torch
+    fn()
     fn()
     torch.cuda.synchronize()
     ts = []
     for _ in range(reps):
-        _flush()
+        if before is not None:
+            before()
+        else:
+            _flush()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
         fn()
@@ -122,10 +130,19 @@ def _synthetic(ns, nt, seed=0):
     return [d(st[k]) for k in ("x", "y", "z", "m", "eps")], [d(t[q]) for q in range(3)]
 
 
-def tune_eval(mode, ns, nt, reps=5, nsplits=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256), log=print):
-    """Best (variant, nsplit) for one evaluation of nt targets from ns sources."""
+def tune_eval(mode, ns, nt, reps=5, nsplits=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256), log=print,
+              in_sequence=True):
+    """Best (variant, nsplit) for one evaluation of nt targets from ns sources.  in_sequence: each
+    timed candidate follows the reverse evaluation (the nt targets as sources onto the ns sources,
+    from a second context), as the bridge alternates the two directions, instead of a plain copy."""
     S, T = _synthetic(ns, nt)
     table = []
+    partner = None
+    if in_sequence:
+        g = GravityField(mode, validate=False, tune_cache=None)
+        import torch
+        g.set_sources(T[0], T[1], T[2], torch.ones_like(T[0]))
+        partner = (g, lambda: g.evaluate(None, S[0], S[1], S[2]))
     with GravityField(mode, validate=False, tune_cache=None) as f:
         f.set_sources(*S)
         for v in variants(mode):
@@ -136,26 +153,32 @@ def tune_eval(mode, ns, nt, reps=5, nsplits=(0, 1, 2, 4, 8, 16, 32, 64, 128, 256
                 if sp > 1 and -(-nt // tile) >= 4096:
                     continue      # plenty of target tiles already
                 f.set_config(variant=v["variant"], nsplit=sp)
-                ms = _time(lambda: f.evaluate(None, *T), reps)
+                ms = _time(lambda: f.evaluate(None, *T), reps, before=partner[1] if partner else None)
                 table.append(dict(v, nsplit=sp, ms=ms, rate=ns * nt / ms * 1e3))
                 log(f"  {mode} ns={ns} nt={nt} variant={v['variant']} (K={v['k']}, NT={v['threads']})"
                     f" nsplit={sp}: {ms:.3f} ms")
+    if partner:
+        partner[0].close()
     best = min(table, key=lambda r: r["ms"])
     return best, table
 
 
 def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(0, 1, 2),
                log=print):
-    """Best (variant, item size, workgroups per CU) for a batch plan of these problems."""
+    """Best (variant, item size, workgroups per CU) for a batch plan of these problems (numpy or
+    device arrays).  The variant and the workgroups per CU only shape the generic work list: a plan
+    with none (targets or sources all 8^3 blocks) is tuned over the item size alone."""
     import torch
     from .batch import BatchPlan, problems_from_workload
-    dev = problems_from_workload(problems)
+    dev = problems if _on_device(problems) else problems_from_workload(problems)
     lib = _lib.load()
     table = []
+    generic = any(p["targets"].get("block_origin") is None and p["sources"].get("block_origin") is None
+                  for p in problems)
     with GravityField(mode, validate=False, tune_cache=None) as f:
-        for v in variants(mode):
+        for v in (variants(mode) if generic else variants(mode)[:1]):
             for it in items:
-                for w in wg_per_cu:
+                for w in (wg_per_cu if generic else (0,)):
                     f.set_config(variant=v["variant"])
                     lib.tgrav_set_batch_config(f.handle, float(it), int(w))
                     plan = BatchPlan(f, dev)
@@ -169,6 +192,17 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
         lib.tgrav_set_batch_config(f.handle, 0.0, 0)
     best = min(table, key=lambda r: r["ms"])
     return best, table
+
+
+def _on_device(problems):
+    import torch
+    return any(torch.is_tensor(v) for p in problems for side in ("sources", "targets") for v in p[side].values())
+
+
+def c5_plan_problems():
+    """C5 as the bench runs it: stars -> the 8,192 leaf blocks, and the blocks' cells -> the stars."""
+    from .profile_driver import plan_problems
+    return plan_problems("c5")
 
 
 def blocks_key(mode, direction, ns, nblk):
@@ -241,7 +275,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--eval", nargs=2, type=int, action="append", default=[],
                     metavar=("NS", "NT"))
-    ap.add_argument("--batch", action="append", default=[], choices=["c4"])
+    ap.add_argument("--batch", action="append", default=[], choices=["c4", "c5"])
     ap.add_argument("--blocks", action="append", default=[], choices=["c2", "c5"])
     ap.add_argument("--modes", default="mixed,fp64")
     ap.add_argument("--cache", default=CACHE)
@@ -263,8 +297,12 @@ def main():
             save_cache(cache, args.cache)
         for name in args.batch:
             from . import workloads as W
-            probs = W.c4(1004)
-            best, _ = tune_batch(mode, probs, reps=max(2, args.reps // 2))
+            if name == "c4":
+                probs = W.c4(1004)
+                best, _ = tune_batch(mode, probs, reps=max(2, args.reps // 2))
+            else:
+                probs = c5_plan_problems()
+                best, _ = tune_batch(mode, probs, reps=2, items=(0.0, 2.5e7, 5e7, 1e8, 2e8, 4e8, 8e8))
             cache[batch_key(mode, probs)] = dict(variant=best["variant"], item=best["item"],
                                                  wg_per_cu=best["wg_per_cu"], ms=best["ms"],
                                                  k=best["k"], threads=best["threads"])

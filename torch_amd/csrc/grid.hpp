@@ -46,22 +46,39 @@ __device__ __forceinline__ void grid_tile_f64(const double4* __restrict__ s4, co
                                               double tx, double ty, const double (&tz)[8],
                                               double (&ax)[8], double (&ay)[8], double (&az)[8],
                                               double (&ap)[8]) {
-#pragma unroll 2
+  // The 8 cells' dependency chains are written step by step across the cells (each step for all 8
+  // before the next), so that independent FP64 operations sit next to each other: written cell by
+  // cell, the compiler keeps each chain of ~16 dependent FP64 operations in one piece (measured 2-4%
+  // slower).
+#pragma unroll 1
   for (int j = 0; j < 64; ++j) {
     const double4 s = s4[j];
     const double dx = s.x - tx, dy = s.y - ty;
     const double base = fma(dx, dx, fma(dy, dy, se2[j]));
+    double dz[8], r2[8], y[8], e[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const double dz = s.z - tz[k];
-      const double r2 = fma(dz, dz, base);
-      const double y = rsqrt_f64<FL>(r2);
-      const double mr = s.w * y;
+      dz[k] = s.z - tz[k];
+      r2[k] = fma(dz[k], dz[k], base);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (FL & F_SEED32) y[k] = (double)__builtin_amdgcn_rsqf((float)r2[k]);
+      else y[k] = __builtin_amdgcn_rsq(r2[k]);
+      if (FL & F_GUARD) y[k] = r2[k] > 0.0 ? y[k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = fma(-(r2[k] * y[k]), y[k], 1.0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) y[k] = fma(y[k] * e[k], fma(e[k], 0.375, 0.5), y[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double mr = s.w * y[k];
       if (!(FL & F_NOPOT)) ap[k] += mr;
-      const double w = mr * (y * y);
+      const double w = mr * (y[k] * y[k]);
       ax[k] = fma(w, dx, ax[k]);
       ay[k] = fma(w, dy, ay[k]);
-      az[k] = fma(w, dz, az[k]);
+      az[k] = fma(w, dz[k], az[k]);
     }
   }
 }
