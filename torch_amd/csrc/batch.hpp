@@ -83,14 +83,15 @@ __device__ __forceinline__ int find_segment(const int64_t* __restrict__ prefix, 
 // Packs every problem's sources into one {x,y,z,m} + eps^2 array and folds each problem's
 // source statistics (ST_*) into stats[ST_WORDS p ...].  Source blocks are expanded to their cell
 // centres (rounded as k_pack_source_blocks rounds them) and also give their cell-centre axes
-// (sax, 24 per block) and eps^2 (be2) for the block-source kernel.  A workgroup inside one
+// (sax, 24 per block), eps^2 (be2) and contiguous cell masses (bms, 512 per block) for the
+// block-source kernel.  A workgroup inside one
 // problem reduces first and issues one atomic per word; a workgroup straddling problems falls
 // back to atomics per lane.  Workgroups of 256 lanes.
 __global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __restrict__ srcs,
                                                     const int64_t* __restrict__ prefix,
                                                     double4* __restrict__ s4, double* __restrict__ se2,
                                                     double* __restrict__ sax, double* __restrict__ be2,
-                                                    unsigned long long* stats) {
+                                                    double* __restrict__ bms, unsigned long long* stats) {
   __shared__ int sp[2];
   const int64_t total = prefix[nprob];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,6 +120,7 @@ __global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __res
       if (ci == 0 && ck == 0) sax[gb * 24 + 8 + cj] = v.y;
       if (ci == 0 && cj == 0) sax[gb * 24 + 16 + ck] = v.z;
       if (c == 0) be2[gb] = e * e;
+      bms[gb * 512 + c] = v.w;
     } else {
       v = make_double4(S.x[j], S.y[j], S.z[j], S.m[j]);
       e = S.eps ? S.eps[j] : 0.0;
@@ -255,7 +257,7 @@ template <int K, int NT, bool MIXED>
 __global__ __launch_bounds__(NT) void k_batch_eval_gridsrc(const BProb* __restrict__ probs,
                                                            const BItem* __restrict__ items,
                                                            int n_items, unsigned int* counter,
-                                                           const double4* __restrict__ s4,
+                                                           const double* __restrict__ bms,
                                                            const double* __restrict__ sax,
                                                            const double* __restrict__ be2,
                                                            const unsigned long long* __restrict__ stats,
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval_gridsrc(const BProb* __restri
       o.part = nullptr;
     }
     gridsrc_tile<K, NT, MIXED>(L, P.T, (int64_t)w.tile * (K * NT), sax + P.sblk_off * 24,
-                               be2 + P.sblk_off, s4 + P.src_off, b0, b1, sst, o);
+                               be2 + P.sblk_off, bms + P.sblk_off * 512, b0, b1, sst, o);
   }
 }
 

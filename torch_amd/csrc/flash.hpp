@@ -380,7 +380,7 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
 template <int K, int NT>
 __global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double* __restrict__ sax,
                                                            const double* __restrict__ be2,
-                                                           const double4* __restrict__ bm,
+                                                           const double* __restrict__ bm,
                                                            int64_t nblk, int64_t chunk, Flash P,
                                                            const double* sst, Out O) {
   __shared__ LdsGS<false> L;

@@ -448,11 +448,11 @@ __device__ __forceinline__ void gs_block(const LdsGS<MIXED>& L,
 template <bool MIXED>
 __device__ __forceinline__ void gs_stage(LdsGS<MIXED>& L, const double* __restrict__ sax,
                                          const double* __restrict__ be2,
-                                         const double4* __restrict__ s4, int64_t blk, double e2u,
+                                         const double* __restrict__ bm, int64_t blk, double e2u,
                                          double ax_, double ay_, double az_) {
   typedef typename LdsGS<MIXED>::T T;
   const int tid = threadIdx.x;
-  for (int q = tid; q < 512; q += blockDim.x) L.m[q] = (T)s4[blk * 512 + q].w;
+  for (int q = tid; q < 512; q += blockDim.x) L.m[q] = (T)bm[blk * 512 + q];
   if (tid < 24) {
     const int axis = tid >> 3, idx = tid & 7;
     const double c = sax[blk * 24 + tid];
@@ -470,7 +470,7 @@ struct GsRun {
   typedef typename LdsGS<MIXED>::T T;
   template <int FL>
   static __device__ __forceinline__ void run(LdsGS<MIXED>& L, const double* sax, const double* be2,
-                                             const double4* bm, int64_t b0, int64_t b1, double e2u,
+                                             const double* bm, int64_t b0, int64_t b1, double e2u,
                                              double ax_, double ay_, double az_, const T (&tx)[K],
                                              const T (&ty)[K], const T (&tz)[K], const T (&te2)[K],
                                              double (&hx)[K], double (&hy)[K], double (&hz)[K],
@@ -496,14 +496,14 @@ struct GsRun {
 };
 
 // One target tile (K per lane, targets tile0 + k NT + lane) against source blocks [b0, b1):
-// sax / be2 / bm are the set's cell-centre axes, block eps^2 and packed cells (mass in .w); o
+// sax / be2 / bm are the set's cell-centre axes, block eps^2 and contiguous cell masses; o
 // already points at this block chunk's partials if the blocks are split.  Shared by
 // k_eval_gridsrc and the persistent batch kernel.
 template <int K, int NT, bool MIXED>
 __device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int64_t tile0,
                                              const double* __restrict__ sax,
                                              const double* __restrict__ be2,
-                                             const double4* __restrict__ bm, int64_t b0, int64_t b1,
+                                             const double* __restrict__ bm, int64_t b0, int64_t b1,
                                              const double* sst, const Out& o) {
   typedef typename LdsGS<MIXED>::T TT;
   const int tid = threadIdx.x;
@@ -561,7 +561,7 @@ __device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int6
 template <int K, int NT, bool MIXED>
 __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
                                                      const double* __restrict__ be2,
-                                                     const double4* __restrict__ bm, int64_t nblk,
+                                                     const double* __restrict__ bm, int64_t nblk,
                                                      int64_t chunk, const double* sst, Out O) {
   __shared__ LdsGS<MIXED> L;
   if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
@@ -574,15 +574,16 @@ __global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __rest
 }
 
 // Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them),
-// writes each block's eps^2 (be2) and cell-centre axes (sax, staged by the block-source
-// kernels) and folds the source statistics (ST_*) into stat; one lane per cell, workgroups of
-// 256 lanes.
+// writes each block's eps^2 (be2), cell-centre axes (sax) and the cell masses as one contiguous
+// array (sm; the three staged by the block-source kernels, which then read 8 B per cell instead
+// of a 32 B record) and folds the source statistics (ST_*) into stat; one lane per cell,
+// workgroups of 256 lanes.
 __global__ __launch_bounds__(256) void k_pack_source_blocks(int64_t nblk, const double* __restrict__ ox,
                                      const double* __restrict__ oy, const double* __restrict__ oz,
                                      const double* __restrict__ dx, const double* __restrict__ m,
                                      const double* __restrict__ eps, double4* __restrict__ s4,
                                      double* __restrict__ se2, double* __restrict__ be2,
-                                     double* __restrict__ sax,
+                                     double* __restrict__ sax, double* __restrict__ sm,
                                      unsigned long long* stat) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long e2min = 0x7ff0000000000000ULL, rmax = 0ULL, e2max = 0ULL, mmax = 0ULL;
@@ -601,6 +602,7 @@ __global__ __launch_bounds__(256) void k_pack_source_blocks(int64_t nblk, const 
     const double mi = m[i];
     s4[i] = make_double4(x, y, z, mi);
     se2[i] = e * e;
+    sm[i] = mi;
     const int ci = c >> 6, cj = (c >> 3) & 7, ck = c & 7;
     if (cj == 0 && ck == 0) sax[b * 24 + ci] = x;
     if (ci == 0 && ck == 0) sax[b * 24 + 8 + cj] = y;
@@ -615,11 +617,12 @@ __global__ __launch_bounds__(256) void k_pack_source_blocks(int64_t nblk, const 
 
 // Explicit sources that are 8^3 leaf blocks in C order (cell c = 64 i + 8 j + k of block b at
 // b * 512 + c): x depends on i only, y on j only, z on k only, and eps^2 is the block's.  Reads
-// the packed sources; writes the axis arrays sax (as k_pack_source_blocks does) and the block
-// eps^2, and clears flag[0] (set to 1 before) on any violation.  One thread per source.
+// the packed sources; writes the axis arrays sax, the block eps^2 and the contiguous cell masses
+// sm (as k_pack_source_blocks does), and clears flag[0] (set to 1 before) on any violation.  One
+// thread per source.
 __global__ void k_detect_source_blocks(int64_t nblk, const double4* __restrict__ s4,
                                        const double* __restrict__ se2, double* __restrict__ sax,
-                                       double* __restrict__ be2, int* flag) {
+                                       double* __restrict__ be2, double* __restrict__ sm, int* flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nblk * 512) return;
   const int64_t b = i >> 9, base = b * 512;
@@ -629,6 +632,7 @@ __global__ void k_detect_source_blocks(int64_t nblk, const double4* __restrict__
   const bool ok = s.x == s4[base + ci * 64].x && s.y == s4[base + cj * 8].y &&
                   s.z == s4[base + ck].z && se2[i] == se2[base];
   if (!ok) flag[0] = 0;
+  sm[i] = s.w;
   if (cj == 0 && ck == 0) sax[b * 24 + ci] = s.x;
   if (ci == 0 && ck == 0) sax[b * 24 + 8 + cj] = s.y;
   if (ci == 0 && cj == 0) sax[b * 24 + 16 + ck] = s.z;

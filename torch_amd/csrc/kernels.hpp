@@ -584,30 +584,45 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
   return (unsigned long long)__double_as_longlong(v);
 }
 
-// Folds one lane's source statistics (ST_* words) into `stat`: reduced over the workgroup of
-// 256 lanes through LDS, then one atomic per word.  Every lane of the workgroup must call it.
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int off) {
+  return (unsigned long long)__shfl_xor((long long)v, off);
+}
+
+// Folds one lane's source statistics (ST_* words) into `stat`: reduced within each wave by
+// shuffles, across the workgroup's (at most 4) waves through LDS, then at most one atomic per
+// word.  Every lane of the workgroup (<= 256 lanes) must call it.
 __device__ __forceinline__ void fold_source_stats(unsigned long long e2min, int bad, unsigned long long rmax,
                                                   unsigned long long e2max, unsigned long long mmax,
                                                   unsigned long long* stat) {
-  __shared__ unsigned long long sm[4][256];
-  __shared__ int sbad[256];
-  const int t = threadIdx.x;
-  sm[0][t] = e2min;
-  sm[1][t] = rmax;
-  sm[2][t] = e2max;
-  sm[3][t] = mmax;
-  sbad[t] = bad;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      sm[0][t] = min(sm[0][t], sm[0][t + s]);
-      sm[1][t] = max(sm[1][t], sm[1][t + s]);
-      sm[2][t] = max(sm[2][t], sm[2][t + s]);
-      sm[3][t] = max(sm[3][t], sm[3][t + s]);
-      sbad[t] |= sbad[t + s];
-    }
-    __syncthreads();
+  __shared__ unsigned long long sm[4][4];
+  __shared__ int sbad[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e2min = min(e2min, shfl_xor_u64(e2min, off));
+    rmax = max(rmax, shfl_xor_u64(rmax, off));
+    e2max = max(e2max, shfl_xor_u64(e2max, off));
+    mmax = max(mmax, shfl_xor_u64(mmax, off));
+    bad |= __shfl_xor(bad, off);
   }
+  const int t = threadIdx.x, w = t >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((t & 63) == 0) {
+    sm[0][w] = e2min;
+    sm[1][w] = rmax;
+    sm[2][w] = e2max;
+    sm[3][w] = mmax;
+    sbad[w] = bad;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < nw; ++q) {
+      sm[0][0] = min(sm[0][0], sm[0][q]);
+      sm[1][0] = max(sm[1][0], sm[1][q]);
+      sm[2][0] = max(sm[2][0], sm[2][q]);
+      sm[3][0] = max(sm[3][0], sm[3][q]);
+      sbad[0] |= sbad[q];
+    }
+  }
+  __syncthreads();
   // One lane per word; the atomic only where the workgroup's value improves on what the word
   // already holds (a plain read, possibly stale, which only costs an atomic): with thousands of
   // workgroups on the same five words, most then skip their atomics instead of queueing on them.

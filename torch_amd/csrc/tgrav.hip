@@ -325,7 +325,7 @@ struct tgrav_ctx {
   DevBuf s4, se2, stat;
   // sources given as 8^3 blocks: descriptors {ox, oy, oz, dx} and eps^2 per block
   int64_t src_nblk = 0;      // > 0: the current source set is block-structured
-  DevBuf sbe2, sax;          // source blocks: eps^2 and cell-centre axes (24 per block)
+  DevBuf sbe2, sax, sms;     // source blocks: eps^2, cell-centre axes (24 per block), cell masses
   int64_t src_cols = 0;      // > 0: explicit sources that may be 8^3 blocks (scolflag says)
   DevBuf scolflag;
   // staging for host pointers and scratch; flag[0] = NaN flag of the targets
@@ -652,11 +652,11 @@ static int gridsrc_kernel(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Ou
   const dim3 g((unsigned)ntiles, (unsigned)nsplit);
   if (mode == TGRAV_MIXED)
     tg::k_eval_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
-        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
+        T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk,
         c->stat.as<const double>(), o);
   else
     tg::k_eval_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
-        T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk,
+        T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk,
         c->stat.as<const double>(), o);
   LAUNCHCHECK("k_eval_gridsrc");
   return 0;
@@ -727,7 +727,7 @@ static int flash_gridsrc_kernel(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& o
   const int K = TG_FLASH_GS_K, NT = 256;
   const dim3 g((unsigned)ntiles, (unsigned)nsplit);
   tg::k_eval_flash_gridsrc<K, NT><<<g, NT, 0, c->stream>>>(
-      T, c->sax.as<double>(), c->sbe2.as<double>(), c->s4.as<double4>(), nblk, chunk, P,
+      T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk, P,
       c->stat.as<const double>(), o);
   LAUNCHCHECK("k_eval_flash_gridsrc");
   return 0;
@@ -1075,6 +1075,7 @@ int tgrav_destroy(tgrav_ctx* c) {
   c->flag.release();
   c->colflag.release();
   c->sbe2.release();
+  c->sms.release();
   c->sax.release();
   c->scolflag.release();
   c->bp4.release();
@@ -1191,13 +1192,14 @@ int tgrav_set_sources(tgrav_ctx* c, int64_t n, const double* x, const double* y,
   const int64_t nb8 = n / 512;
   if (n % 512 == 0) {   // C-ordered 8^3 blocks? decided on the device (k_detect_source_blocks)
     if ((rc = c->sax.ensure((size_t)nb8 * 24 * sizeof(double))) ||
-        (rc = c->sbe2.ensure((size_t)nb8 * sizeof(double))) || (rc = c->scolflag.ensure(sizeof(int))))
+        (rc = c->sbe2.ensure((size_t)nb8 * sizeof(double))) || (rc = c->sms.ensure((size_t)n * 8)) ||
+        (rc = c->scolflag.ensure(sizeof(int))))
       return rc;
     tg::k_set_flag<<<1, 64, 0, c->stream>>>(c->scolflag.as<int>(), 1);
     LAUNCHCHECK("k_set_flag");
     tg::k_detect_source_blocks<<<grid1(n, 256), 256, 0, c->stream>>>(
         nb8, c->s4.as<double4>(), c->se2.as<double>(), c->sax.as<double>(), c->sbe2.as<double>(),
-        c->scolflag.as<int>());
+        c->sms.as<double>(), c->scolflag.as<int>());
     LAUNCHCHECK("k_detect_source_blocks");
   }
   if (kind == TGRAV_DEVICE) {
@@ -1265,14 +1267,14 @@ int tgrav_set_source_blocks(tgrav_ctx* c, int64_t nblk, int nb, const double* ox
   }
   if ((rc = c->s4.ensure((size_t)n * sizeof(double4))) || (rc = c->se2.ensure((size_t)n * 8)) ||
       (rc = c->stat.ensure(tg::ST_WORDS * sizeof(unsigned long long))) ||
-      (rc = c->sbe2.ensure((size_t)nblk * 8)) ||
+      (rc = c->sbe2.ensure((size_t)nblk * 8)) || (rc = c->sms.ensure((size_t)n * 8)) ||
       (rc = c->sax.ensure((size_t)nblk * 24 * sizeof(double))))
     return rc;
   tg::k_reset_stats<<<1, 64, 0, c->stream>>>(c->stat.as<unsigned long long>());
   LAUNCHCHECK("k_reset_stats");
   tg::k_pack_source_blocks<<<grid1(n, 256), 256, 0, c->stream>>>(
       nblk, a[0], a[1], a[2], a[3], a[5], a[4], c->s4.as<double4>(), c->se2.as<double>(),
-      c->sbe2.as<double>(), c->sax.as<double>(), c->stat.as<unsigned long long>());
+      c->sbe2.as<double>(), c->sax.as<double>(), c->sms.as<double>(), c->stat.as<unsigned long long>());
   LAUNCHCHECK("k_pack_source_blocks");
   if (kind == TGRAV_HOST) HIPCHECK(hipStreamSynchronize(c->stream));
   c->ns = n;
@@ -2134,7 +2136,7 @@ struct tgrav_batch {
   const Variant* v = nullptr;
   int64_t total_src = 0, red_total = 0, cand_total = 0, total_sblk = 0;
   double inter = 0.0;
-  DevBuf probs, items, gitems, sitems, srcs, sprefix, s4, se2, sax, be2, stats, part, counter, ids,
+  DevBuf probs, items, gitems, sitems, srcs, sprefix, s4, se2, sax, be2, bms, stats, part, counter, ids,
       rprefix, cand, cprefix, colflag;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -2147,7 +2149,7 @@ static void batch_free(tgrav_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
   DevBuf* bufs[] = {&b->probs, &b->items, &b->gitems, &b->sitems, &b->srcs, &b->sprefix, &b->s4,
-                    &b->se2, &b->sax, &b->be2, &b->stats, &b->part, &b->counter, &b->ids,
+                    &b->se2, &b->sax, &b->be2, &b->bms, &b->stats, &b->part, &b->counter, &b->ids,
                     &b->rprefix, &b->cand, &b->cprefix, &b->colflag};
   for (DevBuf* d : bufs) d->release();
   delete b;
@@ -2173,7 +2175,8 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   if (b->total_src > 0) {
     tg::k_batch_pack<<<grid1(b->total_src, 256), 256, 0, s>>>(
         b->nprob, b->srcs.as<tg::BSrc>(), b->sprefix.as<int64_t>(), b->s4.as<double4>(),
-        b->se2.as<double>(), b->sax.as<double>(), b->be2.as<double>(), b->stats.as<unsigned long long>());
+        b->se2.as<double>(), b->sax.as<double>(), b->be2.as<double>(), b->bms.as<double>(),
+        b->stats.as<unsigned long long>());
     LAUNCHCHECK("k_batch_pack");
   }
   if (b->ncand > 0 && b->cand_total > 0) {
@@ -2216,12 +2219,12 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
       if (b->mode == TGRAV_MIXED)
         tg::k_batch_eval_gridsrc<K, NT, true><<<b->sgrid, NT, 0, s>>>(
             b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems,
-            b->counter.as<unsigned int>() + 2, b->s4.as<double4>(), b->sax.as<double>(),
+            b->counter.as<unsigned int>() + 2, b->bms.as<double>(), b->sax.as<double>(),
             b->be2.as<double>(), b->stats.as<unsigned long long>(), b->part.as<double>());
       else
         tg::k_batch_eval_gridsrc<K, NT, false><<<b->sgrid, NT, 0, s>>>(
             b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems,
-            b->counter.as<unsigned int>() + 2, b->s4.as<double4>(), b->sax.as<double>(),
+            b->counter.as<unsigned int>() + 2, b->bms.as<double>(), b->sax.as<double>(),
             b->be2.as<double>(), b->stats.as<unsigned long long>(), b->part.as<double>());
       LAUNCHCHECK("k_batch_eval_gridsrc");
       return 0;
@@ -2417,11 +2420,16 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       rprefix.push_back(rprefix.back() + q.n_targets);
     }
     const double per_unit = gs ? 512.0 : 1.0;
-    auto add = [&](std::vector<tg::BItem>& it, std::vector<double>& cs, int64_t tt) -> bool {
+    // Items in tile-major order (a tile's chunks next to each other), or chunk-major (all tiles
+    // of one source chunk next to each other, so that the workgroups running at the same time
+    // share the chunk's sources in L2: the block-source list, whose sources are the large side).
+    // The cost sort below is stable, so equal-cost items keep this order.
+    auto add = [&](std::vector<tg::BItem>& it, std::vector<double>& cs, int64_t tt, bool chunk_major) -> bool {
       const int64_t tiles = (q.n_targets + tt - 1) / tt;
       if (tiles > 0x7fffffffLL) return false;
-      for (int64_t t = 0; t < tiles; ++t)
-        for (int64_t k = 0; k < nch; ++k) {
+      for (int64_t a = 0; a < (chunk_major ? nch : tiles); ++a)
+        for (int64_t z = 0; z < (chunk_major ? tiles : nch); ++z) {
+          const int64_t t = chunk_major ? z : a, k = chunk_major ? a : z;
           const double nt_t = (double)std::min(tt, q.n_targets - t * tt);
           const double ns_k = per_unit * (double)std::min(ch, units - k * ch);
           it.push_back(tg::BItem{p, (int)t, (int)k, 0});
@@ -2430,9 +2438,9 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       return true;
     };
     bool ok = true;
-    if (gs) ok = add(sitems, scost, (int64_t)GK * GNT);
-    if (col) ok = ok && add(gitems, gcost, 512);
-    if (P.form == tg::FORM_GENERIC || P.form == tg::FORM_CAND) ok = ok && add(items, cost, T);
+    if (gs) ok = add(sitems, scost, (int64_t)GK * GNT, true);
+    if (col) ok = ok && add(gitems, gcost, 512, false);
+    if (P.form == tg::FORM_GENERIC || P.form == tg::FORM_CAND) ok = ok && add(items, cost, T, false);
     if (!ok) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
   }
   if (items.size() > 0x7fffffffULL || gitems.size() > 0x7fffffffULL || sitems.size() > 0x7fffffffULL) {
@@ -2471,6 +2479,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       (rc = b->se2.ensure(std::max<int64_t>(b->total_src, 1) * sizeof(double))) ||
       (rc = b->sax.ensure(std::max<int64_t>(sblk_total, 1) * 24 * sizeof(double))) ||
       (rc = b->be2.ensure(std::max<int64_t>(sblk_total, 1) * sizeof(double))) ||
+      (rc = b->bms.ensure(std::max<int64_t>(sblk_total, 1) * 512 * sizeof(double))) ||
       (rc = b->stats.ensure((size_t)std::max(n, 1) * tg::ST_WORDS * sizeof(unsigned long long))) ||
       (rc = b->colflag.ensure((size_t)std::max(n, 1) * sizeof(int))) ||
       (rc = b->part.ensure(std::max<int64_t>(part_total, 1) * sizeof(double))) ||
