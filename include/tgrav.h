@@ -234,6 +234,14 @@ int tgrav_bidirectional_blocks(tgrav_ctx* ctx, int64_t nblk, int nb, const doubl
  * recognised as blocks (tgrav_eval, tgrav_set_sources); on = 0 falls back to the generic
  * kernel (same results to rounding; for tests and comparisons).  Default on. */
 int tgrav_set_block_kernel(tgrav_ctx* ctx, int on);
+/* Waves per workgroup of the column kernels (1, 2 or 4, default 4; no counterpart in the reference, a
+ * launch-shape knob like tgrav_set_config): the W waves of a workgroup hold the same leaf block
+ * and split its sources, then add their partial sums in LDS in wave order, so the sources need
+ * W times fewer chunks (and partial sums in memory) for the same number of waves.  Results are
+ * deterministic for a given W and differ from another W's in rounding only.  Batch plans take
+ * the value the context has when they are created.  tgrav_get_column_waves returns it. */
+int tgrav_set_column_waves(tgrav_ctx* ctx, int waves);
+int tgrav_get_column_waves(tgrav_ctx* ctx);
 /* Fused-pass launch shape (variant, particle chunks; -1 keeps, 0 = automatic chunks). */
 int tgrav_set_bidir_config(tgrav_ctx* ctx, int mode, int variant, int nsplit);
 int tgrav_bidir_variant_info(int mode, int variant, int* targets_per_lane, int* threads);

@@ -1,0 +1,126 @@
+"""Column kernels with several waves per workgroup (tgrav_set_column_waves): the W waves of a
+workgroup hold the same 8^3 leaf block, split its sources in rounds of 64 W, and add their partial
+sums in LDS in wave order.
+
+For every W in (1, 2, 4): FP64 within 1e-12 of the FP64 oracle and mixed within the parity bar
+(<= 1e-6 median, <= 1e-4 max relative acceleration error), with source counts below one round
+(waves with nothing to do), ragged last rounds and source chunks split over workgroups.  For a
+fixed W the block form, the explicit C-order form, the fused kick and the unsplit batch plan give
+the same bits, and a repeated run is bit-identical."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import _util
+from tests.test_gpu_batch import cell_blocks, set_item, to_np
+from torch_amd import GravityField, StellarGravityField, TgravError
+from torch_amd import workloads as W
+from torch_amd.batch import BatchPlan, problems_from_workload
+
+pytestmark = pytest.mark.gpu
+G = W.G_PC_MSUN_KMS
+
+
+@pytest.fixture(scope="module")
+def c2():
+    return W.c2(1002)
+
+
+def check_bar(mode, s):
+    if mode == "fp64":
+        assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, s
+    else:
+        assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, s
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_column_waves_match_oracle(mode, waves, c2):
+    st, ce = c2["stars"], c2["cells"]
+    o, dx = ce["block_origin"][:24], ce["block_size"][:24] / 8
+    bx, by, bz = oracle.block_centres(o[:, 0], o[:, 1], o[:, 2], dx)
+    te = np.repeat(dx, 512)
+    with StellarGravityField(mode) as f:
+        f.set_column_waves(waves)
+        assert f.column_waves() == waves
+        for ns in (1, 100, 257, 1000):          # below one round of 64 W sources, ragged rounds
+            sl = slice(0, ns)
+            S = [st[k][sl] for k in ("x", "y", "z", "m", "eps")]
+            f.set_stars(*S)
+            ref = oracle.eval_plummer(*S, bx, by, bz, te, G, mode=oracle.NEUMAIER, threads=16)
+            for nsplit in (1, 3):
+                f.set_config(nsplit=nsplit)
+                got = f.evaluate_blocks(o, dx, eps=dx)
+                s = _util.summary(got, ref)
+                print(mode, "waves", waves, "ns", ns, "nsplit", nsplit, s, flush=True)
+                check_bar(mode, s)
+                again = f.evaluate_blocks(o, dx, eps=dx)
+                explicit = f.evaluate(te, bx, by, bz)          # detected as columns
+                for p, q, r in zip(got, again, explicit):
+                    assert np.array_equal(p, q)
+                    assert np.array_equal(p, r)
+                v = [np.zeros(len(bx)) for _ in range(3)]
+                f.evaluate_blocks(o, dx, eps=dx, dt=0.5, vel=v)
+                for p, q in zip(v, got[:3]):
+                    assert np.array_equal(p, 0.5 * q)
+            f.set_config(nsplit=0)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_column_waves_batch_equals_eager(mode):
+    """Four waves per column workgroup: block-target and explicit C-order problems in one plan,
+    unsplit, equal the eager column kernels bit for bit; split (default item sizes), the plan meets
+    the bar against the oracle and graph replays equal its eager run."""
+    import torch
+    rng = W.rng_for(73)
+    o, dx, (cx, cy, cz), cm = cell_blocks(rng, 11)
+    st = W.star_cluster(rng, 1500)
+    seps = np.full(1500, 0.02)
+    probs = [dict(sources=dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=seps),
+                  targets=dict(block_origin=o, block_dx=dx)),
+             dict(sources=dict(x=st["x"][:150], y=st["y"][:150], z=st["z"][:150], m=st["m"][:150]),
+                  targets=dict(x=cx, y=cy, z=cz))]
+    dev = problems_from_workload(probs)
+    with GravityField(mode) as f:
+        f.set_column_waves(4)
+        f.set_config(nsplit=1)
+        f.set_sources(st["x"], st["y"], st["z"], st["m"], seps)
+        e0 = f.evaluate_blocks(o, dx)
+        f.set_sources(st["x"][:150], st["y"][:150], st["z"][:150], st["m"][:150])
+        e1 = f.evaluate(None, cx, cy, cz)
+        set_item(f, 1e30)                       # one chunk per problem, as the eager launches
+        plan = BatchPlan(f, dev)
+        got = to_np(plan.run(graph=True))
+        torch.cuda.synchronize()
+        for q in range(4):
+            assert np.array_equal(got[0][q], e0[q]), q
+            assert np.array_equal(got[1][q], e1[q]), q
+        plan.close()
+        set_item(f, 0)
+        f.set_config(nsplit=0)
+        plan = BatchPlan(f, dev)
+        eager = to_np(plan.run(graph=False))
+        torch.cuda.synchronize()
+        for rep in range(2):
+            again = to_np(plan.run(graph=True))
+            torch.cuda.synchronize()
+            for i in range(2):
+                for q in range(4):
+                    assert np.array_equal(again[i][q], eager[i][q]), (rep, i, q)
+        plan.close()
+    refs = [oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], seps, cx, cy, cz, None, G,
+                                mode=oracle.NEUMAIER, threads=16),
+            oracle.eval_plummer(st["x"][:150], st["y"][:150], st["z"][:150], st["m"][:150], None, cx, cy, cz,
+                                None, G, mode=oracle.NEUMAIER, threads=16)]
+    for i in range(2):
+        s = _util.summary(tuple(eager[i]), refs[i])
+        print(mode, "batch waves 4 problem", i, s, flush=True)
+        check_bar(mode, s)
+
+
+def test_column_waves_argument_errors():
+    with GravityField("fp64") as f:
+        for bad in (0, 3, 8, -1):
+            with pytest.raises(TgravError):
+                f.set_column_waves(bad)
+        assert f.column_waves() == 4           # the default

@@ -66,6 +66,8 @@ SIGNATURES = [
     ("tgrav_bidirectional_blocks", _I32, [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P,
                                           _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32]),
     ("tgrav_set_block_kernel", _I32, [_P, _I32]),
+    ("tgrav_set_column_waves", _I32, [_P, _I32]),
+    ("tgrav_get_column_waves", _I32, [_P]),
     ("tgrav_set_softening", _I32, [_P, _I32, _D, _I32]),
     ("tgrav_set_periodic", _I32, [_P, _I32, _D, _D, _D, _I32]),
     ("tgrav_pm_deposit", _I32, [_P, _I64, _P, _P, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P, _P,

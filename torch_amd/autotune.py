@@ -64,11 +64,17 @@ def problem_sizes(p):
     return ns, nt
 
 
-def batch_key(mode, problems):
+def waves_suffix(waves):
+    """Key suffix of entries tuned with more than one wave per column workgroup (the library
+    looks such entries up under the same suffix)."""
+    return f"/w{int(waves)}" if int(waves) > 1 else ""
+
+
+def batch_key(mode, problems, waves=1):
     ns = [problem_sizes(p)[0] for p in problems]
     nt = [problem_sizes(p)[1] for p in problems]
     inter = sum(a * b for a, b in zip(ns, nt))
-    return (f"{device_key()}|batch|{mode}|p{len(problems)}|i{_bucket(inter)}"
+    return (f"{device_key()}|batch{waves_suffix(waves)}|{mode}|p{len(problems)}|i{_bucket(inter)}"
             f"|s{_bucket(np.median(ns))}|t{_bucket(np.median(nt))}")
 
 
@@ -176,6 +182,7 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
     generic = any(p["targets"].get("block_origin") is None and p["sources"].get("block_origin") is None
                   for p in problems)
     with GravityField(mode, validate=False, tune_cache=None) as f:
+        waves = f.column_waves()
         for v in (variants(mode) if generic else variants(mode)[:1]):
             for it in items:
                 for w in (wg_per_cu if generic else (0,)):
@@ -191,7 +198,7 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
                     torch.cuda.synchronize()
         lib.tgrav_set_batch_config(f.handle, 0.0, 0)
     best = min(table, key=lambda r: r["ms"])
-    return best, table
+    return dict(best, waves=waves), table
 
 
 def _on_device(problems):
@@ -205,8 +212,10 @@ def c5_plan_problems():
     return plan_problems("c5")
 
 
-def blocks_key(mode, direction, ns, nblk):
-    return f"{device_key()}|{direction}|{mode}|s{_bucket(ns)}|b{_bucket(nblk)}"
+def blocks_key(mode, direction, ns, nblk, waves=1):
+    """direction "blockfwd" (column kernel; keyed by its waves per workgroup) or "blocksrc"."""
+    suf = waves_suffix(waves) if direction == "blockfwd" else ""
+    return f"{device_key()}|{direction}{suf}|{mode}|s{_bucket(ns)}|b{_bucket(nblk)}"
 
 
 def tune_blocks(mode, workload="c2", reps=5, log=print):
@@ -229,7 +238,7 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
             ms = _time(lambda: f.evaluate_blocks(BO, BD), reps)
             table.append(dict(nsplit=sp, ms=ms))
             log(f"  {mode} block targets nsplit={sp}: {ms:.3f} ms")
-        out[blocks_key(mode, "blockfwd", ns, nblk)] = min(table, key=lambda r: r["ms"])
+        out[blocks_key(mode, "blockfwd", ns, nblk, f.column_waves())] = min(table, key=lambda r: r["ms"])
         f.set_source_blocks(BO, BD, CM)
         table = []
         for sp in (0, 16, 32, 64, 100, 128, 200, 256, 400):
@@ -244,7 +253,7 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
 def apply_blocks(field, direction, ns, nblk, cache=None):
     cache = load_cache() if cache is None else cache
     mode = {v: k for k, v in MODES.items() if k != "plain_fp64"}[field.mode]
-    e = cache.get(blocks_key(mode, direction, ns, nblk))
+    e = cache.get(blocks_key(mode, direction, ns, nblk, field.column_waves()))
     if e:
         field.set_config(nsplit=e["nsplit"])
     return e
@@ -263,7 +272,7 @@ def apply_eval(field, ns, nt, cache=None):
 def apply_batch(field, problems, cache=None):
     cache = load_cache() if cache is None else cache
     mode = {v: k for k, v in MODES.items() if k != "plain_fp64"}[field.mode]
-    e = cache.get(batch_key(mode, problems))
+    e = cache.get(batch_key(mode, problems, field.column_waves()))
     if e:
         field.set_config(variant=e["variant"])
         _lib.check(field.lib.tgrav_set_batch_config(field.handle, float(e["item"]),
@@ -302,11 +311,11 @@ def main():
                 best, _ = tune_batch(mode, probs, reps=max(2, args.reps // 2))
             else:
                 probs = c5_plan_problems()
-                best, _ = tune_batch(mode, probs, reps=2, items=(0.0, 2.5e7, 5e7, 1e8, 2e8, 4e8, 8e8))
-            cache[batch_key(mode, probs)] = dict(variant=best["variant"], item=best["item"],
-                                                 wg_per_cu=best["wg_per_cu"], ms=best["ms"],
-                                                 k=best["k"], threads=best["threads"])
-            print(f"best {mode} batch {name}: {cache[batch_key(mode, probs)]}", flush=True)
+                best, _ = tune_batch(mode, probs, reps=2, items=(0.0, 1e7, 2.5e7, 5e7, 1e8))
+            key = batch_key(mode, probs, best["waves"])
+            cache[key] = dict(variant=best["variant"], item=best["item"], wg_per_cu=best["wg_per_cu"],
+                              ms=best["ms"], k=best["k"], threads=best["threads"])
+            print(f"best {mode} batch {name}: {cache[key]}", flush=True)
             save_cache(cache, args.cache)
     print(f"tuned in {time.time() - t0:.1f} s -> {args.cache}", flush=True)
 

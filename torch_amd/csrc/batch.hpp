@@ -207,10 +207,11 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
 }
 
 // Problems whose targets are 8^3 leaf blocks, or explicit targets with their column structure
-// (found at plan creation): the same persistent scheme with 64-lane workgroups and one target
-// block (group of 512 targets) per work item, evaluated by the column code of k_eval_grid.
-template <bool MIXED>
-__global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict__ probs,
+// (found at plan creation): the same persistent scheme with workgroups of W waves and one
+// target block (group of 512 targets) per work item, evaluated by the column code of k_eval_grid
+// (the W waves split the item's sources and sum their partial results in LDS).
+template <bool MIXED, int W>
+__global__ __launch_bounds__(64 * W) void k_batch_eval_grid(const BProb* __restrict__ probs,
                                                         const BItem* __restrict__ items,
                                                         int n_items, unsigned int* counter,
                                                         const double4* __restrict__ s4,
@@ -218,8 +219,7 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
                                                         const unsigned long long* __restrict__ stats,
                                                         const int* __restrict__ colflag,
                                                         double* __restrict__ part) {
-  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
-  __shared__ Lds L;
+  __shared__ LdsCol<MIXED, W> L;
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) {   // items of problems whose targets are not in C order at this run are skipped
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(64) void k_batch_eval_grid(const BProb* __restrict_
     } else {
       o.part = nullptr;
     }
-    if (P.T.blocks) grid_block<MIXED>(L, P.T, w.tile, s4, se2, sb, se, sst, o);
-    else cols_group<MIXED>(L, P.T, w.tile, s4, se2, sb, se, sst, o);   // explicit columns
+    if (P.T.blocks) grid_block<MIXED, W>(L, P.T, w.tile, s4, se2, sb, se, sst, o);
+    else cols_group<MIXED, W>(L, P.T, w.tile, s4, se2, sb, se, sst, o);   // explicit columns
   }
 }
 

@@ -41,6 +41,18 @@ struct LdsGridF {
   float se2[NT];
 };
 
+// LDS of a column workgroup of W waves that share one target block and split its sources: the
+// source tile (64 sources per wave, staged by the whole workgroup) and, after the source loop, in
+// the same space, the waves' partial sums on their way to one total (32 FP64 values per lane).
+template <bool MIXED, int W>
+struct LdsCol {
+  typedef typename std::conditional<MIXED, LdsGridF<64 * W>, LdsGrid<64 * W>>::type Tile;
+  union {
+    Tile t;
+    double red[W > 1 ? 32 * 64 : 1];
+  };
+};
+
 template <int FL>
 __device__ __forceinline__ void grid_tile_f64(const double4* __restrict__ s4, const double* __restrict__ se2,
                                               double tx, double ty, const double (&tz)[8],
@@ -115,54 +127,59 @@ __device__ __forceinline__ void grid_tile_f32(const float4* __restrict__ s4, con
   }
 }
 
-template <int FL>
-__device__ __forceinline__ void grid_sources_f64(LdsGrid<64>& L, const double4* __restrict__ src4,
+// Sources [sb, se) in rounds of 64 W, staged by all 64 W lanes of the workgroup; wave v takes
+// sources base + 64 v ... + 63 of each round (a wave whose slice starts past se skips the round;
+// a partial slice is padded with massless sources, which add exactly zero).
+template <int FL, int W>
+__device__ __forceinline__ void grid_sources_f64(LdsGrid<64 * W>& L, const double4* __restrict__ src4,
                                                  const double* __restrict__ srce2, int64_t sb,
                                                  int64_t se, double e2t, double tx, double ty,
                                                  const double (&tz)[8], double (&ax)[8],
                                                  double (&ay)[8], double (&az)[8], double (&ap)[8]) {
-  const int lane = threadIdx.x;
-  for (int64_t base = sb; base < se; base += 64) {
+  const int tid = threadIdx.x, v = tid >> 6;
+  for (int64_t base = sb; base < se; base += 64 * W) {
     __syncthreads();
-    const int64_t j = base + lane;
+    const int64_t j = base + tid;
     if (j < se) {
-      L.s4[lane] = src4[j];
-      L.se2[lane] = srce2[j] + e2t;
+      L.s4[tid] = src4[j];
+      L.se2[tid] = srce2[j] + e2t;
     } else {
-      L.s4[lane] = make_double4(0.0, 0.0, 0.0, 0.0);
-      L.se2[lane] = 1.0;
+      L.s4[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+      L.se2[tid] = 1.0;
     }
     __syncthreads();
-    grid_tile_f64<FL>(L.s4, L.se2, tx, ty, tz, ax, ay, az, ap);
+    if (W == 1 || base + 64 * v < se)   // uniform over the wave
+      grid_tile_f64<FL>(L.s4 + 64 * v, L.se2 + 64 * v, tx, ty, tz, ax, ay, az, ap);
   }
 }
 
-template <int FL>
-__device__ __forceinline__ void grid_sources_f32(LdsGridF<64>& L, const double4* __restrict__ src4,
+template <int FL, int W>
+__device__ __forceinline__ void grid_sources_f32(LdsGridF<64 * W>& L, const double4* __restrict__ src4,
                                                  const double* __restrict__ srce2, int64_t sb,
                                                  int64_t se, double e2t, double cx, double cy,
                                                  double cz, float tx, float ty, const float (&tz)[8],
                                                  double (&hx)[8], double (&hy)[8], double (&hz)[8],
                                                  double (&hp)[8]) {
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, v = tid >> 6;
   float ax[8], ay[8], az[8], ap[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.f;
   int tiles = 0;
-  for (int64_t base = sb; base < se; base += 64) {
+  for (int64_t base = sb; base < se; base += 64 * W) {
     __syncthreads();
-    const int64_t j = base + lane;
+    const int64_t j = base + tid;
     if (j < se) {
       const double4 s = src4[j];
-      L.s4[lane] = make_float4((float)(s.x - cx), (float)(s.y - cy), (float)(s.z - cz), (float)s.w);
-      L.se2[lane] = (float)(srce2[j] + e2t);
+      L.s4[tid] = make_float4((float)(s.x - cx), (float)(s.y - cy), (float)(s.z - cz), (float)s.w);
+      L.se2[tid] = (float)(srce2[j] + e2t);
     } else {
-      L.s4[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-      L.se2[lane] = 1.0f;
+      L.s4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+      L.se2[tid] = 1.0f;
     }
     __syncthreads();
-    grid_tile_f32<FL>(L.s4, L.se2, tx, ty, tz, ax, ay, az, ap);
-    if (++tiles == TG_GRID_FOLD_TILES) {   // uniform: every lane counts the same tiles
+    if (W > 1 && base + 64 * v >= se) continue;   // uniform over the wave: nothing in its slice
+    grid_tile_f32<FL>(L.s4 + 64 * v, L.se2 + 64 * v, tx, ty, tz, ax, ay, az, ap);
+    if (++tiles == TG_GRID_FOLD_TILES) {   // uniform: every lane of the wave counts the same tiles
       tiles = 0;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -183,25 +200,27 @@ __device__ __forceinline__ void grid_sources_f32(LdsGridF<64>& L, const double4*
   }
 }
 
+template <int W>
 struct GridF64Run {
   template <int FL>
-  static __device__ __forceinline__ void run(LdsGrid<64>& L, const double4* src4, const double* srce2,
+  static __device__ __forceinline__ void run(LdsGrid<64 * W>& L, const double4* src4, const double* srce2,
                                              int64_t sb, int64_t se, double e2t, double tx,
                                              double ty, const double (&tz)[8], double (&ax)[8],
                                              double (&ay)[8], double (&az)[8], double (&ap)[8]) {
-    grid_sources_f64<FL | F_UNIF>(L, src4, srce2, sb, se, e2t, tx, ty, tz, ax, ay, az, ap);
+    grid_sources_f64<FL | F_UNIF, W>(L, src4, srce2, sb, se, e2t, tx, ty, tz, ax, ay, az, ap);
   }
 };
 
+template <int W>
 struct GridF32Run {
   template <int FL>
-  static __device__ __forceinline__ void run(LdsGridF<64>& L, const double4* src4, const double* srce2,
+  static __device__ __forceinline__ void run(LdsGridF<64 * W>& L, const double4* src4, const double* srce2,
                                              int64_t sb, int64_t se, double e2t, double cx,
                                              double cy, double cz, float tx, float ty,
                                              const float (&tz)[8], double (&hx)[8], double (&hy)[8],
                                              double (&hz)[8], double (&hp)[8]) {
-    grid_sources_f32<FL | F_UNIF>(L, src4, srce2, sb, se, e2t, cx, cy, cz, tx, ty, tz, hx, hy, hz,
-                                  hp);
+    grid_sources_f32<FL | F_UNIF, W>(L, src4, srce2, sb, se, e2t, cx, cy, cz, tx, ty, tz, hx, hy, hz,
+                                     hp);
   }
 };
 
@@ -215,14 +234,52 @@ struct GridF32Run {
     default: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                      \
   }
 
-// One wave of column targets against sources [sb, se): lane l owns 8 targets t0 + k that share
-// x, y and eps^2 (e2t, uniform over the wave) and differ in z[k]; (cx, cy, cz) is the FP64 anchor
-// of the mixed form (the centre of the wave's bounding box); targets t0 + k with k >= nk are not
-// stored.  `o` already points at this source chunk's partials if the sources are split.  sst: the
-// source statistics; rt: the largest |coordinate| of the wave's targets (FP64 seed choice).
-template <bool MIXED>
+// Sums the W waves' partial results (a, b, c, d: 8 cells each) of a column workgroup in wave
+// order, ((s_0 + s_1) + s_2) + ..., through L.red; returns true on wave W - 1, which then holds
+// the total.  Every wave of the workgroup must call it, after its source loop.
+template <bool MIXED, int W>
+__device__ __forceinline__ bool wave_sum(LdsCol<MIXED, W>& L, double (&a)[8], double (&b)[8],
+                                         double (&c)[8], double (&d)[8]) {
+  if constexpr (W == 1) {
+    return true;
+  } else {
+    const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+    __syncthreads();                       // the source tile (same space) is no longer read
+    for (int w = 0; w < W; ++w) {
+      if (v == w) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          double* r = L.red + k * 256 + lane;
+          if (w > 0) {
+            a[k] = r[0] + a[k];
+            b[k] = r[64] + b[k];
+            c[k] = r[128] + c[k];
+            d[k] = r[192] + d[k];
+          }
+          if (w < W - 1) {
+            r[0] = a[k];
+            r[64] = b[k];
+            r[128] = c[k];
+            r[192] = d[k];
+          }
+        }
+      }
+      if (w < W - 1) __syncthreads();
+    }
+    return v == W - 1;
+  }
+}
+
+// One target column group against sources [sb, se), by a workgroup of W waves that all hold the
+// same targets and split the sources (grid_sources_*): lane l (of every wave) owns 8 targets
+// t0 + k that share x, y and eps^2 (e2t, uniform over the wave) and differ in z[k]; (cx, cy, cz) is
+// the FP64 anchor of the mixed form (the centre of the group's bounding box); targets t0 + k with
+// k >= nk are not stored.  `o` already points at this source chunk's partials if the sources are
+// split over workgroups.  sst: the source statistics; rt: the largest |coordinate| of the targets
+// (FP64 seed choice).
+template <bool MIXED, int W>
 __device__ __forceinline__ void grid_columns(
-    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, double x, double y,
+    LdsCol<MIXED, W>& L, double x, double y,
     const double (&z)[8], double e2t, double cx, double cy, double cz, int64_t t0, int nk,
     const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
     const double* sst, double rt, const Out& o) {
@@ -235,33 +292,38 @@ __device__ __forceinline__ void grid_columns(
     double hx[8], hy[8], hz[8], hp[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
-    TG_DISPATCH_GRID(GridF32Run::run, fl & ~F_SEED32, L, src4, srce2, sb, se, e2t, cx, cy, cz, tx,
-                     ty, tz, hx, hy, hz, hp)
+    TG_DISPATCH_GRID(GridF32Run<W>::template run, fl & ~F_SEED32, L.t, src4, srce2, sb, se, e2t, cx,
+                     cy, cz, tx, ty, tz, hx, hy, hz, hp)
+    if (wave_sum<MIXED, W>(L, hx, hy, hz, hp)) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < nk) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
+      for (int k = 0; k < 8; ++k)
+        if (k < nk) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
+    }
   } else {
     double ax[8], ay[8], az[8], ap[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
-    TG_DISPATCH_GRID(GridF64Run::run, fl, L, src4, srce2, sb, se, e2t, x, y, z, ax, ay, az, ap)
+    TG_DISPATCH_GRID(GridF64Run<W>::template run, fl, L.t, src4, srce2, sb, se, e2t, x, y, z, ax, ay,
+                     az, ap)
+    if (wave_sum<MIXED, W>(L, ax, ay, az, ap)) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < nk) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
+      for (int k = 0; k < 8; ++k)
+        if (k < nk) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
+    }
   }
 }
 
-// One leaf block b of block-structured targets against sources [sb, se), by a 64-lane
-// workgroup (lane = cell column (i, j) = (l / 8, l % 8)).  Shared by k_eval_grid and the
+// One leaf block b of block-structured targets against sources [sb, se), by a workgroup of W
+// waves (lane l of each = cell column (i, j) = (l / 8, l % 8)).  Shared by k_eval_grid and the
 // persistent batch kernel.  The mixed anchor is the centre of the bounding box of the block's
 // cell centres, (c_0 + c_7) / 2 per axis, the same value the explicit-target form below
 // computes from the same centres.
-template <bool MIXED>
-__device__ __forceinline__ void grid_block(
-    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t b,
-    const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
-    const double* sst, const Out& o) {
-  const int lane = threadIdx.x;
+template <bool MIXED, int W>
+__device__ __forceinline__ void grid_block(LdsCol<MIXED, W>& L, const Tgt& T, int64_t b,
+                                           const double4* __restrict__ src4,
+                                           const double* __restrict__ srce2, int64_t sb, int64_t se,
+                                           const double* sst, const Out& o) {
+  const int lane = threadIdx.x & 63;
   const int i = lane >> 3, jj = lane & 7;
   const double d = T.bdx[b];
   const double e = T.beps ? T.beps[b] : 0.0;
@@ -280,36 +342,35 @@ __device__ __forceinline__ void grid_block(
   // every cell centre lies within [o, o + 8 d] on each axis
   const double rt = fmax(max_abs3(T.ox[b], T.oy[b], T.oz[b]),
                          max_abs3(T.ox[b] + 8.0 * d, T.oy[b] + 8.0 * d, T.oz[b] + 8.0 * d));
-  grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, b * 512 + (int64_t)lane * 8, 8, src4, srce2,
-                      sb, se, sst, rt, o);
+  grid_columns<MIXED, W>(L, x, y, z, e * e, cx, cy, cz, b * 512 + (int64_t)lane * 8, 8, src4, srce2,
+                         sb, se, sst, rt, o);
 }
 
 // blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
-// workgroup of 64 lanes.
-template <bool MIXED>
-__global__ __launch_bounds__(64) void k_eval_grid(Tgt T, const double4* __restrict__ src4,
-                                                  const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* sst, Out O) {
-  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
-  __shared__ Lds L;
+// workgroup of 64 W lanes.
+template <bool MIXED, int W>
+__global__ __launch_bounds__(64 * W) void k_eval_grid(Tgt T, const double4* __restrict__ src4,
+                                                      const double* __restrict__ srce2, int64_t ns,
+                                                      int64_t chunk, const double* sst, Out O) {
+  __shared__ LdsCol<MIXED, W> L;
   const int64_t sb = (int64_t)blockIdx.y * chunk;
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  grid_block<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, sst, o);
+  grid_block<MIXED, W>(L, T, blockIdx.x, src4, srce2, sb, se, sst, o);
 }
 
 // Explicit targets with column structure (every aligned group of 8 consecutive targets shares
 // x and y, every aligned group of 512 shares eps: the C order of 8^3 leaf-block meshgrids, as
 // checked by k_check_cols): the column code of k_eval_grid on explicit coordinates.  Group g =
-// targets 512 g ... 512 g + 511, one wave, lane l owns targets 512 g + 8 l ... + 7.  Shared by
-// k_eval_cols and the persistent batch kernel.
-template <bool MIXED>
-__device__ __forceinline__ void cols_group(
-    typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type& L, const Tgt& T, int64_t g,
-    const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
-    const double* sst, const Out& o) {
-  const int lane = threadIdx.x;
+// targets 512 g ... 512 g + 511, a workgroup of W waves, lane l of each owns targets
+// 512 g + 8 l ... + 7.  Shared by k_eval_cols and the persistent batch kernel.
+template <bool MIXED, int W>
+__device__ __forceinline__ void cols_group(LdsCol<MIXED, W>& L, const Tgt& T, int64_t g,
+                                           const double4* __restrict__ src4,
+                                           const double* __restrict__ srce2, int64_t sb, int64_t se,
+                                           const double* sst, const Out& o) {
+  const int lane = threadIdx.x & 63;
   const int64_t t0 = g * 512 + (int64_t)lane * 8;
   const int64_t last = T.n - 1;
   const int64_t tc = t0 < last ? t0 : last;
@@ -320,7 +381,7 @@ __device__ __forceinline__ void cols_group(
   const int64_t te = g * 512 < last ? g * 512 : last;
   const double e = T.eps ? T.eps[te] : 0.0;
   double cx = 0.0, cy = 0.0, cz = 0.0;
-  if constexpr (MIXED) {   // centre of the bounding box of the wave's targets
+  if constexpr (MIXED) {   // centre of the bounding box of the group's targets
     double mn[3] = {x, y, z[0]}, mx[3] = {x, y, z[0]};
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
@@ -339,7 +400,7 @@ __device__ __forceinline__ void cols_group(
     cy = 0.5 * (mn[1] + mx[1]);
     cz = 0.5 * (mn[2] + mx[2]);
   }
-  double rt = 0.0;   // the wave's largest |coordinate| (the FP64 seed choice)
+  double rt = 0.0;   // the group's largest |coordinate| (the FP64 seed choice)
   if constexpr (!MIXED) {
     rt = fmax(fabs(x), fabs(y));
 #pragma unroll
@@ -348,24 +409,23 @@ __device__ __forceinline__ void cols_group(
     for (int off = 32; off > 0; off >>= 1) rt = fmax(rt, __shfl_xor(rt, off));
   }
   const int64_t left = T.n - t0;
-  grid_columns<MIXED>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
-                      sb, se, sst, rt, o);
+  grid_columns<MIXED, W>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
+                         sb, se, sst, rt, o);
 }
 
 // blockIdx.x = group of 512 targets, blockIdx.y = source chunk.  Gated (Out::gate) against
 // the generic kernel on the check's flag.
-template <bool MIXED>
-__global__ __launch_bounds__(64) void k_eval_cols(Tgt T, const double4* __restrict__ src4,
-                                                  const double* __restrict__ srce2, int64_t ns,
-                                                  int64_t chunk, const double* sst, Out O) {
-  typedef typename std::conditional<MIXED, LdsGridF<64>, LdsGrid<64>>::type Lds;
-  __shared__ Lds L;
+template <bool MIXED, int W>
+__global__ __launch_bounds__(64 * W) void k_eval_cols(Tgt T, const double4* __restrict__ src4,
+                                                      const double* __restrict__ srce2, int64_t ns,
+                                                      int64_t chunk, const double* sst, Out O) {
+  __shared__ LdsCol<MIXED, W> L;
   if (gated_off(O)) return;
   const int64_t sb = (int64_t)blockIdx.y * chunk;
   const int64_t se = sb + chunk < ns ? sb + chunk : ns;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  cols_group<MIXED>(L, T, blockIdx.x, src4, srce2, sb, se, sst, o);
+  cols_group<MIXED, W>(L, T, blockIdx.x, src4, srce2, sb, se, sst, o);
 }
 
 // ---------------------------------------------------------------- block sources ----

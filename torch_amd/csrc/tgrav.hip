@@ -59,6 +59,9 @@ static int fail(int code, const char* fmt, ...) {
 // address hipGetSymbolAddress gave for the current device, kept per device in g_scratch_ptr.
 #define TG_STATIC_SCRATCH_BYTES (64ull << 20)
 #define TG_MAX_DEVICES 64
+#ifndef TG_COL_WAVES
+#define TG_COL_WAVES 4   // default waves per column workgroup (tgrav_set_column_waves)
+#endif
 static __device__ __align__(256) unsigned char scratch[TG_STATIC_SCRATCH_BYTES];
 static std::mutex g_scratch_mu;
 static void* g_scratch_ptr[TG_MAX_DEVICES] = {};
@@ -308,6 +311,7 @@ struct tgrav_ctx {
   int variant[2] = {kDefaultFp64, kDefaultMixed};
   int nsplit[2] = {0, 0};
   int block_kernel = 1;      // 8^3 block targets use the column kernel (k_eval_grid)
+  int col_waves = TG_COL_WAVES;   // waves per column workgroup (k_eval_grid / cols, batch: 1, 2, 4)
   int bvariant[2] = {1, 1};  // fused bidirectional kernel: variant and particle chunks
   int bnsplit[2] = {0, 0};
   // softening kernel: TGRAV_SOFT_* (FLASH kernels use the global radius rsoft), and the
@@ -353,10 +357,13 @@ static const char* mode_name(int mode) { return mode == TGRAV_MIXED ? "mixed" : 
 // Tuned source split of a column kernel ("blockfwd" / "blocksrc"), or -1.  The key's sizes are
 // the explicit point count (the sources of "blockfwd", the targets of "blocksrc") and the
 // number of leaf blocks, as the autotuner writes them.
-static int tuned_block_split(const tgrav_ctx* c, int mode, const char* dir, int64_t ns, int64_t nblk) {
+static int tuned_block_split(const tgrav_ctx* c, int mode, const char* dir, int64_t ns, int64_t nblk,
+                             int waves = 1) {
   if (c->user_cfg[mode] || c->tune.empty()) return -1;
-  char key[256];
-  snprintf(key, sizeof key, "%s|%s|%s|s%d|b%d", c->devkey.c_str(), dir, mode_name(mode),
+  char key[256], d[64];
+  if (waves > 1) snprintf(d, sizeof d, "%s/w%d", dir, waves);   // a split tuned for this workgroup
+  else snprintf(d, sizeof d, "%s", dir);
+  snprintf(key, sizeof key, "%s|%s|%s|s%d|b%d", c->devkey.c_str(), d, mode_name(mode),
            size_bucket((double)ns), size_bucket((double)nblk));
   const TuneEntry* t = tune_lookup(c, key);
   return t ? t->nsplit : -1;
@@ -563,22 +570,23 @@ static void choose_split(const tgrav_ctx* c, int mode, const Variant* v, int64_t
   if (*nsplit < 1) *nsplit = 1;
 }
 
-// 8^3 block targets: one 64-lane workgroup per leaf block (k_eval_grid), sources split over
-// blockIdx.y until there are about 16 waves per CU.
+// 8^3 block targets: one workgroup of W = col_waves waves per leaf block (k_eval_grid; the waves
+// split the workgroup's sources), sources split further over blockIdx.y.
 static void grid_shape(const tgrav_ctx* c, int mode, int64_t ns, int64_t nblk, int* nsplit,
                        int64_t* chunk) {
+  const int W = c->col_waves;
   int want = c->nsplit[mode];
-  const int tuned = tuned_block_split(c, mode, "blockfwd", ns, nblk);
+  const int tuned = tuned_block_split(c, mode, "blockfwd", ns, nblk, W);
   if (tuned > 0) want = tuned;
   if (want <= 0) {
-    // One-wave workgroups, ~3 resident per SIMD: aim at ~8 rounds of them so the last round's
-    // imbalance is small, with at most 12 chunks (each adds 32 B of partial sums per target)
-    // and at least 4 source tiles per chunk -- unless that leaves fewer workgroups than CUs
-    // (few blocks, few sources): then down to one source tile per chunk.
+    // ~3 waves resident per SIMD: aim at ~8 rounds of waves so the last round's imbalance is
+    // small, with at most 12 chunks (each adds 32 B of partial sums per target) and at least 4
+    // source tiles per wave and chunk -- unless that leaves fewer waves than CUs (few blocks, few
+    // sources): then down to one source tile per wave and chunk.
     const int64_t target = 8LL * 12 * c->num_cu;
-    const int64_t per = (target + nblk - 1) / nblk;
-    int64_t w = std::min<int64_t>(per, std::min<int64_t>(12, std::max<int64_t>(1, ns / 256)));
-    if (nblk * w < c->num_cu) w = std::min<int64_t>(per, std::max<int64_t>(1, ns / 64));
+    const int64_t per = (target + nblk * W - 1) / (nblk * W);
+    int64_t w = std::min<int64_t>(per, std::min<int64_t>(12, std::max<int64_t>(1, ns / (256 * W))));
+    if (nblk * W * w < c->num_cu) w = std::min<int64_t>(per, std::max<int64_t>(1, ns / (64 * W)));
     want = (int)std::min<int64_t>(w, 65535);
   }
   if (want > 65535) want = 65535;
@@ -586,6 +594,40 @@ static void grid_shape(const tgrav_ctx* c, int mode, int64_t ns, int64_t nblk, i
   ch = (ch + 63) / 64 * 64;
   *chunk = ch;
   *nsplit = (int)((ns + ch - 1) / ch);
+}
+
+// The column kernels for the context's waves per workgroup: k_eval_grid (block targets) or
+// k_eval_cols (explicit targets in the blocks' C order), grid (groups, source chunks).
+template <bool MIXED, int W>
+static void column_kernel_w(tgrav_ctx* c, bool blocks, dim3 g, const tg::Tgt& T, int64_t ns,
+                            int64_t chunk, const tg::Out& o) {
+  if (blocks)
+    tg::k_eval_grid<MIXED, W><<<g, 64 * W, 0, c->stream>>>(T, c->s4.as<double4>(), c->se2.as<double>(), ns,
+                                                            chunk, c->stat.as<const double>(), o);
+  else
+    tg::k_eval_cols<MIXED, W><<<g, 64 * W, 0, c->stream>>>(T, c->s4.as<double4>(), c->se2.as<double>(), ns,
+                                                            chunk, c->stat.as<const double>(), o);
+}
+
+static int column_kernel(tgrav_ctx* c, int mode, bool blocks, int64_t ngrp, int nsplit,
+                         const tg::Tgt& T, int64_t ns, int64_t chunk, const tg::Out& o) {
+  const dim3 g((unsigned)ngrp, (unsigned)nsplit);
+  const bool mixed = mode == TGRAV_MIXED;
+  switch (c->col_waves) {
+    case 4:
+      if (mixed) column_kernel_w<true, 4>(c, blocks, g, T, ns, chunk, o);
+      else column_kernel_w<false, 4>(c, blocks, g, T, ns, chunk, o);
+      break;
+    case 2:
+      if (mixed) column_kernel_w<true, 2>(c, blocks, g, T, ns, chunk, o);
+      else column_kernel_w<false, 2>(c, blocks, g, T, ns, chunk, o);
+      break;
+    default:
+      if (mixed) column_kernel_w<true, 1>(c, blocks, g, T, ns, chunk, o);
+      else column_kernel_w<false, 1>(c, blocks, g, T, ns, chunk, o);
+  }
+  LAUNCHCHECK(blocks ? "k_eval_grid" : "k_eval_cols");
+  return 0;
 }
 
 static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& O) {
@@ -604,14 +646,7 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
     o.part = nullptr;
   }
   int rc = timed_launch(c, nt, ns, [&]() -> int {
-    if (mode == TGRAV_MIXED)
-      tg::k_eval_grid<true><<<dim3((unsigned)nblk, (unsigned)nsplit), 64, 0, c->stream>>>(
-          T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
-    else
-      tg::k_eval_grid<false><<<dim3((unsigned)nblk, (unsigned)nsplit), 64, 0, c->stream>>>(
-          T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
-    LAUNCHCHECK("k_eval_grid");
-    return 0;
+    return column_kernel(c, mode, true, nblk, nsplit, T, ns, chunk, o);
   });
   if (rc) return rc;
   if (nsplit > 1) {
@@ -961,13 +996,8 @@ static int launch_eval(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
       if (r) return r;
     }
     if (cols) {
-      if (mode == TGRAV_MIXED)
-        tg::k_eval_cols<true><<<dim3((unsigned)ngrp, (unsigned)nsplit_c), 64, 0, c->stream>>>(
-            T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk_c, c->stat.as<const double>(), oc);
-      else
-        tg::k_eval_cols<false><<<dim3((unsigned)ngrp, (unsigned)nsplit_c), 64, 0, c->stream>>>(
-            T, c->s4.as<double4>(), c->se2.as<double>(), ns, chunk_c, c->stat.as<const double>(), oc);
-      LAUNCHCHECK("k_eval_cols");
+      int r = column_kernel(c, mode, false, ngrp, nsplit_c, T, ns, chunk_c, oc);
+      if (r) return r;
     }
     v->fn(dim3((unsigned)ntiles, (unsigned)nsplit), dim3(v->nt), c->stream, T, c->s4.as<double4>(),
           c->se2.as<double>(), ns, chunk, c->stat.as<const double>(), o);
@@ -2089,6 +2119,19 @@ int tgrav_set_block_kernel(tgrav_ctx* c, int on) {
   return 0;
 }
 
+int tgrav_set_column_waves(tgrav_ctx* c, int waves) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  if (waves != 1 && waves != 2 && waves != 4)
+    return fail(TGRAV_EINVAL, "waves per column workgroup must be 1, 2 or 4 (got %d)", waves);
+  c->col_waves = waves;
+  return 0;
+}
+
+int tgrav_get_column_waves(tgrav_ctx* c) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  return c->col_waves;
+}
+
 int tgrav_set_timing(tgrav_ctx* c, int on) {
   if (!c) return fail(TGRAV_EINVAL, "context is NULL");
   c->timing = on ? 1 : 0;
@@ -2131,6 +2174,7 @@ struct tgrav_batch {
   tgrav_ctx* c = nullptr;
   int mode = 0, nprob = 0, n_items = 0, nred = 0, grid = 0;
   int n_gitems = 0, ggrid = 0;   // column work list (block targets, explicit targets in C order)
+  int waves = 1;                 // waves per column workgroup (the context's, at creation)
   int n_sitems = 0, sgrid = 0;   // block-source work list (k_eval_gridsrc's column walk)
   int ncand = 0;                 // problems whose column structure is tested at every run
   const Variant* v = nullptr;
@@ -2158,6 +2202,21 @@ static void batch_free(tgrav_batch* b) {
 // Block-source work items: K targets per lane, NT lanes (the eager kernel's shape).
 #define TG_BATCH_GS_K 4
 #define TG_BATCH_GS_NT 256
+
+// The persistent column kernel for W waves per workgroup.
+static const void* batch_grid_kernel(bool mixed, int W) {
+  if (W == 4) return mixed ? (const void*)tg::k_batch_eval_grid<true, 4> : (const void*)tg::k_batch_eval_grid<false, 4>;
+  if (W == 2) return mixed ? (const void*)tg::k_batch_eval_grid<true, 2> : (const void*)tg::k_batch_eval_grid<false, 2>;
+  return mixed ? (const void*)tg::k_batch_eval_grid<true, 1> : (const void*)tg::k_batch_eval_grid<false, 1>;
+}
+
+template <bool MIXED, int W>
+static void batch_grid_launch(tgrav_batch* b, hipStream_t s) {
+  tg::k_batch_eval_grid<MIXED, W><<<b->ggrid, 64 * W, 0, s>>>(
+      b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems, b->counter.as<unsigned int>() + 1,
+      b->s4.as<double4>(), b->se2.as<double>(), b->stats.as<unsigned long long>(), b->colflag.as<int>(),
+      b->part.as<double>());
+}
 
 template <class T>
 static int upload(DevBuf& d, const std::vector<T>& v) {
@@ -2197,16 +2256,12 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   }
   if (b->n_gitems > 0) {
     auto go = [&]() -> int {
-      if (b->mode == TGRAV_MIXED)
-        tg::k_batch_eval_grid<true><<<b->ggrid, 64, 0, s>>>(
-            b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems,
-            b->counter.as<unsigned int>() + 1, b->s4.as<double4>(), b->se2.as<double>(),
-            b->stats.as<unsigned long long>(), b->colflag.as<int>(), b->part.as<double>());
-      else
-        tg::k_batch_eval_grid<false><<<b->ggrid, 64, 0, s>>>(
-            b->probs.as<tg::BProb>(), b->gitems.as<tg::BItem>(), b->n_gitems,
-            b->counter.as<unsigned int>() + 1, b->s4.as<double4>(), b->se2.as<double>(),
-            b->stats.as<unsigned long long>(), b->colflag.as<int>(), b->part.as<double>());
+      const bool mixed = b->mode == TGRAV_MIXED;
+      switch (b->waves) {
+        case 4: mixed ? batch_grid_launch<true, 4>(b, s) : batch_grid_launch<false, 4>(b, s); break;
+        case 2: mixed ? batch_grid_launch<true, 2>(b, s) : batch_grid_launch<false, 2>(b, s); break;
+        default: mixed ? batch_grid_launch<true, 1>(b, s) : batch_grid_launch<false, 1>(b, s);
+      }
       LAUNCHCHECK("k_batch_eval_grid");
       return 0;
     };
@@ -2285,6 +2340,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   tgrav_batch* b = new tgrav_batch();
   b->c = c;
   b->mode = mode;
+  b->waves = c->col_waves;
   b->nprob = n;
   b->v = variant_of(mode, c->variant[mode]);
   double total = 0.0;
@@ -2302,7 +2358,11 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       return v.size() % 2 ? v[m] : 0.5 * (v[m - 1] + v[m]);
     };
     char key[256];
-    snprintf(key, sizeof key, "%s|batch|%s|p%d|i%d|s%d|t%d", c->devkey.c_str(), mode_name(mode), n,
+    // plans tuned for more than one wave per column workgroup carry it in the key
+    char bk[32];
+    if (c->col_waves > 1) snprintf(bk, sizeof bk, "batch/w%d", c->col_waves);
+    else snprintf(bk, sizeof bk, "batch");
+    snprintf(key, sizeof key, "%s|%s|%s|p%d|i%d|s%d|t%d", c->devkey.c_str(), bk, mode_name(mode), n,
              size_bucket(total), size_bucket(median(vs)), size_bucket(median(vt)));
     const TuneEntry* t = tune_lookup(c, key);
     if (t) {
@@ -2324,8 +2384,8 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   };
   const int GK = TG_BATCH_GS_K, GNT = TG_BATCH_GS_NT;
   const int slots = occupancy(b->v->bkernel(), b->v->nt, wg_per_cu);
-  const int gslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_grid<true>
-                                                   : (const void*)tg::k_batch_eval_grid<false>, 64, 0);
+  const int CW = c->col_waves;
+  const int gslots = occupancy(batch_grid_kernel(mode == TGRAV_MIXED, CW), 64 * CW, 0);
   const int sslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_gridsrc<GK, GNT, true>
                                                    : (const void*)tg::k_batch_eval_gridsrc<GK, GNT, false>, GNT, 0);
   // Form of each problem's work items.
@@ -2349,7 +2409,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const int sl = li == 0 ? slots : li == 1 ? gslots : sslots;
     return std::min(C, std::max(floor_cost, tot_list[li] / (8.0 * sl)));
   };
-  const double C_gen = list_cost(0, (double)T * NT * 4.0), C_col = list_cost(1, 512.0 * 64 * 4),
+  const double C_gen = list_cost(0, (double)T * NT * 4.0), C_col = list_cost(1, 512.0 * 64 * 4 * CW),
                C_gs = list_cost(2, (double)GK * GNT * 512.0 * 4);
   std::vector<tg::BProb> probs(n);
   std::vector<tg::BSrc> srcs(n);

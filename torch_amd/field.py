@@ -392,6 +392,15 @@ class GravityField:
         and explicit arrays recognised as leaf blocks in C order."""
         _lib.check(self.lib.tgrav_set_block_kernel(self._ctx, int(bool(on))), "tgrav_set_block_kernel")
 
+    def set_column_waves(self, waves):
+        """Waves per workgroup of the 8^3 column kernels (1, 2 or 4; default 4): the waves of a workgroup
+        share one leaf block and split its sources, so fewer source chunks (and partial sums in
+        memory) give the same parallelism.  Batch plans use the value set when they are created."""
+        _lib.check(self.lib.tgrav_set_column_waves(self._ctx, int(waves)), "tgrav_set_column_waves")
+
+    def column_waves(self):
+        return int(self.lib.tgrav_get_column_waves(self._ctx))
+
     def set_bidir_config(self, variant=-1, nsplit=-1):
         _lib.check(self.lib.tgrav_set_bidir_config(self._ctx, self.mode, int(variant), int(nsplit)),
                    "tgrav_set_bidir_config")
