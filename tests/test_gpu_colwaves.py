@@ -1,6 +1,7 @@
-"""Column kernels with several waves per workgroup (tgrav_set_column_waves): the W waves of a
-workgroup hold the same 8^3 leaf block, split its sources in rounds of 64 W, and add their partial
-sums in LDS in wave order.
+"""Block kernels with several waves per workgroup (tgrav_set_column_waves).  Column kernel (block
+targets): the W waves of a workgroup hold the same 8^3 leaf block, split its sources in rounds of
+64 W, and add their partial sums in LDS in wave order.  Block-source kernel: the W waves hold the
+same 256 targets and split the source blocks (wave v takes every W-th block).
 
 For every W in (1, 2, 4): FP64 within 1e-12 of the FP64 oracle and mixed within the parity bar
 (<= 1e-6 median, <= 1e-4 max relative acceleration error), with source counts below one round
@@ -13,7 +14,7 @@ import pytest
 import oracle
 from tests import _util
 from tests.test_gpu_batch import cell_blocks, set_item, to_np
-from torch_amd import GravityField, StellarGravityField, TgravError
+from torch_amd import GasGravityField, GravityField, StellarGravityField, TgravError
 from torch_amd import workloads as W
 from torch_amd.batch import BatchPlan, problems_from_workload
 
@@ -116,6 +117,36 @@ def test_column_waves_batch_equals_eager(mode):
         s = _util.summary(tuple(eager[i]), refs[i])
         print(mode, "batch waves 4 problem", i, s, flush=True)
         check_bar(mode, s)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_block_source_waves_match_oracle(mode, waves, c2):
+    """13 source blocks (ragged rounds of W blocks, chunks of 1, 2 and 5 blocks) onto 700 stars
+    (a partial last tile of 256): the oracle's field to the bar; repeated runs and explicit
+    cells in the blocks' C order (detected as blocks) give the same bits."""
+    st, ce = c2["stars"], c2["cells"]
+    nb = 13
+    sl = slice(0, nb * 512)
+    o, dxb = ce["block_origin"][:nb], ce["block_size"][:nb] / 8
+    tx, ty, tz = st["x"][:700], st["y"][:700], st["z"][:700]
+    ref = oracle.eval_plummer(ce["x"][sl], ce["y"][sl], ce["z"][sl], ce["m"][sl], ce["eps"][sl], tx, ty, tz,
+                              None, G, mode=oracle.NEUMAIER, threads=16)
+    with GasGravityField(mode) as g:
+        g.set_column_waves(waves)
+        for nsplit in (13, 7, 3, 1):            # chunks of 1, 2, 5 and 13 blocks
+            g.set_config(nsplit=nsplit)
+            g.set_blocks(o, dxb, ce["m"][sl])
+            got = g.evaluate(None, tx, ty, tz)
+            s = _util.summary(got, ref)
+            print(mode, "block sources waves", waves, "nsplit", nsplit, s, flush=True)
+            check_bar(mode, s)
+            again = g.evaluate(None, tx, ty, tz)
+            g.set_sources(ce["x"][sl], ce["y"][sl], ce["z"][sl], ce["m"][sl], ce["eps"][sl])
+            detected = g.evaluate(None, tx, ty, tz)
+            for p, q, r in zip(got, again, detected):
+                assert np.array_equal(p, q)
+                assert np.array_equal(p, r)
 
 
 def test_column_waves_argument_errors():

@@ -647,8 +647,9 @@ def test_library_applies_the_tune_cache(c2):
     import json
     from torch_amd import autotune
     cache = os.path.join(HERE, "..", "torch_amd", "tune_cache.json")
-    with GravityField("fp64") as f0:          # the column kernel's entries are keyed by its waves
-        fwd = f"|blockfwd{autotune.waves_suffix(f0.column_waves())}|fp64|s13|b11"
+    with GravityField("fp64") as f0:          # the block kernels' entries are keyed by their waves
+        suf = autotune.waves_suffix(f0.column_waves())
+    fwd, src = f"|blockfwd{suf}|fp64|s13|b11", f"|blocksrc{suf}|fp64|s13|b11"
     tuned = [v["nsplit"] for k, v in json.load(open(cache)).items() if fwd in k][0]
     other = 5 if tuned != 5 else 7
     st, ce = c2["stars"], c2["cells"]
@@ -665,7 +666,7 @@ def test_library_applies_the_tune_cache(c2):
         np.testing.assert_array_equal(res[2][q], res[3][q])          # explicit split wins over the cache
     assert any(not np.array_equal(res[0][q], res[2][q]) for q in range(4))
     # block sources: the entry is keyed by the target (star) count and the block count
-    tuned_s = [v["nsplit"] for k, v in json.load(open(cache)).items() if "|blocksrc|fp64|s13|b11" in k][0]
+    tuned_s = [v["nsplit"] for k, v in json.load(open(cache)).items() if src in k][0]
     res = []
     for tune, split in ((cache, None), (None, tuned_s), (None, 64)):
         with GasGravityField("fp64", tune_cache=tune) as g:

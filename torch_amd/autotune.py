@@ -213,9 +213,8 @@ def c5_plan_problems():
 
 
 def blocks_key(mode, direction, ns, nblk, waves=1):
-    """direction "blockfwd" (column kernel; keyed by its waves per workgroup) or "blocksrc"."""
-    suf = waves_suffix(waves) if direction == "blockfwd" else ""
-    return f"{device_key()}|{direction}{suf}|{mode}|s{_bucket(ns)}|b{_bucket(nblk)}"
+    """direction "blockfwd" or "blocksrc", keyed by the kernels' waves per workgroup."""
+    return f"{device_key()}|{direction}{waves_suffix(waves)}|{mode}|s{_bucket(ns)}|b{_bucket(nblk)}"
 
 
 def tune_blocks(mode, workload="c2", reps=5, log=print):
@@ -241,12 +240,12 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
         out[blocks_key(mode, "blockfwd", ns, nblk, f.column_waves())] = min(table, key=lambda r: r["ms"])
         f.set_source_blocks(BO, BD, CM)
         table = []
-        for sp in (0, 16, 32, 64, 100, 128, 200, 256, 400):
+        for sp in (0, 8, 16, 24, 32, 48, 64, 100, 128, 200, 256):
             f.set_config(nsplit=sp)
             ms = _time(lambda: f.evaluate(None, S[0], S[1], S[2]), reps)
             table.append(dict(nsplit=sp, ms=ms))
             log(f"  {mode} block sources nsplit={sp}: {ms:.3f} ms")
-        out[blocks_key(mode, "blocksrc", ns, nblk)] = min(table, key=lambda r: r["ms"])
+        out[blocks_key(mode, "blocksrc", ns, nblk, f.column_waves())] = min(table, key=lambda r: r["ms"])
     return out
 
 

@@ -251,10 +251,10 @@ __global__ __launch_bounds__(64 * W) void k_batch_eval_grid(const BProb* __restr
 }
 
 // Problems whose sources are 8^3 leaf blocks: the same persistent scheme over (target tile of
-// K NT, chunk of source blocks) items, evaluated by the column walk of k_eval_gridsrc
-// (the per-block source loop of Gravity_getAccelAtPoint.F90:168-204).
-template <int K, int NT, bool MIXED>
-__global__ __launch_bounds__(NT) void k_batch_eval_gridsrc(const BProb* __restrict__ probs,
+// K x 64, chunk of source blocks) items, evaluated by the column walk of k_eval_gridsrc (the W
+// waves split the chunk's blocks; the per-block source loop of Gravity_getAccelAtPoint.F90:168-204).
+template <int K, int W, bool MIXED>
+__global__ __launch_bounds__(64 * W) void k_batch_eval_gridsrc(const BProb* __restrict__ probs,
                                                            const BItem* __restrict__ items,
                                                            int n_items, unsigned int* counter,
                                                            const double* __restrict__ bms,
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval_gridsrc(const BProb* __restri
                                                            const double* __restrict__ be2,
                                                            const unsigned long long* __restrict__ stats,
                                                            double* __restrict__ part) {
-  __shared__ LdsGS<MIXED> L;
+  __shared__ LdsGSW<MIXED, W> L;
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval_gridsrc(const BProb* __restri
     } else {
       o.part = nullptr;
     }
-    gridsrc_tile<K, NT, MIXED>(L, P.T, (int64_t)w.tile * (K * NT), sax + P.sblk_off * 24,
+    gridsrc_tile<K, W, MIXED>(L, P.T, (int64_t)w.tile * (K * 64), sax + P.sblk_off * 24,
                                be2 + P.sblk_off, bms + P.sblk_off * 512, b0, b1, sst, o);
   }
 }

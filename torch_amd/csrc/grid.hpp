@@ -234,22 +234,23 @@ struct GridF32Run {
     default: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                      \
   }
 
-// Sums the W waves' partial results (a, b, c, d: 8 cells each) of a column workgroup in wave
-// order, ((s_0 + s_1) + s_2) + ..., through L.red; returns true on wave W - 1, which then holds
-// the total.  Every wave of the workgroup must call it, after its source loop.
-template <bool MIXED, int W>
-__device__ __forceinline__ bool wave_sum(LdsCol<MIXED, W>& L, double (&a)[8], double (&b)[8],
-                                         double (&c)[8], double (&d)[8]) {
+// Sums the W waves' partial results (a, b, c, d: N values per lane each) of a workgroup whose
+// waves split one sum, in wave order, ((s_0 + s_1) + s_2) + ..., through red (4 N 64 doubles of
+// LDS); returns true on wave W - 1, which then holds the total.  Every wave of the workgroup must
+// call it, after its own loop.
+template <int W, int N>
+__device__ __forceinline__ bool wave_sum(double* red, double (&a)[N], double (&b)[N], double (&c)[N],
+                                         double (&d)[N]) {
   if constexpr (W == 1) {
     return true;
   } else {
     const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
-    __syncthreads();                       // the source tile (same space) is no longer read
+    __syncthreads();                       // the loop's LDS (possibly the same space) is no longer read
     for (int w = 0; w < W; ++w) {
       if (v == w) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          double* r = L.red + k * 256 + lane;
+        for (int k = 0; k < N; ++k) {
+          double* r = red + k * 256 + lane;
           if (w > 0) {
             a[k] = r[0] + a[k];
             b[k] = r[64] + b[k];
@@ -294,7 +295,7 @@ __device__ __forceinline__ void grid_columns(
     for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
     TG_DISPATCH_GRID(GridF32Run<W>::template run, fl & ~F_SEED32, L.t, src4, srce2, sb, se, e2t, cx,
                      cy, cz, tx, ty, tz, hx, hy, hz, hp)
-    if (wave_sum<MIXED, W>(L, hx, hy, hz, hp)) {
+    if (wave_sum<W, 8>(L.red, hx, hy, hz, hp)) {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < nk) store_result(o, t0 + k, hx[k], hy[k], hz[k], hp[k]);
@@ -305,7 +306,7 @@ __device__ __forceinline__ void grid_columns(
     for (int k = 0; k < 8; ++k) ax[k] = ay[k] = az[k] = ap[k] = 0.0;
     TG_DISPATCH_GRID(GridF64Run<W>::template run, fl, L.t, src4, srce2, sb, se, e2t, x, y, z, ax, ay,
                      az, ap)
-    if (wave_sum<MIXED, W>(L, ax, ay, az, ap)) {
+    if (wave_sum<W, 8>(L.red, ax, ay, az, ap)) {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < nk) store_result(o, t0 + k, ax[k], ay[k], az[k], ap[k]);
@@ -525,24 +526,57 @@ __device__ __forceinline__ void gs_stage(LdsGS<MIXED>& L, const double* __restri
   if (tid == 0) L.e2 = (T)(be2[blk] + e2u);
 }
 
-template <int K, int NT, bool MIXED>
+// The block-source kernels' workgroup: W waves that hold the same K x 64 targets and split the
+// source blocks (wave v takes blocks b0 + v, b0 + v + W, ...; each stages its own block), then
+// add their sums in wave order.
+template <bool MIXED, int W>
+struct LdsGSW {
+  LdsGS<MIXED> s[W];           // the block each wave is on
+  double red[W > 1 ? 16 * 64 : 1];   // cross-wave sums: 4 K doubles per lane (K <= 4)
+};
+
+// Stages source block blk into one wave's slot, by that wave's 64 lanes.
+template <bool MIXED>
+__device__ __forceinline__ void gs_stage_wave(LdsGS<MIXED>& L, const double* __restrict__ sax,
+                                              const double* __restrict__ be2,
+                                              const double* __restrict__ bm, int64_t blk, double e2u,
+                                              double ax_, double ay_, double az_) {
+  typedef typename LdsGS<MIXED>::T T;
+  const int lane = threadIdx.x & 63;
+  for (int q = lane; q < 512; q += 64) L.m[q] = (T)bm[blk * 512 + q];
+  if (lane < 24) {
+    const int axis = lane >> 3, idx = lane & 7;
+    const double c = sax[blk * 24 + lane];
+    const double a = axis == 0 ? ax_ : axis == 1 ? ay_ : az_;
+    T v = MIXED ? (T)(c - a) : (T)c;
+    if (axis == 0) L.cx[idx] = v;
+    else if (axis == 1) L.cy[idx] = v;
+    else L.cz[idx] = v;
+  }
+  if (lane == 0) L.e2 = (T)(be2[blk] + e2u);
+}
+
+template <int K, int W, bool MIXED>
 struct GsRun {
   typedef typename LdsGS<MIXED>::T T;
   template <int FL>
-  static __device__ __forceinline__ void run(LdsGS<MIXED>& L, const double* sax, const double* be2,
+  static __device__ __forceinline__ void run(LdsGSW<MIXED, W>& L, const double* sax, const double* be2,
                                              const double* bm, int64_t b0, int64_t b1, double e2u,
                                              double ax_, double ay_, double az_, const T (&tx)[K],
                                              const T (&ty)[K], const T (&tz)[K], const T (&te2)[K],
                                              double (&hx)[K], double (&hy)[K], double (&hz)[K],
                                              double (&hp)[K]) {
-    for (int64_t blk = b0; blk < b1; ++blk) {
+    const int v = threadIdx.x >> 6;
+    for (int64_t base = b0; base < b1; base += W) {   // uniform over the workgroup
+      const int64_t blk = base + v;
       __syncthreads();
-      gs_stage<MIXED>(L, sax, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
+      if (blk < b1) gs_stage_wave<MIXED>(L.s[v], sax, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
       __syncthreads();
+      if (blk >= b1) continue;          // uniform over the wave: no block left for it this round
       T ax[K], ay[K], az[K], ap[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = (T)0;
-      gs_block<K, MIXED, FL>(L, tx, ty, tz, te2, ax, ay, az, ap);
+      gs_block<K, MIXED, FL>(L.s[v], tx, ty, tz, te2, ax, ay, az, ap);
       // per-block partials folded into FP64 accumulators (mixed: one FP32 sum per 512 cells)
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -555,24 +589,25 @@ struct GsRun {
   }
 };
 
-// One target tile (K per lane, targets tile0 + k NT + lane) against source blocks [b0, b1):
-// sax / be2 / bm are the set's cell-centre axes, block eps^2 and contiguous cell masses; o
-// already points at this block chunk's partials if the blocks are split.  Shared by
-// k_eval_gridsrc and the persistent batch kernel.
-template <int K, int NT, bool MIXED>
-__device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int64_t tile0,
+// One target tile (K per lane, targets tile0 + 64 k + lane, the same in every wave) against source
+// blocks [b0, b1), split over the W waves: sax / be2 / bm are the set's cell-centre axes, block
+// eps^2 and contiguous cell masses; o already points at this block chunk's partials if the blocks
+// are split over workgroups.  Shared by k_eval_gridsrc and the persistent batch kernel.
+template <int K, int W, bool MIXED>
+__device__ __forceinline__ void gridsrc_tile(LdsGSW<MIXED, W>& L, const Tgt& T, int64_t tile0,
                                              const double* __restrict__ sax,
                                              const double* __restrict__ be2,
                                              const double* __restrict__ bm, int64_t b0, int64_t b1,
                                              const double* sst, const Out& o) {
+  static_assert(K <= 4, "the cross-wave buffer holds 4 K <= 16 values per lane");
   typedef typename LdsGS<MIXED>::T TT;
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   double px[K], py[K], pz[K], pe2[K];
   double mn[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
   double mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    const int64_t t = tile0 + (int64_t)k * 64 + lane;
     load_target(T, t < T.n ? t : T.n - 1, px[k], py[k], pz[k], pe2[k]);
     if (t < T.n) {
       mn[0] = fmin(mn[0], px[k]); mx[0] = fmax(mx[0], px[k]);
@@ -581,7 +616,14 @@ __device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int6
       mn[3] = fmin(mn[3], pe2[k]); mx[3] = fmax(mx[3], pe2[k]);
     }
   }
-  block_minmax<NT, 4>(mn, mx, L.red);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {   // over the wave: every wave holds the same tile
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      mn[q] = fmin(mn[q], __shfl_xor(mn[q], off));
+      mx[q] = fmax(mx[q], __shfl_xor(mx[q], off));
+    }
+  }
   double ax_ = 0.0, ay_ = 0.0, az_ = 0.0;
   if (MIXED) {
     ax_ = 0.5 * (mn[0] + mx[0]);
@@ -601,7 +643,7 @@ __device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int6
   double hx[K], hy[K], hz[K], hp[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
-  typedef GsRun<K, NT, MIXED> R;
+  typedef GsRun<K, W, MIXED> R;
   if constexpr (MIXED) {
     TG_DISPATCH8(R::template run, fl, L, sax, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
                  hx, hy, hz, hp)
@@ -609,28 +651,29 @@ __device__ __forceinline__ void gridsrc_tile(LdsGS<MIXED>& L, const Tgt& T, int6
     TG_DISPATCH(R::template run, fl, L, sax, be2, bm, b0, b1, mn[3], ax_, ay_, az_, tx, ty, tz, te2,
                 hx, hy, hz, hp)
   }
+  if (!wave_sum<W, K>(L.red, hx, hy, hz, hp)) return;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int64_t t = tile0 + (int64_t)k * NT + tid;
+    const int64_t t = tile0 + (int64_t)k * 64 + lane;
     if (t < T.n) store_result(o, t, hx[k], hy[k], hz[k], hp[k]);
   }
 }
 
-// Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
-// blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
-template <int K, int NT, bool MIXED>
-__global__ __launch_bounds__(NT) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
-                                                     const double* __restrict__ be2,
-                                                     const double* __restrict__ bm, int64_t nblk,
-                                                     int64_t chunk, const double* sst, Out O) {
-  __shared__ LdsGS<MIXED> L;
+// Explicit targets (K per lane, K x 64 per workgroup of W waves) against 8^3 source blocks;
+// blockIdx.x = target tile, blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
+template <int K, int W, bool MIXED>
+__global__ __launch_bounds__(64 * W) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
+                                                         const double* __restrict__ be2,
+                                                         const double* __restrict__ bm, int64_t nblk,
+                                                         int64_t chunk, const double* sst, Out O) {
+  __shared__ LdsGSW<MIXED, W> L;
   if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
-  const int64_t tile0 = (int64_t)blockIdx.x * (K * NT);
+  const int64_t tile0 = (int64_t)blockIdx.x * (K * 64);
   const int64_t b0 = (int64_t)blockIdx.y * chunk;
   const int64_t b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  gridsrc_tile<K, NT, MIXED>(L, T, tile0, sax, be2, bm, b0, b1, sst, o);
+  gridsrc_tile<K, W, MIXED>(L, T, tile0, sax, be2, bm, b0, b1, sst, o);
 }
 
 // Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them),

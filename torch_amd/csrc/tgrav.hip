@@ -658,22 +658,24 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   return 0;
 }
 
-// Explicit targets against 8^3 source blocks: K = 4 targets per lane, 256 lanes, source
-// blocks split over blockIdx.y until there are about 4 workgroups per CU.
+// Explicit targets against 8^3 source blocks: K = 4 targets per lane, tiles of 256 targets, a
+// workgroup of W = col_waves waves per tile (splitting its source blocks), source blocks split
+// further over blockIdx.y.
+#define TG_GS_K 4
 static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk, int64_t* ntiles,
                           int* nsplit, int64_t* chunk) {
-  const int K = 4, NT = 256;
-  *ntiles = (nt + K * NT - 1) / (K * NT);
+  const int K = TG_GS_K, W = c->col_waves;
+  *ntiles = (nt + K * 64 - 1) / (K * 64);
   int want = c->nsplit[mode];
-  const int tuned = tuned_block_split(c, mode, "blocksrc", nt, nblk);
+  const int tuned = tuned_block_split(c, mode, "blocksrc", nt, nblk, W);
   if (tuned > 0) want = tuned;
   if (want <= 0) {
-    // Four-wave workgroups, ~3 resident per CU: aim at ~16 rounds, with at least 4 source
-    // blocks per chunk (1 when that leaves fewer workgroups than CUs).
-    const int64_t target = 16LL * 3 * c->num_cu;
-    const int64_t per = (target + *ntiles - 1) / *ntiles;
-    int64_t w = std::min<int64_t>(per, std::max<int64_t>(1, nblk / 4));
-    if (*ntiles * w < c->num_cu) w = std::min<int64_t>(per, nblk);
+    // ~12 waves resident per CU: aim at ~16 rounds of waves, with at least 4 source blocks per
+    // wave and chunk (1 when that leaves fewer waves than CUs).
+    const int64_t target = 16LL * 12 * c->num_cu;
+    const int64_t per = (target + *ntiles * W - 1) / (*ntiles * W);
+    int64_t w = std::min<int64_t>(per, std::max<int64_t>(1, nblk / (4 * W)));
+    if (*ntiles * W * w < c->num_cu) w = std::min<int64_t>(per, nblk);
     want = (int)std::min<int64_t>(w, 65535);
   }
   if (want > 65535) want = 65535;
@@ -681,18 +683,23 @@ static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk
   *nsplit = (int)((nblk + *chunk - 1) / *chunk);
 }
 
+template <bool MIXED, int W>
+static void gridsrc_kernel_w(tgrav_ctx* c, dim3 g, const tg::Tgt& T, const tg::Out& o, int64_t chunk,
+                             int64_t nblk) {
+  tg::k_eval_gridsrc<TG_GS_K, W, MIXED><<<g, 64 * W, 0, c->stream>>>(
+      T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk,
+      c->stat.as<const double>(), o);
+}
+
 static int gridsrc_kernel(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& o, int64_t ntiles,
                           int nsplit, int64_t chunk, int64_t nblk) {
-  const int K = 4, NT = 256;
   const dim3 g((unsigned)ntiles, (unsigned)nsplit);
-  if (mode == TGRAV_MIXED)
-    tg::k_eval_gridsrc<K, NT, true><<<g, NT, 0, c->stream>>>(
-        T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk,
-        c->stat.as<const double>(), o);
-  else
-    tg::k_eval_gridsrc<K, NT, false><<<g, NT, 0, c->stream>>>(
-        T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk,
-        c->stat.as<const double>(), o);
+  const bool mixed = mode == TGRAV_MIXED;
+  switch (c->col_waves) {
+    case 4: mixed ? gridsrc_kernel_w<true, 4>(c, g, T, o, chunk, nblk) : gridsrc_kernel_w<false, 4>(c, g, T, o, chunk, nblk); break;
+    case 2: mixed ? gridsrc_kernel_w<true, 2>(c, g, T, o, chunk, nblk) : gridsrc_kernel_w<false, 2>(c, g, T, o, chunk, nblk); break;
+    default: mixed ? gridsrc_kernel_w<true, 1>(c, g, T, o, chunk, nblk) : gridsrc_kernel_w<false, 1>(c, g, T, o, chunk, nblk);
+  }
   LAUNCHCHECK("k_eval_gridsrc");
   return 0;
 }
@@ -2199,15 +2206,27 @@ static void batch_free(tgrav_batch* b) {
   delete b;
 }
 
-// Block-source work items: K targets per lane, NT lanes (the eager kernel's shape).
-#define TG_BATCH_GS_K 4
-#define TG_BATCH_GS_NT 256
+// Block-source work items: tiles of TG_GS_K x 64 targets, the eager kernel's shape.
 
 // The persistent column kernel for W waves per workgroup.
 static const void* batch_grid_kernel(bool mixed, int W) {
   if (W == 4) return mixed ? (const void*)tg::k_batch_eval_grid<true, 4> : (const void*)tg::k_batch_eval_grid<false, 4>;
   if (W == 2) return mixed ? (const void*)tg::k_batch_eval_grid<true, 2> : (const void*)tg::k_batch_eval_grid<false, 2>;
   return mixed ? (const void*)tg::k_batch_eval_grid<true, 1> : (const void*)tg::k_batch_eval_grid<false, 1>;
+}
+
+template <bool MIXED, int W>
+static void batch_gridsrc_launch(tgrav_batch* b, hipStream_t s) {
+  tg::k_batch_eval_gridsrc<TG_GS_K, W, MIXED><<<b->sgrid, 64 * W, 0, s>>>(
+      b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems, b->counter.as<unsigned int>() + 2,
+      b->bms.as<double>(), b->sax.as<double>(), b->be2.as<double>(), b->stats.as<unsigned long long>(),
+      b->part.as<double>());
+}
+
+static const void* batch_gridsrc_kernel(bool mixed, int W) {
+  if (W == 4) return mixed ? (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 4, true> : (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 4, false>;
+  if (W == 2) return mixed ? (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 2, true> : (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 2, false>;
+  return mixed ? (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 1, true> : (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 1, false>;
 }
 
 template <bool MIXED, int W>
@@ -2270,17 +2289,12 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   }
   if (b->n_sitems > 0) {
     auto go = [&]() -> int {
-      const int K = TG_BATCH_GS_K, NT = TG_BATCH_GS_NT;
-      if (b->mode == TGRAV_MIXED)
-        tg::k_batch_eval_gridsrc<K, NT, true><<<b->sgrid, NT, 0, s>>>(
-            b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems,
-            b->counter.as<unsigned int>() + 2, b->bms.as<double>(), b->sax.as<double>(),
-            b->be2.as<double>(), b->stats.as<unsigned long long>(), b->part.as<double>());
-      else
-        tg::k_batch_eval_gridsrc<K, NT, false><<<b->sgrid, NT, 0, s>>>(
-            b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems,
-            b->counter.as<unsigned int>() + 2, b->bms.as<double>(), b->sax.as<double>(),
-            b->be2.as<double>(), b->stats.as<unsigned long long>(), b->part.as<double>());
+      const bool mixed = b->mode == TGRAV_MIXED;
+      switch (b->waves) {
+        case 4: mixed ? batch_gridsrc_launch<true, 4>(b, s) : batch_gridsrc_launch<false, 4>(b, s); break;
+        case 2: mixed ? batch_gridsrc_launch<true, 2>(b, s) : batch_gridsrc_launch<false, 2>(b, s); break;
+        default: mixed ? batch_gridsrc_launch<true, 1>(b, s) : batch_gridsrc_launch<false, 1>(b, s);
+      }
       LAUNCHCHECK("k_batch_eval_gridsrc");
       return 0;
     };
@@ -2382,12 +2396,11 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const int per_cu = limit > 0 ? std::min(limit, o) : o;
     return per_cu * c->num_cu;
   };
-  const int GK = TG_BATCH_GS_K, GNT = TG_BATCH_GS_NT;
   const int slots = occupancy(b->v->bkernel(), b->v->nt, wg_per_cu);
   const int CW = c->col_waves;
   const int gslots = occupancy(batch_grid_kernel(mode == TGRAV_MIXED, CW), 64 * CW, 0);
-  const int sslots = occupancy(mode == TGRAV_MIXED ? (const void*)tg::k_batch_eval_gridsrc<GK, GNT, true>
-                                                   : (const void*)tg::k_batch_eval_gridsrc<GK, GNT, false>, GNT, 0);
+  const int sslots = occupancy(batch_gridsrc_kernel(mode == TGRAV_MIXED, CW), 64 * CW, 0);
+  const int64_t GT = (int64_t)TG_GS_K * 64;   // targets per block-source tile
   // Form of each problem's work items.
   std::vector<int> form(n, tg::FORM_GENERIC);
   double tot_list[3] = {0.0, 0.0, 0.0};   // interactions on the generic, column, block-source lists
@@ -2410,7 +2423,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     return std::min(C, std::max(floor_cost, tot_list[li] / (8.0 * sl)));
   };
   const double C_gen = list_cost(0, (double)T * NT * 4.0), C_col = list_cost(1, 512.0 * 64 * 4 * CW),
-               C_gs = list_cost(2, (double)GK * GNT * 512.0 * 4);
+               C_gs = list_cost(2, (double)GT * 512.0 * 4 * CW);
   std::vector<tg::BProb> probs(n);
   std::vector<tg::BSrc> srcs(n);
   std::vector<int64_t> sprefix(n + 1, 0);
@@ -2461,7 +2474,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     // kernel that may take the problem's items (64 columns, NT generic; NT is a multiple of 64) --
     // NT for all, so that targets in the blocks' C order and the same targets as blocks get the
     // same chunks, hence the same bits.
-    const int64_t TT = gs ? (int64_t)GK * GNT : col ? 512 : T;
+    const int64_t TT = gs ? GT : col ? 512 : T;
     const int64_t SG = gs ? 1 : NT;
     const int64_t units = gs ? q.n_source_blocks : q.n_sources;
     const double per_tile = (double)std::min<int64_t>(TT, q.n_targets) * q.n_sources;
@@ -2498,7 +2511,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       return true;
     };
     bool ok = true;
-    if (gs) ok = add(sitems, scost, (int64_t)GK * GNT, true);
+    if (gs) ok = add(sitems, scost, GT, true);
     if (col) ok = ok && add(gitems, gcost, 512, false);
     if (P.form == tg::FORM_GENERIC || P.form == tg::FORM_CAND) ok = ok && add(items, cost, T, false);
     if (!ok) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
