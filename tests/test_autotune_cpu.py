@@ -1,0 +1,27 @@
+"""The autotuner's choice among timed candidates (no GPU needed): the fastest candidate, except
+that among candidates within 1% of it the one that writes the fewest partial sums wins."""
+from torch_amd import autotune
+
+
+def test_pick_prefers_fewer_chunks_within_the_tie():
+    table = [dict(nsplit=0, ms=90.475), dict(nsplit=8, ms=99.807), dict(nsplit=32, ms=90.325),
+             dict(nsplit=64, ms=89.944), dict(nsplit=256, ms=89.741)]
+    assert autotune.pick(table, autotune._chunks)["nsplit"] == 32
+
+
+def test_pick_takes_the_fastest_outside_the_tie():
+    table = [dict(nsplit=8, ms=4.335), dict(nsplit=64, ms=2.417), dict(nsplit=256, ms=2.357)]
+    assert autotune.pick(table, autotune._chunks)["nsplit"] == 256
+
+
+def test_heuristic_split_ranks_last_among_ties():
+    table = [dict(nsplit=0, ms=10.0), dict(nsplit=12, ms=10.05)]
+    assert autotune.pick(table, autotune._chunks)["nsplit"] == 12
+    table = [dict(nsplit=0, ms=10.0), dict(nsplit=12, ms=10.2)]
+    assert autotune.pick(table, autotune._chunks)["nsplit"] == 0
+
+
+def test_batch_rows_tie_break_on_work_items():
+    table = [dict(item=0.0, items_n=49152, ms=190.5), dict(item=2.5e7, items_n=24576, ms=191.9),
+             dict(item=1e8, items_n=8192, ms=200.0)]
+    assert autotune.pick(table, lambda r: r["items_n"])["item"] == 2.5e7

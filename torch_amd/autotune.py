@@ -197,7 +197,7 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
                     plan.close()
                     torch.cuda.synchronize()
         lib.tgrav_set_batch_config(f.handle, 0.0, 0)
-    best = min(table, key=lambda r: r["ms"])
+    best = pick(table, lambda r: r["items_n"])   # fewer work items: fewer source chunks
     return dict(best, waves=waves), table
 
 
@@ -210,6 +210,24 @@ def c5_plan_problems():
     """C5 as the bench runs it: stars -> the 8,192 leaf blocks, and the blocks' cells -> the stars."""
     from .profile_driver import plan_problems
     return plan_problems("c5")
+
+
+TIE = 0.01   # candidates within 1% of the fastest count as equally fast
+
+
+def pick(table, fewer_partials):
+    """The fastest row, except that among the rows within TIE of it the one that writes the fewest
+    partial sums to memory wins (fewer source chunks: less memory and less traffic for no
+    measurable time).  fewer_partials(row) orders the rows by that (smaller = fewer)."""
+    best = min(r["ms"] for r in table)
+    near = [r for r in table if r["ms"] <= best * (1.0 + TIE)]
+    return min(near, key=lambda r: (fewer_partials(r), r["ms"]))
+
+
+def _chunks(r):
+    """Source chunks of a block-kernel row: nsplit, with 0 (the built-in heuristic, whose count
+    is not known here) ranked last."""
+    return r["nsplit"] if r["nsplit"] > 0 else float("inf")
 
 
 def blocks_key(mode, direction, ns, nblk, waves=1):
@@ -237,7 +255,7 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
             ms = _time(lambda: f.evaluate_blocks(BO, BD), reps)
             table.append(dict(nsplit=sp, ms=ms))
             log(f"  {mode} block targets nsplit={sp}: {ms:.3f} ms")
-        out[blocks_key(mode, "blockfwd", ns, nblk, f.column_waves())] = min(table, key=lambda r: r["ms"])
+        out[blocks_key(mode, "blockfwd", ns, nblk, f.column_waves())] = pick(table, _chunks)
         f.set_source_blocks(BO, BD, CM)
         table = []
         for sp in (0, 8, 16, 24, 32, 48, 64, 100, 128, 200, 256):
@@ -245,7 +263,7 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
             ms = _time(lambda: f.evaluate(None, S[0], S[1], S[2]), reps)
             table.append(dict(nsplit=sp, ms=ms))
             log(f"  {mode} block sources nsplit={sp}: {ms:.3f} ms")
-        out[blocks_key(mode, "blocksrc", ns, nblk, f.column_waves())] = min(table, key=lambda r: r["ms"])
+        out[blocks_key(mode, "blocksrc", ns, nblk, f.column_waves())] = pick(table, _chunks)
     return out
 
 
