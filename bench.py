@@ -60,12 +60,20 @@ PEAK_FP32_UNPACKED_TFLOPS = 78.6   # one FP32 FMA per lane per clock (packed FP3
 BYTES_PER_TARGET = 64              # x, y, z, eps in + ax, ay, az, phi out (FP64)
 BYTES_PER_SOURCE = 40              # x, y, z, m, eps (FP64), read once per launch
 
-# HBM bytes per launch of the c2 forward block kernel k_eval_grid, by (mode, source split): rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE passes over `bench.py --steps 2 --warmup 1 --no-cpu` (medians over
-# the dispatches), FETCH_SIZE divided by 0.5 and WRITE_SIZE by 1.0, the factors those counters read
-# for a plain 512 MiB device copy.  Reported only when the bench runs the launch shape measured.
-C2_FWD_TRAFFIC = {("mixed", 8): (3881 / 0.5 + 300373 / 1.0) * 1024,
-                  ("fp64", 6): (3124 / 0.5 + 284458 / 1.0) * 1024}
+# HBM bytes per launch of the persistent or block kernels, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over `python -m torch_amd.profile_driver` running the same launch (KiB per
+# dispatch, means over the dispatches), FETCH_SIZE divided by 0.5 and WRITE_SIZE by 1.0: the
+# factors those counters read for a plain 512 MiB device copy in the same runs.  Keyed by the launch
+# shape measured (mode, source split or work items, waves per workgroup); reported only when the
+# bench runs that shape, null otherwise.
+C2_FWD_TRAFFIC = {("mixed", 4, 4): (3144.97 / 0.5 + 128436 / 1.0) * 1024,        # k_eval_grid
+                  ("fp64", 6, 4): (3578.75 / 0.5 + 192301 / 1.0) * 1024}
+C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (56704.7 / 0.5 + 339881 / 1.0) * 1024,   # k_batch_eval_grid
+                   ("fp64", 21199, 4): (58554.3 / 0.5 + 339884 / 1.0) * 1024}
+C5_PLAN_TRAFFIC = {("mixed", 49152, 4): {"fwd": (19076.3 / 0.5 + 788021 / 1.0) * 1024,
+                                         "rev": (235770 / 0.5 + 338844 / 1.0) * 1024},
+                   ("fp64", 24576, 4): {"fwd": (21731.1 / 0.5 + 394016 / 1.0) * 1024,
+                                        "rev": (208362 / 0.5 + 134916 / 1.0) * 1024}}
 
 
 def log(*a):
@@ -104,6 +112,7 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
         rev = GasGravityField(mode, validate=False)
         nblk = nc // 512
         cfg = (autotune.apply_blocks(fwd, "blockfwd", ns, nblk), autotune.apply_blocks(rev, "blocksrc", ns, nblk))
+        waves = fwd.column_waves()
     else:
         # The library applies the tuned shapes itself: explicit arrays recognised as 8^3 blocks
         # take the block kernels' entries, others the generic kernels' (reported here).
@@ -112,6 +121,7 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
         cache = autotune.load_cache()
         m = "mixed" if mode == "mixed" else "fp64"
         cfg = (cache.get(autotune.eval_key(m, ns, nc)), cache.get(autotune.eval_key(m, nc, ns)))
+        waves = fwd.column_waves()
     log(f"{mode}: {'block' if blocks else 'explicit'} format, tuned launch shapes fwd={cfg[0]} rev={cfg[1]}")
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
@@ -147,7 +157,7 @@ def run_pair(mode, dev, ns, nc, steps, warmup, blocks=True):
     fwd.close()
     rev.close()
     return dict(ms=t0.elapsed_time(t1) / steps, fwd_kernel_ms=float(np.mean(kf)),
-                rev_kernel_ms=float(np.mean(kr)), wall=wall, res=res, cfg=cfg)
+                rev_kernel_ms=float(np.mean(kr)), wall=wall, res=res, cfg=cfg, waves=waves)
 
 
 # ---------------------------------------------------------------------- c4 ----
@@ -186,7 +196,7 @@ def run_batch(mode, probs_dev, steps, warmup, eager_timed=3):
     wall = time.perf_counter() - wall0
     res = [tuple(a.cpu().numpy() for a in o) for o in out]
     info = dict(ms=t0.elapsed_time(t1) / steps, kernel_ms=float(np.sum(per_kernel)) if per_kernel else None,
-                kernels_ms=per_kernel, wall=wall, res=res, cfg=cfg, items=plan.n_items,
+                kernels_ms=per_kernel, wall=wall, res=res, cfg=cfg, items=plan.n_items, waves=f.column_waves(),
                 interactions=plan.interactions)
     plan.close()
     f.close()
@@ -544,9 +554,13 @@ def c5_record(args, modes, d, steps, warmup):
         kf, kr = (r["kernels_ms"] + [None, None])[:2]
         peak = PEAK_FP32_TFLOPS if mode == "mixed" else PEAK_FP64_TFLOPS
         frac = lambda k: None if k is None else FLOP_PER_INTERACTION * ns * nc / (k * 1e-3) / 1e12 / peak  # noqa
+        tr = C5_PLAN_TRAFFIC.get((mode, r["items"], r["waves"]), {})
         rec[mode] = {"value": inter / (r["ms"] * 1e-3), "ms_per_step": r["ms"], "fwd_kernel_ms": kf,
                      "rev_kernel_ms": kr, "fwd_roofline_frac": frac(kf), "rev_roofline_frac": frac(kr),
-                     "peak": peak, "launch_shape": r["cfg"], "work_items": r["items"]}
+                     "peak": peak, "launch_shape": r["cfg"], "work_items": r["items"], "waves": r["waves"],
+                     "fwd_traffic": tr.get("fwd"), "rev_traffic": tr.get("rev"),
+                     "fwd_algorithmic_bytes": BYTES_PER_TARGET * nc + BYTES_PER_SOURCE * ns,
+                     "rev_algorithmic_bytes": BYTES_PER_TARGET * ns + BYTES_PER_SOURCE * nc}
     if "mixed" in results and "fp64" in results:
         rec["mixed_over_fp64"] = rec["mixed"]["value"] / rec["fp64"]["value"]
         errs = {}
@@ -596,7 +610,8 @@ def bench_c4_c5(args, line, modes, head, d):
                      "config": {k: rec[k] for k in ("workload", "seed", "n_stars", "n_cells", "interactions_per_step")}})
         line["config"]["mode"] = head
         line["roofline"] = roofline_of(head, float(rec["n_stars"]) * rec["n_cells"], r["fwd_kernel_ms"],
-                                       f"k_batch_eval_grid<{head}> (forward, stars -> 8^3 cell blocks)")
+                                       f"k_batch_eval_grid<{head}> (forward, stars -> 8^3 cell blocks)",
+                                       traffic=r["fwd_traffic"], algorithmic_bytes=r["fwd_algorithmic_bytes"])
         for k in ("fp64", "mixed_over_fp64", "mixed_error_vs_fp64", "error_vs_oracle", "fp64_vs_oracle",
                   "error_gate_met"):
             if k in rec:
@@ -635,7 +650,8 @@ def bench_c4_c5(args, line, modes, head, d):
     kern_bytes = sum(BYTES_PER_TARGET * b + BYTES_PER_SOURCE * a for a, b in sizes)
     line["roofline"] = roofline_of(head, inter_step, r["kernel_ms"],
                                    f"k_batch_eval_grid<{head}> (persistent column kernel over the 64 problems' "
-                                   f"leaf blocks)", algorithmic_bytes=kern_bytes)
+                                   f"leaf blocks)", algorithmic_bytes=kern_bytes,
+                                   traffic=C4_PLAN_TRAFFIC.get((head, r["items"], r["waves"])))
     line["explicit_targets"] = explicit
     if "fp64" in results and head == "mixed":
         r64 = results["fp64"]
@@ -941,7 +957,7 @@ def main():
     peak = PEAK_FP32_TFLOPS if head == "mixed" else PEAK_FP64_TFLOPS
     traffic = None
     if args.workload == "c2" and results[head]["cfg"][0]:
-        traffic = C2_FWD_TRAFFIC.get((head, results[head]["cfg"][0]["nsplit"]))
+        traffic = C2_FWD_TRAFFIC.get((head, results[head]["cfg"][0]["nsplit"], results[head]["waves"]))
     line["roofline"] = {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": peak,
                         "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                         "kernel_ms": kern_ms, "flop_per_interaction": FLOP_PER_INTERACTION,

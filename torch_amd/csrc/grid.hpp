@@ -659,8 +659,12 @@ __device__ __forceinline__ void gridsrc_tile(LdsGSW<MIXED, W>& L, const Tgt& T, 
   }
 }
 
-// Explicit targets (K per lane, K x 64 per workgroup of W waves) against 8^3 source blocks;
-// blockIdx.x = target tile, blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
+// Explicit targets (K per lane, K x 64 per workgroup of W waves) against 8^3 source blocks: a grid
+// of (target tiles, block chunks [q chunk, min((q + 1) chunk, nblk))).  Workgroup L = x + y gridDim.x
+// is dispatched to XCD L mod 8; with gridDim.y a multiple of 8 the kernel maps its workgroups so that
+// all tiles of block chunk q run on XCD q mod 8 -- each chunk's masses are then fetched into one
+// L2, not eight -- and workgroups whose chunk starts past the last block (the padding of
+// gridDim.y to a multiple of 8) return.  Otherwise tile = x, chunk = y.
 template <int K, int W, bool MIXED>
 __global__ __launch_bounds__(64 * W) void k_eval_gridsrc(Tgt T, const double* __restrict__ sax,
                                                          const double* __restrict__ be2,
@@ -668,12 +672,19 @@ __global__ __launch_bounds__(64 * W) void k_eval_gridsrc(Tgt T, const double* __
                                                          int64_t chunk, const double* sst, Out O) {
   __shared__ LdsGSW<MIXED, W> L;
   if (gated_off(O)) return;   // explicit sources that turned out not to be 8^3 blocks
-  const int64_t tile0 = (int64_t)blockIdx.x * (K * 64);
-  const int64_t b0 = (int64_t)blockIdx.y * chunk;
+  int64_t tile = blockIdx.x, q = blockIdx.y;
+  if (gridDim.y % 8 == 0) {
+    const int64_t l = (int64_t)blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+    const int64_t s = l >> 3;
+    tile = s % gridDim.x;
+    q = (s / gridDim.x) * 8 + (l & 7);
+  }
+  const int64_t b0 = q * chunk;
+  if (b0 >= nblk) return;     // uniform over the workgroup
   const int64_t b1 = b0 + chunk < nblk ? b0 + chunk : nblk;
   Out o = O;
-  if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
-  gridsrc_tile<K, W, MIXED>(L, T, tile0, sax, be2, bm, b0, b1, sst, o);
+  if (o.part) o.part += q * 4 * o.pstride;
+  gridsrc_tile<K, W, MIXED>(L, T, tile * (K * 64), sax, be2, bm, b0, b1, sst, o);
 }
 
 // Expands 8^3 source blocks into the packed per-cell form (so every kernel can use them),

@@ -693,7 +693,11 @@ static void gridsrc_kernel_w(tgrav_ctx* c, dim3 g, const tg::Tgt& T, const tg::O
 
 static int gridsrc_kernel(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& o, int64_t ntiles,
                           int nsplit, int64_t chunk, int64_t nblk) {
-  const dim3 g((unsigned)ntiles, (unsigned)nsplit);
+  // blockIdx.y rounded up to a multiple of 8: the kernel then maps the tiles of each block chunk to
+  // one XCD (the padding workgroups have no chunk and return).
+  const unsigned ny = nsplit >= 8 ? (unsigned)((nsplit + 7) / 8 * 8) : (unsigned)nsplit;
+  if (ny > 65535) return fail(TGRAV_ELIMIT, "too many block chunks (%d)", nsplit);
+  const dim3 g((unsigned)ntiles, ny);
   const bool mixed = mode == TGRAV_MIXED;
   switch (c->col_waves) {
     case 4: mixed ? gridsrc_kernel_w<true, 4>(c, g, T, o, chunk, nblk) : gridsrc_kernel_w<false, 4>(c, g, T, o, chunk, nblk); break;
@@ -2493,10 +2497,10 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       rprefix.push_back(rprefix.back() + q.n_targets);
     }
     const double per_unit = gs ? 512.0 : 1.0;
-    // Items in tile-major order (a tile's chunks next to each other), or chunk-major (all tiles
-    // of one source chunk next to each other, so that the workgroups running at the same time
-    // share the chunk's sources in L2: the block-source list, whose sources are the large side).
-    // The cost sort below is stable, so equal-cost items keep this order.
+    // Items in tile-major order (a tile's chunks next to each other: the generic list), or
+    // chunk-major (all tiles of one source chunk next to each other, so that the workgroups
+    // running at the same time share the chunk's sources in L2: the column and block-source
+    // lists).  The cost sort below is stable, so equal-cost items keep this order.
     auto add = [&](std::vector<tg::BItem>& it, std::vector<double>& cs, int64_t tt, bool chunk_major) -> bool {
       const int64_t tiles = (q.n_targets + tt - 1) / tt;
       if (tiles > 0x7fffffffLL) return false;
@@ -2512,7 +2516,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     };
     bool ok = true;
     if (gs) ok = add(sitems, scost, GT, true);
-    if (col) ok = ok && add(gitems, gcost, 512, false);
+    if (col) ok = ok && add(gitems, gcost, 512, true);
     if (P.form == tg::FORM_GENERIC || P.form == tg::FORM_CAND) ok = ok && add(items, cost, T, false);
     if (!ok) { batch_free(b); return fail(TGRAV_ELIMIT, "problem %d too large", p); }
   }
