@@ -369,13 +369,22 @@ def host_threads():
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it (what lscpu prints as the model name), with its
+    family / model / stepping numbers, which identify the part when the name is generic."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = (x.strip() for x in line.split(":", 1))
+                    info.setdefault(k, v)
+                elif info:
+                    break                       # first processor only
     except OSError:
         pass
+    if "model name" in info:
+        ids = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+        return info["model name"] + (f" ({', '.join(ids)})" if ids else "")
     import platform
     return platform.processor() or "unknown"
 

@@ -8,7 +8,8 @@ rev = cells -> stars, fwdb / revb = the same with the cells as 8^3 leaf blocks (
 kernels; VARIANT is ignored there); variant/nsplit default to the tuned shape of the run's
 kernel (the block entries for fwdb / revb).  Workloads c4 and c5 take MODE:plan: the bench's
 batch plan (c4: the 64 problems with block targets; c5: stars -> blocks and blocks -> stars),
-run eagerly --reps times with the tuned shape.  After the evaluations it performs one plain
+run eagerly --reps times with the tuned shape.  --waves sets the waves per workgroup of the 8^3
+block kernels (default: the library's).  After the evaluations it performs one plain
 device-to-device copy of --copy-mb MiB (torch copy_), whose FETCH_SIZE / WRITE_SIZE readings
 calibrate the traffic counters against a known byte count.  Everything runs on torch's current
 stream; the program prints one line per step.
@@ -50,6 +51,8 @@ def run_plans(args):
     for spec in args.runs:
         mode = spec.split(":")[0]
         with GravityField(mode, validate=False) as f:
+            if args.waves:
+                f.set_column_waves(args.waves)
             e = autotune.apply_batch(f, probs)
             plan = BatchPlan(f, probs)
             f.set_timing(True)
@@ -78,6 +81,8 @@ def run_points(args):
         mode, direction = parts[0], parts[1]
         src, tgt = (S, C) if direction.startswith("fwd") else (C, S)
         with GravityField(mode, validate=False) as f:
+            if args.waves:
+                f.set_column_waves(args.waves)
             nblk = int(BD.numel())
             if direction == "fwdb":
                 e = autotune.apply_blocks(f, "blockfwd", S[0].numel(), nblk)
@@ -108,6 +113,7 @@ def main():
     ap.add_argument("--runs", nargs="+", default=["mixed:fwd", "fp64:fwd"])
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mb", type=int, default=512)
+    ap.add_argument("--waves", type=int, default=0, choices=[0, 1, 2, 4])
     args = ap.parse_args()
 
     import torch
