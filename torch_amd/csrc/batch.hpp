@@ -84,9 +84,8 @@ __device__ __forceinline__ int find_segment(const int64_t* __restrict__ prefix, 
 // source statistics (ST_*) into stats[ST_WORDS p ...].  Source blocks are expanded to their cell
 // centres (rounded as k_pack_source_blocks rounds them) and also give their cell-centre axes
 // (sax, 24 per block), eps^2 (be2) and contiguous cell masses (bms, 512 per block) for the
-// block-source kernel.  A workgroup inside one
-// problem reduces first and issues one atomic per word; a workgroup straddling problems falls
-// back to atomics per lane.  Workgroups of 256 lanes.
+// block-source kernel.  Each workgroup reduces the statistics of every problem it touches and issues
+// at most one atomic per word and problem.  Workgroups of 256 lanes.
 __global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __restrict__ srcs,
                                                     const int64_t* __restrict__ prefix,
                                                     double4* __restrict__ s4, double* __restrict__ se2,
@@ -137,14 +136,16 @@ __global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __res
     sp[1] = last >= 0 ? find_segment(prefix, nprob, last) : -1;
   }
   __syncthreads();
-  if (sp[0] == sp[1]) {            // uniform over the workgroup
-    if (sp[0] >= 0) fold_source_stats(e2min, 0, rmax, e2max, mmax, stats + (int64_t)ST_WORDS * sp[0]);
-  } else if (p >= 0) {
-    unsigned long long* st = stats + (int64_t)ST_WORDS * p;
-    atomicMin(&st[ST_E2MIN], e2min);
-    atomicMax(&st[ST_RMAX], rmax);
-    atomicMax(&st[ST_E2MAX], e2max);
-    atomicMax(&st[ST_MMAX], mmax);
+  // The workgroup's sources belong to problems sp[0] .. sp[1] (contiguous in the packed order): one
+  // workgroup reduction per problem, each lane contributing to its own problem only (per-lane
+  // atomics on the few words of a problem would queue up behind each other).
+  const int p0 = sp[0], p1 = sp[1];   // uniform over the workgroup
+  if (p0 >= 0) {
+    for (int q = p0; q <= p1; ++q) {
+      const bool mine = p == q;
+      fold_source_stats(mine ? e2min : 0x7ff0000000000000ULL, 0, mine ? rmax : 0ULL, mine ? e2max : 0ULL,
+                        mine ? mmax : 0ULL, stats + (int64_t)ST_WORDS * q);
+    }
   }
 }
 
