@@ -49,15 +49,18 @@ struct BItem {             // work item: problem, target tile (or 8^3 target blo
   int prob, tile, chunk, pad;
 };
 
-// counters = 0 (generic, column and block-source work lists), per-problem source statistics
-// (ST_WORDS each, see kernels.hpp) reset, column flags = 1 (k_batch_check_cols clears them)
+// counters = 0 (generic, column and block-source work lists), counter[3] = skip_generic (1 when
+// every generic item belongs to a column candidate: then the generic list has nothing to do unless
+// a candidate fails its test, which clears the word), per-problem source statistics (ST_WORDS
+// each, see kernels.hpp) reset, column flags = 1 (k_batch_check_cols clears them)
 __global__ void k_batch_reset(int nprob, unsigned int* counter, unsigned long long* stats,
-                              int* colflag) {
+                              int* colflag, unsigned int skip_generic) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     counter[0] = 0u;
     counter[1] = 0u;
     counter[2] = 0u;
+    counter[3] = skip_generic;
   }
   if (i < nprob) {
     colflag[i] = 1;
@@ -151,16 +154,19 @@ __global__ __launch_bounds__(256) void k_batch_pack(int nprob, const BSrc* __res
 
 // The column-structure test of k_check_cols over the plan's candidate problems (ids[q], explicit
 // targets, a multiple of 512): lane i checks target i - prefix[q] of candidate q = the segment of
-// i in prefix (targets per candidate) and clears colflag[ids[q]] on a violation.  Runs after
-// k_batch_reset set the flags to 1.
+// i in prefix (targets per candidate) and clears colflag[ids[q]] and the skip word counter[3] on a
+// violation.  Runs after k_batch_reset set the flags.
 __global__ void k_batch_check_cols(int ncand, const int* __restrict__ ids,
                                    const int64_t* __restrict__ prefix, const BProb* __restrict__ probs,
-                                   int* colflag) {
+                                   int* colflag, unsigned int* counter) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= prefix[ncand]) return;
   const int q = find_segment(prefix, ncand, i);
   const int p = ids[q];
-  if (!column_ok(probs[p].T, i - prefix[q])) colflag[p] = 0;
+  if (!column_ok(probs[p].T, i - prefix[q])) {
+    colflag[p] = 0;
+    counter[3] = 0u;
+  }
 }
 
 template <int K, int NT, bool MIXED>
@@ -175,6 +181,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
   typedef typename std::conditional<MIXED, LdsF32<NT>, LdsF64<NT>>::type Lds;
   __shared__ Lds L;
   __shared__ int s_item;
+  if (counter[3] != 0u) return;   // every generic item belongs to a candidate that took the columns
   for (;;) {
     if (threadIdx.x == 0) {   // items of problems whose targets took the column kernel are skipped here
       int i;

@@ -2188,6 +2188,7 @@ struct tgrav_batch {
   int waves = 1;                 // waves per column workgroup (the context's, at creation)
   int n_sitems = 0, sgrid = 0;   // block-source work list (k_eval_gridsrc's column walk)
   int ncand = 0;                 // problems whose column structure is tested at every run
+  bool skip_generic = false;     // every generic item belongs to a candidate (skipped when all pass)
   const Variant* v = nullptr;
   int64_t total_src = 0, red_total = 0, cand_total = 0, total_sblk = 0;
   double inter = 0.0;
@@ -2252,7 +2253,8 @@ static int upload(DevBuf& d, const std::vector<T>& v) {
 // Enqueues reset -> pack -> column check -> persistent evals -> chunk reduction on `s`.
 static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   tg::k_batch_reset<<<grid1(std::max(b->nprob, 1), 256), 256, 0, s>>>(
-      b->nprob, b->counter.as<unsigned int>(), b->stats.as<unsigned long long>(), b->colflag.as<int>());
+      b->nprob, b->counter.as<unsigned int>(), b->stats.as<unsigned long long>(), b->colflag.as<int>(),
+      b->skip_generic ? 1u : 0u);
   LAUNCHCHECK("k_batch_reset");
   if (b->total_src > 0) {
     tg::k_batch_pack<<<grid1(b->total_src, 256), 256, 0, s>>>(
@@ -2263,7 +2265,8 @@ static int batch_enqueue(tgrav_batch* b, hipStream_t s, bool timed) {
   }
   if (b->ncand > 0 && b->cand_total > 0) {
     tg::k_batch_check_cols<<<grid1(b->cand_total, 256), 256, 0, s>>>(
-        b->ncand, b->cand.as<int>(), b->cprefix.as<int64_t>(), b->probs.as<tg::BProb>(), b->colflag.as<int>());
+        b->ncand, b->cand.as<int>(), b->cprefix.as<int64_t>(), b->probs.as<tg::BProb>(), b->colflag.as<int>(),
+        b->counter.as<unsigned int>());
     LAUNCHCHECK("k_batch_check_cols");
   }
   if (b->n_items > 0) {
@@ -2542,6 +2545,9 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   b->total_src = sprefix[n];
   b->total_sblk = sblk_total;
   b->ncand = (int)cand.size();
+  b->skip_generic = !sorted.empty() && std::all_of(sorted.begin(), sorted.end(), [&](const tg::BItem& w) {
+    return probs[w.prob].form == tg::FORM_CAND;
+  });
   b->cand_total = cprefix.back();
   b->grid = std::max(1, std::min(b->n_items, slots));
   b->ggrid = std::max(1, std::min(b->n_gitems, gslots));
@@ -2560,7 +2566,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       (rc = b->stats.ensure((size_t)std::max(n, 1) * tg::ST_WORDS * sizeof(unsigned long long))) ||
       (rc = b->colflag.ensure((size_t)std::max(n, 1) * sizeof(int))) ||
       (rc = b->part.ensure(std::max<int64_t>(part_total, 1) * sizeof(double))) ||
-      (rc = b->counter.ensure(3 * sizeof(unsigned int)))) {
+      (rc = b->counter.ensure(4 * sizeof(unsigned int)))) {
     batch_free(b);
     return rc;
   }
