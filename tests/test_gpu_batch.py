@@ -138,7 +138,9 @@ def test_batch_argument_errors():
 
 
 def test_c4_batch_mixed_against_fp64():
-    """The 64-problem uneven batch (c4): mixed within the error gate of the FP64 batch."""
+    """The 64-problem uneven batch (c4, graph replay): mixed within the error gate of the FP64 batch
+    over every target, and both modes against the oracle on 64 seeded targets of every problem
+    (FP64 within 1e-12; mixed within the gate)."""
     import torch
     probs = W.c4(1004)
     dev = problems_from_workload(probs)
@@ -152,14 +154,19 @@ def test_c4_batch_mixed_against_fp64():
     ea = np.concatenate([_util.rel_acc_err(m, r) for m, r in zip(res["mixed"], res["fp64"])])
     print("c4 mixed vs fp64: median", np.median(ea), "max", ea.max(), flush=True)
     assert np.median(ea) <= 1e-6 and ea.max() <= 1e-4
-    # FP64 batch against the oracle on a few targets of a few problems
-    for i in (0, 17, 40):
+    # both modes against the oracle on 64 targets of every problem
+    em = []
+    for i in range(len(probs)):
         S, T = probs[i]["sources"], probs[i]["targets"]
         idx = _util.subsample(len(T["x"]), 64, i)
         ref = oracle.eval_plummer(S["x"], S["y"], S["z"], S["m"], S["eps"], T["x"][idx], T["y"][idx],
                                   T["z"][idx], None, G, mode=oracle.NEUMAIER, threads=16)
         s = _util.summary(tuple(a[idx] for a in res["fp64"][i]), ref)
         assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+        em.append(_util.rel_acc_err(tuple(a[idx] for a in res["mixed"][i]), ref))
+    em = np.concatenate(em)
+    print("c4 mixed vs oracle: median", np.median(em), "max", em.max(), flush=True)
+    assert np.median(em) <= 1e-6 and em.max() <= 1e-4
 
 
 def block_batch(seed=41, n=9):
