@@ -101,7 +101,11 @@ __device__ __forceinline__ void image_range(double d, double iL, double rem, int
     hi = 0;
     return;
   }
-  const double s = (double)__builtin_amdgcn_sqrtf((float)rem);   // raw v_sqrt_f32, ~1 ulp
+  // raw v_sqrt_f32 (~1 ulp); rem above 1e30 is scaled by 2^-100 first and s by 2^50 after (both
+  // exact), so the FP32 range holds rem up to ~4e68 (cgs boxes of kpc: rem ~ 1e44)
+  const bool big = rem > 1e30;
+  double s = (double)__builtin_amdgcn_sqrtf((float)(big ? rem * 0x1p-100 : rem));
+  if (big) s *= 0x1p50;
   double a = (-s - d) * iL - 1e-4, b = (s - d) * iL + 1e-4;
   a = fmax(a, -(double)N - 1.0);   // also keeps NaN / inf out of the int conversion
   b = fmin(b, (double)N + 1.0);
@@ -141,6 +145,61 @@ __device__ __forceinline__ void flash_images(const Flash& P, double dxi, double 
       int z0, z1;
       image_range(dzi, P.ilz, P.cut_thr - xy, N, z0, z1);
       const double fz1 = (double)z1;
+      if (xy > P.soft_thr && P.cut_thr < 1e68) {
+        // No image of this row lies inside rsoft (r2 >= xy, also as rounded).  The images strictly
+        // between the range's ends lie inside the cutoff: the range is the integers of an interval
+        // widened by 1e-4 of an image spacing around the exact one (whose half-width carries a
+        // relative error of ~2^-22), so every inner image is more than 0.9999 spacings inside it.
+        // (cut_thr < 1e68 keeps that half-width inside image_range's FP32 range, so it is finite.)
+        // They are summed without tests, with r2 formed by FMAs; only the two end images decide
+        // the cutoff, with the reference's rounding.  The x and y sums take the row's sum of
+        // weights (dx, dy are fixed along the row).
+        double skv = 0.0;
+        if (z0 <= z1) {
+          double dz, r2;
+          {
+#pragma clang fp contract(off)
+            dz = dzi + (double)z0 * P.lz;
+            r2 = xy + dz * dz;
+          }
+          if (r2 <= P.cut_thr) {
+            const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(r2);
+            const double kv = (y * y) * y;
+            skv += kv;
+            pz = fma(kv, dz, pz);
+            if (QUIRKS) pp = y;
+            else pp += y;
+          }
+        }
+        for (double fz = (double)z0 + 1.0; fz < fz1; fz += 1.0) {
+          const double dz = fma(fz, P.lz, dzi);
+          const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(fma(dz, dz, xy));
+          const double kv = (y * y) * y;
+          skv += kv;
+          pz = fma(kv, dz, pz);
+          if (QUIRKS) pp = y;
+          else pp += y;
+        }
+        if (z1 > z0) {
+          double dz, r2;
+          {
+#pragma clang fp contract(off)
+            dz = dzi + fz1 * P.lz;
+            r2 = xy + dz * dz;
+          }
+          if (r2 <= P.cut_thr) {
+            const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(r2);
+            const double kv = (y * y) * y;
+            skv += kv;
+            pz = fma(kv, dz, pz);
+            if (QUIRKS) pp = y;
+            else pp += y;
+          }
+        }
+        px = fma(skv, dx, px);
+        py = fma(skv, dy, py);
+        continue;
+      }
       for (double fz = (double)z0; fz <= fz1; fz += 1.0) {
         double dz, r2;
         {
