@@ -212,7 +212,8 @@ def c5_plan_problems():
     return plan_problems("c5")
 
 
-TIE = 0.01   # candidates within 1% of the fastest count as equally fast
+TIE = 0.01   # candidates within 1% of the fastest count as equally fast (repeated timings of one
+             # shape on one box differ by about that much)
 
 
 def pick(table, fewer_partials):

@@ -49,7 +49,8 @@ struct BItem {             // work item: problem, target tile (or 8^3 target blo
   int prob, tile, chunk, pad;
 };
 
-// counters = 0 (generic, column and block-source work lists), counter[3] = skip_generic (1 when
+// counters = 0 (generic and column work lists, counter[4 ... 11] the block-source groups),
+// counter[3] = skip_generic (1 when
 // every generic item belongs to a column candidate: then the generic list has nothing to do unless
 // a candidate fails its test, which clears the word), per-problem source statistics (ST_WORDS
 // each, see kernels.hpp) reset, column flags = 1 (k_batch_check_cols clears them)
@@ -61,6 +62,8 @@ __global__ void k_batch_reset(int nprob, unsigned int* counter, unsigned long lo
     counter[1] = 0u;
     counter[2] = 0u;
     counter[3] = skip_generic;
+#pragma unroll
+    for (int q = 4; q < 12; ++q) counter[q] = 0u;
   }
   if (i < nprob) {
     colflag[i] = 1;
@@ -261,10 +264,16 @@ __global__ __launch_bounds__(64 * W) void k_batch_eval_grid(const BProb* __restr
 // Problems whose sources are 8^3 leaf blocks: the same persistent scheme over (target tile of
 // K x 64, chunk of source blocks) items, evaluated by the column walk of k_eval_gridsrc (the W
 // waves split the chunk's blocks; the per-block source loop of Gravity_getAccelAtPoint.F90:168-204).
+// The items come in 8 groups, group q = the items whose chunk is q mod 8 (items goff[q] ...
+// goff[q + 1] - 1, each group cost-sorted and chunk-major), pulled through counters gcount[q].
+// Workgroup b starts on group b mod 8 -- the XCD the hardware dispatches it to -- so each chunk's
+// masses are read into one XCD's L2, and moves on to the next group when its own is empty, so the
+// load still balances.  Which workgroup runs an item changes no result.
 template <int K, int W, bool MIXED>
 __global__ __launch_bounds__(64 * W) void k_batch_eval_gridsrc(const BProb* __restrict__ probs,
                                                            const BItem* __restrict__ items,
-                                                           int n_items, unsigned int* counter,
+                                                           const int* __restrict__ goff,
+                                                           unsigned int* gcount,
                                                            const double* __restrict__ bms,
                                                            const double* __restrict__ sax,
                                                            const double* __restrict__ be2,
@@ -272,12 +281,25 @@ __global__ __launch_bounds__(64 * W) void k_batch_eval_gridsrc(const BProb* __re
                                                            double* __restrict__ part) {
   __shared__ LdsGSW<MIXED, W> L;
   __shared__ int s_item;
+  int cur = blockIdx.x & 7;             // lane 0's group
   for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {
+      int it = -1;
+      for (int k = 0; k < 8 && it < 0; ++k) {   // its group, then the others in turn
+        const int q = (cur + k) & 7;
+        if (goff[q] == goff[q + 1]) continue;
+        const int i = goff[q] + (int)atomicAdd(&gcount[q], 1u);
+        if (i < goff[q + 1]) {
+          it = i;
+          cur = q;
+        }
+      }
+      s_item = it;                      // -1: every group is empty
+    }
     __syncthreads();
     const int it = s_item;
     __syncthreads();                    // all lanes hold `it` before lane 0 overwrites it
-    if (it >= n_items) break;           // every lane of the workgroup leaves together
+    if (it < 0) break;                  // every lane of the workgroup leaves together
     const BItem w = items[it];
     const BProb& P = probs[w.prob];
     const int64_t b0 = (int64_t)w.chunk * P.chunk;

@@ -2192,7 +2192,7 @@ struct tgrav_batch {
   const Variant* v = nullptr;
   int64_t total_src = 0, red_total = 0, cand_total = 0, total_sblk = 0;
   double inter = 0.0;
-  DevBuf probs, items, gitems, sitems, srcs, sprefix, s4, se2, sax, be2, bms, stats, part, counter, ids,
+  DevBuf probs, items, gitems, sitems, sgoff, srcs, sprefix, s4, se2, sax, be2, bms, stats, part, counter, ids,
       rprefix, cand, cprefix, colflag;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -2204,7 +2204,7 @@ static void batch_free(tgrav_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
-  DevBuf* bufs[] = {&b->probs, &b->items, &b->gitems, &b->sitems, &b->srcs, &b->sprefix, &b->s4,
+  DevBuf* bufs[] = {&b->probs, &b->items, &b->gitems, &b->sitems, &b->sgoff, &b->srcs, &b->sprefix, &b->s4,
                     &b->se2, &b->sax, &b->be2, &b->bms, &b->stats, &b->part, &b->counter, &b->ids,
                     &b->rprefix, &b->cand, &b->cprefix, &b->colflag};
   for (DevBuf* d : bufs) d->release();
@@ -2223,7 +2223,7 @@ static const void* batch_grid_kernel(bool mixed, int W) {
 template <bool MIXED, int W>
 static void batch_gridsrc_launch(tgrav_batch* b, hipStream_t s) {
   tg::k_batch_eval_gridsrc<TG_GS_K, W, MIXED><<<b->sgrid, 64 * W, 0, s>>>(
-      b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->n_sitems, b->counter.as<unsigned int>() + 2,
+      b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->sgoff.as<int>(), b->counter.as<unsigned int>() + 4,
       b->bms.as<double>(), b->sax.as<double>(), b->be2.as<double>(), b->stats.as<unsigned long long>(),
       b->part.as<double>());
 }
@@ -2536,7 +2536,18 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     return out;
   };
   const std::vector<tg::BItem> sorted = by_cost(items, cost), gsorted = by_cost(gitems, gcost),
-                               ssorted = by_cost(sitems, scost);
+                               ssorted_all = by_cost(sitems, scost);
+  // Block-source items in 8 groups by chunk mod 8 (each group keeps the cost order), so that the
+  // workgroups on one XCD share their chunks' masses in its L2 (k_batch_eval_gridsrc).
+  std::vector<tg::BItem> ssorted;
+  std::vector<int> sgoff(9, 0);
+  ssorted.reserve(ssorted_all.size());
+  for (int q = 0; q < 8; ++q) {
+    sgoff[q] = (int)ssorted.size();
+    for (const tg::BItem& w : ssorted_all)
+      if ((w.chunk & 7) == q) ssorted.push_back(w);
+  }
+  sgoff[8] = (int)ssorted.size();
   b->n_items = (int)sorted.size();
   b->n_gitems = (int)gsorted.size();
   b->n_sitems = (int)ssorted.size();
@@ -2554,7 +2565,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   b->sgrid = std::max(1, std::min(b->n_sitems, sslots));
 
   if ((rc = upload(b->probs, probs)) || (rc = upload(b->items, sorted)) ||
-      (rc = upload(b->gitems, gsorted)) || (rc = upload(b->sitems, ssorted)) ||
+      (rc = upload(b->gitems, gsorted)) || (rc = upload(b->sitems, ssorted)) || (rc = upload(b->sgoff, sgoff)) ||
       (rc = upload(b->srcs, srcs)) || (rc = upload(b->sprefix, sprefix)) ||
       (rc = upload(b->ids, ids)) || (rc = upload(b->rprefix, rprefix)) ||
       (rc = upload(b->cand, cand)) || (rc = upload(b->cprefix, cprefix)) ||
@@ -2566,7 +2577,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
       (rc = b->stats.ensure((size_t)std::max(n, 1) * tg::ST_WORDS * sizeof(unsigned long long))) ||
       (rc = b->colflag.ensure((size_t)std::max(n, 1) * sizeof(int))) ||
       (rc = b->part.ensure(std::max<int64_t>(part_total, 1) * sizeof(double))) ||
-      (rc = b->counter.ensure(4 * sizeof(unsigned int)))) {
+      (rc = b->counter.ensure(12 * sizeof(unsigned int)))) {
     batch_free(b);
     return rc;
   }
