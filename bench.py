@@ -66,14 +66,14 @@ BYTES_PER_SOURCE = 40              # x, y, z, m, eps (FP64), read once per launc
 # factors those counters read for a plain 512 MiB device copy in the same runs.  Keyed by the launch
 # shape measured (mode, source split or work items, waves per workgroup); reported only when the
 # bench runs that shape, null otherwise.
-C2_FWD_TRAFFIC = {("mixed", 4, 4): (3144.97 / 0.5 + 128436 / 1.0) * 1024,        # k_eval_grid
-                  ("fp64", 6, 4): (3578.75 / 0.5 + 192301 / 1.0) * 1024}
-C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (56704.7 / 0.5 + 339881 / 1.0) * 1024,   # k_batch_eval_grid
-                   ("fp64", 21199, 4): (58554.3 / 0.5 + 339884 / 1.0) * 1024}
-C5_PLAN_TRAFFIC = {("mixed", 49152, 4): {"fwd": (19076.3 / 0.5 + 788021 / 1.0) * 1024,
-                                         "rev": (235770 / 0.5 + 338844 / 1.0) * 1024},
-                   ("fp64", 24576, 4): {"fwd": (21731.1 / 0.5 + 394016 / 1.0) * 1024,
-                                        "rev": (208362 / 0.5 + 134916 / 1.0) * 1024}}
+C2_FWD_TRAFFIC = {("mixed", 2, 4): (2511.5 / 0.5 + 64283.9 / 1.0) * 1024,        # k_eval_grid
+                  ("fp64", 6, 4): (3591.22 / 0.5 + 192287 / 1.0) * 1024}
+C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (57482.8 / 0.5 + 339881 / 1.0) * 1024,   # k_batch_eval_grid
+                   ("fp64", 21199, 4): (59483.2 / 0.5 + 339886 / 1.0) * 1024}
+C5_PLAN_TRAFFIC = {("mixed", 24576, 4): {"fwd": (19204.7 / 0.5 + 394018 / 1.0) * 1024,
+                                         "rev": (91801.7 / 0.5 + 135142 / 1.0) * 1024},
+                   ("fp64", 24576, 4): {"fwd": (21665.6 / 0.5 + 394017 / 1.0) * 1024,
+                                        "rev": (91794.7 / 0.5 + 135075 / 1.0) * 1024}}
 
 
 def log(*a):
