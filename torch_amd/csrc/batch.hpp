@@ -49,8 +49,8 @@ struct BItem {             // work item: problem, target tile (or 8^3 target blo
   int prob, tile, chunk, pad;
 };
 
-// counters = 0 (generic and column work lists, counter[4 ... 11] the block-source groups),
-// counter[3] = skip_generic (1 when
+// counters = 0 (counter[0] the generic and counter[1] the column work list, counter[4 ... 11] the
+// block-source groups; counter[2] is unused), counter[3] = skip_generic (1 when
 // every generic item belongs to a column candidate: then the generic list has nothing to do unless
 // a candidate fails its test, which clears the word), per-problem source statistics (ST_WORDS
 // each, see kernels.hpp) reset, column flags = 1 (k_batch_check_cols clears them)

@@ -9,7 +9,9 @@ kernels; VARIANT is ignored there); variant/nsplit default to the tuned shape of
 kernel (the block entries for fwdb / revb).  Workloads c4 and c5 take MODE:plan: the bench's
 batch plan (c4: the 64 problems with block targets; c5: stars -> blocks and blocks -> stars),
 run eagerly --reps times with the tuned shape.  --waves sets the waves per workgroup of the 8^3
-block kernels (default: the library's).  After the evaluations it performs one plain
+block kernels (default: the library's); --alternate runs the point runs in turn (rep 1 of each run,
+then rep 2, ...), as a bridge step alternates its two directions, instead of one run after the
+other; --no-block-kernel keeps explicit arrays on the generic kernels.  After the evaluations it performs one plain
 device-to-device copy of --copy-mb MiB (torch copy_), whose FETCH_SIZE / WRITE_SIZE readings
 calibrate the traffic counters against a known byte count.  Everything runs on torch's current
 stream; the program prints one line per step.
@@ -76,35 +78,48 @@ def run_points(args):
     torch.cuda.synchronize()
     BO = _dev(ce["block_origin"])
     BD = _dev(ce["block_size"] / 8)
+    runs = []
     for spec in args.runs:
         parts = spec.split(":")
         mode, direction = parts[0], parts[1]
         src, tgt = (S, C) if direction.startswith("fwd") else (C, S)
-        with GravityField(mode, validate=False) as f:
-            if args.waves:
-                f.set_column_waves(args.waves)
-            nblk = int(BD.numel())
-            if direction == "fwdb":
-                e = autotune.apply_blocks(f, "blockfwd", S[0].numel(), nblk)
-            elif direction == "revb":
-                e = autotune.apply_blocks(f, "blocksrc", S[0].numel(), nblk)
-            else:
-                e = autotune.apply_eval(f, src[0].numel(), tgt[0].numel())
-            if len(parts) > 2:
-                f.set_config(variant=int(parts[2]), nsplit=int(parts[3]) if len(parts) > 3 else 0)
-            if direction == "revb":
-                f.set_source_blocks(BO, BD, C[3])
-            else:
-                f.set_sources(*src)
-            f.set_timing(True)
+        f = GravityField(mode, validate=False)
+        if args.waves:
+            f.set_column_waves(args.waves)
+        if args.no_block_kernel:
+            f.set_block_kernel(False)
+        nblk = int(BD.numel())
+        if direction == "fwdb":
+            e = autotune.apply_blocks(f, "blockfwd", S[0].numel(), nblk)
+        elif direction == "revb":
+            e = autotune.apply_blocks(f, "blocksrc", S[0].numel(), nblk)
+        else:
+            e = autotune.apply_eval(f, src[0].numel(), tgt[0].numel())
+        if len(parts) > 2:
+            f.set_config(variant=int(parts[2]), nsplit=int(parts[3]) if len(parts) > 3 else 0)
+        if direction == "revb":
+            f.set_source_blocks(BO, BD, C[3])
+        else:
+            f.set_sources(*src)
+        f.set_timing(True)
+        if direction == "fwdb":
+            call = (lambda f=f: f.evaluate_blocks(BO, BD))
+        else:
+            call = (lambda f=f, tgt=tgt: f.evaluate(None, tgt[0], tgt[1], tgt[2]))
+        runs.append((spec, e, f, call))
+    if args.alternate:                 # the runs in turn, as a bridge step alternates its directions
+        for _ in range(args.reps):
+            for _, _, _, call in runs:
+                call()
+    else:
+        for _, _, _, call in runs:
             for _ in range(args.reps):
-                if direction == "fwdb":
-                    f.evaluate_blocks(BO, BD)
-                else:
-                    f.evaluate(None, tgt[0], tgt[1], tgt[2])
-            torch.cuda.synchronize()
-            ks = f.kernel_times()
-            print(f"{spec}: tuned={e} kernel ms={[round(k[0], 3) for k in ks]}", flush=True)
+                call()
+    torch.cuda.synchronize()
+    for spec, e, f, _ in runs:
+        ks = f.kernel_times()
+        print(f"{spec}: tuned={e} kernel ms={[round(k[0], 3) for k in ks]}", flush=True)
+        f.close()
 
 
 def main():
@@ -114,6 +129,9 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mb", type=int, default=512)
     ap.add_argument("--waves", type=int, default=0, choices=[0, 1, 2, 4])
+    ap.add_argument("--alternate", action="store_true", help="run the point runs in turn, rep by rep")
+    ap.add_argument("--no-block-kernel", action="store_true",
+                    help="generic kernels for explicit arrays even when they are 8^3 blocks in C order")
     args = ap.parse_args()
 
     import torch
