@@ -5,8 +5,11 @@
 // dx, dy and dx^2 + dy^2 + eps^2 once and only dz and r^2 per cell:
 //   per interaction  dz (1) + r^2 (1) + rsqrt + m y, y^2, w (3) + phi (1) + a (3)
 //   per source/lane  dx, dy (2) + dx^2 + dy^2 + eps^2 (2), shared by 8 cells
-// i.e. ~9.5 VALU instructions instead of 13 (FP32) per interaction.  Output index
-// b*512 + (i*8 + j)*8 + k is the C order of the block format.
+// i.e. ~9.5 VALU instructions instead of 13 (FP32) per interaction.  The 64 columns of a leaf
+// block have the same 8 cell z, so a source's 8 dz are the same in every lane: the workgroup
+// forms them once per source when it stages the source in LDS (F_ZROW), and each lane reads them
+// (two broadcast LDS reads per source) instead of forming them, 8.5 instructions per interaction.
+// Output index b*512 + (i*8 + j)*8 + k is the C order of the block format.
 //
 // Block sources (k_eval_gridsrc): the same sharing with the roles swapped -- the cells of a
 // source block are walked column by column, and every lane forms dx, dy and the partial r^2
@@ -30,15 +33,21 @@
 
 namespace tg {
 
+// Column flag (set by grid_columns only, never by tile_flags): every lane of the workgroup holds
+// the same 8 target z, and the staged sources carry their 8 dz.
+enum { F_ZROW = 16 };
+
 template <int NT>
 struct LdsGrid {
   double4 s4[NT];
   double se2[NT];
+  double dz[NT][8];   // F_ZROW: z_s - z_k of the 8 target cells
 };
 template <int NT>
 struct LdsGridF {
   float4 s4[NT];
   float se2[NT];
+  float4 dz[NT][2];   // F_ZROW: fp32(z_s - c) - fp32(z_k - c) of the 8 target cells
 };
 
 // LDS of a column workgroup of W waves that share one target block and split its sources: the
@@ -55,9 +64,9 @@ struct LdsCol {
 
 template <int FL>
 __device__ __forceinline__ void grid_tile_f64(const double4* __restrict__ s4, const double* __restrict__ se2,
-                                              double tx, double ty, const double (&tz)[8],
-                                              double (&ax)[8], double (&ay)[8], double (&az)[8],
-                                              double (&ap)[8]) {
+                                              const double (*__restrict__ sdz)[8], double tx, double ty,
+                                              const double (&tz)[8], double (&ax)[8], double (&ay)[8],
+                                              double (&az)[8], double (&ap)[8]) {
   // The 8 cells' dependency chains are written step by step across the cells (each step for all 8
   // before the next), so that independent FP64 operations sit next to each other: written cell by
   // cell, the compiler keeps each chain of ~16 dependent FP64 operations in one piece (measured 2-4%
@@ -70,7 +79,7 @@ __device__ __forceinline__ void grid_tile_f64(const double4* __restrict__ s4, co
     double dz[8], r2[8], y[8], e[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      dz[k] = s.z - tz[k];
+      dz[k] = (FL & F_ZROW) ? sdz[j][k] : s.z - tz[k];
       r2[k] = fma(dz[k], dz[k], base);
     }
 #pragma unroll
@@ -97,17 +106,23 @@ __device__ __forceinline__ void grid_tile_f64(const double4* __restrict__ s4, co
 
 template <int FL>
 __device__ __forceinline__ void grid_tile_f32(const float4* __restrict__ s4, const float* __restrict__ se2,
-                                              float tx, float ty, const float (&tz)[8],
-                                              float (&ax)[8], float (&ay)[8], float (&az)[8],
-                                              float (&ap)[8]) {
+                                              const float4 (*__restrict__ sdz)[2], float tx, float ty,
+                                              const float (&tz)[8], float (&ax)[8], float (&ay)[8],
+                                              float (&az)[8], float (&ap)[8]) {
 #pragma unroll 2
   for (int j = 0; j < 64; ++j) {
     const float4 s = s4[j];
     const float dx = s.x - tx, dy = s.y - ty;
     const float base = fmaf(dx, dx, fmaf(dy, dy, se2[j]));
+    float zr[8];
+    if (FL & F_ZROW) {
+      const float4 d0 = sdz[j][0], d1 = sdz[j][1];
+      zr[0] = d0.x; zr[1] = d0.y; zr[2] = d0.z; zr[3] = d0.w;
+      zr[4] = d1.x; zr[5] = d1.y; zr[6] = d1.z; zr[7] = d1.w;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float dz = s.z - tz[k];
+      const float dz = (FL & F_ZROW) ? zr[k] : s.z - tz[k];
       const float r2 = fmaf(dz, dz, base);
       float y = __builtin_amdgcn_rsqf(r2);
       if (FL & F_GUARD) y = r2 > 0.0f ? y : 0.0f;
@@ -140,16 +155,16 @@ __device__ __forceinline__ void grid_sources_f64(LdsGrid<64 * W>& L, const doubl
   for (int64_t base = sb; base < se; base += 64 * W) {
     __syncthreads();
     const int64_t j = base + tid;
-    if (j < se) {
-      L.s4[tid] = src4[j];
-      L.se2[tid] = srce2[j] + e2t;
-    } else {
-      L.s4[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
-      L.se2[tid] = 1.0;
+    const double4 s = j < se ? src4[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+    L.s4[tid] = s;
+    L.se2[tid] = j < se ? srce2[j] + e2t : 1.0;
+    if (FL & F_ZROW) {   // this lane's tz are every lane's
+#pragma unroll
+      for (int k = 0; k < 8; ++k) L.dz[tid][k] = s.z - tz[k];
     }
     __syncthreads();
     if (W == 1 || base + 64 * v < se)   // uniform over the wave
-      grid_tile_f64<FL>(L.s4 + 64 * v, L.se2 + 64 * v, tx, ty, tz, ax, ay, az, ap);
+      grid_tile_f64<FL>(L.s4 + 64 * v, L.se2 + 64 * v, L.dz + 64 * v, tx, ty, tz, ax, ay, az, ap);
   }
 }
 
@@ -168,17 +183,20 @@ __device__ __forceinline__ void grid_sources_f32(LdsGridF<64 * W>& L, const doub
   for (int64_t base = sb; base < se; base += 64 * W) {
     __syncthreads();
     const int64_t j = base + tid;
+    float4 s4 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (j < se) {
       const double4 s = src4[j];
-      L.s4[tid] = make_float4((float)(s.x - cx), (float)(s.y - cy), (float)(s.z - cz), (float)s.w);
-      L.se2[tid] = (float)(srce2[j] + e2t);
-    } else {
-      L.s4[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
-      L.se2[tid] = 1.0f;
+      s4 = make_float4((float)(s.x - cx), (float)(s.y - cy), (float)(s.z - cz), (float)s.w);
+    }
+    L.s4[tid] = s4;
+    L.se2[tid] = j < se ? (float)(srce2[j] + e2t) : 1.0f;
+    if (FL & F_ZROW) {   // this lane's tz are every lane's: the dz the tile loop would form
+      L.dz[tid][0] = make_float4(s4.z - tz[0], s4.z - tz[1], s4.z - tz[2], s4.z - tz[3]);
+      L.dz[tid][1] = make_float4(s4.z - tz[4], s4.z - tz[5], s4.z - tz[6], s4.z - tz[7]);
     }
     __syncthreads();
     if (W > 1 && base + 64 * v >= se) continue;   // uniform over the wave: nothing in its slice
-    grid_tile_f32<FL>(L.s4 + 64 * v, L.se2 + 64 * v, tx, ty, tz, ax, ay, az, ap);
+    grid_tile_f32<FL>(L.s4 + 64 * v, L.se2 + 64 * v, L.dz + 64 * v, tx, ty, tz, ax, ay, az, ap);
     if (++tiles == TG_GRID_FOLD_TILES) {   // uniform: every lane of the wave counts the same tiles
       tiles = 0;
 #pragma unroll
@@ -224,14 +242,20 @@ struct GridF32Run {
   }
 };
 
-#define TG_DISPATCH_GRID(FN, fl, ...)                                         \
+#define TG_DISPATCH_GRID_(FN, Z, fl, ...)                                     \
   switch (fl & (F_GUARD | F_NOPOT | F_SEED32)) {                              \
-    case 0: FN<0>(__VA_ARGS__); break;                                        \
-    case F_GUARD: FN<F_GUARD>(__VA_ARGS__); break;                            \
-    case F_NOPOT: FN<F_NOPOT>(__VA_ARGS__); break;                            \
-    case F_NOPOT | F_GUARD: FN<F_NOPOT | F_GUARD>(__VA_ARGS__); break;        \
-    case F_SEED32: FN<F_SEED32>(__VA_ARGS__); break;                          \
-    default: FN<F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                      \
+    case 0: FN<Z>(__VA_ARGS__); break;                                        \
+    case F_GUARD: FN<Z | F_GUARD>(__VA_ARGS__); break;                        \
+    case F_NOPOT: FN<Z | F_NOPOT>(__VA_ARGS__); break;                        \
+    case F_NOPOT | F_GUARD: FN<Z | F_NOPOT | F_GUARD>(__VA_ARGS__); break;    \
+    case F_SEED32: FN<Z | F_SEED32>(__VA_ARGS__); break;                      \
+    default: FN<Z | F_SEED32 | F_NOPOT>(__VA_ARGS__); break;                  \
+  }
+#define TG_DISPATCH_GRID(FN, fl, ...)                                         \
+  if (fl & F_ZROW) {                                                          \
+    TG_DISPATCH_GRID_(FN, F_ZROW, fl, __VA_ARGS__)                            \
+  } else {                                                                    \
+    TG_DISPATCH_GRID_(FN, 0, fl, __VA_ARGS__)                                 \
   }
 
 // Sums the W waves' partial results (a, b, c, d: N values per lane each) of a workgroup whose
@@ -277,14 +301,15 @@ __device__ __forceinline__ bool wave_sum(double* red, double (&a)[N], double (&b
 // the FP64 anchor of the mixed form (the centre of the group's bounding box); targets t0 + k with
 // k >= nk are not stored.  `o` already points at this source chunk's partials if the sources are
 // split over workgroups.  sst: the source statistics; rt: the largest |coordinate| of the targets
-// (FP64 seed choice).
+// (FP64 seed choice); zrow: every lane of the workgroup holds the same z[8] (F_ZROW).
 template <bool MIXED, int W>
 __device__ __forceinline__ void grid_columns(
     LdsCol<MIXED, W>& L, double x, double y,
     const double (&z)[8], double e2t, double cx, double cy, double cz, int64_t t0, int nk,
     const double4* __restrict__ src4, const double* __restrict__ srce2, int64_t sb, int64_t se,
-    const double* sst, double rt, const Out& o) {
-  const int fl = tile_flags(sst[ST_E2MIN], e2t, e2t, o, MIXED, r2_bound(sst, rt, e2t));
+    const double* sst, double rt, const Out& o, bool zrow) {
+  const int fl = tile_flags(sst[ST_E2MIN], e2t, e2t, o, MIXED, r2_bound(sst, rt, e2t)) |
+                 (zrow ? F_ZROW : 0);
   if constexpr (MIXED) {
     const float tx = (float)(x - cx), ty = (float)(y - cy);
     float tz[8];
@@ -344,7 +369,7 @@ __device__ __forceinline__ void grid_block(LdsCol<MIXED, W>& L, const Tgt& T, in
   const double rt = fmax(max_abs3(T.ox[b], T.oy[b], T.oz[b]),
                          max_abs3(T.ox[b] + 8.0 * d, T.oy[b] + 8.0 * d, T.oz[b] + 8.0 * d));
   grid_columns<MIXED, W>(L, x, y, z, e * e, cx, cy, cz, b * 512 + (int64_t)lane * 8, 8, src4, srce2,
-                         sb, se, sst, rt, o);
+                         sb, se, sst, rt, o, true);
 }
 
 // blockIdx.x = leaf block, blockIdx.y = source chunk.  Requires T.blocks && T.nb == 8 and a
@@ -409,9 +434,15 @@ __device__ __forceinline__ void cols_group(LdsCol<MIXED, W>& L, const Tgt& T, in
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) rt = fmax(rt, __shfl_xor(rt, off));
   }
+  // the same 8 z in every lane (every wave of the workgroup holds these targets): the C order of
+  // 8^3 leaf blocks has it; a ragged group's clamped lanes and other layouts take the per-lane dz
+  bool same = true;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) same = same && z[k] == __shfl(z[k], 0);
+  const bool zrow = __all(same) != 0;
   const int64_t left = T.n - t0;
   grid_columns<MIXED, W>(L, x, y, z, e * e, cx, cy, cz, t0, left < 8 ? (int)left : 8, src4, srce2,
-                         sb, se, sst, rt, o);
+                         sb, se, sst, rt, o, zrow);
 }
 
 // blockIdx.x = group of 512 targets, blockIdx.y = source chunk.  Gated (Out::gate) against

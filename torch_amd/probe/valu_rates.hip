@@ -2,8 +2,14 @@
 // runs on: every lane runs 8 independent chains of one operation for a fixed number of steps
 // (the kernels hold 8 cells or K targets per lane the same way), and every result is stored.
 // Prints, per kind, the time of 5 launches and the rate in instructions and flop per second.
+// Then the cost of the smallest FP32 matrix instruction, v_mfma_f32_4x4x1_16b_f32 (16 blocks of
+// 4x4 outer products: 256 FMAs per wave), alone and between FP32 FMA streams, in cycles per
+// wave-step of one SIMD: whether the matrix core could take the direct sum's a += w d updates
+// off the vector ALU.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
+
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 #define CHK(x)                                                                     \
   do {                                                                             \
@@ -75,6 +81,55 @@ static int run(double* d, int nblk, int steps, int ncu) {
   return 0;
 }
 
+// NF FP32 FMAs (8 chains) then NM 4x4x1 matrix instructions (8 accumulators) per step.
+template <int NF, int NM>
+__global__ __launch_bounds__(256) void k_mfma_mix(float* out, int steps, float seed) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const float b = 1e-3f * (float)(threadIdx.x & 63);
+  float f[8];
+  v4f acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    f[i] = seed + t * 1e-9f + i;
+    acc[i] = (v4f){0.f, 0.f, 0.f, 0.f};
+  }
+  for (int s = 0; s < steps; ++s) {
+#pragma unroll
+    for (int r = 0; r < NF / 8; ++r)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) f[i] = fmaf(f[i], 0.999999f, 1e-7f);
+#pragma unroll
+    for (int i = 0; i < NM; ++i) acc[i] = __builtin_amdgcn_mfma_f32_4x4x1f32(f[i], b, acc[i], 0, 0, 0);
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s += f[i] + acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3];
+  out[t] = s;
+}
+
+template <int NF, int NM>
+static int run_mfma(float* d, int nblk, int steps, int ncu) {
+  hipEvent_t e0, e1;
+  CHK(hipEventCreate(&e0));
+  CHK(hipEventCreate(&e1));
+  for (int w = 0; w < 3; ++w) k_mfma_mix<NF, NM><<<nblk, 256>>>(d, steps, 1.0f);
+  CHK(hipGetLastError());
+  CHK(hipDeviceSynchronize());
+  CHK(hipEventRecord(e0));
+  for (int r = 0; r < 5; ++r) k_mfma_mix<NF, NM><<<nblk, 256>>>(d, steps, 1.0f);
+  CHK(hipEventRecord(e1));
+  CHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CHK(hipEventElapsedTime(&ms, e0, e1));
+  const double wave_steps_per_simd = 5.0 * nblk * 4.0 * steps / (4.0 * ncu);
+  const double ns = ms * 1e6 / wave_steps_per_simd;
+  printf("%2d v_fma_f32 + %d v_mfma_f32_4x4x1_16b_f32 per step %9.3f ms  %7.2f ns = %6.1f cycles at 2.4 GHz\n",
+         NF, NM, ms, ns, ns * 2.4);
+  CHK(hipEventDestroy(e0));
+  CHK(hipEventDestroy(e1));
+  return 0;
+}
+
 int main() {
   int dev = 0, ncu = 0, clk = 0;
   CHK(hipGetDevice(&dev));
@@ -88,6 +143,12 @@ int main() {
   if (run<FMA64>(d, nblk, steps, ncu) || run<MUL64>(d, nblk, steps, ncu) || run<ADD64>(d, nblk, steps, ncu) ||
       run<FMA32>(d, nblk, steps, ncu) || run<MUL32>(d, nblk, steps, ncu) || run<ADD32>(d, nblk, steps, ncu) ||
       run<CVT>(d, nblk, steps, ncu) || run<RSQ32>(d, nblk, steps, ncu) || run<RSQ64>(d, nblk, steps, ncu))
+    return 1;
+  printf("per wave-step of one SIMD (8 chains / accumulators per lane, %d steps):\n", steps / 2);
+  float* f = reinterpret_cast<float*>(d);
+  if (run_mfma<8, 0>(f, nblk, steps / 2, ncu) || run_mfma<0, 8>(f, nblk, steps / 2, ncu) ||
+      run_mfma<8, 8>(f, nblk, steps / 2, ncu) || run_mfma<48, 0>(f, nblk, steps / 2, ncu) ||
+      run_mfma<48, 8>(f, nblk, steps / 2, ncu))
     return 1;
   CHK(hipFree(d));
   return 0;
