@@ -222,7 +222,7 @@ __global__ __launch_bounds__(NT) void k_batch_eval(const BProb* __restrict__ pro
 // target block (group of 512 targets) per work item, evaluated by the column code of k_eval_grid
 // (the W waves split the item's sources and sum their partial results in LDS).
 template <bool MIXED, int W>
-__global__ __launch_bounds__(64 * W) void k_batch_eval_grid(const BProb* __restrict__ probs,
+__global__ __launch_bounds__(64 * W, 3) void k_batch_eval_grid(const BProb* __restrict__ probs,
                                                         const BItem* __restrict__ items,
                                                         int n_items, unsigned int* counter,
                                                         const double4* __restrict__ s4,
