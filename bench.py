@@ -66,10 +66,12 @@ BYTES_PER_SOURCE = 40              # x, y, z, m, eps (FP64), read once per launc
 # factors those counters read for a plain 512 MiB device copy in the same runs.  Keyed by the launch
 # shape measured (mode, source split or work items, waves per workgroup); reported only when the
 # bench runs that shape, null otherwise.
-C2_FWD_TRAFFIC = {("mixed", 2, 4): (2511.5 / 0.5 + 64283.9 / 1.0) * 1024,        # k_eval_grid
+C2_FWD_TRAFFIC = {("mixed", 6, 4): (3602.66 / 0.5 + 193676 / 1.0) * 1024,       # k_eval_grid
+                  ("fp64", 2, 4): (2494.88 / 0.5 + 64014.9 / 1.0) * 1024,
+                  ("mixed", 2, 4): (2511.5 / 0.5 + 64283.9 / 1.0) * 1024,
                   ("fp64", 6, 4): (3591.22 / 0.5 + 192287 / 1.0) * 1024}
-C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (57482.8 / 0.5 + 339881 / 1.0) * 1024,   # k_batch_eval_grid
-                   ("fp64", 21199, 4): (59483.2 / 0.5 + 339886 / 1.0) * 1024}
+C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (56107.1 / 0.5 + 339897 / 1.0) * 1024,   # k_batch_eval_grid
+                   ("fp64", 21199, 4): (59845.0 / 0.5 + 339885 / 1.0) * 1024}
 C5_PLAN_TRAFFIC = {("mixed", 24576, 4): {"fwd": (19204.7 / 0.5 + 394018 / 1.0) * 1024,
                                          "rev": (91801.7 / 0.5 + 135142 / 1.0) * 1024},
                    ("fp64", 24576, 4): {"fwd": (21665.6 / 0.5 + 394017 / 1.0) * 1024,
@@ -509,11 +511,12 @@ def oracle_check(pairs, G):
 
 
 # ------------------------------------------------------------------ c4 / c5 ----
-# Flops the column kernels execute per interaction (the algorithmic count is 21): per cell dz, the
-# r^2 FMA, rsqrt, m y, the potential add, (m y) y y and three FMAs = 14; per source and column of 8
-# cells dx, dy and two FMAs for dx^2 + dy^2 + eps^2 = 6, i.e. 0.75 per cell.  FP64 adds the
-# refinement of the FP32 seed (r2 y, two FMAs, y e, one FMA = 8).
-EXECUTED_FLOP_COLUMN = {"mixed": 14.75, "fp64": 22.75}
+# Flops the column kernels execute per interaction (the algorithmic count is 21): per cell the r^2
+# FMA, rsqrt, m y, the potential add, (m y) y y and three FMAs = 13; per source the 8 dz, formed once
+# for the workgroup's 64 columns (1/64 per cell); per source and column of 8 cells dx, dy and two FMAs
+# for dx^2 + dy^2 + eps^2 = 6, i.e. 0.75 per cell.  FP64 adds the refinement of the FP32 seed (r2 y,
+# two FMAs, y e, one FMA = 8).
+EXECUTED_FLOP_COLUMN = {"mixed": 13.77, "fp64": 21.77}
 
 
 def roofline_of(mode, interactions, kernel_ms, kernel, traffic=None, algorithmic_bytes=None):
