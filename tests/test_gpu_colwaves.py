@@ -120,6 +120,57 @@ def test_column_waves_batch_equals_eager(mode):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("waves", [1, 4])
+def test_shared_and_per_lane_dz_agree(mode, waves, c2):
+    """Explicit column targets whose 64 columns of a group hold the same 8 z take the shared dz (each
+    source's 8 dz formed once, by the lane that stages the source); a group whose columns differ in z
+    forms them per lane.  Reversing the 8 z of one column per group keeps every group's bounding box
+    (the mixed anchor) and sends each group down the per-lane path: every other column gives the same
+    bits as before, the reversed column the same values in reverse order, eagerly and in a batch plan
+    (the persistent kernel), and both layouts meet the bar against the oracle."""
+    import torch
+    st, ce = c2["stars"], c2["cells"]
+    o, dx = ce["block_origin"][:16], ce["block_size"][:16] / 8
+    bx, by, bz = oracle.block_centres(o[:, 0], o[:, 1], o[:, 2], dx)
+    te = np.repeat(dx, 512)
+    rz = bz.reshape(-1, 64, 8).copy()
+    rz[:, 37, :] = rz[:, 37, ::-1]                  # column 37 of every group, z reversed
+    rz = rz.ravel()
+    col = np.zeros(len(bz), dtype=bool)
+    col.reshape(-1, 64, 8)[:, 37, :] = True
+    perm = np.arange(len(bz)).reshape(-1, 64, 8)
+    perm[:, 37, :] = perm[:, 37, ::-1]               # b[t] corresponds to a[perm[t]]
+    perm = perm.ravel()
+    S = [st[k][:2000] for k in ("x", "y", "z", "m", "eps")]
+    with StellarGravityField(mode) as f:
+        f.set_column_waves(waves)
+        f.set_stars(*S)
+        f.set_config(nsplit=1)
+        a = f.evaluate(te, bx, by, bz)
+        b = f.evaluate(te, bx, by, rz)
+        for p, q in zip(a, b):
+            assert np.array_equal(p[~col], q[~col])
+            assert np.array_equal(p[perm[col]], q[col])
+        ref = oracle.eval_plummer(*S, bx, by, rz, te, G, mode=oracle.NEUMAIER, threads=16)
+        s = _util.summary(b, ref)
+        print(mode, "waves", waves, "per-lane dz vs oracle", s, flush=True)
+        check_bar(mode, s)
+        if waves == 4:
+            dev = problems_from_workload([dict(sources=dict(zip(("x", "y", "z", "m", "eps"), S)),
+                                               targets=dict(x=bx, y=by, z=z, eps=te)) for z in (bz, rz)])
+            set_item(f, 1e30)                       # one chunk per problem, as the eager launches
+            plan = BatchPlan(f, dev)
+            got = to_np(plan.run(graph=True))
+            torch.cuda.synchronize()
+            plan.close()
+            set_item(f, 0)
+            for q in range(4):
+                assert np.array_equal(got[0][q], a[q]), q
+                assert np.array_equal(got[1][q], b[q]), q
+        f.set_config(nsplit=0)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
 @pytest.mark.parametrize("waves", [1, 2, 4])
 def test_block_source_waves_match_oracle(mode, waves, c2):
     """13 source blocks (ragged rounds of W blocks, chunks of 1, 2 and 5 blocks) onto 700 stars
