@@ -1,5 +1,6 @@
 """The autotuner's choice among timed candidates (no GPU needed): the fastest candidate, except
-that among candidates within 1% of it the one that writes the fewest partial sums wins."""
+that among candidates within 1% of it (2% for the block-target split, whose chunks each write half
+a launch's algorithmic bytes of partial sums) the one that writes the fewest partial sums wins."""
 from torch_amd import autotune
 
 
@@ -25,3 +26,12 @@ def test_batch_rows_tie_break_on_work_items():
     table = [dict(item=0.0, items_n=49152, ms=190.5), dict(item=2.5e7, items_n=24576, ms=191.9),
              dict(item=1e8, items_n=8192, ms=200.0)]
     assert autotune.pick(table, lambda r: r["items_n"])["item"] == 2.5e7
+
+
+def test_block_target_split_takes_the_wider_tie():
+    # the c2 mixed forward row of one tuning run: 8 chunks fastest, 2 chunks 2.0% slower
+    table = [dict(nsplit=0, ms=2.657), dict(nsplit=1, ms=2.482), dict(nsplit=2, ms=2.405),
+             dict(nsplit=3, ms=2.436), dict(nsplit=4, ms=2.384), dict(nsplit=6, ms=2.377),
+             dict(nsplit=8, ms=2.358), dict(nsplit=12, ms=2.613), dict(nsplit=16, ms=2.673)]
+    assert autotune.pick(table, autotune._chunks)["nsplit"] == 6
+    assert autotune.pick(table, autotune._chunks, autotune.TIE_BLOCK_TARGETS)["nsplit"] == 2

@@ -214,14 +214,18 @@ def c5_plan_problems():
 
 TIE = 0.01   # candidates within 1% of the fastest count as equally fast (repeated timings of one
              # shape on one box differ by about that much)
+# Each source chunk of the block-target kernel writes 32 B of FP64 partial sums per cell, half the
+# algorithmic bytes of the launch (64 B per cell), and the reduction reads them back: its split keeps
+# the fewest chunks within 2% of the fastest.
+TIE_BLOCK_TARGETS = 0.02
 
 
-def pick(table, fewer_partials):
-    """The fastest row, except that among the rows within TIE of it the one that writes the fewest
-    partial sums to memory wins (fewer source chunks: less memory and less traffic for no
+def pick(table, fewer_partials, tie=TIE):
+    """The fastest row, except that among the rows within `tie` of it the one that writes the
+    fewest partial sums to memory wins (fewer source chunks: less memory and less traffic for no
     measurable time).  fewer_partials(row) orders the rows by that (smaller = fewer)."""
     best = min(r["ms"] for r in table)
-    near = [r for r in table if r["ms"] <= best * (1.0 + TIE)]
+    near = [r for r in table if r["ms"] <= best * (1.0 + tie)]
     return min(near, key=lambda r: (fewer_partials(r), r["ms"]))
 
 
@@ -256,7 +260,8 @@ def tune_blocks(mode, workload="c2", reps=5, log=print):
             ms = _time(lambda: f.evaluate_blocks(BO, BD), reps)
             table.append(dict(nsplit=sp, ms=ms))
             log(f"  {mode} block targets nsplit={sp}: {ms:.3f} ms")
-        out[blocks_key(mode, "blockfwd", ns, nblk, f.column_waves())] = pick(table, _chunks)
+        out[blocks_key(mode, "blockfwd", ns, nblk, f.column_waves())] = pick(table, _chunks,
+                                                                              TIE_BLOCK_TARGETS)
         f.set_source_blocks(BO, BD, CM)
         table = []
         for sp in (0, 8, 16, 24, 32, 48, 64, 100, 128, 200, 256):
