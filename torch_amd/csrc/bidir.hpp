@@ -293,9 +293,12 @@ struct LdsBiGrid {
   typename std::conditional<MIXED, float4, double4>::type p4[64];
   T pe2[64];
   typename std::conditional<MIXED, float4, double4>::type acc[64];   // particle sums of the tile
+  // ZROW (every lane holds the same 8 cell z): z_p - z_k of each staged particle, cells 0-3 in
+  // dz[0], 4-7 in dz[1], formed once by the lane that stages the particle
+  typename std::conditional<MIXED, float4, double4>::type dz[2][64];
 };
 
-template <bool MIXED, bool GUARD>
+template <bool MIXED, bool GUARD, bool ZROW>
 __device__ __forceinline__ void bi_grid_tiles(
     LdsBiGrid<MIXED>& L, const double4* __restrict__ p4, const double* __restrict__ pe2,
     int64_t pb, int64_t pe, double e2c, double ax_, double ay_, double az_,
@@ -310,17 +313,23 @@ __device__ __forceinline__ void bi_grid_tiles(
   for (int64_t base = pb; base < pe; base += 64) {
     __syncthreads();
     const int64_t j = base + lane;
+    T pz = (T)0;
     if (j < pe) {
       const double4 s = p4[j];
       if constexpr (MIXED)
         L.p4[lane] = make_float4((float)(s.x - ax_), (float)(s.y - ay_), (float)(s.z - az_), (float)s.w);
       else
         L.p4[lane] = s;
+      pz = MIXED ? (T)(s.z - az_) : (T)s.z;
       L.pe2[lane] = (T)(pe2[j] + e2c);
     } else {                      // padding particle: zero mass, eps^2 = 1
       if constexpr (MIXED) L.p4[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
       else L.p4[lane] = make_double4(0.0, 0.0, 0.0, 0.0);
       L.pe2[lane] = (T)1.0;
+    }
+    if constexpr (ZROW) {         // this lane's qz are every lane's: the dz the loop would form
+      L.dz[0][lane] = {pz - qz[0], pz - qz[1], pz - qz[2], pz - qz[3]};
+      L.dz[1][lane] = {pz - qz[4], pz - qz[5], pz - qz[6], pz - qz[7]};
     }
     if constexpr (MIXED) L.acc[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     else L.acc[lane] = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -337,9 +346,15 @@ __device__ __forceinline__ void bi_grid_tiles(
       constexpr bool sumw = TG_BIGRID_SUMW == 2 || (TG_BIGRID_SUMW == 1 && !MIXED) ||
                             (TG_BIGRID_SUMW == 3 && MIXED);
       T ux = 0, uy = 0, uz = 0, up = 0, uw = 0;
+      T zr[8];
+      if constexpr (ZROW) {
+        const auto d0 = L.dz[0][jj], d1 = L.dz[1][jj];
+        zr[0] = d0.x; zr[1] = d0.y; zr[2] = d0.z; zr[3] = d0.w;
+        zr[4] = d1.x; zr[5] = d1.y; zr[6] = d1.z; zr[7] = d1.w;
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const T dz = P.z - qz[k];
+        const T dz = ZROW ? zr[k] : P.z - qz[k];
         const T r2 = fma(dz, dz, base2);
         T y;
         if constexpr (MIXED) {
@@ -467,8 +482,18 @@ __global__ __launch_bounds__(64) void k_bidir_grid(CellBlocks C, const double4* 
 #pragma unroll
   for (int k = 0; k < 8; ++k) hx[k] = hy[k] = hz[k] = hp[k] = 0.0;
   double* pp = ppart + b * 4 * np;
-  if (guard) bi_grid_tiles<MIXED, true>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
-  else bi_grid_tiles<MIXED, false>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+  // the same 8 z in every lane: leaf blocks always, explicit cells in their C order (wave vote)
+  bool same = true;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) same = same && z[k] == __shfl(z[k], 0);
+  const bool zrow = !C.cx || __all(same) != 0;
+  if (zrow) {
+    if (guard) bi_grid_tiles<MIXED, true, true>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+    else bi_grid_tiles<MIXED, false, true>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+  } else {
+    if (guard) bi_grid_tiles<MIXED, true, false>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+    else bi_grid_tiles<MIXED, false, false>(L, p4, pe2, pb, pe, e2c, ax_, ay_, az_, qx, qy, qz, qm, hx, hy, hz, hp, pp, np);
+  }
   Out o = CO;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
 #pragma unroll
