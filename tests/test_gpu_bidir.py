@@ -124,6 +124,35 @@ def test_blocks_match_explicit_and_oracle(mode, nblk, npart):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_shared_and_per_lane_dz_agree(mode):
+    """Explicit cells in the blocks' C order, whose 64 columns per block hold the same 8 z, take the
+    column kernel with each staged particle's 8 dz formed once; listing the 8 cells of one column per
+    block in reverse z order keeps the column structure and every block's bounding box (the mixed
+    anchor) but not the common z, so those blocks form dz per lane.  Each cell's result is the same
+    bits at its new place; the particles' sums (added over a column's cells in the other order)
+    agree within the bar, and both meet it against the oracle."""
+    B, C, P = blocks_and_particles(4242, 4, 500)
+    perm = np.arange(len(C["x"])).reshape(-1, 64, 8)
+    perm[:, 21, :] = perm[:, 21, ::-1]
+    perm = perm.ravel()
+    R = {k: v[perm] for k, v in C.items()}
+    with GravityField(mode) as f:
+        ca, pa = run(f, C, P)
+        cb, pb = run(f, R, P)
+    for q in range(4):
+        np.testing.assert_array_equal(cb[q], ca[q][perm])
+    (qa, qb, qc, qp), _ = oracle.bidirectional(
+        C["x"], C["y"], C["z"], C["m"], C["eps"], P["x"], P["y"], P["z"], P["m"], P["eps"], G)
+    for name, got, ref in (("shared vs per-lane", pb, pa), ("per-lane vs oracle", pb, (qa, qb, qc, qp))):
+        s = _util.summary(got, ref)
+        print(mode, "particles", name, s, flush=True)
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (name, s)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (name, s)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_block_order_with_varied_softening_in_a_block(mode):
     """Cells in the blocks' C order whose softening is not uniform within one 512-cell block fail the
     structure test (the column kernel reads one eps per block) and must take the generic fused
