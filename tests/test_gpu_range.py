@@ -1,9 +1,10 @@
 """FP64 parity in cgs, where r^2 leaves the FP32 range (needs an MI355X).
 
 FLASH works in cgs (the hydro side's converter, src/torch_mainloop.py:126): positions of
-1e19-1e20 cm, so r^2 reaches 1e41, beyond FLT_MAX = 3.4e38.  The FP64 kernels seed 1/sqrt in
-FP32 only for tiles whose bound on r^2 (source statistics and the tile's targets) stays below
-1e36; these inputs span both sides of that bound (+-30 pc boxes, and a compact 0.1 pc cluster),
+1e19-1e20 cm, so r^2 reaches 1e41, beyond FLT_MAX = 3.4e38.  The FP64 kernels seed 1/sqrt with
+v_rsq_f64 (built with TG_FP64_SEED32=1, in FP32 for the tiles whose bound on r^2, from the source
+statistics and the tile's targets, stays below 1e36); these inputs span both sides of that bound
+(+-30 pc boxes, and a compact 0.1 pc cluster),
 through every FP64 kernel family: explicit points, 8^3 block targets (column kernel), block
 sources, explicit points in block order, the batch plan, and FLASH's kernels isolated and
 periodic.  Bar: 1e-12 of each target's sum of |terms| against the oracle in cgs.  The mixed

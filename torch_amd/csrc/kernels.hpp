@@ -113,8 +113,9 @@ __device__ __forceinline__ void block_minmax(double (&mn)[NV], double (&mx)[NV],
 //   F_GUARD  the tile's smallest possible eps^2 is 0: zero r2 -> the pair contributes 0
 //   F_UNIF   all targets of the tile share eps_t^2: folded into the staged source eps^2
 //   F_NOPOT  the caller did not ask for the potential
-//   F_SEED32 (FP64 only) r2 >= 1e-30 is guaranteed: seed 1/sqrt from v_rsq_f32 instead of the
-//            quarter-rate v_rsq_f64 (both seeds are ~2^-24 accurate; the refinement is the same)
+//   F_SEED32 (FP64 only, built with TG_FP64_SEED32 = 1) r2 >= 1e-30 is guaranteed: seed 1/sqrt
+//            from v_rsq_f32 instead of the quarter-rate v_rsq_f64 (both seeds are ~2^-24 accurate;
+//            the refinement is the same)
 enum { F_GUARD = 1, F_UNIF = 2, F_NOPOT = 4, F_SEED32 = 8 };
 
 // --------------------------------------------------------------- FP64 tile loop ----
@@ -218,6 +219,15 @@ struct LdsF32 {
 // seed converts r^2 to float, and FLT_MAX is 3.4e38 (a separation of 1.8e19 in the caller's
 // units; cgs coordinates of a few kpc reach it).
 #define TG_F32_HUGE_R2 1e36
+// 1: the FP64 kernels seed 1/sqrt from v_rsq_f32 of the converted r2 where the tile's range allows
+// (F_SEED32, P.s32); 0: from v_rsq_f64 everywhere.  Measured on MI355X with the block kernels'
+// instruction mixes, the v_rsq_f64 seed (one quarter-rate transcendental, which the FP64 arithmetic
+// around it hides) is as fast or faster than the two conversions plus v_rsq_f32: c4 FP64 column
+// kernel 25.25 against 27.13 ms on one box and 26.55 against 26.69 on another, c5 forward 200.0
+// against 202.2 ms, c2 5.20 / 5.03 against 5.29 / 5.08 ms; the FLASH kernels within 1% either way.
+#ifndef TG_FP64_SEED32
+#define TG_FP64_SEED32 0
+#endif
 
 // Statistics of a packed source set, device words written by the pack kernels: doubles stored as
 // their bits (non-negative, so integer atomicMin / atomicMax order them like the values).
@@ -280,7 +290,7 @@ __device__ __forceinline__ int tile_flags(double e2s_min, double e2t_min, double
     if (!(lo > TG_F32_TINY_E2)) fl |= F_GUARD;
   } else {
     if (!(lo > 0.0)) fl |= F_GUARD;
-    else if (lo > TG_F32_TINY_E2 && r2max < TG_F32_HUGE_R2) fl |= F_SEED32;
+    else if (TG_FP64_SEED32 && lo > TG_F32_TINY_E2 && r2max < TG_F32_HUGE_R2) fl |= F_SEED32;
   }
   if (O.nopot) fl |= F_NOPOT;
   return fl;

@@ -807,7 +807,7 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   P.nrep = per ? c->nrep : 0;
   // Pairs reaching the refined 1/sqrt have r2 >= rsoft^2 (the rest take the softened
   // branch); the FP32 seed is valid from 1e-30 up, and up to the tile's r^2 bound (in the kernels).
-  P.s32 = c->rsoft * c->rsoft > TG_F32_TINY_E2;
+  P.s32 = TG_FP64_SEED32 && c->rsoft * c->rsoft > TG_F32_TINY_E2;
   P.reach = per ? (double)(c->nrep + 1) * std::max(P.lx, std::max(P.ly, P.lz)) : 0.0;
   // Sources given as 8^3 blocks take the block-source kernel.  Explicit sources that may be
   // C-ordered 8^3 blocks (flagged on the device when they were set) launch it too, gated against
