@@ -33,6 +33,11 @@
 
 namespace tg {
 
+// Block-source kernels (mixed): the next source block's masses loaded into registers while the
+// current block is walked (c5 reverse 91.9 -> 90.4 ms).
+#ifndef TG_GS_PREFETCH
+#define TG_GS_PREFETCH 1
+#endif
 #ifndef TG_WAVE_LOCAL_STAGE
 #define TG_WAVE_LOCAL_STAGE 1
 #endif
@@ -620,14 +625,43 @@ struct GsRun {
                                              double (&hx)[K], double (&hy)[K], double (&hz)[K],
                                              double (&hp)[K]) {
     const int v = threadIdx.x >> 6;
+    // PF: the wave's next block (masses, axis value, eps^2) in registers while the current one is
+    // walked (mixed only: the FP64 kernel has no registers to spare)
+    constexpr bool PF = TG_GS_PREFETCH && MIXED;
+    const int lane = threadIdx.x & 63;
+    double pm[8], pa = 0.0, pe = 0.0;
+    auto fetch = [&](int64_t b) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) pm[i] = bm[b * 512 + lane + 64 * i];
+      if (lane < 24) pa = sax[b * 24 + lane];
+      if (lane == 0) pe = be2[b];
+    };
+    if (PF && b0 + v < b1) fetch(b0 + v);
     for (int64_t base = b0; base < b1; base += W) {   // uniform over the workgroup
       const int64_t blk = base + v;
       // Each wave stages and reads only its own block here too, but with the waves' rounds left
       // unsynchronised this loop measured 0.3% slower (c5 reverse); it keeps the workgroup barriers.
       __syncthreads();
-      if (blk < b1) gs_stage_wave<MIXED>(L.s[v], sax, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
+      if (PF && blk < b1) {             // the values gs_stage_wave would load, from the registers
+        typedef typename LdsGS<MIXED>::T T;
+        LdsGS<MIXED>& S = L.s[v];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) S.m[lane + 64 * i] = (T)pm[i];
+        if (lane < 24) {
+          const int axis = lane >> 3, idx = lane & 7;
+          const double a = axis == 0 ? ax_ : axis == 1 ? ay_ : az_;
+          const T val = MIXED ? (T)(pa - a) : (T)pa;
+          if (axis == 0) S.cx[idx] = val;
+          else if (axis == 1) S.cy[idx] = val;
+          else S.cz[idx] = val;
+        }
+        if (lane == 0) S.e2 = (T)(pe + ((FL & F_UNIF) ? e2u : 0.0));
+      } else if (!PF && blk < b1) {
+        gs_stage_wave<MIXED>(L.s[v], sax, be2, bm, blk, (FL & F_UNIF) ? e2u : 0.0, ax_, ay_, az_);
+      }
       __syncthreads();
       if (blk >= b1) continue;          // uniform over the wave: no block left for it this round
+      if (PF && blk + W < b1) fetch(blk + W);
       T ax[K], ay[K], az[K], ap[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = ap[k] = (T)0;
