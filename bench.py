@@ -72,8 +72,8 @@ C2_FWD_TRAFFIC = {("mixed", 6, 4): (3602.66 / 0.5 + 193676 / 1.0) * 1024,       
                   ("fp64", 6, 4): (3591.22 / 0.5 + 192287 / 1.0) * 1024}
 C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (56107.1 / 0.5 + 339897 / 1.0) * 1024,   # k_batch_eval_grid
                    ("fp64", 21199, 4): (59845.0 / 0.5 + 339885 / 1.0) * 1024}
-C5_PLAN_TRAFFIC = {("mixed", 24576, 4): {"fwd": (19204.7 / 0.5 + 394018 / 1.0) * 1024,
-                                         "rev": (91801.7 / 0.5 + 135142 / 1.0) * 1024},
+C5_PLAN_TRAFFIC = {("mixed", 49152, 4): {"fwd": (18082.8 / 0.5 + 787997 / 1.0) * 1024,
+                                         "rev": (106837 / 0.5 + 170382 / 1.0) * 1024},
                    ("fp64", 24576, 4): {"fwd": (21665.6 / 0.5 + 394017 / 1.0) * 1024,
                                         "rev": (91794.7 / 0.5 + 135075 / 1.0) * 1024}}
 

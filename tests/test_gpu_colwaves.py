@@ -1,7 +1,8 @@
 """Block kernels with several waves per workgroup (tgrav_set_column_waves).  Column kernel (block
 targets): the W waves of a workgroup hold the same 8^3 leaf block, split its sources in rounds of
 64 W, and add their partial sums in LDS in wave order.  Block-source kernel: the W waves hold the
-same 256 targets and split the source blocks (wave v takes every W-th block).
+same tile of targets (2 per lane mixed, 4 FP64) and split the source blocks (wave v takes every
+W-th block).
 
 For every W in (1, 2, 4): FP64 within 1e-12 of the FP64 oracle and mixed within the parity bar
 (<= 1e-6 median, <= 1e-4 max relative acceleration error), with source counts below one round

@@ -658,13 +658,24 @@ static int launch_grid(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out& 
   return 0;
 }
 
-// Explicit targets against 8^3 source blocks: K = 4 targets per lane, tiles of 256 targets, a
+// Explicit targets against 8^3 source blocks: K targets per lane, tiles of 64 K targets, a
 // workgroup of W = col_waves waves per tile (splitting its source blocks), source blocks split
-// further over blockIdx.y.
-#define TG_GS_K 4
+// further over blockIdx.y.  K = 2 in the mixed kernel, 4 in the FP64 one: measured on c5's reverse
+// (plan) 87.7 against 90.2 ms (mixed, K = 2 against 4) and 204.8 against 206.4 ms (FP64, K = 4
+// against 2); the FP64 kernel holds each block's dz for its K targets and already needs 240 VGPRs.
+#ifndef TG_GS_K_MIXED
+#define TG_GS_K_MIXED 2
+#endif
+#ifndef TG_GS_K_FP64
+#define TG_GS_K_FP64 4
+#endif
+template <bool MIXED>
+static constexpr int gs_k() { return MIXED ? TG_GS_K_MIXED : TG_GS_K_FP64; }
+static int gs_k_mode(int mode) { return mode == TGRAV_MIXED ? TG_GS_K_MIXED : TG_GS_K_FP64; }
+
 static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk, int64_t* ntiles,
                           int* nsplit, int64_t* chunk) {
-  const int K = TG_GS_K, W = c->col_waves;
+  const int K = gs_k_mode(mode), W = c->col_waves;
   *ntiles = (nt + K * 64 - 1) / (K * 64);
   int want = c->nsplit[mode];
   const int tuned = tuned_block_split(c, mode, "blocksrc", nt, nblk, W);
@@ -686,7 +697,7 @@ static void gridsrc_shape(const tgrav_ctx* c, int mode, int64_t nt, int64_t nblk
 template <bool MIXED, int W>
 static void gridsrc_kernel_w(tgrav_ctx* c, dim3 g, const tg::Tgt& T, const tg::Out& o, int64_t chunk,
                              int64_t nblk) {
-  tg::k_eval_gridsrc<TG_GS_K, W, MIXED><<<g, 64 * W, 0, c->stream>>>(
+  tg::k_eval_gridsrc<gs_k<MIXED>(), W, MIXED><<<g, 64 * W, 0, c->stream>>>(
       T, c->sax.as<double>(), c->sbe2.as<double>(), c->sms.as<double>(), nblk, chunk,
       c->stat.as<const double>(), o);
 }
@@ -2211,7 +2222,7 @@ static void batch_free(tgrav_batch* b) {
   delete b;
 }
 
-// Block-source work items: tiles of TG_GS_K x 64 targets, the eager kernel's shape.
+// Block-source work items: tiles of K x 64 targets (gs_k), the eager kernel's shape.
 
 // The persistent column kernel for W waves per workgroup.
 static const void* batch_grid_kernel(bool mixed, int W) {
@@ -2222,16 +2233,16 @@ static const void* batch_grid_kernel(bool mixed, int W) {
 
 template <bool MIXED, int W>
 static void batch_gridsrc_launch(tgrav_batch* b, hipStream_t s) {
-  tg::k_batch_eval_gridsrc<TG_GS_K, W, MIXED><<<b->sgrid, 64 * W, 0, s>>>(
+  tg::k_batch_eval_gridsrc<gs_k<MIXED>(), W, MIXED><<<b->sgrid, 64 * W, 0, s>>>(
       b->probs.as<tg::BProb>(), b->sitems.as<tg::BItem>(), b->sgoff.as<int>(), b->counter.as<unsigned int>() + 4,
       b->bms.as<double>(), b->sax.as<double>(), b->be2.as<double>(), b->stats.as<unsigned long long>(),
       b->part.as<double>());
 }
 
 static const void* batch_gridsrc_kernel(bool mixed, int W) {
-  if (W == 4) return mixed ? (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 4, true> : (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 4, false>;
-  if (W == 2) return mixed ? (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 2, true> : (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 2, false>;
-  return mixed ? (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 1, true> : (const void*)tg::k_batch_eval_gridsrc<TG_GS_K, 1, false>;
+  if (W == 4) return mixed ? (const void*)tg::k_batch_eval_gridsrc<gs_k<true>(), 4, true> : (const void*)tg::k_batch_eval_gridsrc<gs_k<false>(), 4, false>;
+  if (W == 2) return mixed ? (const void*)tg::k_batch_eval_gridsrc<gs_k<true>(), 2, true> : (const void*)tg::k_batch_eval_gridsrc<gs_k<false>(), 2, false>;
+  return mixed ? (const void*)tg::k_batch_eval_gridsrc<gs_k<true>(), 1, true> : (const void*)tg::k_batch_eval_gridsrc<gs_k<false>(), 1, false>;
 }
 
 template <bool MIXED, int W>
@@ -2407,7 +2418,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   const int CW = c->col_waves;
   const int gslots = occupancy(batch_grid_kernel(mode == TGRAV_MIXED, CW), 64 * CW, 0);
   const int sslots = occupancy(batch_gridsrc_kernel(mode == TGRAV_MIXED, CW), 64 * CW, 0);
-  const int64_t GT = (int64_t)TG_GS_K * 64;   // targets per block-source tile
+  const int64_t GT = (int64_t)gs_k_mode(mode) * 64;   // targets per block-source tile
   // Form of each problem's work items.
   std::vector<int> form(n, tg::FORM_GENERIC);
   double tot_list[3] = {0.0, 0.0, 0.0};   // interactions on the generic, column, block-source lists
