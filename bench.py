@@ -28,7 +28,9 @@ takes the autotuner's launch shape (torch_amd/tune_cache.json).
       potential), FP64.
   c1periodic: c1 reverse (4,096 cells -> 1,024 stars, 4 pc box) with FLASH's spline softening
       and its periodic image sum (nrep = 6: images within 24 pc), FP64; the rate counts pairs,
-      `image_interactions_per_step` the images summed.
+      `image_interactions_per_step` the images summed.  The pairs are grouped by separation
+      (sort, grouped walk, per-star sums, all in the step); `per_target_walk` times each star's
+      image walk on its own lane beside it.
   c2pm: the particle-mesh pieces beside the kick on c2's 2,000 leaf blocks with 4 guard layers
       (8,192,000 cells): CIC deposit of the 9,174 stars inside the box, finite-difference
       acceleration fused with the grid kick, CIC interpolation back to the stars (the Poisson
@@ -272,16 +274,17 @@ def run_fused(mode, dev, ns, nc, steps, warmup):
 
 
 # ------------------------------------------------------- c2flash / c1periodic ----
-def run_flash(dev, steps, warmup, rsoft, box=None, nrep=6):
+def run_flash(dev, steps, warmup, rsoft, box=None, nrep=6, per_target=False):
     """Cells -> stars with FLASH's spline kernel (reference potential), FP64; cells as 8^3
-    leaf blocks when the workload has them, else as explicit arrays."""
+    leaf blocks when the workload has them, else as explicit arrays.  Periodic: the pairs
+    grouped by separation, or (per_target) each star's image walk on its own lane."""
     import torch
 
     from torch_amd import GasGravityField
     rev = GasGravityField("fp64", validate=False)
     rev.set_softening("spline", rsoft, quirks=True)
     if box is not None:
-        rev.set_periodic(box, nrep)
+        rev.set_periodic(box, nrep, per_target=per_target)
     S = [dev[k] for k in ("sx", "sy", "sz", "sm", "se")]
     C = [dev[k] for k in ("cx", "cy", "cz", "cm", "ce")]
 
@@ -835,8 +838,18 @@ def main():
                           "image_interactions_per_step": inter_step * images}
         kern_inter = inter_step * images
         achieved = FLOP_PER_INTERACTION * kern_inter / (r["kernel_ms"] * 1e-3) / 1e12
-        line["roofline"] = {"bound": "valu", "kernel": ("k_eval_flash<1,256,periodic>" if periodic
-                                                          else "k_eval_flash_gridsrc<2,256>"),
+        if periodic:   # the per-target image walk on the same inputs, for comparison
+            rt = run_flash(dev, args.steps, args.warmup, rsoft, box, nrep, per_target=True)
+            ach_t = FLOP_PER_INTERACTION * inter_step * images / (rt["kernel_ms"] * 1e-3) / 1e12
+            ea_t, ep_t = rel_err(rt["res"], r["res"])
+            log(f"c1periodic per-target walk: {rt['ms']:.3f} ms/step, kernel {rt['kernel_ms']:.3f} ms "
+                f"({ach_t / PEAK_FP64_TFLOPS:.3f} of peak)")
+            line["per_target_walk"] = {"kernel": "k_eval_flash<1,256,periodic>", "ms_per_step": rt["ms"],
+                                       "kernel_ms": rt["kernel_ms"], "frac": ach_t / PEAK_FP64_TFLOPS,
+                                       "max_rel_acc_vs_grouped": float(ea_t.max()),
+                                       "max_rel_pot_vs_grouped": float(ep_t.max())}
+        line["roofline"] = {"bound": "valu", "kernel": ("k_eval_flash_pairs (pairs grouped by separation)"
+                                                          if periodic else "k_eval_flash_gridsrc<2,256>"),
                             "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                             "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel_ms": r["kernel_ms"],
                             "flop_per_interaction": FLOP_PER_INTERACTION,

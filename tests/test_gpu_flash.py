@@ -178,11 +178,14 @@ def test_flash_decisions_at_rsoft(field):
             np.testing.assert_allclose(got[c], ref[c], rtol=1e-13, atol=1e-13 * G / rs ** 2)
 
 
+@pytest.mark.parametrize("per_target", [False, True], ids=["pairs", "per_target"])
 @pytest.mark.parametrize("kernel", ["spline", "linear"])
 @pytest.mark.parametrize("quirks", [False, True])
-def test_flash_periodic_matches_oracle(field, kernel, quirks):
+def test_flash_periodic_matches_oracle(field, kernel, quirks, per_target):
     """Periodic image sum with the reference's nrep = 6 in an anisotropic box; some points lie
-    outside the box."""
+    outside the box.  Both image walks: the pairs grouped by separation (few pairs here, so a
+    wave's separations spread widely and most images take the reference's test) and the
+    per-target walk."""
     rng = W.rng_for(43)
     box = (1.0, 1.3, 0.9)
     ns, nt = 80, 70
@@ -192,7 +195,7 @@ def test_flash_periodic_matches_oracle(field, kernel, quirks):
     m = rng.uniform(0.1, 3.0, ns)
     rs = 0.15
     field.set_softening(kernel, rs, quirks=quirks)
-    field.set_periodic(box, nrep=6)
+    field.set_periodic(box, nrep=6, per_target=per_target)
     field.set_sources(s[:, 0], s[:, 1], s[:, 2], m, None)
     got = field.evaluate(None, t[:, 0], t[:, 1], t[:, 2])
     ref = oracle.eval_flash(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], G, rs,
@@ -201,16 +204,17 @@ def test_flash_periodic_matches_oracle(field, kernel, quirks):
     check(got, ref, sa, sp)
 
 
-def test_flash_periodic_decisions_at_cutoff(field):
+@pytest.mark.parametrize("per_target", [False, True], ids=["pairs", "per_target"])
+def test_flash_periodic_decisions_at_cutoff(field, per_target):
     """Images exactly at and one ulp around the cutoff nrep * min(L) are kept or dropped as
-    the reference drops them (radius < maxradius_pbc)."""
+    the reference drops them (radius < maxradius_pbc), in both image walks."""
     box, nrep, rs = (1.0, 1.0, 1.0), 2, 0.1
     xs = [0.0, 2.0 ** -50, -(2.0 ** -50), 2.0 ** -40, 0.5, np.nextafter(0.5, 1), 1e-17]
     t = np.array([[x, 0.0, 0.0] for x in xs] + [[0.0, x, 0.0] for x in xs] +
                  [[0.3, 0.4, 0.0], [0.6, 0.8, 0.0], [0.0, 0.6, 0.8]])
     one = np.zeros(1)
     field.set_softening("spline", rs, quirks=True)
-    field.set_periodic(box, nrep=nrep)
+    field.set_periodic(box, nrep=nrep, per_target=per_target)
     field.set_sources(one, one, one, np.ones(1), None)
     got = field.evaluate(None, t[:, 0], t[:, 1], t[:, 2])
     ref = oracle.eval_flash(one, one, one, np.ones(1), t[:, 0], t[:, 1], t[:, 2], G, rs, quirks=True,
@@ -219,6 +223,49 @@ def test_flash_periodic_decisions_at_cutoff(field):
     for c in range(3):
         np.testing.assert_allclose(got[c], ref[c], rtol=1e-12, atol=1e-12 * scale)
     np.testing.assert_allclose(got[3], ref[3], rtol=1e-12, atol=1e-12 * G / rs)
+
+
+def test_flash_periodic_nrep_beyond_the_row_table(field):
+    """nrep = 8 (more image rows than the grouped walk's table holds) takes the per-target walk
+    by itself: against the oracle."""
+    rng = W.rng_for(47)
+    box = (1.0, 1.1, 1.2)
+    s = rng.uniform(0, 1, (30, 3)) * box
+    t = rng.uniform(0, 1, (40, 3)) * box
+    m = rng.uniform(0.5, 2.0, 30)
+    rs = 0.12
+    field.set_softening("spline", rs, quirks=False)
+    field.set_periodic(box, nrep=8)
+    field.set_sources(s[:, 0], s[:, 1], s[:, 2], m, None)
+    got = field.evaluate(None, t[:, 0], t[:, 1], t[:, 2])
+    ref = oracle.eval_flash(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], G, rs,
+                            kernel="spline", quirks=False, box=box, nrep=8, threads=16)
+    sa, sp = term_scales(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], rs, box=box, nrep=8)
+    check(got, ref, sa, sp)
+
+
+def test_flash_periodic_c1_scale(c1):
+    """c1periodic (4,096 cells -> 1,024 stars, 4 pc box, nrep = 6, spline with the reference's
+    potential): 4.2 M pairs, dense enough in separation that most images of a wave's rows are
+    summed as certain.  Both image walks against the oracle at 1e-12 (relative to the terms'
+    magnitudes) on 48 seeded stars."""
+    from torch_amd import GasGravityField
+    st, ce = c1["stars"], c1["cells"]
+    box, rs = (4.0, 4.0, 4.0), 0.25
+    out = {}
+    for per_target in (False, True):
+        with GasGravityField("fp64") as f:
+            f.set_softening("spline", rs, quirks=True)
+            f.set_periodic(box, nrep=6, per_target=per_target)
+            f.set_sources(ce["x"], ce["y"], ce["z"], ce["m"], None)
+            out[per_target] = f.evaluate(None, st["x"], st["y"], st["z"])
+    idx = np.sort(W.rng_for(48).choice(len(st["x"]), 48, replace=False))
+    ref = oracle.eval_flash(ce["x"], ce["y"], ce["z"], ce["m"], st["x"][idx], st["y"][idx], st["z"][idx], G, rs,
+                            kernel="spline", quirks=True, box=box, nrep=6, threads=16)
+    sa, sp = term_scales(ce["x"], ce["y"], ce["z"], ce["m"], st["x"][idx], st["y"][idx], st["z"][idx], rs,
+                         box=box, nrep=6)
+    for per_target in (False, True):
+        check(tuple(a[idx] for a in out[per_target]), ref, sa, sp)
 
 
 def test_flash_periodic_chunked_deterministic(field):

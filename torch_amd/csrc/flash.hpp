@@ -354,6 +354,257 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
   }
 }
 
+// ------------------------------------------ periodic pairs grouped by separation ----
+// The periodic branch's work is the image walk of each (target, cell) pair, and its rows and
+// ranges depend only on the pair's separation d = x_t - x_c.  k_eval_flash gives a wave 64
+// targets and one cell: their separations differ, so the lanes' rows and ranges differ (lane
+// efficiency ~0.8) and every lane forms its own ranges (~44 instructions per row).  Here every
+// pair gets the Morton code of d, quantized to 1/64 of the box and clamped to 8 boxes either way,
+// the pairs are sorted by it, and each wave takes 64 consecutive pairs.  Their separations are
+// close, so the 64 lanes build one table of rows from the wave's bounding box of d (one candidate
+// row per lane at a time), and walk it together:
+//   - a row's z images certainly inside the cutoff for every d in the box (and outside rsoft)
+//     are summed without tests, r2 formed by FMAs;
+//   - the images that some d of the box may have inside are tested per lane with the reference's
+//     rounding (Gravity_getAccelAtPoint.F90:262-270);
+//   - rows that may come within rsoft take the general per-image loop with the softened kernel.
+// The walk keeps the reference's order (nx, ny, nz ascending), so with quirks pp is the pk of the
+// pair's last image inside the cutoff.  Each lane writes its pair's image sums (px, py, pz, pp) at
+// the pair's index t * ns + c; k_pair_reduce sums them per target with the masses.
+#define TG_PAIR_NREP 7                                   // rows of the table: (2 N + 1)^2 <= 225
+#ifndef TG_PAIR_BITS
+#define TG_PAIR_BITS 10      // separation key: bits per axis (bins of L / 2^(bits - 4), 8 boxes either way)
+#endif
+#define TG_PAIR_SLACK 1e-4   // image spacings; the square roots (v_rsq_f64, ~2^-24) are off by < 1e-5
+#define TG_PAIR_ROWS ((2 * TG_PAIR_NREP + 1) * (2 * TG_PAIR_NREP + 1))
+
+struct PairRow {
+  double fxl, fyl;          // fl(nx lx), fl(ny ly): the reference forms d + n L from these
+  double zu0, zu1;          // z images some pair of the wave may have inside the cutoff
+  double zc0, zc1;          // of those, the ones inside for every pair; zc0 = -1e300: a row that
+};                          //   may come within rsoft (general loop over [zu0, zu1])
+
+__device__ __forceinline__ unsigned int morton_spread10(unsigned int v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ unsigned int sep_bin(double d, double ibin) {
+  const double h = (double)(1 << (TG_PAIR_BITS - 1));
+  const double q = fmin(fmax(floor(d * ibin) + h, 0.0), 2.0 * h - 1.0);   // NaN -> 0
+  return (unsigned int)q;
+}
+
+// keys[i], idx[i] for pair i = t * ns + c: Morton code of the quantized separation, and i.
+__global__ __launch_bounds__(256) void k_pair_keys(Tgt T, const double4* __restrict__ src4, int64_t ns,
+                                                   int64_t npairs, double ibx, double iby, double ibz,
+                                                   unsigned int* __restrict__ keys,
+                                                   unsigned int* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  const int64_t t = i / ns, c = i - t * ns;
+  double tx, ty, tz, e2;
+  load_target(T, t, tx, ty, tz, e2);
+  const double4 s = src4[c];
+  keys[i] = morton_spread10(sep_bin(tx - s.x, ibx)) | (morton_spread10(sep_bin(ty - s.y, iby)) << 1) |
+            (morton_spread10(sep_bin(tz - s.z, ibz)) << 2);
+  idx[i] = (unsigned int)i;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Smallest and largest of fl(d + c)^2 over d in [lo, hi] (fl rounds monotonically, so a pair's
+// own rounded square lies between them).
+__device__ __forceinline__ void sq_range(double lo, double hi, double c, double& mn, double& mx) {
+  double a, b;
+  {
+#pragma clang fp contract(off)
+    a = lo + c;
+    b = hi + c;
+  }
+  const double a2 = a * a, b2 = b * b;
+  mx = fmax(a2, b2);
+  mn = (a <= 0.0 && b >= 0.0) ? 0.0 : fmin(a2, b2);
+}
+
+// One wave per workgroup: sorted pairs [64 w, 64 w + 64) of `idx`.  The table's 225 rows (10.8 KB)
+// hold nrep <= 7; sizing it by nrep instead (8.1 KB at nrep = 6, more waves per CU) was 4% slower.
+template <bool QUIRKS>
+__global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, const double4* __restrict__ src4, int64_t ns,
+                                                         int64_t npairs, const unsigned int* __restrict__ idx,
+                                                         Flash P, double4* __restrict__ pairbuf) {
+  __shared__ PairRow rows[TG_PAIR_ROWS];
+  const int lane = threadIdx.x;
+  const int64_t pi = (int64_t)blockIdx.x * 64 + lane;
+  const bool valid = pi < npairs;
+  const int64_t pr = valid ? (int64_t)idx[pi] : 0;
+  const int64_t t = pr / ns, c = pr - t * ns;
+  double tx, ty, tz, e2;
+  load_target(T, t, tx, ty, tz, e2);
+  const double4 s = src4[c];
+  const double dxi = tx - s.x, dyi = ty - s.y, dzi = tz - s.z;
+  const double INF = __builtin_huge_val();
+  const double dxl = wave_min(valid ? dxi : INF), dxh = wave_max(valid ? dxi : -INF);
+  const double dyl = wave_min(valid ? dyi : INF), dyh = wave_max(valid ? dyi : -INF);
+  const double dzl = wave_min(valid ? dzi : INF), dzh = wave_max(valid ? dzi : -INF);
+  const int N = P.nrep, W2 = 2 * N + 1;
+  const double fN = (double)N, cut = P.cut_thr;
+  // the row table, in (nx, ny) order: lane j takes candidate 64 r + j
+  int nrows = 0;
+  for (int r0 = 0; r0 < W2 * W2; r0 += 64) {
+    const int cand = r0 + lane;
+    bool keep = false;
+    PairRow R;
+    if (cand < W2 * W2) {
+      const double fx = (double)(cand / W2 - N), fy = (double)(cand % W2 - N);
+      R.fxl = fx * P.lx;
+      R.fyl = fy * P.ly;
+      double x2l, x2h, y2l, y2h;
+      sq_range(dxl, dxh, R.fxl, x2l, x2h);
+      sq_range(dyl, dyh, R.fyl, y2l, y2h);
+      double xyl, xyh, remh, reml;
+      {
+#pragma clang fp contract(off)
+        xyl = x2l + y2l;
+        xyh = x2h + y2h;
+        remh = cut - xyl;   // no pair of the box has a larger remainder cut_thr - xy
+        reml = cut - xyh;   // nor a smaller one (the same rounded operations, monotone)
+      }
+      if (remh >= 0.0) {
+        const double sh = remh * __builtin_amdgcn_rsq(remh + 0x1p-1022);
+        R.zu0 = fmax(ceil((-sh - dzh) * P.ilz - TG_PAIR_SLACK), -fN);
+        R.zu1 = fmin(floor((sh - dzl) * P.ilz + TG_PAIR_SLACK), fN);
+        keep = R.zu0 <= R.zu1;
+        R.zc0 = R.zu1 + 1.0;   // no certain images unless shown below
+        R.zc1 = R.zu1;
+        if (!(xyl > P.soft_thr)) {
+          R.zc0 = -1e300;      // some image of the row may lie within rsoft
+        } else if (reml > 0.0) {
+          // every pair of the box: |dz + n L| <= sqrt(reml) - m L keeps the reference's rounded
+          // r2 below cut_thr (m: the error of the square root and 2.7u cut / sqrt(reml) for the
+          // rounding of dz^2 and r2)
+          const double yl = __builtin_amdgcn_rsq(reml);
+          const double sl = reml * yl;
+          const double m = fma(cut * P.ilz * 6e-16, yl, TG_PAIR_SLACK);
+          const double c0 = fmax(ceil((-sl - dzl) * P.ilz + m), R.zu0);
+          const double c1 = fmin(floor((sl - dzh) * P.ilz - m), R.zu1);
+          if (c0 <= c1) {
+            R.zc0 = c0;
+            R.zc1 = c1;
+          }
+        }
+      }
+    }
+    const unsigned long long b = __builtin_amdgcn_ballot_w64(keep);
+    if (keep) rows[nrows + __builtin_popcountll(b & ((1ULL << lane) - 1ULL))] = R;
+    nrows += __builtin_popcountll(b);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double px = 0.0, py = 0.0, pz = 0.0, pp = 0.0;
+  for (int r = 0; r < nrows; ++r) {
+    const PairRow R = rows[r];
+    double dx, dy, xy;
+    {
+#pragma clang fp contract(off)
+      dx = dxi + R.fxl;
+      dy = dyi + R.fyl;
+      xy = dx * dx + dy * dy;
+    }
+    if (R.zc0 < -1e299) {   // (uniform) the general loop: the reference's test and kernel per image
+      for (double fz = R.zu0; fz <= R.zu1; fz += 1.0) {
+        double dz, r2;
+        {
+#pragma clang fp contract(off)
+          dz = dzi + fz * P.lz;
+          r2 = xy + dz * dz;
+        }
+        if (r2 <= cut) {
+          double kv, pk;
+          flash_pair<false>(P, r2, kv, pk);
+          px = fma(kv, dx, px);
+          py = fma(kv, dy, py);
+          pz = fma(kv, dz, pz);
+          if (QUIRKS) pp = pk;
+          else pp += pk;
+        }
+      }
+      continue;
+    }
+    double skv = 0.0;
+    auto tested = [&](double fz) {
+      double dz, r2;
+      {
+#pragma clang fp contract(off)
+        dz = dzi + fz * P.lz;
+        r2 = xy + dz * dz;
+      }
+      if (r2 <= cut) {
+        const double y = rsqrt_f64<0>(r2);
+        const double kv = (y * y) * y;
+        skv += kv;
+        pz = fma(kv, dz, pz);
+        if (QUIRKS) pp = y;
+        else pp += y;
+      }
+    };
+    for (double fz = R.zu0; fz < R.zc0; fz += 1.0) tested(fz);
+    for (double fz = R.zc0; fz <= R.zc1; fz += 1.0) {
+      const double dz = fma(fz, P.lz, dzi);
+      const double y = rsqrt_f64<0>(fma(dz, dz, xy));
+      const double kv = (y * y) * y;
+      skv += kv;
+      pz = fma(kv, dz, pz);
+      if (QUIRKS) pp = y;
+      else pp += y;
+    }
+    for (double fz = R.zc1 + 1.0; fz <= R.zu1; fz += 1.0) tested(fz);
+    px = fma(skv, dx, px);
+    py = fma(skv, dy, py);
+  }
+  if (valid) pairbuf[pr] = make_double4(px, py, pz, pp);
+}
+
+// Per target t (one wave each): s -= m_c (px, py, pz), s_p += m_c pp over its cells c, lane j
+// taking c = j, j + 64, ... in order, then the lanes' sums in a fixed butterfly order.
+__global__ __launch_bounds__(64) void k_pair_reduce(const double4* __restrict__ pairbuf,
+                                                    const double4* __restrict__ src4, int64_t ns, Out O) {
+  const int64_t t = blockIdx.x;
+  const int lane = threadIdx.x;
+  double ax = 0.0, ay = 0.0, az = 0.0, ap = 0.0;
+  const double4* pb = pairbuf + t * ns;
+  for (int64_t c = lane; c < ns; c += 64) {
+    const double4 r = pb[c];
+    const double m = src4[c].w;
+    ax = fma(-m, r.x, ax);
+    ay = fma(-m, r.y, ay);
+    az = fma(-m, r.z, az);
+    ap = fma(m, r.w, ap);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ax += __shfl_xor(ax, o);
+    ay += __shfl_xor(ay, o);
+    az += __shfl_xor(az, o);
+    ap += __shfl_xor(ap, o);
+  }
+  if (lane == 0) store_result(O, t, ax, ay, az, ap);
+}
+
 // ------------------------------------------------------- 8^3 block sources ----
 // Isolated FLASH kernel against cells handed over as 8^3 leaf blocks (tgrav_set_source_blocks):
 // the column walk of k_eval_gridsrc (dx, dy and dx^2 + dy^2 formed once per column and

@@ -5,6 +5,7 @@
 // reduction.  Device code lives in kernels.hpp (grid kernels) and batch.hpp
 // (persistent batch kernel).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <math.h>
 #include <stdarg.h>
@@ -319,7 +320,7 @@ struct tgrav_ctx {
   int soft = TGRAV_SOFT_PLUMMER;
   double rsoft = 0.0;
   int soft_quirks = 0;
-  int periodic = 0;
+  int periodic = 0;           // 1: periodic images (2: always the per-target image walk)
   double box[3] = {0.0, 0.0, 0.0};
   int nrep = 6;
   double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
@@ -338,6 +339,8 @@ struct tgrav_ctx {
   DevBuf bp4, bpe2, bstat, bppart, bpgrp;   // fused passes: packed particles, partials
   DevBuf pm[16], pm_acc[3], pm_stat;  // particle-mesh pieces: staging, fixed-point sums, stat
   DevBuf rstat;              // range statistics of the mixed mode's check (2 x tg::ST_WORDS)
+  DevBuf pkey, pidx, ptmp, pbuf;   // periodic pairs: sort keys and indices (2 x each), sort scratch,
+                                   //   per-pair image sums
   unsigned long long* host_flag = nullptr;   // pinned, kHostWords words
   // instrumentation: HIP events around each evaluation kernel (tgrav_set_timing)
   int timing = 0;
@@ -790,6 +793,44 @@ static int flash_gridsrc_kernel(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& o
   return 0;
 }
 
+// Periodic FLASH sums with the pairs grouped by separation (k_eval_flash_pairs): pair keys, a
+// stable radix sort of them, the grouped walk (the timed kernel), the per-target sums.
+static int launch_flash_pairs(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O, const tg::Flash& P) {
+  const int64_t nt = T.n, ns = c->ns, npairs = nt * ns;
+  size_t tmp = 0;
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const unsigned int*)nullptr, (unsigned int*)nullptr,
+                                             (const unsigned int*)nullptr, (unsigned int*)nullptr, (int)npairs, 0,
+                                             3 * TG_PAIR_BITS, c->stream));
+  int rc;
+  if ((rc = c->pkey.ensure((size_t)npairs * 2 * sizeof(unsigned int))) ||
+      (rc = c->pidx.ensure((size_t)npairs * 2 * sizeof(unsigned int))) || (rc = c->ptmp.ensure(tmp)) ||
+      (rc = c->pbuf.ensure((size_t)npairs * sizeof(double4))))
+    return rc;
+  unsigned int *k0 = c->pkey.as<unsigned int>(), *k1 = k0 + npairs;
+  unsigned int *i0 = c->pidx.as<unsigned int>(), *i1 = i0 + npairs;
+  const double4* s4 = c->s4.as<double4>();
+  const double bins = (double)(1 << (TG_PAIR_BITS - 4));   // per box length
+  tg::k_pair_keys<<<grid1(npairs, 256), 256, 0, c->stream>>>(T, s4, ns, npairs, bins / P.lx, bins / P.ly,
+                                                             bins / P.lz, k0, i0);
+  LAUNCHCHECK("k_pair_keys");
+  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(c->ptmp.p, tmp, k0, k1, i0, i1, (int)npairs, 0, 3 * TG_PAIR_BITS,
+                                             c->stream));
+  double4* pb = c->pbuf.as<double4>();
+  rc = timed_launch(c, nt, ns, [&]() -> int {
+    const unsigned int nw = (unsigned int)((npairs + 63) / 64);
+    if (P.quirks) tg::k_eval_flash_pairs<true><<<nw, 64, 0, c->stream>>>(T, s4, ns, npairs, i1, P, pb);
+    else tg::k_eval_flash_pairs<false><<<nw, 64, 0, c->stream>>>(T, s4, ns, npairs, i1, P, pb);
+    LAUNCHCHECK("k_eval_flash_pairs");
+    return 0;
+  });
+  if (rc) return rc;
+  tg::Out fin = O;
+  fin.part = nullptr;
+  tg::k_pair_reduce<<<(unsigned int)nt, 64, 0, c->stream>>>(pb, s4, ns, fin);
+  LAUNCHCHECK("k_pair_reduce");
+  return 0;
+}
+
 // FLASH softening kernels (FP64 only).  Isolated: K = 2 targets per lane, sources split until
 // there are about 16 workgroups per CU (4 waves each, so the last round of workgroups is a
 // small share of the launch); periodic: K = 1 (hundreds of images per pair), about 8.
@@ -820,6 +861,12 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   // branch); the FP32 seed is valid from 1e-30 up, and up to the tile's r^2 bound (in the kernels).
   P.s32 = TG_FP64_SEED32 && c->rsoft * c->rsoft > TG_F32_TINY_E2;
   P.reach = per ? (double)(c->nrep + 1) * std::max(P.lx, std::max(P.ly, P.lz)) : 0.0;
+  // Periodic sums with the pairs grouped by separation, where the row table fits (nrep <= 7) and
+  // the pair arrays stay modest (2^25 pairs: 1.3 GB); tgrav_set_periodic(on = 2) keeps the
+  // per-target walk.
+  if (per && c->periodic == 1 && c->nrep <= TG_PAIR_NREP && nt > 0 && ns > 0 &&
+      (double)nt * (double)ns <= (double)(1LL << 25))
+    return launch_flash_pairs(c, T, O, P);
   // Sources given as 8^3 blocks take the block-source kernel.  Explicit sources that may be
   // C-ordered 8^3 blocks (flagged on the device when they were set) launch it too, gated against
   // the generic kernel on that flag, as in the Plummer path.
@@ -1138,6 +1185,10 @@ int tgrav_destroy(tgrav_ctx* c) {
   for (auto& b : c->pm) b.release();
   for (auto& b : c->pm_acc) b.release();
   c->pm_stat.release();
+  c->pkey.release();
+  c->pidx.release();
+  c->ptmp.release();
+  c->pbuf.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
@@ -1907,7 +1958,7 @@ int tgrav_set_periodic(tgrav_ctx* c, int on, double lx, double ly, double lz, in
   if (!(lx > 0.0 && ly > 0.0 && lz > 0.0 && lx < 1e150 && ly < 1e150 && lz < 1e150))
     return fail(TGRAV_EINVAL, "box lengths must be positive and finite");
   if (nrep < 1 || nrep > 64) return fail(TGRAV_EINVAL, "nrep must be in [1, 64], got %d", nrep);
-  c->periodic = 1;
+  c->periodic = on == 2 ? 2 : 1;
   c->box[0] = lx;
   c->box[1] = ly;
   c->box[2] = lz;

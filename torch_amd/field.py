@@ -378,14 +378,17 @@ class GravityField:
         _lib.check(self.lib.tgrav_set_softening(self._ctx, self.SOFTENING[kernel], float(rsoft),
                                                 int(bool(quirks))), "tgrav_set_softening")
 
-    def set_periodic(self, box=None, nrep=6):
+    def set_periodic(self, box=None, nrep=6, per_target=False):
         """Periodic image sum of the FLASH kernels over n in [-nrep, nrep]^3 within
-        nrep * min(box) (Gravity_getAccelAtPoint.F90:250-312); box=None: isolated."""
+        nrep * min(box) (Gravity_getAccelAtPoint.F90:250-312); box=None: isolated.
+        per_target=True keeps the per-target image walk instead of the pairs grouped by
+        separation (the default where nrep <= 7 and the pairs number at most 2^25)."""
         if box is None:
             _lib.check(self.lib.tgrav_set_periodic(self._ctx, 0, 0.0, 0.0, 0.0, 0), "tgrav_set_periodic")
             return
         lx, ly, lz = (float(self._plain((b, "length"))[0][0]) for b in box)
-        _lib.check(self.lib.tgrav_set_periodic(self._ctx, 1, lx, ly, lz, int(nrep)), "tgrav_set_periodic")
+        _lib.check(self.lib.tgrav_set_periodic(self._ctx, 2 if per_target else 1, lx, ly, lz, int(nrep)),
+                   "tgrav_set_periodic")
 
     def set_block_kernel(self, on=True):
         """Use (default) or bypass the 8^3 column kernels: block-structured targets and sources,
