@@ -610,8 +610,10 @@ __global__ __launch_bounds__(64) void k_pair_reduce(const double4* __restrict__ 
 // the column walk of k_eval_gridsrc (dx, dy and dx^2 + dy^2 formed once per column and
 // target) with the branch-free Newtonian loop; a wave-uniform bit per cell of the column marks
 // cells that came closer than the clamp to any of the wave's targets, and those are redone
-// exactly (reference-rounded r2, exact threshold) as flash_fix does.
-template <int K, bool S32>
+// exactly (reference-rounded r2, exact threshold) as flash_fix does.  FAR: no target of the wave
+// comes within the clamp of the block (block_far), so no pair is clamped or flagged -- the same
+// arithmetic without the clamp, the flags and their ballots.
+template <int K, bool S32, bool FAR>
 __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false>& L,
                                                const double (&tx)[K], const double (&ty)[K],
                                                const double (&tz)[K], double (&ax)[K],
@@ -637,15 +639,15 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
       for (int k = 0; k < K; ++k) {
         const double dz = tz[k] - zs;
         const double r2 = fma(dz, dz, base[k]);
-        nj = nj || r2 < P.near2;
-        const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(fmax(r2, P.near2));
+        if (!FAR) nj = nj || r2 < P.near2;
+        const double y = rsqrt_f64<S32 ? F_SEED32 : 0>(FAR ? r2 : fmax(r2, P.near2));
         const double mr = m * y;
         ap[k] += mr;
         const double w = mr * (y * y);
         sw[k] += w;
         az[k] = fma(-w, dz, az[k]);
       }
-      if (__builtin_amdgcn_ballot_w64(nj)) nm |= 1u << kk;
+      if (!FAR && __builtin_amdgcn_ballot_w64(nj)) nm |= 1u << kk;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -685,6 +687,26 @@ __device__ __forceinline__ void flash_gs_block(const Flash& P, const LdsGS<false
   }
 }
 
+// Whether every target of the wave lies farther than the clamp sqrt(near2) from the box of the
+// staged block's cell centres: that distance bounds every pair's exact r^2 from below, which the
+// FMA-formed r^2 undercuts by a few ulps at most (the 1e-9 covers it and the rounding here).
+template <int K>
+__device__ __forceinline__ bool block_far(const Flash& P, const LdsGS<false>& L, const double (&tx)[K],
+                                          const double (&ty)[K], const double (&tz)[K]) {
+  const double x0 = fmin(L.cx[0], L.cx[7]), x1 = fmax(L.cx[0], L.cx[7]);
+  const double y0 = fmin(L.cy[0], L.cy[7]), y1 = fmax(L.cy[0], L.cy[7]);
+  const double z0 = fmin(L.cz[0], L.cz[7]), z1 = fmax(L.cz[0], L.cz[7]);
+  bool far = true;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double gx = fmax(fmax(x0 - tx[k], tx[k] - x1), 0.0);
+    const double gy = fmax(fmax(y0 - ty[k], ty[k] - y1), 0.0);
+    const double gz = fmax(fmax(z0 - tz[k], tz[k] - z1), 0.0);
+    far = far && gx * gx + gy * gy + gz * gz > P.near2 * (1.0 + 1e-9);   // NaN: not far
+  }
+  return __all(far);
+}
+
 // Explicit targets (K per lane) against 8^3 source blocks; blockIdx.x = target tile,
 // blockIdx.y = chunk of source blocks [y*chunk, min((y+1)*chunk, nblk)).
 template <int K, int NT>
@@ -713,8 +735,14 @@ __global__ __launch_bounds__(NT) void k_eval_flash_gridsrc(Tgt T, const double* 
     __syncthreads();
     gs_stage<false>(L, sax, be2, bm, blk, 0.0, 0.0, 0.0, 0.0);
     __syncthreads();
-    if (s32) flash_gs_block<K, true>(P, L, tx, ty, tz, ax, ay, az, ap);
-    else flash_gs_block<K, false>(P, L, tx, ty, tz, ax, ay, az, ap);
+    const bool far = block_far<K>(P, L, tx, ty, tz);   // uniform over the wave
+    if (s32) {
+      if (far) flash_gs_block<K, true, true>(P, L, tx, ty, tz, ax, ay, az, ap);
+      else flash_gs_block<K, true, false>(P, L, tx, ty, tz, ax, ay, az, ap);
+    } else {
+      if (far) flash_gs_block<K, false, true>(P, L, tx, ty, tz, ax, ay, az, ap);
+      else flash_gs_block<K, false, false>(P, L, tx, ty, tz, ax, ay, az, ap);
+    }
   }
   Out o = O;
   if (o.part) o.part += (int64_t)blockIdx.y * 4 * o.pstride;
