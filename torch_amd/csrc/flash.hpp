@@ -562,8 +562,13 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, const double4* _
         else pp += y;
       }
     };
-    for (double fz = R.zu0; fz < R.zc0; fz += 1.0) tested(fz);
-    for (double fz = R.zc0; fz <= R.zc1; fz += 1.0) {
+    // the trip counts are the same in every lane: counted on the scalar unit
+    const int nlo = __builtin_amdgcn_readfirstlane((int)(R.zc0 - R.zu0));
+    const int nc = __builtin_amdgcn_readfirstlane((int)(R.zc1 - R.zc0) + 1);
+    const int nhi = __builtin_amdgcn_readfirstlane((int)(R.zu1 - R.zc1));
+    double fz = R.zu0;
+    for (int i = 0; i < nlo; ++i, fz += 1.0) tested(fz);
+    for (int i = 0; i < nc; ++i, fz += 1.0) {
       const double dz = fma(fz, P.lz, dzi);
       const double y = rsqrt_f64<0>(fma(dz, dz, xy));
       const double kv = (y * y) * y;
@@ -572,7 +577,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, const double4* _
       if (QUIRKS) pp = y;
       else pp += y;
     }
-    for (double fz = R.zc1 + 1.0; fz <= R.zu1; fz += 1.0) tested(fz);
+    for (int i = 0; i < nhi; ++i, fz += 1.0) tested(fz);
     px = fma(skv, dx, px);
     py = fma(skv, dy, py);
   }
