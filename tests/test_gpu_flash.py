@@ -268,6 +268,32 @@ def test_flash_periodic_c1_scale(c1):
         check(tuple(a[idx] for a in out[per_target]), ref, sa, sp)
 
 
+def test_flash_periodic_block_targets_equal_centres():
+    """Periodic sums onto 8^3 leaf-block targets (the grouped walk reads their centres from the
+    blocks): bit-identical to the explicit centres, and against the oracle on a subsample."""
+    rng = W.rng_for(49)
+    box = (2.0, 2.0, 2.0)
+    s = rng.uniform(0, 2, (60, 3))
+    m = rng.uniform(0.5, 2.0, 60)
+    nblk = 2
+    ox, oy, oz = (rng.uniform(0.0, 1.6, nblk) for _ in range(3))
+    bdx = rng.uniform(0.02, 0.05, nblk)
+    x, y, z = oracle.block_centres(ox, oy, oz, bdx)
+    with GravityField("fp64") as f:
+        f.set_softening("spline", 0.1, quirks=True)
+        f.set_periodic(box, nrep=3)
+        f.set_sources(s[:, 0], s[:, 1], s[:, 2], m, None)
+        a = f.evaluate(None, x, y, z)
+        b = f.evaluate_blocks(np.stack([ox, oy, oz], 1), bdx, nb=8)
+    for c in range(4):
+        np.testing.assert_array_equal(a[c], b[c])
+    idx = np.sort(W.rng_for(50).choice(len(x), 64, replace=False))
+    ref = oracle.eval_flash(s[:, 0], s[:, 1], s[:, 2], m, x[idx], y[idx], z[idx], G, 0.1, kernel="spline",
+                            quirks=True, box=box, nrep=3, threads=16)
+    sa, sp = term_scales(s[:, 0], s[:, 1], s[:, 2], m, x[idx], y[idx], z[idx], 0.1, box=box, nrep=3)
+    check(tuple(q[idx] for q in a), ref, sa, sp)
+
+
 def test_flash_periodic_chunked_deterministic(field):
     """Many source chunks (few targets): repeated runs are bit-identical and the kick is
     v + dt * a rounded as the reference rounds it."""
