@@ -216,8 +216,9 @@ int tgrav_set_softening(tgrav_ctx* ctx, int kernel, double rsoft, int flags);
  * loop order (it assigns, not adds, per image, Gravity_getPotentialAtPoint.F90:262-274).
  * on = 0 restores isolated boundaries.  nrep in [1, 64].  on = 1 sorts the (target, cell)
  * pairs by separation and walks each wave's image rows together where nrep <= 7 and there are
- * at most 2^25 pairs (it allocates ~40 B per pair); on = 2 always walks each target's images
- * on its own lane (the only way above those limits).  Both give the reference's decisions. */
+ * at most 2^25 cells, in batches of targets of at most 2^25 pairs (it allocates ~40 B per pair
+ * of a batch, up to 1.3 GB); on = 2 always walks each target's images on its own lane (the only
+ * way above those limits).  Both give the reference's decisions. */
 int tgrav_set_periodic(tgrav_ctx* ctx, int on, double lx, double ly, double lz, int nrep);
 
 /* The fused pass with the gas cells handed over as 8^3 leaf blocks (nb must be 8): block b has

@@ -268,6 +268,30 @@ def test_flash_periodic_c1_scale(c1):
         check(tuple(a[idx] for a in out[per_target]), ref, sa, sp)
 
 
+def test_flash_periodic_pairs_in_target_batches(c1):
+    """9,000 stars x 4,096 cells = 36.9 M pairs, more than one batch of the grouped walk (2^25
+    pairs, 8,192 stars here): stars on both sides of the batch boundary and at the ends against
+    the oracle, and both image walks against each other there."""
+    from torch_amd import GasGravityField
+    ce = c1["cells"]
+    st = W.star_cluster(W.rng_for(51), 9000, r_vir=1.0, eps=0.01)
+    box, rs = (4.0, 4.0, 4.0), 0.25
+    out = {}
+    for per_target in (False, True):
+        with GasGravityField("fp64") as f:
+            f.set_softening("linear", rs, quirks=False)
+            f.set_periodic(box, nrep=6, per_target=per_target)
+            f.set_sources(ce["x"], ce["y"], ce["z"], ce["m"], None)
+            out[per_target] = f.evaluate(None, st["x"], st["y"], st["z"])
+    idx = np.array([0, 1, 2, 8189, 8190, 8191, 8192, 8193, 8194, 8997, 8998, 8999])
+    ref = oracle.eval_flash(ce["x"], ce["y"], ce["z"], ce["m"], st["x"][idx], st["y"][idx], st["z"][idx], G, rs,
+                            kernel="linear", quirks=False, box=box, nrep=6, threads=16)
+    sa, sp = term_scales(ce["x"], ce["y"], ce["z"], ce["m"], st["x"][idx], st["y"][idx], st["z"][idx], rs,
+                         box=box, nrep=6)
+    for per_target in (False, True):
+        check(tuple(a[idx] for a in out[per_target]), ref, sa, sp)
+
+
 def test_flash_periodic_block_targets_equal_centres():
     """Periodic sums onto 8^3 leaf-block targets (the grouped walk reads their centres from the
     blocks): bit-identical to the explicit centres, and against the oracle on a subsample."""

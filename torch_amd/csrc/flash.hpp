@@ -399,16 +399,17 @@ __device__ __forceinline__ unsigned int sep_bin(double d, double ibin) {
   return (unsigned int)q;
 }
 
-// keys[i], idx[i] for pair i = t * ns + c: Morton code of the quantized separation, and i.
-__global__ __launch_bounds__(256) void k_pair_keys(Tgt T, const double4* __restrict__ src4, int64_t ns,
-                                                   int64_t npairs, double ibx, double iby, double ibz,
-                                                   unsigned int* __restrict__ keys,
+// keys[i], idx[i] for pair i = t * ns + c (target t0 + t): Morton code of the quantized
+// separation, and i.
+__global__ __launch_bounds__(256) void k_pair_keys(Tgt T, int64_t t0, const double4* __restrict__ src4,
+                                                   int64_t ns, int64_t npairs, double ibx, double iby,
+                                                   double ibz, unsigned int* __restrict__ keys,
                                                    unsigned int* __restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   const int64_t t = i / ns, c = i - t * ns;
   double tx, ty, tz, e2;
-  load_target(T, t, tx, ty, tz, e2);
+  load_target(T, t0 + t, tx, ty, tz, e2);
   const double4 s = src4[c];
   keys[i] = morton_spread10(sep_bin(tx - s.x, ibx)) | (morton_spread10(sep_bin(ty - s.y, iby)) << 1) |
             (morton_spread10(sep_bin(tz - s.z, ibz)) << 2);
@@ -443,9 +444,10 @@ __device__ __forceinline__ void sq_range(double lo, double hi, double c, double&
 // One wave per workgroup: sorted pairs [64 w, 64 w + 64) of `idx`.  The table's 225 rows (10.8 KB)
 // hold nrep <= 7; sizing it by nrep instead (8.1 KB at nrep = 6, more waves per CU) was 4% slower.
 template <bool QUIRKS>
-__global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, const double4* __restrict__ src4, int64_t ns,
-                                                         int64_t npairs, const unsigned int* __restrict__ idx,
-                                                         Flash P, double4* __restrict__ pairbuf) {
+__global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, const double4* __restrict__ src4,
+                                                         int64_t ns, int64_t npairs,
+                                                         const unsigned int* __restrict__ idx, Flash P,
+                                                         double4* __restrict__ pairbuf) {
   __shared__ PairRow rows[TG_PAIR_ROWS];
   const int lane = threadIdx.x;
   const int64_t pi = (int64_t)blockIdx.x * 64 + lane;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, const double4* _
   const int64_t pr = valid ? (int64_t)idx[pi] : 0;
   const int64_t t = pr / ns, c = pr - t * ns;
   double tx, ty, tz, e2;
-  load_target(T, t, tx, ty, tz, e2);
+  load_target(T, t0 + t, tx, ty, tz, e2);
   const double4 s = src4[c];
   const double dxi = tx - s.x, dyi = ty - s.y, dzi = tz - s.z;
   const double INF = __builtin_huge_val();
@@ -584,10 +586,11 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, const double4* _
   if (valid) pairbuf[pr] = make_double4(px, py, pz, pp);
 }
 
-// Per target t (one wave each): s -= m_c (px, py, pz), s_p += m_c pp over its cells c, lane j
-// taking c = j, j + 64, ... in order, then the lanes' sums in a fixed butterfly order.
+// Per target t0 + t (one wave each): s -= m_c (px, py, pz), s_p += m_c pp over its cells c, lane
+// j taking c = j, j + 64, ... in order, then the lanes' sums in a fixed butterfly order.
 __global__ __launch_bounds__(64) void k_pair_reduce(const double4* __restrict__ pairbuf,
-                                                    const double4* __restrict__ src4, int64_t ns, Out O) {
+                                                    const double4* __restrict__ src4, int64_t ns, int64_t t0,
+                                                    Out O) {
   const int64_t t = blockIdx.x;
   const int lane = threadIdx.x;
   double ax = 0.0, ay = 0.0, az = 0.0, ap = 0.0;
@@ -607,7 +610,7 @@ __global__ __launch_bounds__(64) void k_pair_reduce(const double4* __restrict__ 
     az += __shfl_xor(az, o);
     ap += __shfl_xor(ap, o);
   }
-  if (lane == 0) store_result(O, t, ax, ay, az, ap);
+  if (lane == 0) store_result(O, t0 + t, ax, ay, az, ap);
 }
 
 // ------------------------------------------------------- 8^3 block sources ----

@@ -793,41 +793,48 @@ static int flash_gridsrc_kernel(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& o
   return 0;
 }
 
-// Periodic FLASH sums with the pairs grouped by separation (k_eval_flash_pairs): pair keys, a
-// stable radix sort of them, the grouped walk (the timed kernel), the per-target sums.
+// Periodic FLASH sums with the pairs grouped by separation (k_eval_flash_pairs), in batches of
+// targets of at most kMaxPairs pairs: pair keys, a stable radix sort of them, the grouped walk
+// (the timed kernel), the per-target sums.  ns <= kMaxPairs (the caller checks).
+static const int64_t kMaxPairs = 1LL << 25;
 static int launch_flash_pairs(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O, const tg::Flash& P) {
-  const int64_t nt = T.n, ns = c->ns, npairs = nt * ns;
+  const int64_t nt = T.n, ns = c->ns;
+  const int64_t tb = std::max<int64_t>(1, std::min<int64_t>(nt, kMaxPairs / ns)), nmax = tb * ns;
   size_t tmp = 0;
   HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const unsigned int*)nullptr, (unsigned int*)nullptr,
-                                             (const unsigned int*)nullptr, (unsigned int*)nullptr, (int)npairs, 0,
+                                             (const unsigned int*)nullptr, (unsigned int*)nullptr, (int)nmax, 0,
                                              3 * TG_PAIR_BITS, c->stream));
   int rc;
-  if ((rc = c->pkey.ensure((size_t)npairs * 2 * sizeof(unsigned int))) ||
-      (rc = c->pidx.ensure((size_t)npairs * 2 * sizeof(unsigned int))) || (rc = c->ptmp.ensure(tmp)) ||
-      (rc = c->pbuf.ensure((size_t)npairs * sizeof(double4))))
+  if ((rc = c->pkey.ensure((size_t)nmax * 2 * sizeof(unsigned int))) ||
+      (rc = c->pidx.ensure((size_t)nmax * 2 * sizeof(unsigned int))) || (rc = c->ptmp.ensure(tmp)) ||
+      (rc = c->pbuf.ensure((size_t)nmax * sizeof(double4))))
     return rc;
-  unsigned int *k0 = c->pkey.as<unsigned int>(), *k1 = k0 + npairs;
-  unsigned int *i0 = c->pidx.as<unsigned int>(), *i1 = i0 + npairs;
+  unsigned int *k0 = c->pkey.as<unsigned int>(), *k1 = k0 + nmax;
+  unsigned int *i0 = c->pidx.as<unsigned int>(), *i1 = i0 + nmax;
   const double4* s4 = c->s4.as<double4>();
-  const double bins = (double)(1 << (TG_PAIR_BITS - 4));   // per box length
-  tg::k_pair_keys<<<grid1(npairs, 256), 256, 0, c->stream>>>(T, s4, ns, npairs, bins / P.lx, bins / P.ly,
-                                                             bins / P.lz, k0, i0);
-  LAUNCHCHECK("k_pair_keys");
-  HIPCHECK(hipcub::DeviceRadixSort::SortPairs(c->ptmp.p, tmp, k0, k1, i0, i1, (int)npairs, 0, 3 * TG_PAIR_BITS,
-                                             c->stream));
   double4* pb = c->pbuf.as<double4>();
-  rc = timed_launch(c, nt, ns, [&]() -> int {
-    const unsigned int nw = (unsigned int)((npairs + 63) / 64);
-    if (P.quirks) tg::k_eval_flash_pairs<true><<<nw, 64, 0, c->stream>>>(T, s4, ns, npairs, i1, P, pb);
-    else tg::k_eval_flash_pairs<false><<<nw, 64, 0, c->stream>>>(T, s4, ns, npairs, i1, P, pb);
-    LAUNCHCHECK("k_eval_flash_pairs");
-    return 0;
-  });
-  if (rc) return rc;
+  const double bins = (double)(1 << (TG_PAIR_BITS - 4));   // per box length
   tg::Out fin = O;
   fin.part = nullptr;
-  tg::k_pair_reduce<<<(unsigned int)nt, 64, 0, c->stream>>>(pb, s4, ns, fin);
-  LAUNCHCHECK("k_pair_reduce");
+  for (int64_t t0 = 0; t0 < nt; t0 += tb) {
+    const int64_t nb = std::min<int64_t>(tb, nt - t0), np = nb * ns;
+    tg::k_pair_keys<<<grid1(np, 256), 256, 0, c->stream>>>(T, t0, s4, ns, np, bins / P.lx, bins / P.ly,
+                                                           bins / P.lz, k0, i0);
+    LAUNCHCHECK("k_pair_keys");
+    size_t tmp_b = tmp;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(c->ptmp.p, tmp_b, k0, k1, i0, i1, (int)np, 0, 3 * TG_PAIR_BITS,
+                                               c->stream));
+    rc = timed_launch(c, nb, ns, [&]() -> int {
+      const unsigned int nw = (unsigned int)((np + 63) / 64);
+      if (P.quirks) tg::k_eval_flash_pairs<true><<<nw, 64, 0, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+      else tg::k_eval_flash_pairs<false><<<nw, 64, 0, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+      LAUNCHCHECK("k_eval_flash_pairs");
+      return 0;
+    });
+    if (rc) return rc;
+    tg::k_pair_reduce<<<(unsigned int)nb, 64, 0, c->stream>>>(pb, s4, ns, t0, fin);
+    LAUNCHCHECK("k_pair_reduce");
+  }
   return 0;
 }
 
@@ -861,11 +868,11 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   // branch); the FP32 seed is valid from 1e-30 up, and up to the tile's r^2 bound (in the kernels).
   P.s32 = TG_FP64_SEED32 && c->rsoft * c->rsoft > TG_F32_TINY_E2;
   P.reach = per ? (double)(c->nrep + 1) * std::max(P.lx, std::max(P.ly, P.lz)) : 0.0;
-  // Periodic sums with the pairs grouped by separation, where the row table fits (nrep <= 7) and
-  // the pair arrays stay modest (2^25 pairs: 1.3 GB); tgrav_set_periodic(on = 2) keeps the
-  // per-target walk.
-  if (per && c->periodic == 1 && c->nrep <= TG_PAIR_NREP && nt > 0 && ns > 0 &&
-      (double)nt * (double)ns <= (double)(1LL << 25))
+  // Periodic sums with the pairs grouped by separation, where the row table fits (nrep <= 7), in
+  // batches of targets of at most 2^25 pairs (1.3 GB of pair arrays); tgrav_set_periodic(on = 2)
+  // keeps the per-target walk.
+  if (per && c->periodic == 1 && c->nrep <= TG_PAIR_NREP && nt > 0 && ns > 0 && ns <= kMaxPairs &&
+      nt <= 0x7fffffffLL)
     return launch_flash_pairs(c, T, O, P);
   // Sources given as 8^3 blocks take the block-source kernel.  Explicit sources that may be
   // C-ordered 8^3 blocks (flagged on the device when they were set) launch it too, gated against
