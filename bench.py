@@ -66,18 +66,27 @@ BYTES_PER_SOURCE = 40              # x, y, z, m, eps (FP64), read once per launc
 # WRITE_SIZE passes over `python -m torch_amd.profile_driver` running the same launch (KiB per
 # dispatch, means over the dispatches), FETCH_SIZE divided by 0.5 and WRITE_SIZE by 1.0: the
 # factors those counters read for a plain 512 MiB device copy in the same runs.  Keyed by the launch
-# shape measured (mode, source split or work items, waves per workgroup); reported only when the
-# bench runs that shape, null otherwise.
+# shape measured (c2: mode, source split, waves per workgroup; the plans: shape_key); reported only
+# when the bench runs that shape, null otherwise.
 C2_FWD_TRAFFIC = {("mixed", 6, 4): (3602.66 / 0.5 + 193676 / 1.0) * 1024,       # k_eval_grid
                   ("fp64", 2, 4): (2494.88 / 0.5 + 64014.9 / 1.0) * 1024,
                   ("mixed", 2, 4): (2511.5 / 0.5 + 64283.9 / 1.0) * 1024,
                   ("fp64", 6, 4): (3591.22 / 0.5 + 192287 / 1.0) * 1024}
-C4_PLAN_TRAFFIC = {("mixed", 21199, 4): (56107.1 / 0.5 + 339897 / 1.0) * 1024,   # k_batch_eval_grid
-                   ("fp64", 21199, 4): (59845.0 / 0.5 + 339885 / 1.0) * 1024}
-C5_PLAN_TRAFFIC = {("mixed", 49152, 4): {"fwd": (18082.8 / 0.5 + 787997 / 1.0) * 1024,
-                                         "rev": (106837 / 0.5 + 170382 / 1.0) * 1024},
-                   ("fp64", 24576, 4): {"fwd": (21665.6 / 0.5 + 394017 / 1.0) * 1024,
-                                        "rev": (91794.7 / 0.5 + 135075 / 1.0) * 1024}}
+
+
+def shape_key(mode, r):
+    """(mode, item size, column / block-source item scales, waves) of a plan's tuned launch."""
+    c = r.get("cfg") or {}
+    return (mode, float(c.get("item", 0.0)), float(c.get("col_scale", 1.0)), float(c.get("gs_scale", 1.0)),
+            r.get("waves"))
+
+
+C4_PLAN_TRAFFIC = {("mixed", 8e6, 2.5, 1.0, 4): (61747.7 / 0.5 + 292839 / 1.0) * 1024,   # k_batch_eval_grid
+                   ("fp64", 0.0, 1.0, 1.0, 4): (59845.0 / 0.5 + 339885 / 1.0) * 1024}
+C5_PLAN_TRAFFIC = {("mixed", 1e7, 2.5, 1.5, 4): {"fwd": (16814.2 / 0.5 + 394011 / 1.0) * 1024,
+                                                 "rev": (91200.4 / 0.5 + 113708 / 1.0) * 1024},
+                   ("fp64", 2.5e7, 1.0, 1.0, 4): {"fwd": (19378.3 / 0.5 + 394009 / 1.0) * 1024,
+                                                  "rev": (89607.5 / 0.5 + 135066 / 1.0) * 1024}}
 
 
 def log(*a):
@@ -569,7 +578,7 @@ def c5_record(args, modes, d, steps, warmup):
         kf, kr = (r["kernels_ms"] + [None, None])[:2]
         peak = PEAK_FP32_TFLOPS if mode == "mixed" else PEAK_FP64_TFLOPS
         frac = lambda k: None if k is None else FLOP_PER_INTERACTION * ns * nc / (k * 1e-3) / 1e12 / peak  # noqa
-        tr = C5_PLAN_TRAFFIC.get((mode, r["items"], r["waves"]), {})
+        tr = C5_PLAN_TRAFFIC.get(shape_key(mode, r), {})
         rec[mode] = {"value": inter / (r["ms"] * 1e-3), "ms_per_step": r["ms"], "fwd_kernel_ms": kf,
                      "rev_kernel_ms": kr, "fwd_roofline_frac": frac(kf), "rev_roofline_frac": frac(kr),
                      "peak": peak, "launch_shape": r["cfg"], "work_items": r["items"], "waves": r["waves"],
@@ -666,7 +675,7 @@ def bench_c4_c5(args, line, modes, head, d):
     line["roofline"] = roofline_of(head, inter_step, r["kernel_ms"],
                                    f"k_batch_eval_grid<{head}> (persistent column kernel over the 64 problems' "
                                    f"leaf blocks)", algorithmic_bytes=kern_bytes,
-                                   traffic=C4_PLAN_TRAFFIC.get((head, r["items"], r["waves"])))
+                                   traffic=C4_PLAN_TRAFFIC.get(shape_key(head, r)))
     line["explicit_targets"] = explicit
     if "fp64" in results and head == "mixed":
         r64 = results["fp64"]

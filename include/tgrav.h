@@ -301,6 +301,8 @@ int tgrav_batch_create(tgrav_ctx* ctx, int32_t n_problems, const tgrav_problem* 
  * persistent evaluation, chunk reduction + kick) in a hipGraph on first use and
  * replays it afterwards. */
 int tgrav_batch_run(tgrav_batch* b, int use_graph);
+/* Destroy every plan before its context (tgrav_destroy): a plan uses the context's device and
+ * stream. */
 int tgrav_batch_destroy(tgrav_batch* b);
 /* Work-list statistics: items, total interactions. */
 int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions);
@@ -308,6 +310,15 @@ int tgrav_batch_info(tgrav_batch* b, int64_t* n_items, double* interactions);
  * (0 = total / (8 x CUs)) and persistent workgroups per CU (0 = occupancy limit).
  * The kernel variant is the one tgrav_set_config chose for the mode.  Set by the autotuner. */
 int tgrav_set_batch_config(tgrav_ctx* ctx, double item_interactions, int wg_per_cu);
+/* Plans created afterwards cut the problems of their column work list (targets that are 8^3 leaf
+ * blocks, or explicit targets in their C order) into items of column_scale x the item size, and
+ * those of their block-source work list (sources that are 8^3 leaf blocks) into items of
+ * block_source_scale x it: every source chunk of a problem writes 32 B of partial sums per target,
+ * and a list with many items balances with larger ones.  Scales in (0, 64]; both 0 return to the
+ * tune cache's values (or 1).  Set by the autotuner.  No counterpart in the reference. */
+int tgrav_set_batch_scales(tgrav_ctx* ctx, double column_scale, double block_source_scale);
+/* Bytes of chunk partial sums a run of the plan writes (and its reduction reads back). */
+int tgrav_batch_partials(tgrav_batch* b, int64_t* bytes);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* on != 0: record HIP events on the context's stream around every evaluation kernel

@@ -35,3 +35,17 @@ def test_block_target_split_takes_the_wider_tie():
              dict(nsplit=8, ms=2.358), dict(nsplit=12, ms=2.613), dict(nsplit=16, ms=2.673)]
     assert autotune.pick(table, autotune._chunks)["nsplit"] == 6
     assert autotune.pick(table, autotune._chunks, autotune.TIE_BLOCK_TARGETS)["nsplit"] == 2
+
+
+def test_list_scales_kept_only_when_as_fast():
+    # c5 mixed plan rows (one box): larger column / block-source items cut the partial sums
+    key = lambda r: (r["partial_bytes"], r["items_n"])
+    rows = [dict(item=1e7, col_scale=1.0, gs_scale=1.0, ms=175.54, items_n=49152, partial_bytes=978e6),
+            dict(item=2.5e7, col_scale=1.0, gs_scale=1.0, ms=177.7, items_n=24576, partial_bytes=472e6),
+            dict(item=1e7, col_scale=2.5, gs_scale=1.0, ms=175.85, items_n=24576, partial_bytes=576e6),
+            dict(item=1e7, col_scale=2.5, gs_scale=1.5, ms=176.1, items_n=24576, partial_bytes=518e6)]
+    best = autotune.pick(rows, key)
+    assert (best["item"], best["col_scale"], best["gs_scale"]) == (1e7, 2.5, 1.5)
+    rows[3]["ms"] = 178.0                 # outside the 1% band of the fastest: not taken
+    best = autotune.pick(rows, key)
+    assert (best["item"], best["col_scale"], best["gs_scale"]) == (1e7, 2.5, 1.0)

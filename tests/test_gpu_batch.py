@@ -246,6 +246,72 @@ def test_block_target_problems(mode):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_column_items_scaled(mode):
+    """tgrav_set_batch_scales: the column list's items at 2.5x the item size give fewer work items
+    and partial sums, the block and explicit forms stay bit-identical, graph replays equal the
+    eager run, and the results agree with the unscaled plan (FP64 within 1e-12, mixed within the
+    gate)."""
+    import torch
+    probs = block_batch(seed=45)
+    explicit = [dict(sources=p["sources"], targets={k: p["targets"][k] for k in ("x", "y", "z")})
+                for p in probs]
+    dev_b, dev_e = problems_from_workload(probs), problems_from_workload(explicit)
+    lib = _lib.load()
+    res = {}
+    with GravityField(mode) as f:
+        set_item(f, 2e5)
+        for scale in (1.0, 2.5):
+            assert lib.tgrav_set_batch_scales(f.handle, scale, 1.0) == 0
+            pb, pe = BatchPlan(f, dev_b), BatchPlan(f, dev_e)
+            eager = to_np(pb.run(graph=False))
+            again = to_np(pb.run(graph=True))
+            expl = to_np(pe.run(graph=True))
+            torch.cuda.synchronize()
+            for i in range(len(probs)):
+                for q in range(4):
+                    assert np.array_equal(again[i][q], eager[i][q]), (scale, i, q)
+                    assert np.array_equal(expl[i][q], eager[i][q]), (scale, i, q)
+            res[scale] = (eager, pb.n_items, pb.partial_bytes)
+            pb.close()
+            pe.close()
+        assert lib.tgrav_set_batch_scales(f.handle, 0.0, 0.0) == 0
+        assert lib.tgrav_set_batch_scales(f.handle, -1.0, 1.0) == _lib.EINVAL
+    assert res[2.5][1] < res[1.0][1] and res[2.5][2] < res[1.0][2]
+    for i in range(len(probs)):
+        if len(probs[i]["sources"]["x"]) == 0:
+            continue
+        s = _util.summary(res[2.5][0][i], res[1.0][0][i])
+        if mode == "fp64":
+            assert s["acc_max"] <= 1e-12 and s["pot_max"] <= 1e-12, (i, s)
+        else:
+            assert s["acc_median"] <= 1e-6 and s["acc_max"] <= 1e-4, (i, s)
+
+
+def test_column_chunks_stay_within_the_l2():
+    """A block-target problem with 60,000 sources and an item size that would give one chunk is
+    cut into two (a chunk's sources stay within 2 MB): two work items, FP64 within 1e-12 of the
+    oracle."""
+    rng = W.rng_for(46)
+    o = rng.uniform(-1.0, 0.5, (1, 3))
+    s = np.array([0.25])
+    tx, ty, tz, _ = W.block_cell_centres(o, s)
+    st = W.star_cluster(rng, 60000)
+    eps = rng.uniform(0.005, 0.05, 60000)
+    probs = [dict(sources=dict(x=st["x"], y=st["y"], z=st["z"], m=st["m"], eps=eps),
+                  targets=dict(x=tx, y=ty, z=tz, block_origin=o, block_dx=s / 8))]
+    with GravityField("fp64") as f:
+        set_item(f, 1e30)
+        plan = BatchPlan(f, problems_from_workload(probs))
+        assert plan.n_items == 2
+        got = to_np(plan.run(graph=True))[0]
+        plan.close()
+    ref = oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], eps, tx, ty, tz, None, G,
+                              mode=oracle.NEUMAIER, threads=16)
+    s2 = _util.summary(got, ref)
+    assert s2["acc_max"] <= 1e-12 and s2["pot_max"] <= 1e-12, s2
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
 def test_explicit_targets_without_column_structure(mode):
     """Explicit targets whose count is a multiple of 512 but which are not in the blocks' C order
     (shuffled cell centres) stay on the generic work list: against the oracle (FP64) and the
@@ -337,7 +403,27 @@ def test_block_source_problems(mode):
             for i in range(3):
                 for q in range(4):
                     assert np.array_equal(again[i][q], eager[i][q]), (rep, i, q)
+        n0 = plan.partial_bytes
         plan.close()
+        # the block-source list's items at 2.5x (tgrav_set_batch_scales): fewer partial sums, the
+        # same field (FP64 within 1e-12, mixed within the gate)
+        set_item(f, 2e5)
+        plan = BatchPlan(f, dev)
+        base = to_np(plan.run(graph=True))
+        nb0 = plan.partial_bytes
+        plan.close()
+        _lib.check(_lib.load().tgrav_set_batch_scales(f.handle, 1.0, 2.5), "tgrav_set_batch_scales")
+        plan = BatchPlan(f, dev)
+        scaled = to_np(plan.run(graph=True))
+        assert plan.partial_bytes < nb0, (plan.partial_bytes, nb0, n0)
+        plan.close()
+        _lib.check(_lib.load().tgrav_set_batch_scales(f.handle, 0.0, 0.0), "tgrav_set_batch_scales")
+        set_item(f, 0)
+    s_sc = _util.summary(scaled[0], base[0])
+    if mode == "fp64":
+        assert s_sc["acc_max"] <= 1e-12 and s_sc["pot_max"] <= 1e-12, s_sc
+    else:
+        assert s_sc["acc_median"] <= 1e-6 and s_sc["acc_max"] <= 1e-4, s_sc
     refs = [oracle.eval_plummer(cx, cy, cz, cm, ce, st["x"], st["y"], st["z"], None, G, mode=oracle.NEUMAIER,
                                 threads=16),
             oracle.eval_plummer(st["x"], st["y"], st["z"], st["m"], np.full(900, 0.01), cx, cy, cz, None, G,
@@ -391,3 +477,18 @@ def test_column_structure_checked_at_every_run(mode):
                     assert np.array_equal(got[q], ref_blk[q]), q
         plan.close()
         blk.close()
+
+
+def test_plan_left_open_is_closed_with_its_field():
+    """A plan the caller never closed is closed by its field before the field's context goes
+    (the plan uses the context's device and stream), so dropping it afterwards is harmless."""
+    import gc
+    probs = block_batch(seed=47, n=2)
+    f = GravityField("fp64")
+    plan = BatchPlan(f, problems_from_workload(probs))
+    plan.run(graph=False)
+    f.close()
+    assert plan._h is None
+    plan.close()
+    del plan
+    gc.collect()

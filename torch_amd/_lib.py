@@ -83,6 +83,8 @@ SIGNATURES = [
     ("tgrav_batch_destroy", _I32, [_P]),
     ("tgrav_batch_info", _I32, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
     ("tgrav_set_batch_config", _I32, [_P, _D, _I32]),
+    ("tgrav_set_batch_scales", _I32, [_P, _D, _D]),
+    ("tgrav_batch_partials", _I32, [_P, ctypes.POINTER(ctypes.c_int64)]),
     ("tgrav_set_timing", _I32, [_P, _I32]),
     ("tgrav_timing_read", _I32, [_P, _I32, ctypes.POINTER(_D), ctypes.POINTER(_I64),
                                  ctypes.POINTER(_I64)]),

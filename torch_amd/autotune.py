@@ -191,13 +191,35 @@ def tune_batch(mode, problems, reps=3, items=(0.0, 2e6, 8e6, 3.2e7), wg_per_cu=(
                     plan = BatchPlan(f, dev)
                     ms = _time(lambda: plan.run(graph=True), reps)
                     table.append(dict(v, item=it, wg_per_cu=w, ms=ms, items_n=plan.n_items,
+                                      partial_bytes=plan.partial_bytes, col_scale=1.0, gs_scale=1.0,
                                       rate=plan.interactions / ms * 1e3))
                     log(f"  batch {mode} variant={v['variant']} (K={v['k']}, NT={v['threads']}) "
                         f"item={it:.1e} wg/cu={w}: {ms:.3f} ms, {plan.n_items} items")
                     plan.close()
                     torch.cuda.synchronize()
         lib.tgrav_set_batch_config(f.handle, 0.0, 0)
-    best = pick(table, lambda r: r["items_n"])   # fewer work items: fewer source chunks
+        blocks = [any(p[side].get("block_origin") is not None for p in problems) for side in ("targets", "sources")]
+        if any(blocks):
+            # Larger items for the block lists at the fastest shape's item size: each source chunk
+            # writes 32 B of partial sums per target, and these lists have many items to balance.
+            fast = min(table, key=lambda r: r["ms"])
+            for cs, gs in LIST_SCALES:
+                if (cs != 1.0 and not blocks[0]) or (gs != 1.0 and not blocks[1]):
+                    continue
+                f.set_config(variant=fast["variant"])
+                lib.tgrav_set_batch_config(f.handle, float(fast["item"]), int(fast["wg_per_cu"]))
+                lib.tgrav_set_batch_scales(f.handle, float(cs), float(gs))
+                plan = BatchPlan(f, dev)
+                ms = _time(lambda: plan.run(graph=True), reps)
+                table.append(dict(fast, col_scale=cs, gs_scale=gs, ms=ms, items_n=plan.n_items,
+                                  partial_bytes=plan.partial_bytes, rate=plan.interactions / ms * 1e3))
+                log(f"  batch {mode} item={fast['item']:.1e} list items x{cs:g} / x{gs:g}: {ms:.3f} ms, "
+                    f"{plan.partial_bytes / 1e6:.0f} MB of partials")
+                plan.close()
+                torch.cuda.synchronize()
+            lib.tgrav_set_batch_scales(f.handle, 0.0, 0.0)
+            lib.tgrav_set_batch_config(f.handle, 0.0, 0)
+    best = pick(table, lambda r: (r["partial_bytes"], r["items_n"]))   # fewer partial sums
     return dict(best, waves=waves), table
 
 
@@ -218,6 +240,10 @@ TIE = 0.01   # candidates within 1% of the fastest count as equally fast (repeat
 # algorithmic bytes of the launch (64 B per cell), and the reduction reads them back: its split keeps
 # the fewest chunks within 2% of the fastest.
 TIE_BLOCK_TARGETS = 0.02
+
+
+# (column, block-source) work lists' item sizes against the plan's, tried by tune_batch
+LIST_SCALES = ((1.0, 1.0), (2.5, 1.0), (1.0, 1.5), (2.5, 1.5))
 
 
 def pick(table, fewer_partials, tie=TIE):
@@ -300,6 +326,8 @@ def apply_batch(field, problems, cache=None):
         field.set_config(variant=e["variant"])
         _lib.check(field.lib.tgrav_set_batch_config(field.handle, float(e["item"]),
                                                      int(e["wg_per_cu"])), "tgrav_set_batch_config")
+        _lib.check(field.lib.tgrav_set_batch_scales(field.handle, float(e.get("col_scale", 1.0)),
+                                                     float(e.get("gs_scale", 1.0))), "tgrav_set_batch_scales")
     return e
 
 
@@ -337,7 +365,8 @@ def main():
                 best, _ = tune_batch(mode, probs, reps=2, items=(0.0, 1e7, 2.5e7, 5e7, 1e8))
             key = batch_key(mode, probs, best["waves"])
             cache[key] = dict(variant=best["variant"], item=best["item"], wg_per_cu=best["wg_per_cu"],
-                              ms=best["ms"], k=best["k"], threads=best["threads"])
+                              col_scale=best["col_scale"], gs_scale=best["gs_scale"], ms=best["ms"], k=best["k"],
+                              threads=best["threads"])
             print(f"best {mode} batch {name}: {cache[key]}", flush=True)
             save_cache(cache, args.cache)
     print(f"tuned in {time.time() - t0:.1f} s -> {args.cache}", flush=True)

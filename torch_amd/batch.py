@@ -102,11 +102,15 @@ class BatchPlan:
                                          ctypes.byref(h))
         _lib.check(rc, "tgrav_batch_create")
         self._h = h
+        field._plans.add(self)   # the field closes its plans before its context
         n_items, inter = ctypes.c_int64(), ctypes.c_double()
         _lib.check(self.lib.tgrav_batch_info(h, ctypes.byref(n_items), ctypes.byref(inter)),
                    "tgrav_batch_info")
         self.n_items = n_items.value
         self.interactions = inter.value
+        pbytes = ctypes.c_int64()
+        _lib.check(self.lib.tgrav_batch_partials(h, ctypes.byref(pbytes)), "tgrav_batch_partials")
+        self.partial_bytes = pbytes.value   # chunk partial sums written per run
 
     def run(self, graph=True):
         """Enqueue the plan on torch's current stream (graph=True: captured once, then

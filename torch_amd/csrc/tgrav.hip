@@ -192,7 +192,7 @@ const Variant* variant_of(int mode, int v) {
 // (variant, nsplit, item, wg_per_cu, ...).  Size buckets are round(log2(n)).
 struct TuneEntry {
   int variant = -1, nsplit = -1, wg_per_cu = -1;
-  double item = -1.0;
+  double item = -1.0, col_scale = -1.0, gs_scale = -1.0;
 };
 
 struct JsonIn {
@@ -270,6 +270,8 @@ static int parse_tune_cache(const std::string& text, std::map<std::string, TuneE
           else if (f == "nsplit") t.nsplit = (int)v;
           else if (f == "wg_per_cu") t.wg_per_cu = (int)v;
           else if (f == "item") t.item = v;
+          else if (f == "col_scale") t.col_scale = v;
+          else if (f == "gs_scale") t.gs_scale = v;
         } else if (!in.skip_value()) {
           return -1;
         }
@@ -325,6 +327,8 @@ struct tgrav_ctx {
   int nrep = 6;
   double batch_item = 0.0;   // target interactions per batch work item (0 = automatic)
   int batch_wg_per_cu = 0;   // persistent workgroups per CU (0 = occupancy limit)
+  double batch_scale[2] = {1.0, 1.0};   // column / block-source work lists: items of this many
+  bool user_scales = false;             //   times the item size; tgrav_set_batch_scales was called
   // packed sources; stat: their statistics (tg::ST_*, read by the kernels; ST_BAD = NaN flag)
   int64_t ns = -1;
   DevBuf s4, se2, stat;
@@ -2248,6 +2252,17 @@ int tgrav_set_batch_config(tgrav_ctx* c, double item_interactions, int wg_per_cu
   return 0;
 }
 
+int tgrav_set_batch_scales(tgrav_ctx* c, double column_scale, double block_source_scale) {
+  if (!c) return fail(TGRAV_EINVAL, "context is NULL");
+  if (!(column_scale >= 0.0 && column_scale <= 64.0 && block_source_scale >= 0.0 && block_source_scale <= 64.0))
+    return fail(TGRAV_EINVAL, "item scales must be in [0, 64]");
+  const bool reset = column_scale == 0.0 && block_source_scale == 0.0;
+  c->batch_scale[0] = column_scale > 0.0 ? column_scale : 1.0;
+  c->batch_scale[1] = block_source_scale > 0.0 ? block_source_scale : 1.0;
+  c->user_scales = !reset;
+  return 0;
+}
+
 }  // extern "C"
 
 struct tgrav_batch {
@@ -2260,6 +2275,7 @@ struct tgrav_batch {
   bool skip_generic = false;     // every generic item belongs to a candidate (skipped when all pass)
   const Variant* v = nullptr;
   int64_t total_src = 0, red_total = 0, cand_total = 0, total_sblk = 0;
+  int64_t part_bytes = 0;          // chunk partial sums written per run (bytes)
   double inter = 0.0;
   DevBuf probs, items, gitems, sitems, sgoff, srcs, sprefix, s4, se2, sax, be2, bms, stats, part, counter, ids,
       rprefix, cand, cprefix, colflag;
@@ -2438,6 +2454,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   b->inter = total;
   double item = c->batch_item;
   int wg_per_cu = c->batch_wg_per_cu;
+  double col_scale = c->batch_scale[0], gs_scale = c->batch_scale[1];
   bool tuned_item = c->user_batch && item > 0;
   if (!c->tune.empty() && n > 0) {   // the autotuner's shape for batches like this one
     std::vector<double> vs(n), vt(n);
@@ -2462,6 +2479,8 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
         if (t->wg_per_cu >= 0) wg_per_cu = t->wg_per_cu;
         tuned_item = item > 0;
       }
+      if (!c->user_scales && t->col_scale > 0) col_scale = t->col_scale;
+      if (!c->user_scales && t->gs_scale > 0) gs_scale = t->gs_scale;
     }
   }
   // Resident workgroups of each persistent kernel: the occupancy limit, or for the generic kernel
@@ -2498,8 +2517,8 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const int sl = li == 0 ? slots : li == 1 ? gslots : sslots;
     return std::min(C, std::max(floor_cost, tot_list[li] / (8.0 * sl)));
   };
-  const double C_gen = list_cost(0, (double)T * NT * 4.0), C_col = list_cost(1, 512.0 * 64 * 4 * CW),
-               C_gs = list_cost(2, (double)GT * 512.0 * 4 * CW);
+  const double C_gen = list_cost(0, (double)T * NT * 4.0), C_col = col_scale * list_cost(1, 512.0 * 64 * 4 * CW),
+               C_gs = gs_scale * list_cost(2, (double)GT * 512.0 * 4 * CW);
   std::vector<tg::BProb> probs(n);
   std::vector<tg::BSrc> srcs(n);
   std::vector<int64_t> sprefix(n + 1, 0);
@@ -2556,6 +2575,11 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
     const double per_tile = (double)std::min<int64_t>(TT, q.n_targets) * q.n_sources;
     const double Cp = gs ? C_gs : col ? C_col : C_gen;
     int64_t nch = per_tile > Cp ? (int64_t)ceil(per_tile / Cp) : 1;
+    // A column chunk's sources (40 B each with eps^2) stay within half an XCD's L2, so that the
+    // workgroups in flight, which walk the same chunk out of step, share it there: kColChunk
+    // sources (whole-block items over 100,000 sources read 12x the bytes from beyond the L2).
+    const int64_t kColChunk = ((int64_t)(2 << 20) / 40) / NT * NT;
+    if (col) nch = std::max<int64_t>(nch, (units + kColChunk - 1) / kColChunk);
     int64_t ch = (units + nch - 1) / nch;
     ch = (ch + SG - 1) / SG * SG;
     nch = (units + ch - 1) / ch;
@@ -2624,6 +2648,7 @@ int tgrav_batch_create(tgrav_ctx* c, int32_t n, const tgrav_problem* pr, int mod
   b->red_total = rprefix.back();
   b->total_src = sprefix[n];
   b->total_sblk = sblk_total;
+  b->part_bytes = part_total * (int64_t)sizeof(double);
   b->ncand = (int)cand.size();
   b->skip_generic = !sorted.empty() && std::all_of(sorted.begin(), sorted.end(), [&](const tg::BItem& w) {
     return probs[w.prob].form == tg::FORM_CAND;
@@ -2691,6 +2716,12 @@ int tgrav_batch_destroy(tgrav_batch* b) {
   (void)hipSetDevice(b->c->device);
   (void)hipStreamSynchronize(b->c->stream);
   batch_free(b);
+  return 0;
+}
+
+int tgrav_batch_partials(tgrav_batch* b, int64_t* bytes) {
+  if (!b || !bytes) return fail(TGRAV_EINVAL, "batch or output is NULL");
+  *bytes = b->part_bytes;
   return 0;
 }
 

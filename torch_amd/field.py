@@ -33,6 +33,7 @@ speed^2.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -156,9 +157,12 @@ class GravityField:
         _lib.check(self.lib.tgrav_set_constant(ctx, float(G)), "tgrav_set_constant")
         _lib.check(self.lib.tgrav_set_validation(ctx, int(bool(validate))), "tgrav_set_validation")
         self.n_sources = None
+        self._plans = weakref.WeakSet()   # batch plans on this context: closed before it
 
     # -- lifetime ---------------------------------------------------------------
     def close(self):
+        for plan in list(getattr(self, "_plans", ())):
+            plan.close()
         if getattr(self, "_ctx", None):
             self.lib.tgrav_destroy(self._ctx)
             self._ctx = None
