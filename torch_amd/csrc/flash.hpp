@@ -570,8 +570,10 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
     const int nhi = __builtin_amdgcn_readfirstlane((int)(R.zu1 - R.zc1));
     double fz = R.zu0;
     for (int i = 0; i < nlo; ++i, fz += 1.0) tested(fz);
-    for (int i = 0; i < nc; ++i, fz += 1.0) {
-      const double dz = fma(fz, P.lz, dzi);
+    // the certain images, dz stepped by lz: one add per image instead of an index step and an FMA
+    // (4% of the walk); the steps round dz to within nc/2 ulps of the row's largest |dz|
+    double dz = fma(R.zc0, P.lz, dzi);
+    for (int i = 0; i < nc; ++i, dz += P.lz) {
       const double y = rsqrt_f64<0>(fma(dz, dz, xy));
       const double kv = (y * y) * y;
       skv += kv;
@@ -579,6 +581,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
       if (QUIRKS) pp = y;
       else pp += y;
     }
+    fz = R.zc1 + 1.0;
     for (int i = 0; i < nhi; ++i, fz += 1.0) tested(fz);
     px = fma(skv, dx, px);
     py = fma(skv, dy, py);
