@@ -378,11 +378,15 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
 #define TG_PAIR_SLACK 1e-4   // image spacings; the square roots (v_rsq_f64, ~2^-24) are off by < 1e-5
 #define TG_PAIR_ROWS ((2 * TG_PAIR_NREP + 1) * (2 * TG_PAIR_NREP + 1))
 
-struct PairRow {
+// The z images of a row: zu0 .. zu0 + nlo - 1 tested, then nc inside the cutoff for every pair of
+// the wave, then zh0 .. zh0 + nhi - 1 tested; soft: a row that may come within rsoft, its nlo images
+// from zu0 all taking the general test and kernel.  The counts are formed when the table is built,
+// one row per lane, so the walk reads them ready for the scalar unit.
+struct __align__(16) PairRow {
   double fxl, fyl;          // fl(nx lx), fl(ny ly): the reference forms d + n L from these
-  double zu0, zu1;          // z images some pair of the wave may have inside the cutoff
-  double zc0, zc1;          // of those, the ones inside for every pair; zc0 = -1e300: a row that
-};                          //   may come within rsoft (general loop over [zu0, zu1])
+  double zu0, zh0;
+  int nlo, nc, nhi, soft;
+};
 
 __device__ __forceinline__ unsigned int morton_spread10(unsigned int v) {
   v &= 0x3ffu;
@@ -487,27 +491,30 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
       }
       if (remh >= 0.0) {
         const double sh = remh * __builtin_amdgcn_rsq(remh + 0x1p-1022);
-        R.zu0 = fmax(ceil((-sh - dzh) * P.ilz - TG_PAIR_SLACK), -fN);
-        R.zu1 = fmin(floor((sh - dzl) * P.ilz + TG_PAIR_SLACK), fN);
-        keep = R.zu0 <= R.zu1;
-        R.zc0 = R.zu1 + 1.0;   // no certain images unless shown below
-        R.zc1 = R.zu1;
-        if (!(xyl > P.soft_thr)) {
-          R.zc0 = -1e300;      // some image of the row may lie within rsoft
-        } else if (reml > 0.0) {
+        const double zu0 = fmax(ceil((-sh - dzh) * P.ilz - TG_PAIR_SLACK), -fN);
+        const double zu1 = fmin(floor((sh - dzl) * P.ilz + TG_PAIR_SLACK), fN);
+        keep = zu0 <= zu1;
+        double zc0 = zu1 + 1.0, zc1 = zu1;   // no certain images unless shown below
+        R.soft = !(xyl > P.soft_thr);        // some image of the row may lie within rsoft
+        if (!R.soft && reml > 0.0) {
           // every pair of the box: |dz + n L| <= sqrt(reml) - m L keeps the reference's rounded
           // r2 below cut_thr (m: the error of the square root and 2.7u cut / sqrt(reml) for the
           // rounding of dz^2 and r2)
           const double yl = __builtin_amdgcn_rsq(reml);
           const double sl = reml * yl;
           const double m = fma(cut * P.ilz * 6e-16, yl, TG_PAIR_SLACK);
-          const double c0 = fmax(ceil((-sl - dzl) * P.ilz + m), R.zu0);
-          const double c1 = fmin(floor((sl - dzh) * P.ilz - m), R.zu1);
+          const double c0 = fmax(ceil((-sl - dzl) * P.ilz + m), zu0);
+          const double c1 = fmin(floor((sl - dzh) * P.ilz - m), zu1);
           if (c0 <= c1) {
-            R.zc0 = c0;
-            R.zc1 = c1;
+            zc0 = c0;
+            zc1 = c1;
           }
         }
+        R.zu0 = zu0;
+        R.zh0 = zc1 + 1.0;
+        R.nlo = R.soft ? (int)(zu1 - zu0) + 1 : (int)(zc0 - zu0);
+        R.nc = R.soft ? 0 : (int)(zc1 - zc0) + 1;
+        R.nhi = R.soft ? 0 : (int)(zu1 - zc1);
       }
     }
     const unsigned long long b = __builtin_amdgcn_ballot_w64(keep);
@@ -527,8 +534,14 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
       dy = dyi + R.fyl;
       xy = dx * dx + dy * dy;
     }
-    if (R.zc0 < -1e299) {   // (uniform) the general loop: the reference's test and kernel per image
-      for (double fz = R.zu0; fz <= R.zu1; fz += 1.0) {
+    // the row's counts are the same in every lane: kept on the scalar unit
+    const int nlo = __builtin_amdgcn_readfirstlane(R.nlo);
+    const int nc = __builtin_amdgcn_readfirstlane(R.nc);
+    const int nhi = __builtin_amdgcn_readfirstlane(R.nhi);
+    double skv = 0.0;   // the row's sum of kv: dx and dy are fixed along it
+    double fz = R.zu0;
+    if (__builtin_amdgcn_readfirstlane(R.soft)) {   // the reference's test and kernel per image
+      for (int i = 0; i < nlo; ++i, fz += 1.0) {
         double dz, r2;
         {
 #pragma clang fp contract(off)
@@ -538,51 +551,45 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
         if (r2 <= cut) {
           double kv, pk;
           flash_pair<false>(P, r2, kv, pk);
-          px = fma(kv, dx, px);
-          py = fma(kv, dy, py);
+          skv += kv;
           pz = fma(kv, dz, pz);
           if (QUIRKS) pp = pk;
           else pp += pk;
         }
       }
-      continue;
-    }
-    double skv = 0.0;
-    auto tested = [&](double fz) {
-      double dz, r2;
-      {
+    } else {
+      auto tested = [&](double fz) {
+        double dz, r2;
+        {
 #pragma clang fp contract(off)
-        dz = dzi + fz * P.lz;
-        r2 = xy + dz * dz;
-      }
-      if (r2 <= cut) {
-        const double y = rsqrt_f64<0>(r2);
+          dz = dzi + fz * P.lz;
+          r2 = xy + dz * dz;
+        }
+        if (r2 <= cut) {
+          const double y = rsqrt_f64<0>(r2);
+          const double kv = (y * y) * y;
+          skv += kv;
+          pz = fma(kv, dz, pz);
+          if (QUIRKS) pp = y;
+          else pp += y;
+        }
+      };
+      for (int i = 0; i < nlo; ++i, fz += 1.0) tested(fz);
+      // the certain images from fz = zu0 + nlo, dz stepped by lz: one add per image instead of an
+      // index step and an FMA (4% of the walk); the steps round dz to within nc/2 ulps of the row's
+      // largest |dz|
+      double dz = fma(fz, P.lz, dzi);
+      for (int i = 0; i < nc; ++i, dz += P.lz) {
+        const double y = rsqrt_f64<0>(fma(dz, dz, xy));
         const double kv = (y * y) * y;
         skv += kv;
         pz = fma(kv, dz, pz);
         if (QUIRKS) pp = y;
         else pp += y;
       }
-    };
-    // the trip counts are the same in every lane: counted on the scalar unit
-    const int nlo = __builtin_amdgcn_readfirstlane((int)(R.zc0 - R.zu0));
-    const int nc = __builtin_amdgcn_readfirstlane((int)(R.zc1 - R.zc0) + 1);
-    const int nhi = __builtin_amdgcn_readfirstlane((int)(R.zu1 - R.zc1));
-    double fz = R.zu0;
-    for (int i = 0; i < nlo; ++i, fz += 1.0) tested(fz);
-    // the certain images, dz stepped by lz: one add per image instead of an index step and an FMA
-    // (4% of the walk); the steps round dz to within nc/2 ulps of the row's largest |dz|
-    double dz = fma(R.zc0, P.lz, dzi);
-    for (int i = 0; i < nc; ++i, dz += P.lz) {
-      const double y = rsqrt_f64<0>(fma(dz, dz, xy));
-      const double kv = (y * y) * y;
-      skv += kv;
-      pz = fma(kv, dz, pz);
-      if (QUIRKS) pp = y;
-      else pp += y;
+      fz = R.zh0;
+      for (int i = 0; i < nhi; ++i, fz += 1.0) tested(fz);
     }
-    fz = R.zc1 + 1.0;
-    for (int i = 0; i < nhi; ++i, fz += 1.0) tested(fz);
     px = fma(skv, dx, px);
     py = fma(skv, dy, py);
   }
