@@ -381,11 +381,12 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
 // The z images of a row: zu0 .. zu0 + nlo - 1 tested, then nc inside the cutoff for every pair of
 // the wave, then zh0 .. zh0 + nhi - 1 tested; soft: a row that may come within rsoft, its nlo images
 // from zu0 all taking the general test and kernel.  The counts are formed when the table is built,
-// one row per lane, so the walk reads them ready for the scalar unit.
+// one row per lane, and packed into one word (nlo | nc << 8 | nhi << 16 | soft << 24, each < 256)
+// that the walk moves to the scalar unit in one instruction.
 struct __align__(16) PairRow {
   double fxl, fyl;          // fl(nx lx), fl(ny ly): the reference forms d + n L from these
   double zu0, zh0;
-  int nlo, nc, nhi, soft;
+  int counts, pad[3];
 };
 
 __device__ __forceinline__ unsigned int morton_spread10(unsigned int v) {
@@ -495,8 +496,8 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
         const double zu1 = fmin(floor((sh - dzl) * P.ilz + TG_PAIR_SLACK), fN);
         keep = zu0 <= zu1;
         double zc0 = zu1 + 1.0, zc1 = zu1;   // no certain images unless shown below
-        R.soft = !(xyl > P.soft_thr);        // some image of the row may lie within rsoft
-        if (!R.soft && reml > 0.0) {
+        const bool soft = !(xyl > P.soft_thr);   // some image of the row may lie within rsoft
+        if (!soft && reml > 0.0) {
           // every pair of the box: |dz + n L| <= sqrt(reml) - m L keeps the reference's rounded
           // r2 below cut_thr (m: the error of the square root and 2.7u cut / sqrt(reml) for the
           // rounding of dz^2 and r2)
@@ -512,9 +513,8 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
         }
         R.zu0 = zu0;
         R.zh0 = zc1 + 1.0;
-        R.nlo = R.soft ? (int)(zu1 - zu0) + 1 : (int)(zc0 - zu0);
-        R.nc = R.soft ? 0 : (int)(zc1 - zc0) + 1;
-        R.nhi = R.soft ? 0 : (int)(zu1 - zc1);
+        R.counts = soft ? ((int)(zu1 - zu0) + 1) | 1 << 24
+                        : (int)(zc0 - zu0) | ((int)(zc1 - zc0) + 1) << 8 | (int)(zu1 - zc1) << 16;
       }
     }
     const unsigned long long b = __builtin_amdgcn_ballot_w64(keep);
@@ -535,12 +535,11 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
       xy = dx * dx + dy * dy;
     }
     // the row's counts are the same in every lane: kept on the scalar unit
-    const int nlo = __builtin_amdgcn_readfirstlane(R.nlo);
-    const int nc = __builtin_amdgcn_readfirstlane(R.nc);
-    const int nhi = __builtin_amdgcn_readfirstlane(R.nhi);
+    const int counts = __builtin_amdgcn_readfirstlane(R.counts);
+    const int nlo = counts & 0xff, nc = (counts >> 8) & 0xff, nhi = (counts >> 16) & 0xff;
     double skv = 0.0;   // the row's sum of kv: dx and dy are fixed along it
     double fz = R.zu0;
-    if (__builtin_amdgcn_readfirstlane(R.soft)) {   // the reference's test and kernel per image
+    if (counts >> 24) {   // the reference's test and kernel per image
       for (int i = 0; i < nlo; ++i, fz += 1.0) {
         double dz, r2;
         {
