@@ -382,11 +382,12 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
 // the wave, then zh0 .. zh0 + nhi - 1 tested; soft: a row that may come within rsoft, its nlo images
 // from zu0 all taking the general test and kernel.  The counts are formed when the table is built,
 // one row per lane, and packed into one word (nlo | nc << 8 | nhi << 16 | soft << 24, each < 256)
-// that the walk moves to the scalar unit in one instruction.
+// that the walk moves to the scalar unit in one instruction.  Soft rows fill the table from its end;
+// `before` is the number of the other rows that precede a soft row in (nx, ny) order.
 struct __align__(16) PairRow {
   double fxl, fyl;          // fl(nx lx), fl(ny ly): the reference forms d + n L from these
   double zu0, zh0;
-  int counts, pad[3];
+  int counts, before, pad[2];
 };
 
 __device__ __forceinline__ unsigned int morton_spread10(unsigned int v) {
@@ -469,8 +470,9 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
   const double dzl = wave_min(valid ? dzi : INF), dzh = wave_max(valid ? dzi : -INF);
   const int N = P.nrep, W2 = 2 * N + 1;
   const double fN = (double)N, cut = P.cut_thr;
-  // the row table, in (nx, ny) order: lane j takes candidate 64 r + j
-  int nrows = 0;
+  // the row table, in (nx, ny) order: lane j takes candidate 64 r + j; rows that may come within
+  // rsoft (rare) go to the table's end, so the walk over the others has no branch for them
+  int nrows = 0, nsoft = 0;
   for (int r0 = 0; r0 < W2 * W2; r0 += 64) {
     const int cand = r0 + lane;
     bool keep = false;
@@ -517,46 +519,42 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
                         : (int)(zc0 - zu0) | ((int)(zc1 - zc0) + 1) << 8 | (int)(zu1 - zc1) << 16;
       }
     }
-    const unsigned long long b = __builtin_amdgcn_ballot_w64(keep);
-    if (keep) rows[nrows + __builtin_popcountll(b & ((1ULL << lane) - 1ULL))] = R;
-    nrows += __builtin_popcountll(b);
+    const bool soft = keep && (R.counts >> 24) != 0;
+    const unsigned long long lt = (1ULL << lane) - 1ULL;
+    const unsigned long long bh = __builtin_amdgcn_ballot_w64(keep && !soft);
+    const unsigned long long bs = __builtin_amdgcn_ballot_w64(soft);
+    const int before = nrows + __builtin_popcountll(bh & lt);
+    if (keep && !soft) rows[before] = R;
+    if (soft) {
+      R.before = before;
+      rows[TG_PAIR_ROWS - 1 - (nsoft + __builtin_popcountll(bs & lt))] = R;
+    }
+    nrows += __builtin_popcountll(bh);
+    nsoft += __builtin_popcountll(bs);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   double px = 0.0, py = 0.0, pz = 0.0, pp = 0.0;
-  for (int r = 0; r < nrows; ++r) {
-    const PairRow R = rows[r];
-    double dx, dy, xy;
-    {
+  // the rows in (nx, ny) order (the quirk's potential is the last image's): the runs of rows without
+  // a soft row, each followed by the soft row that ends it
+  int r = 0;
+  for (int k = 0; k <= nsoft; ++k) {
+    int end = nrows;
+    if (k < nsoft) end = __builtin_amdgcn_readfirstlane(rows[TG_PAIR_ROWS - 1 - k].before);
+    for (; r < end; ++r) {
+      const PairRow R = rows[r];
+      double dx, dy, xy;
+      {
 #pragma clang fp contract(off)
-      dx = dxi + R.fxl;
-      dy = dyi + R.fyl;
-      xy = dx * dx + dy * dy;
-    }
-    // the row's counts are the same in every lane: kept on the scalar unit
-    const int counts = __builtin_amdgcn_readfirstlane(R.counts);
-    const int nlo = counts & 0xff, nc = (counts >> 8) & 0xff, nhi = (counts >> 16) & 0xff;
-    double skv = 0.0;   // the row's sum of kv: dx and dy are fixed along it
-    double fz = R.zu0;
-    if (counts >> 24) {   // the reference's test and kernel per image
-      for (int i = 0; i < nlo; ++i, fz += 1.0) {
-        double dz, r2;
-        {
-#pragma clang fp contract(off)
-          dz = dzi + fz * P.lz;
-          r2 = xy + dz * dz;
-        }
-        if (r2 <= cut) {
-          double kv, pk;
-          flash_pair<false>(P, r2, kv, pk);
-          skv += kv;
-          pz = fma(kv, dz, pz);
-          if (QUIRKS) pp = pk;
-          else pp += pk;
-        }
+        dx = dxi + R.fxl;
+        dy = dyi + R.fyl;
+        xy = dx * dx + dy * dy;
       }
-    } else {
+      // the row's counts are the same in every lane: kept on the scalar unit
+      const int counts = __builtin_amdgcn_readfirstlane(R.counts);
+      const int nlo = counts & 0xff, nc = (counts >> 8) & 0xff, nhi = (counts >> 16) & 0xff;
+      double skv = 0.0;   // the row's sum of kv: dx and dy are fixed along it
       auto tested = [&](double fz) {
         double dz, r2;
         {
@@ -573,6 +571,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
           else pp += y;
         }
       };
+      double fz = R.zu0;
       for (int i = 0; i < nlo; ++i, fz += 1.0) tested(fz);
       // the certain images from fz = zu0 + nlo, dz stepped by lz: one add per image instead of an
       // index step and an FMA (4% of the walk); the steps round dz to within nc/2 ulps of the row's
@@ -588,9 +587,40 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
       }
       fz = R.zh0;
       for (int i = 0; i < nhi; ++i, fz += 1.0) tested(fz);
+      px = fma(skv, dx, px);
+      py = fma(skv, dy, py);
     }
-    px = fma(skv, dx, px);
-    py = fma(skv, dy, py);
+    if (k < nsoft) {   // a soft row: the reference's test and kernel per image
+      const PairRow R = rows[TG_PAIR_ROWS - 1 - k];
+      double dx, dy, xy;
+      {
+#pragma clang fp contract(off)
+        dx = dxi + R.fxl;
+        dy = dyi + R.fyl;
+        xy = dx * dx + dy * dy;
+      }
+      const int n = __builtin_amdgcn_readfirstlane(R.counts) & 0xff;
+      double skv = 0.0;
+      double fz = R.zu0;
+      for (int i = 0; i < n; ++i, fz += 1.0) {
+        double dz, r2;
+        {
+#pragma clang fp contract(off)
+          dz = dzi + fz * P.lz;
+          r2 = xy + dz * dz;
+        }
+        if (r2 <= cut) {
+          double kv, pk;
+          flash_pair<false>(P, r2, kv, pk);
+          skv += kv;
+          pz = fma(kv, dz, pz);
+          if (QUIRKS) pp = pk;
+          else pp += pk;
+        }
+      }
+      px = fma(skv, dx, px);
+      py = fma(skv, dy, py);
+    }
   }
   if (valid) pairbuf[pr] = make_double4(px, py, pz, pp);
 }
