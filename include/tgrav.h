@@ -215,7 +215,7 @@ int tgrav_set_softening(tgrav_ctx* ctx, int kernel, double rsoft, int flags);
  * TGRAV_FLASH_QUIRKS the potential of a cell is that of its last image in the reference's
  * loop order (it assigns, not adds, per image, Gravity_getPotentialAtPoint.F90:262-274).
  * on = 0 restores isolated boundaries.  nrep in [1, 64].  on = 1 sorts the (target, cell)
- * pairs by separation and walks each wave's image rows together where nrep <= 7 and there are
+ * pairs by separation and walks each wave's image rows together where nrep <= 10 and there are
  * at most 2^25 cells, in batches of targets of at most 2^25 pairs (it allocates ~40 B per pair
  * of a batch, up to 1.3 GB); on = 2 always walks each target's images on its own lane (the only
  * way above those limits).  Both give the reference's decisions. */

@@ -225,22 +225,23 @@ def test_flash_periodic_decisions_at_cutoff(field, per_target):
     np.testing.assert_allclose(got[3], ref[3], rtol=1e-12, atol=1e-12 * G / rs)
 
 
-@pytest.mark.parametrize("nrep", [7, 8])
-def test_flash_periodic_nrep_beyond_the_row_table(field, nrep):
-    """nrep = 7 fills the grouped walk's row table (15 x 15 rows); nrep = 8 has more rows than it
-    holds and takes the per-target walk by itself: both against the oracle."""
+@pytest.mark.parametrize("nrep,quirks", [(7, False), (8, False), (9, True), (10, False), (11, False)])
+def test_flash_periodic_nrep_beyond_the_row_table(field, nrep, quirks):
+    """nrep = 7 fills the grouped walk's static row table (15 x 15 rows); nrep 8 .. 10 size the table
+    in LDS at launch ((2 nrep + 1)^2 rows, up to 21 KB); nrep = 11 takes the per-target walk by
+    itself: all against the oracle."""
     rng = W.rng_for(47)
     box = (1.0, 1.1, 1.2)
     s = rng.uniform(0, 1, (30, 3)) * box
     t = rng.uniform(0, 1, (40, 3)) * box
     m = rng.uniform(0.5, 2.0, 30)
     rs = 0.12
-    field.set_softening("spline", rs, quirks=False)
+    field.set_softening("spline", rs, quirks=quirks)
     field.set_periodic(box, nrep=nrep)
     field.set_sources(s[:, 0], s[:, 1], s[:, 2], m, None)
     got = field.evaluate(None, t[:, 0], t[:, 1], t[:, 2])
     ref = oracle.eval_flash(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], G, rs,
-                            kernel="spline", quirks=False, box=box, nrep=nrep, threads=16)
+                            kernel="spline", quirks=quirks, box=box, nrep=nrep, threads=16)
     sa, sp = term_scales(s[:, 0], s[:, 1], s[:, 2], m, t[:, 0], t[:, 1], t[:, 2], rs, box=box, nrep=nrep)
     check(got, ref, sa, sp)
 

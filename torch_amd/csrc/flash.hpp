@@ -377,6 +377,10 @@ __global__ __launch_bounds__(NT) void k_eval_flash(Tgt T, const double4* __restr
 #endif
 #define TG_PAIR_SLACK 1e-4   // image spacings; the square roots (v_rsq_f64, ~2^-24) are off by < 1e-5
 #define TG_PAIR_ROWS ((2 * TG_PAIR_NREP + 1) * (2 * TG_PAIR_NREP + 1))
+// nrep 8 .. 10: a table of (2 nrep + 1)^2 rows in dynamic LDS (<= 21.2 KB).  On c1's inputs it ran at
+// 0.50 / 0.45 of FP64 peak at nrep 8 / 10 against 0.37 / 0.40 for the per-target walk; the gap
+// narrows as the table takes more LDS (fewer waves per CU), so larger nrep keep the per-target walk.
+#define TG_PAIR_NREP_DYN 10
 
 // The z images of a row: zu0 .. zu0 + nlo - 1 tested, then nc inside the cutoff for every pair of
 // the wave, then zh0 .. zh0 + nhi - 1 tested; soft: a row that may come within rsoft, its nlo images
@@ -447,14 +451,23 @@ __device__ __forceinline__ void sq_range(double lo, double hi, double c, double&
   mn = (a <= 0.0 && b >= 0.0) ? 0.0 : fmin(a2, b2);
 }
 
-// One wave per workgroup: sorted pairs [64 w, 64 w + 64) of `idx`.  The table's 225 rows (10.8 KB)
-// hold nrep <= 7; sizing it by nrep instead (8.1 KB at nrep = 6, more waves per CU) was 4% slower.
-template <bool QUIRKS>
+// One wave per workgroup: sorted pairs [64 w, 64 w + 64) of `idx`.  DYN = false: a static table of
+// 225 rows (10.8 KB) for nrep <= 7 (sizing it by nrep instead, 8.1 KB at nrep = 6 and more waves per
+// CU, was 4% slower); DYN = true: (2 nrep + 1)^2 rows of dynamic LDS, set at launch, for nrep 8 .. 10.
+template <bool QUIRKS, bool DYN>
 __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, const double4* __restrict__ src4,
                                                          int64_t ns, int64_t npairs,
                                                          const unsigned int* __restrict__ idx, Flash P,
                                                          double4* __restrict__ pairbuf) {
-  __shared__ PairRow rows[TG_PAIR_ROWS];
+  PairRow* rows;
+  if constexpr (DYN) {
+    extern __shared__ PairRow rows_dyn[];
+    rows = rows_dyn;
+  } else {
+    __shared__ PairRow rows_static[TG_PAIR_ROWS];
+    rows = rows_static;
+  }
+  const int ROWS = DYN ? (2 * P.nrep + 1) * (2 * P.nrep + 1) : TG_PAIR_ROWS;
   const int lane = threadIdx.x;
   const int64_t pi = (int64_t)blockIdx.x * 64 + lane;
   const bool valid = pi < npairs;
@@ -527,7 +540,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
     if (keep && !soft) rows[before] = R;
     if (soft) {
       R.before = before;
-      rows[TG_PAIR_ROWS - 1 - (nsoft + __builtin_popcountll(bs & lt))] = R;
+      rows[ROWS - 1 - (nsoft + __builtin_popcountll(bs & lt))] = R;
     }
     nrows += __builtin_popcountll(bh);
     nsoft += __builtin_popcountll(bs);
@@ -541,7 +554,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
   int r = 0;
   for (int k = 0; k <= nsoft; ++k) {
     int end = nrows;
-    if (k < nsoft) end = __builtin_amdgcn_readfirstlane(rows[TG_PAIR_ROWS - 1 - k].before);
+    if (k < nsoft) end = __builtin_amdgcn_readfirstlane(rows[ROWS - 1 - k].before);
     for (; r < end; ++r) {
       const PairRow R = rows[r];
       double dx, dy, xy;
@@ -591,7 +604,7 @@ __global__ __launch_bounds__(64) void k_eval_flash_pairs(Tgt T, int64_t t0, cons
       py = fma(skv, dy, py);
     }
     if (k < nsoft) {   // a soft row: the reference's test and kernel per image
-      const PairRow R = rows[TG_PAIR_ROWS - 1 - k];
+      const PairRow R = rows[ROWS - 1 - k];
       double dx, dy, xy;
       {
 #pragma clang fp contract(off)

@@ -830,8 +830,14 @@ static int launch_flash_pairs(tgrav_ctx* c, const tg::Tgt& T, const tg::Out& O, 
                                                c->stream));
     rc = timed_launch(c, nb, ns, [&]() -> int {
       const unsigned int nw = (unsigned int)((np + 63) / 64);
-      if (P.quirks) tg::k_eval_flash_pairs<true><<<nw, 64, 0, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
-      else tg::k_eval_flash_pairs<false><<<nw, 64, 0, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+      if (c->nrep <= TG_PAIR_NREP) {
+        if (P.quirks) tg::k_eval_flash_pairs<true, false><<<nw, 64, 0, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+        else tg::k_eval_flash_pairs<false, false><<<nw, 64, 0, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+      } else {   // nrep <= TG_PAIR_NREP_DYN (the caller checks): the table in dynamic LDS
+        const size_t lds = (size_t)(2 * c->nrep + 1) * (2 * c->nrep + 1) * sizeof(tg::PairRow);
+        if (P.quirks) tg::k_eval_flash_pairs<true, true><<<nw, 64, lds, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+        else tg::k_eval_flash_pairs<false, true><<<nw, 64, lds, c->stream>>>(T, t0, s4, ns, np, i1, P, pb);
+      }
       LAUNCHCHECK("k_eval_flash_pairs");
       return 0;
     });
@@ -872,10 +878,10 @@ static int launch_flash(tgrav_ctx* c, int mode, const tg::Tgt& T, const tg::Out&
   // branch); the FP32 seed is valid from 1e-30 up, and up to the tile's r^2 bound (in the kernels).
   P.s32 = TG_FP64_SEED32 && c->rsoft * c->rsoft > TG_F32_TINY_E2;
   P.reach = per ? (double)(c->nrep + 1) * std::max(P.lx, std::max(P.ly, P.lz)) : 0.0;
-  // Periodic sums with the pairs grouped by separation, where the row table fits (nrep <= 7), in
-  // batches of targets of at most 2^25 pairs (1.3 GB of pair arrays); tgrav_set_periodic(on = 2)
-  // keeps the per-target walk.
-  if (per && c->periodic == 1 && c->nrep <= TG_PAIR_NREP && nt > 0 && ns > 0 && ns <= kMaxPairs &&
+  // Periodic sums with the pairs grouped by separation, where the row table fits in LDS (nrep <= 7
+  // static, nrep <= 10 sized at launch), in batches of targets of at most 2^25 pairs (1.3 GB of pair
+  // arrays); tgrav_set_periodic(on = 2) keeps the per-target walk.
+  if (per && c->periodic == 1 && c->nrep <= TG_PAIR_NREP_DYN && nt > 0 && ns > 0 && ns <= kMaxPairs &&
       nt <= 0x7fffffffLL)
     return launch_flash_pairs(c, T, O, P);
   // Sources given as 8^3 blocks take the block-source kernel.  Explicit sources that may be

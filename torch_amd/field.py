@@ -386,7 +386,7 @@ class GravityField:
         """Periodic image sum of the FLASH kernels over n in [-nrep, nrep]^3 within
         nrep * min(box) (Gravity_getAccelAtPoint.F90:250-312); box=None: isolated.
         per_target=True keeps the per-target image walk instead of the pairs grouped by
-        separation (the default where nrep <= 7 and there are at most 2^25 cells; the targets
+        separation (the default where nrep <= 10 and there are at most 2^25 cells; the targets
         go in batches of at most 2^25 pairs)."""
         if box is None:
             _lib.check(self.lib.tgrav_set_periodic(self._ctx, 0, 0.0, 0.0, 0.0, 0), "tgrav_set_periodic")
